@@ -1,0 +1,129 @@
+/*
+ * mose_hip.h — C ABI of the MI355X (gfx950) explicit residual-and-update path.
+ *
+ * The reference (open-hydra/MOSE, Fortran) runs one explicit time step as
+ *   Compute_dt -> per RK stage { Fill_Ghost_Cell -> Fluxes -> BC_Fluxes -> RK_Newstate }
+ *   -> Compute_Residual
+ * (src/lib/numerics/time/explicit/Mod_Explicit.f90:10-146).  Its operator API for this
+ * path is a set of per-stage module calls on `type(MOSE_domain_type)` plus the
+ * `rlimiter` / `Riemann` / `RK_Newstate` procedure pointers bound at setup
+ * (src/lib/config/Assign_Setup.f90:41-56).  A per-face scalar callback cannot cross
+ * into a GPU efficiently, so this ABI replaces the whole stage loop: the Fortran host
+ * keeps MOSE's loop structure and calls these entry points through ISO_C_BINDING
+ * (mose_amd/fortran/mose_hip_iface.f90).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a nonzero MOSE_E_* code otherwise; the
+ *    block/cell of the failure is available from mose_last_error().
+ *  - Block numbers and cell indices are 1-based, as in the reference (b, i, j, k).
+ *  - Primitive state arrays use the reference layout of blk%P:
+ *    P(nprim, 1-gc:ni+gc, 1-gc:nj+gc, 1-gc:nk+gc), Fortran order, nprim=5, gc=2
+ *    (src/lib/driver/Mod_Allocate_Data.f90:104-136; src/lib/base/Global_m.f90:17).
+ *  - Node arrays use the layout of blk%node: (3, 0:ni, 0:nj, 0:nk), Fortran order.
+ *  - The library owns its device buffers; host pointers are only read or written for
+ *    the duration of a call.  A context is not thread-safe: call it from one thread.
+ *  - The gas is single-species and calorically perfect (gamma, R).
+ */
+#ifndef MOSE_HIP_H
+#define MOSE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Limiters: rlimiter_MINMOD/VANLEER/VANALBADA/IORD, src/lib/numerics/space/Lib_Limiters.f90:24-75 */
+enum { MOSE_LIM_MINMOD = 0, MOSE_LIM_VANLEER = 1, MOSE_LIM_VANALBADA = 2, MOSE_LIM_FIRST_ORDER = 3 };
+/* Riemann solvers: riemann_HLLC (Lib_Riemann_HLL.f90:147-258), riemann_AUSMp (Lib_Riemann_AUSM.f90:150-238) */
+enum { MOSE_RS_HLLC = 0, MOSE_RS_AUSMP = 1 };
+/* Arithmetic of reconstruction + Riemann flux; state, accumulation and reductions are always FP64. */
+enum { MOSE_PREC_FP64 = 0, MOSE_PREC_FP32_FLUX = 1 };
+
+/* Error codes (the reference `stop`s at the same places). */
+enum {
+  MOSE_OK = 0,
+  MOSE_E_ARG = 1,          /* bad argument / unsupported configuration                        */
+  MOSE_E_HIP = 2,          /* HIP runtime error                                                */
+  MOSE_E_STATE = 3,        /* NaN or p<0 after an update: Lib_Newstate.f90:334-338             */
+  MOSE_E_INTERFACE = 4,    /* non-physical reconstructed face state: Lib_Convective.f90:45-50  */
+  MOSE_E_METRIC = 5,       /* det=0 or non-positive volume: Lib_Metrics.f90:286-288,353-356    */
+  MOSE_E_BC = 6,           /* boundary list inconsistent with the blocks: IO_BC.f90:123-127    */
+  MOSE_E_NOGPU = 7         /* no HIP device                                                    */
+};
+
+typedef struct {
+  double  gamma;            /* ratio of specific heats                                         */
+  double  Rgas;             /* gas constant; cp = gamma R/(gamma-1)                            */
+  double  cfl;              /* [MOSE-Numerics] cfl                                             */
+  int32_t limiter;          /* [MOSE-Numerics] flux-limiter       (MOSE_LIM_*)                 */
+  int32_t riemann;          /* [MOSE-Numerics] riemann-solver     (MOSE_RS_*)                  */
+  int32_t n_rk;             /* [MOSE-Numerics] time-scheme: 1 euler, 2 RK2, 3 RK3              */
+  int32_t time_accurate;    /* [MOSE-Numerics] time-accurate: 1 global dt, 0 local dt          */
+  int32_t rampa_cfl_iter;   /* CFL ramp length in iterations (0: off), Mod_dt.f90:45           */
+  int32_t precision;        /* MOSE_PREC_*                                                     */
+  int32_t reserved[4];
+} mose_cfg;
+
+/* One boundary-face-cell record of bc.txt (src/lib/io/IO_BC.f90:136-414).
+ * type 101/201 use (bs,is,js,ks,fs,d11,d12,d21,d22); type 405 uses (mach,T,p,alpha,beta,rel_fac)
+ * with alpha/beta in radians.  Supported types: 0, 101, 201, 300, 400, 405. */
+typedef struct {
+  int32_t b, i, j, k, f, type;
+  int32_t bs, is, js, ks, fs;
+  int32_t d11, d12, d21, d22;
+  int32_t pad_;
+  double  mach, T, p, alpha, beta, rel_fac;
+} mose_bc_entry;
+
+typedef struct { int32_t code, block, i, j, k; } mose_err;
+
+typedef struct mose_ctx mose_ctx;
+
+/* Setup: replaces MOSE_setup's Setup_Data_Structure/Setup_BC/Setup_Metrics
+ * (src/lib/driver/Wrap_Setup.f90:60-82). */
+int  mose_create(const mose_cfg* cfg, mose_ctx** out);
+/* Allocate_Block (Mod_Allocate_Data.f90:104-136) for block b of nblocks; nodes as blk%node. */
+int  mose_add_block(mose_ctx* ctx, int32_t b, const int32_t dims[3], const double* nodes);
+/* Read_BCfile (IO_BC.f90:136-414) — the entries in file order. */
+int  mose_set_bc(mose_ctx* ctx, const mose_bc_entry* entries, int64_t n);
+/* Setup_Metrics (Mod_Metrics.f90:8-107) + device allocation; call once after blocks and BCs. */
+int  mose_finalize(mose_ctx* ctx);
+
+/* blk%P in / out (primitive, with ghost layers; ghosts are ignored on input). */
+int  mose_put_state(mose_ctx* ctx, int32_t b, const double* P);
+int  mose_get_state(mose_ctx* ctx, int32_t b, double* P);
+
+/* Compute_dt (Mod_dt.f90:10-107) preceded by dtglobal=1e5 (Mod_Explicit.f90:44), using
+ * iteration number `iter` for the CFL ramp; writes the global minimum dt. */
+int  mose_compute_dt(mose_ctx* ctx, int32_t iter, double* dtglobal);
+
+/* Explicit_Step (Mod_Explicit.f90:10-113): iter+=1, dt (+time+=dt when time-accurate),
+ * Copy_State, n_rk x {Fill_Ghost_Cell, Fluxes, BC_Fluxes, RK_Newstate}, Compute_Residual.
+ * dt_out / time_out / norms (5 values: |d rho|, |du|, |dv|, |dw|, |dp| L2) may be NULL. */
+int  mose_step(mose_ctx* ctx, double* dt_out, double* time_out, double* norms);
+
+/* nsteps calls of mose_step without host synchronisation in between (the step is a
+ * captured hipGraph).  norms (if not NULL) receives the last step's norms. */
+int  mose_advance(mose_ctx* ctx, int32_t nsteps, double* time_out, double* norms);
+
+/* One Fill_Ghost_Cell + Fluxes + BC_Fluxes on the current state, no update: the residual
+ * R(nprim, 1-gc:ni+gc, ...) of block b in the blk%R layout (interior cells filled, ghosts 0).
+ * This is the value the reference holds in blk%R right before RK_Newstate. */
+int  mose_residual(mose_ctx* ctx, int32_t b, double* R);
+
+/* Simulation clock and iteration counter (domain%time, domain%iter). */
+int  mose_get_time(mose_ctx* ctx, double* time, int32_t* iter);
+int  mose_set_time(mose_ctx* ctx, double time, int32_t iter);
+
+/* Total interior cells over all blocks. */
+int64_t mose_num_cells(const mose_ctx* ctx);
+
+int  mose_last_error(const mose_ctx* ctx, mose_err* err);
+const char* mose_strerror(int32_t code);
+void mose_destroy(mose_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOSE_HIP_H */

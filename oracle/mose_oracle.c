@@ -1,0 +1,1034 @@
+/*
+ * mose_oracle.c — CPU oracle of MOSE's explicit residual-and-update path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C FP64 restatement of the reference
+ * algorithm, used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+ * the checker.  The product (mose_amd/csrc, libmose_hip.so) never links or calls it.
+ *
+ * Each routine cites the reference routine it restates (paths relative to the reference
+ * repository root).  Data layout follows the reference's own arrays (AoS per cell,
+ * variable fastest, ghost depth gc=2) so that the restatement reads line by line against
+ * the Fortran, not like the GPU design.
+ *
+ * Thermodynamics: the reference takes every gas property from the FLINT submodule, which is
+ * absent from the reference snapshot.  This oracle uses a single-species calorically
+ * perfect gas (h = cp T, cp const).  With a table that is exactly linear in T the
+ * reference's table interpolation (roe_averages, Lib_Riemann_HLL.f90:581-626) reduces to
+ * these closed forms up to round-off; FLINT's own arithmetic is parity-unpinned.
+ *
+ * Parity pins (tests/test_oracle_*.py): limiter closed forms and brute-force tables;
+ * reconstruction exactness on linear data; Riemann consistency F(U,U)=physical flux,
+ * supersonic upwinding, stationary-contact preservation, n -> -n symmetry; free-stream
+ * preservation and discrete conservation of the composed residual on curvilinear and
+ * multi-block grids; block-decomposition invariance; metric exactness on affine grids;
+ * SSP-RK coefficient order test; Sod against the reference's committed exact.txt.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#include "mose_oracle.h"
+
+#define NV 5      /* nprim for one species: rho, u, v, w, p        (Mod_Allocate_Data.f90:22-26) */
+#define GC 2      /* ghost layers                                   (Global_m.f90:17)            */
+
+/* guide(6,3): inward offsets of the six faces (src/lib/base/Parameters_m.f90:10-15) */
+static const int guide[6][3] = {{1,0,0},{-1,0,0},{0,1,0},{0,-1,0},{0,0,1},{0,0,-1}};
+
+typedef struct {
+  int ni, nj, nk, pi, pj, pk;
+  int64_t npad;
+  double* node;                 /* (3, 0:ni, 0:nj, 0:nk)                                   */
+  double *P, *PO, *R;           /* (NV, padded cells)                                       */
+  double *fn[3], *fa[3];        /* dir(d)%f(i,j,k)%n / %a, face between cell i and i+1 ... */
+  double *dl, *M, *vol;         /* (3)/(9)/(1) per padded cell                              */
+  double* dt;                   /* dtlocal per padded cell                                  */
+} OBlk;
+
+struct ob_domain {
+  mose_cfg cfg;
+  double gam, Rg, cp, cv;
+  int nb;
+  OBlk* blk;
+  mose_bc_entry* bc;
+  int64_t nbc;
+  int ndir;
+  double dtglobal, time;
+  int iter;
+  mose_err err;
+};
+
+/* ------------------------------------------------------------------------------------ */
+static inline int64_t cidx(const OBlk* b, int i, int j, int k) {
+  return (int64_t)(i + GC - 1) + (int64_t)b->pi * ((int64_t)(j + GC - 1) + (int64_t)b->pj * (k + GC - 1));
+}
+static inline const double* node_at(const OBlk* b, int i, int j, int k) {
+  return b->node + 3 * ((int64_t)i + (int64_t)(b->ni + 1) * ((int64_t)j + (int64_t)(b->nj + 1) * k));
+}
+static int set_err(ob_domain* d, int code, int b, int i, int j, int k) {
+  if (d->err.code == 0) { d->err.code = code; d->err.block = b; d->err.i = i; d->err.j = j; d->err.k = k; }
+  return code;
+}
+
+/* ---------------------------- thermodynamics (FLINT shim) --------------------------- */
+static inline double th_T(const ob_domain* d, double p, double rho) { return p / (rho * d->Rg); }
+static inline double f_ss(const ob_domain* d, double p, double rho) { return sqrt(d->gam * p / rho); }
+/* H0(p, rho_s, |V|) = h(T) + |V|^2/2 */
+static inline double H0(const ob_domain* d, double p, double rho, double vmag) {
+  return d->cp * th_T(d, p, rho) + 0.5 * vmag * vmag;
+}
+static void prim2cons_d(const ob_domain* d, const double* P, double* U) {
+  double rho = P[0], T = th_T(d, P[4], rho);
+  U[0] = rho; U[1] = rho * P[1]; U[2] = rho * P[2]; U[3] = rho * P[3];
+  U[4] = rho * (d->cv * T + 0.5 * (P[1] * P[1] + P[2] * P[2] + P[3] * P[3]));
+}
+static void cons2prim_d(const ob_domain* d, const double* U, double* P) {
+  double rho = U[0], u = U[1] / rho, v = U[2] / rho, w = U[3] / rho;
+  double e = U[4] / rho - 0.5 * (u * u + v * v + w * w);
+  double T = e / d->cv;
+  P[0] = rho; P[1] = u; P[2] = v; P[3] = w; P[4] = rho * d->Rg * T;
+}
+
+/* ---------------------------------- limiters ----------------------------------------
+ * src/lib/numerics/space/Lib_Limiters.f90:24-75 */
+double ob_limiter(int32_t kind, double x, double y) {
+  double ru, phi, r;
+  switch (kind) {
+    case MOSE_LIM_MINMOD:
+      r = 0.0;
+      if (x * y < 0.0) return r;
+      r = x;
+      if (fabs(x) > fabs(y)) r = y;
+      return r;
+    case MOSE_LIM_VANLEER:
+      ru = y / (x + 1e-30);
+      phi = (ru + fabs(ru)) / (1.0 + fabs(ru));
+      return phi * x;
+    case MOSE_LIM_VANALBADA:
+      ru = y / (x + 1e-30);
+      phi = (ru * ru + ru) / (1.0 + ru * ru);
+      return phi * x;
+    default:
+      return 0.0;   /* rlimiter_IORD */
+  }
+}
+
+/* Reconstruction, src/lib/numerics/space/Lib_Reconstruction.f90:10-29 (beta = 1) */
+static inline void reconstruct(int kind, double prev, double local, double next, double next2,
+                               double dl0, double dl1, double dl2, double dll, double dlr,
+                               double* priml, double* primr) {
+  double slope0 = (local - prev) / dl0;
+  double slope1 = (next - local) / dl1;
+  double slope2 = (next2 - next) / dl2;
+  double slopel = ob_limiter(kind, slope1, slope0);
+  double sloper = ob_limiter(kind, slope2, slope1);
+  *priml = local + 1.0 * slopel * dll;
+  *primr = next - 1.0 * sloper * dlr;
+}
+void ob_reconstruct(int32_t kind, const double q[4], double l0, double l1, double l2,
+                    double lm, double lp, double* ql, double* qr) {
+  reconstruct(kind, q[0], q[1], q[2], q[3], l0, l1, l2, lm, lp, ql, qr);
+}
+
+/* -------------------------------- Riemann solvers ----------------------------------- */
+/* roe_averages, src/lib/numerics/fluxes/riemann/Lib_Riemann_HLL.f90:550-647, one species,
+ * tables h_tab = cp T and cp_tab = cp. */
+static void roe_averages(const ob_domain* d, double dl, double dr, double dltot, double drtot,
+                         double ul, double vl, double wl, double ur, double vr, double wr,
+                         double pl, double pr, double* h0l, double* h0r, double* rho_roe,
+                         double* u_roe, double* v_roe, double* w_roe, double* a_roe, double* h0_roe) {
+  double Rl = d->Rg, Rr = d->Rg;
+  double inv_dltot = 1.0 / dltot, inv_drtot = 1.0 / drtot;
+  double Tl = pl * inv_dltot / Rl, Tr = pr * inv_drtot / Rr;
+  double srL = sqrt(dltot), srR = sqrt(drtot), inv_sr = 1.0 / (srL + srR);
+  *rho_roe = srL * srR;
+  *u_roe = (srR * ur + srL * ul) * inv_sr;
+  *v_roe = (srR * vr + srL * vl) * inv_sr;
+  *w_roe = (srR * wr + srL * wl) * inv_sr;
+  double R_roe = (srR * Rr + srL * Rl) * inv_sr;
+  double T_roe = (srR * Tr + srL * Tl) * inv_sr;
+  double invW = d->Rg;
+  double hl = d->cp * Tl, hr = d->cp * Tr;
+  double el = hl - invW * Tl, er = hr - invW * Tr;
+  double hl0 = hl * dl * inv_dltot, hr0 = hr * dr * inv_drtot;
+  double d_roe = (srR * dr * inv_drtot + srL * dl * inv_dltot) * inv_sr;
+  double cv_roe = d_roe * (0.5 * (d->cp + d->cp) - invW);
+  double sum_ei = d_roe * (srR * er + srL * el) * inv_sr;
+  hl0 += 0.5 * (ul * ul + vl * vl + wl * wl);
+  hr0 += 0.5 * (ur * ur + vr * vr + wr * wr);
+  *h0l = hl0; *h0r = hr0;
+  *h0_roe = (srR * hr0 + srL * hl0) * inv_sr;
+  double gam_roe = 1.0 + R_roe / cv_roe;
+  double vel2 = (*u_roe) * (*u_roe) + (*v_roe) * (*v_roe) + (*w_roe) * (*w_roe);
+  *a_roe = sqrt((gam_roe - 1.0) * (*h0_roe - 0.5 * vel2 + cv_roe * T_roe - sum_ei));
+}
+
+/* riemann_HLLC (Batten), Lib_Riemann_HLL.f90:147-258 */
+static void riemann_hllc(const ob_domain* d, double dl, double ul, double vl, double wl, double pl,
+                         double al, double dltot, double dr, double ur, double vr, double wr,
+                         double pr, double ar, double drtot, double nx, double ny, double nz,
+                         double* F) {
+  double rho_roe, u_roe, v_roe, w_roe, a_roe, h0_roe, h0l, h0r;
+  double unl = ul * nx + vl * ny + wl * nz;
+  double unr = ur * nx + vr * ny + wr * nz;
+  roe_averages(d, dl, dr, dltot, drtot, ul, vl, wl, ur, vr, wr, pl, pr, &h0l, &h0r,
+               &rho_roe, &u_roe, &v_roe, &w_roe, &a_roe, &h0_roe);
+  double un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+  double e0l = h0l - pl / dltot, e0r = h0r - pr / drtot;
+  double S1 = fmin(0.0, fmin(unl - al, un_roe - a_roe));
+  double S4 = fmax(0.0, fmax(unr + ar, un_roe + a_roe));
+  double inv_denom = 1.0 / (dltot * (S1 - unl) - drtot * (S4 - unr));
+  double Sstar = (pr - pl + dltot * unl * (S1 - unl) - drtot * unr * (S4 - unr)) * inv_denom;
+  double pstar = dltot * (unl - S1) * (unl - Sstar) + pl;
+  double Fr, Fu, Fv, Fw, FE, U1S, U2S, U3S, U4S, U5S;
+  if (S1 >= 0.0) {
+    Fr = dltot * unl; Fu = Fr * ul + pl * nx; Fv = Fr * vl + pl * ny; Fw = Fr * wl + pl * nz; FE = Fr * h0l;
+  } else if (Sstar >= 0.0) {
+    Fr = dltot * unl; Fu = Fr * ul + pl * nx; Fv = Fr * vl + pl * ny; Fw = Fr * wl + pl * nz; FE = Fr * h0l;
+    inv_denom = 1.0 / (S1 - Sstar);
+    U1S = dltot * (S1 - unl) * inv_denom;
+    U2S = ((S1 - unl) * dltot * ul + (pstar - pl) * nx) * inv_denom;
+    U3S = ((S1 - unl) * dltot * vl + (pstar - pl) * ny) * inv_denom;
+    U4S = ((S1 - unl) * dltot * wl + (pstar - pl) * nz) * inv_denom;
+    U5S = ((S1 - unl) * dltot * e0l - pl * unl + pstar * Sstar) * inv_denom;
+    Fr += S1 * (U1S - dltot); Fu += S1 * (U2S - dltot * ul); Fv += S1 * (U3S - dltot * vl);
+    Fw += S1 * (U4S - dltot * wl); FE += S1 * (U5S - dltot * e0l);
+  } else if (S4 > 0.0) {
+    Fr = drtot * unr; Fu = Fr * ur + pr * nx; Fv = Fr * vr + pr * ny; Fw = Fr * wr + pr * nz; FE = Fr * h0r;
+    inv_denom = 1.0 / (S4 - Sstar);
+    U1S = drtot * (S4 - unr) * inv_denom;
+    U2S = ((S4 - unr) * drtot * ur + (pstar - pr) * nx) * inv_denom;
+    U3S = ((S4 - unr) * drtot * vr + (pstar - pr) * ny) * inv_denom;
+    U4S = ((S4 - unr) * drtot * wr + (pstar - pr) * nz) * inv_denom;
+    U5S = ((S4 - unr) * drtot * e0r - pr * unr + pstar * Sstar) * inv_denom;
+    Fr += S4 * (U1S - drtot); Fu += S4 * (U2S - drtot * ur); Fv += S4 * (U3S - drtot * vr);
+    Fw += S4 * (U4S - drtot * wr); FE += S4 * (U5S - drtot * e0r);
+  } else {
+    Fr = drtot * unr; Fu = Fr * ur + pr * nx; Fv = Fr * vr + pr * ny; Fw = Fr * wr + pr * nz; FE = Fr * h0r;
+  }
+  F[0] = Fr; F[1] = Fu; F[2] = Fv; F[3] = Fw; F[4] = FE;
+}
+
+/* riemann_AUSMp, Lib_Riemann_AUSM.f90:150-238 (a_half = sqrt(aL aR), :196) */
+static void riemann_ausmp(const ob_domain* d, double dl, double ul, double vl, double wl, double pl,
+                          double al, double dltot, double dr, double ur, double vr, double wr,
+                          double pr, double ar, double drtot, double nx, double ny, double nz,
+                          double* F) {
+  (void)dl; (void)dr;
+  const double beta = 0.125, alpha = 0.1875;
+  double unl = ul * nx + vl * ny + wl * nz;
+  double unr = ur * nx + vr * ny + wr * nz;
+  double H_L = H0(d, pl, dltot, sqrt(ul * ul + vl * vl + wl * wl));
+  double H_R = H0(d, pr, drtot, sqrt(ur * ur + vr * vr + wr * wr));
+  double a_half = sqrt(al * ar);
+  double ML = unl / a_half, MR = unr / a_half;
+  double MplusL, pplusL, MminusR, pminusR;
+  if (fabs(ML) < 1.0) {
+    MplusL = 0.25 * (ML + 1.0) * (ML + 1.0) + beta * (ML * ML - 1.0) * (ML * ML - 1.0);
+    pplusL = 0.25 * (ML + 1.0) * (ML + 1.0) * (2.0 - ML) + alpha * ML * (ML * ML - 1.0) * (ML * ML - 1.0);
+  } else {
+    MplusL = 0.5 * (ML + fabs(ML));
+    pplusL = 0.5 * (1.0 + copysign(1.0, ML));
+  }
+  if (fabs(MR) < 1.0) {
+    MminusR = -0.25 * (MR - 1.0) * (MR - 1.0) - beta * (MR * MR - 1.0) * (MR * MR - 1.0);
+    pminusR = 0.25 * (MR - 1.0) * (MR - 1.0) * (2.0 + MR) - alpha * MR * (MR * MR - 1.0) * (MR * MR - 1.0);
+  } else {
+    MminusR = 0.5 * (MR - fabs(MR));
+    pminusR = 0.5 * (1.0 - copysign(1.0, MR));
+  }
+  double Mhalf = MplusL + MminusR;
+  double phalf = pplusL * pl + pminusR * pr;
+  double mdotp = dltot * a_half * 0.5 * (Mhalf + fabs(Mhalf));
+  double mdotm = drtot * a_half * 0.5 * (Mhalf - fabs(Mhalf));
+  F[0] = mdotp + mdotm;
+  F[1] = mdotp * ul + mdotm * ur + nx * phalf;
+  F[2] = mdotp * vl + mdotm * vr + ny * phalf;
+  F[3] = mdotp * wl + mdotm * wr + nz * phalf;
+  F[4] = mdotp * H_L + mdotm * H_R;
+}
+
+/* The `Riemann` procedure pointer (Mod_Riemann.f90:9-28,32-114) */
+static void riemann(const ob_domain* d, const double* PL, double aL, double rL,
+                    const double* PR, double aR, double rR, const double* n, double* F) {
+  if (d->cfg.riemann == MOSE_RS_AUSMP)
+    riemann_ausmp(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
+                  n[0], n[1], n[2], F);
+  else
+    riemann_hllc(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
+                 n[0], n[1], n[2], F);
+}
+
+void ob_riemann(const mose_cfg* cfg, const double PL[5], const double PR[5], const double n[3], double F[5]) {
+  ob_domain d; memset(&d, 0, sizeof d);
+  d.cfg = *cfg; d.gam = cfg->gamma; d.Rg = cfg->Rgas; d.cp = d.gam * d.Rg / (d.gam - 1.0); d.cv = d.cp - d.Rg;
+  riemann(&d, PL, f_ss(&d, PL[4], PL[0]), PL[0], PR, f_ss(&d, PR[4], PR[0]), PR[0], n, F);
+}
+static void init_gas(ob_domain* d) {
+  d->gam = d->cfg.gamma; d->Rg = d->cfg.Rgas; d->cp = d->gam * d->Rg / (d->gam - 1.0); d->cv = d->cp - d->Rg;
+}
+void ob_prim2cons(const mose_cfg* cfg, const double P[5], double U[5]) {
+  ob_domain d; memset(&d, 0, sizeof d); d.cfg = *cfg; init_gas(&d); prim2cons_d(&d, P, U);
+}
+void ob_cons2prim(const mose_cfg* cfg, const double U[5], double P[5]) {
+  ob_domain d; memset(&d, 0, sizeof d); d.cfg = *cfg; init_gas(&d); cons2prim_d(&d, U, P);
+}
+
+/* check_gas_state (FLINT): non-physical if rho_s <= 0 or p <= 0 (assumed criterion) */
+static inline int gas_bad(const double* P) { return !(P[0] > 0.0) || !(P[4] > 0.0); }
+
+/* ------------------------------ Convective_Flux ------------------------------------
+ * src/lib/numerics/fluxes/Lib_Convective.f90:10-85; returns nonzero on interface error */
+static int convective_flux(const ob_domain* d, const double dl[4], const double* normal, double area,
+                           const double* Pm1, const double* P0, const double* P1, const double* P2,
+                           double* Res0, double* Res1) {
+  double l0 = 0.5 * (dl[0] + dl[1]);
+  double l1 = 0.5 * (dl[1] + dl[2]);
+  double l2 = 0.5 * (dl[2] + dl[3]);
+  double lm = 0.5 * dl[1];
+  double lp = 0.5 * dl[2];
+  double P1s[NV], P4s[NV], F[NV], Flux[NV];
+  for (int v = 0; v < NV; ++v)
+    reconstruct(d->cfg.limiter, Pm1[v], P0[v], P1[v], P2[v], l0, l1, l2, lm, lp, &P1s[v], &P4s[v]);
+  if (gas_bad(P1s) || gas_bad(P4s)) return MOSE_E_INTERFACE;
+  double rotot1 = P1s[0], rotot4 = P4s[0];
+  double a1 = f_ss(d, P1s[4], rotot1), a4 = f_ss(d, P4s[4], rotot4);
+  riemann(d, P1s, a1, rotot1, P4s, a4, rotot4, normal, F);
+  double su = copysign(0.5, F[0]);
+  double sp = 0.5 + su, sm = su - 0.5;
+  Flux[0] = F[0] * area * (sp * (P1s[0] / rotot1) - sm * (P4s[0] / rotot4));
+  Flux[1] = F[1] * area; Flux[2] = F[2] * area; Flux[3] = F[3] * area; Flux[4] = F[4] * area;
+  for (int v = 0; v < NV; ++v) { Res0[v] += Flux[v]; Res1[v] -= Flux[v]; }
+  return 0;
+}
+
+/* ---------------------------------- Fluxes_blk -------------------------------------
+ * src/lib/numerics/fluxes/Mod_Fluxes.f90:46-209 (inviscid, no shock detector) */
+static int fluxes_blk(ob_domain* d, int b) {
+  OBlk* B = &d->blk[b];
+  memset(B->R, 0, sizeof(double) * NV * B->npad);
+  int ni = B->ni, nj = B->nj, nk = B->nk, err = 0;
+  /* i sweep, :89-99 */
+  for (int k = 1; k <= nk; ++k)
+    for (int j = 1; j <= nj; ++j)
+      for (int i = 1; i <= ni - 1; ++i) {
+        double dl[4];
+        for (int s = 0; s < 4; ++s) dl[s] = B->dl[3 * cidx(B, i - 1 + s, j, k) + 0];
+        int64_t f = cidx(B, i, j, k);
+        int e = convective_flux(d, dl, &B->fn[0][3 * f], B->fa[0][f],
+                                &B->P[NV * cidx(B, i - 1, j, k)], &B->P[NV * cidx(B, i, j, k)],
+                                &B->P[NV * cidx(B, i + 1, j, k)], &B->P[NV * cidx(B, i + 2, j, k)],
+                                &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i + 1, j, k)]);
+        if (e && !err) { err = e; set_err(d, e, b + 1, i, j, k); }
+      }
+  /* j sweep, :129-139 */
+  for (int k = 1; k <= nk; ++k)
+    for (int i = 1; i <= ni; ++i)
+      for (int j = 1; j <= nj - 1; ++j) {
+        double dl[4];
+        for (int s = 0; s < 4; ++s) dl[s] = B->dl[3 * cidx(B, i, j - 1 + s, k) + 1];
+        int64_t f = cidx(B, i, j, k);
+        int e = convective_flux(d, dl, &B->fn[1][3 * f], B->fa[1][f],
+                                &B->P[NV * cidx(B, i, j - 1, k)], &B->P[NV * cidx(B, i, j, k)],
+                                &B->P[NV * cidx(B, i, j + 1, k)], &B->P[NV * cidx(B, i, j + 2, k)],
+                                &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i, j + 1, k)]);
+        if (e && !err) { err = e; set_err(d, e, b + 1, i, j, k); }
+      }
+  /* k sweep, :169-179 */
+  for (int j = 1; j <= nj; ++j)
+    for (int i = 1; i <= ni; ++i)
+      for (int k = 1; k <= nk - 1; ++k) {
+        double dl[4];
+        for (int s = 0; s < 4; ++s) dl[s] = B->dl[3 * cidx(B, i, j, k - 1 + s) + 2];
+        int64_t f = cidx(B, i, j, k);
+        int e = convective_flux(d, dl, &B->fn[2][3 * f], B->fa[2][f],
+                                &B->P[NV * cidx(B, i, j, k - 1)], &B->P[NV * cidx(B, i, j, k)],
+                                &B->P[NV * cidx(B, i, j, k + 1)], &B->P[NV * cidx(B, i, j, k + 2)],
+                                &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i, j, k + 1)]);
+        if (e && !err) { err = e; set_err(d, e, b + 1, i, j, k); }
+      }
+  return err;
+}
+
+/* Face_Index (Lib_BC_Fluxes.f90:22-45) -> dir (0-based) and face (i,j,k) */
+static inline void face_index(int f, int i, int j, int k, int* dir, int* fi, int* fj, int* fk) {
+  int m = f % 2;
+  *fi = i; *fj = j; *fk = k;
+  if (f <= 2) { *dir = 0; *fi = i - m; }
+  else if (f <= 4) { *dir = 1; *fj = j - m; }
+  else { *dir = 2; *fk = k - m; }
+}
+
+/* BC_Connection_Eul, Lib_BC_Fluxes_Connection.f90:13-114 */
+static int bc_connection(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm) {
+  int modfm = Fm % 2, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
+  const int* g = guide[Fm - 1];
+  int dir, fi, fj, fk;
+  face_index(Fm, Im, Jm, Km, &dir, &fi, &fj, &fk);
+  int I1 = modfm * (Im - g[0] * GC) + modfm1 * (Im + g[0]);
+  int J1 = modfm * (Jm - g[1] * GC) + modfm1 * (Jm + g[1]);
+  int K1 = modfm * (Km - g[2] * GC) + modfm1 * (Km + g[2]);
+  int I2 = modfm1 * Im + modfm * (Im - g[0]);
+  int J2 = modfm1 * Jm + modfm * (Jm - g[1]);
+  int K2 = modfm1 * Km + modfm * (Km - g[2]);
+  int I3 = modfm1 * (Im - g[0]) + modfm * Im;
+  int J3 = modfm1 * (Jm - g[1]) + modfm * Jm;
+  int K3 = modfm1 * (Km - g[2]) + modfm * Km;
+  int I4 = modfm1 * (Im - g[0] * GC) + modfm * (Im + g[0]);
+  int J4 = modfm1 * (Jm - g[1] * GC) + modfm * (Jm + g[1]);
+  int K4 = modfm1 * (Km - g[2] * GC) + modfm * (Km + g[2]);
+  const double* Pr1 = &B->P[NV * cidx(B, I1, J1, K1)];
+  const double* Pr2 = &B->P[NV * cidx(B, I2, J2, K2)];
+  const double* Pr3 = &B->P[NV * cidx(B, I3, J3, K3)];
+  const double* Pr4 = &B->P[NV * cidx(B, I4, J4, K4)];
+  int64_t f = cidx(B, fi, fj, fk);
+  const double* normal = &B->fn[dir][3 * f];
+  double area = B->fa[dir][f];
+  double Dl1 = B->dl[3 * cidx(B, I1, J1, K1) + dir] * 0.5;
+  double Dl2 = B->dl[3 * cidx(B, I2, J2, K2) + dir] * 0.5;
+  double Dl3 = B->dl[3 * cidx(B, I3, J3, K3) + dir] * 0.5;
+  double Dl4 = B->dl[3 * cidx(B, I4, J4, K4) + dir] * 0.5;
+  double Dist23 = Dl2 + Dl3, Dist12 = Dl1 + Dl2, Dist34 = Dl3 + Dl4;
+  double PL[NV], PR[NV], F[NV], Flux[NV];
+  for (int v = 0; v < NV; ++v)
+    reconstruct(d->cfg.limiter, Pr1[v], Pr2[v], Pr3[v], Pr4[v], Dist12, Dist23, Dist34, Dl2, Dl3, &PL[v], &PR[v]);
+  double rho_L = PL[0], rho_R = PR[0];
+  double a_L = f_ss(d, PL[4], rho_L), a_R = f_ss(d, PR[4], rho_R);
+  riemann(d, PL, a_L, rho_L, PR, a_R, rho_R, normal, F);
+  double su = copysign(0.5, F[0]);
+  double sel_L = 0.5 + su, sel_R = su - 0.5;
+  double Fmass = F[0] * area;
+  Flux[0] = Fmass * (sel_L * PL[0] / rho_L - sel_R * PR[0] / rho_R);
+  Flux[1] = F[1] * area; Flux[2] = F[2] * area; Flux[3] = F[3] * area; Flux[4] = F[4] * area;
+  double* R = &B->R[NV * cidx(B, Im, Jm, Km)];
+  for (int v = 0; v < NV; ++v) R[v] += Flux[v] * modfm2;
+  return 0;
+}
+
+/* BC_Extrapolation, Lib_BC_Fluxes_Extrapolation.f90:13-47 */
+static void bc_extrapolation(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm) {
+  int dir, fi, fj, fk;
+  face_index(Fm, Im, Jm, Km, &dir, &fi, &fj, &fk);
+  int64_t f = cidx(B, fi, fj, fk);
+  const double* n = &B->fn[dir][3 * f];
+  double area = B->fa[dir][f];
+  const double* P = &B->P[NV * cidx(B, Im, Jm, Km)];
+  double Un = P[1] * n[0] + P[2] * n[1] + P[3] * n[2];
+  double Flux[NV];
+  Flux[0] = P[0] * Un * area;
+  double Fmass = Flux[0];
+  Flux[1] = Fmass * P[1] + P[4] * area * n[0];
+  Flux[2] = Fmass * P[2] + P[4] * area * n[1];
+  Flux[3] = Fmass * P[3] + P[4] * area * n[2];
+  Flux[4] = Fmass * H0(d, P[4], P[0], sqrt(P[1] * P[1] + P[2] * P[2] + P[3] * P[3]));
+  int sign = 1 - 2 * (Fm % 2);
+  double* R = &B->R[NV * cidx(B, Im, Jm, Km)];
+  for (int v = 0; v < NV; ++v) R[v] += sign * Flux[v];
+}
+
+/* BC_Symmetry_Eul, Lib_BC_Fluxes_Symmetry.f90:14-125 (non-rotating frame: Un_wall = 0) */
+static void bc_symmetry(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm) {
+  int modfm = Fm % 2, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
+  const int* g = guide[Fm - 1];
+  int Ii = Im + g[0], Ji = Jm + g[1], Ki = Km + g[2];
+  int dir, fi, fj, fk;
+  face_index(Fm, Im, Jm, Km, &dir, &fi, &fj, &fk);
+  int64_t f = cidx(B, fi, fj, fk);
+  const double* n = &B->fn[dir][3 * f];
+  double area = B->fa[dir][f];
+  double Bound_Dl = 0.5 * B->dl[3 * cidx(B, Im, Jm, Km) + dir];
+  double Int_Dl = 0.5 * B->dl[3 * cidx(B, Ii, Ji, Ki) + dir];
+  const double* Bp = &B->P[NV * cidx(B, Im, Jm, Km)];
+  const double* Ip = &B->P[NV * cidx(B, Ii, Ji, Ki)];
+  double Un = Bp[1] * n[0] + Bp[2] * n[1] + Bp[3] * n[2];
+  double Gp[NV], P1[NV], P3[NV], Slope[NV], Face[NV];
+  for (int v = 0; v < NV; ++v) Gp[v] = Bp[v];
+  for (int c = 0; c < 3; ++c) Gp[1 + c] = Gp[1 + c] - 2.0 * Un * n[c];
+  for (int v = 0; v < NV; ++v) {
+    P1[v] = modfm * Gp[v] + modfm1 * Ip[v];
+    P3[v] = modfm * Ip[v] + modfm1 * Gp[v];
+  }
+  double Dist12 = 2.0 * Bound_Dl * modfm + (Bound_Dl + Int_Dl) * modfm1;
+  double Dist23 = (Bound_Dl + Int_Dl) * modfm + 2.0 * Bound_Dl * modfm1;
+  for (int v = 0; v < NV; ++v) {
+    double s32 = (P3[v] - Bp[v]) / Dist23;
+    double s21 = (Bp[v] - P1[v]) / Dist12;
+    Slope[v] = ob_limiter(d->cfg.limiter, s32, s21);   /* s = 1..np (all five here) */
+  }
+  double Dist = Bound_Dl * modfm2;
+  for (int v = 0; v < NV; ++v) Face[v] = Bp[v] + Slope[v] * Dist;
+  double rotot = Face[0];
+  double Sound = f_ss(d, Face[4], rotot);
+  double Unf = Face[1] * n[0] + Face[2] * n[1] + Face[3] * n[2];
+  double Velt[3], U1[3], U4[3];
+  for (int c = 0; c < 3; ++c) Velt[c] = Face[1 + c] - Unf * n[c];
+  double Un1 = Unf * modfm2;
+  for (int c = 0; c < 3; ++c) { U1[c] = Un1 * n[c] + Velt[c]; U4[c] = -Un1 * n[c] + Velt[c]; }
+  double PL[NV] = {Face[0], U1[0], U1[1], U1[2], Face[4]};
+  double PR[NV] = {Face[0], U4[0], U4[1], U4[2], Face[4]};
+  double F[NV];
+  riemann(d, PL, Sound, rotot, PR, Sound, rotot, n, F);
+  double* R = &B->R[NV * cidx(B, Im, Jm, Km)];
+  R[1] += modfm2 * (F[1] * area);
+  R[2] += modfm2 * (F[2] * area);
+  R[3] += modfm2 * (F[3] * area);
+}
+
+/* BC_Inlet_Supersonic_Static + Supersonic_Inflow, Lib_BC_Fluxes_Inflow.f90:244-382.
+ * Returns 1 (error) when the flow leaves through the face; the caller then applies the
+ * symmetry flux (Mod_BC_Fluxes.f90:159-168). */
+static int bc_inflow405(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm, const mose_bc_entry* e) {
+  int modfm = Fm % 2, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
+  const int* g = guide[Fm - 1];
+  int Ii = Im + g[0], Ji = Jm + g[1], Ki = Km + g[2];
+  int dir, fi, fj, fk;
+  face_index(Fm, Im, Jm, Km, &dir, &fi, &fj, &fk);
+  int64_t f = cidx(B, fi, fj, fk);
+  const double* n = &B->fn[dir][3 * f];
+  double area = B->fa[dir][f];
+  const double* P2 = &B->P[NV * cidx(B, Im, Jm, Km)];
+  const double* Ip = &B->P[NV * cidx(B, Ii, Ji, Ki)];
+  double Un = P2[1] * n[0] + P2[2] * n[1] + P2[3] * n[2];
+  double BC_Sign = (double)modfm2;
+  if (BC_Sign * Un >= 0.0) return 1;
+  double flow_Sound = f_ss(d, P2[4], P2[0]);
+  double flow_Mach = sqrt(P2[1] * P2[1] + P2[2] * P2[2] + P2[3] * P2[3]) / flow_Sound;
+  double Mach_l = e->rel_fac * e->mach + (1.0 - e->rel_fac) * flow_Mach;
+  double p_l = e->rel_fac * e->p + (1.0 - e->rel_fac) * P2[4];
+  double alpha = e->alpha, beta = e->beta;   /* explicit angles (Compute_Inflow_Direction :479-496) */
+  double Sup[NV];
+  Sup[4] = p_l;
+  double Sup_rho = Sup[4] / (d->Rg * e->T);
+  Sup[0] = Sup_rho;
+  double Sup_Sound = f_ss(d, Sup[4], Sup_rho);
+  Sup[1] = Mach_l * Sup_Sound * cos(alpha) * cos(beta);
+  Sup[2] = Mach_l * Sup_Sound * sin(alpha) * cos(beta);
+  Sup[3] = Mach_l * Sup_Sound * sin(beta);
+  double P1[NV], P3[NV], Slope[NV], Face[NV], PL[NV], PR[NV], F[NV];
+  for (int v = 0; v < NV; ++v) {
+    P1[v] = modfm * Sup[v] + modfm1 * Ip[v];
+    P3[v] = modfm * Ip[v] + modfm1 * Sup[v];
+  }
+  for (int v = 0; v < NV; ++v) {
+    double d32 = P3[v] - P2[v], d21 = P2[v] - P1[v];
+    Slope[v] = ob_limiter(d->cfg.limiter, d32, d21);
+    Face[v] = P2[v] + 0.5 * Slope[v] * modfm2;
+  }
+  for (int v = 0; v < NV; ++v) {
+    PL[v] = modfm * Sup[v] + modfm1 * Face[v];
+    PR[v] = modfm * Face[v] + modfm1 * Sup[v];
+  }
+  double rho_L = PL[0], rho_R = PR[0];
+  riemann(d, PL, f_ss(d, PL[4], rho_L), rho_L, PR, f_ss(d, PR[4], rho_R), rho_R, n, F);
+  double su = copysign(1.0, F[0]);
+  double Sel_L = (1.0 + su) / 2.0, Sel_R = (su - 1.0) / 2.0;
+  double Fmass = F[0] * area;
+  double Flux[NV];
+  Flux[0] = Fmass * (Sel_L * PL[0] / rho_L - Sel_R * PR[0] / rho_R);
+  Flux[1] = F[1] * area; Flux[2] = F[2] * area; Flux[3] = F[3] * area; Flux[4] = F[4] * area;
+  double* R = &B->R[NV * cidx(B, Im, Jm, Km)];
+  for (int v = 0; v < NV; ++v) R[v] += modfm2 * Flux[v];
+  return 0;
+}
+
+/* BC_Fluxes dispatcher, src/lib/numerics/fluxes/bc/Mod_BC_Fluxes.f90:10-218 */
+static int bc_fluxes(ob_domain* d) {
+  for (int64_t q = 0; q < d->nbc; ++q) {
+    const mose_bc_entry* e = &d->bc[q];
+    OBlk* B = &d->blk[e->b - 1];
+    switch (e->type) {
+      case 101: case 201: bc_connection(d, B, e->i, e->j, e->k, e->f); break;
+      case 300: bc_symmetry(d, B, e->i, e->j, e->k, e->f); break;
+      case 400: bc_extrapolation(d, B, e->i, e->j, e->k, e->f); break;
+      case 405:
+        if (bc_inflow405(d, B, e->i, e->j, e->k, e->f, e)) bc_symmetry(d, B, e->i, e->j, e->k, e->f);
+        break;
+      case 0: break;
+      default: return set_err(d, MOSE_E_BC, e->b, e->i, e->j, e->k);
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------- Fill_Ghost_Cell ----------------------------------
+ * src/lib/numerics/space/Lib_Ghost.f90:11-137 (serial: entries in bc-list order) */
+static void ghost_connection(ob_domain* d, const mose_bc_entry* e) {      /* :140-160 */
+  OBlk* Bm = &d->blk[e->b - 1];
+  OBlk* Bs = &d->blk[e->bs - 1];
+  const int* gm = guide[e->f - 1];
+  const int* gs = guide[e->fs - 1];
+  for (int g = 1; g <= GC; ++g) {
+    int Ig = e->i - gm[0] * g, Jg = e->j - gm[1] * g, Kg = e->k - gm[2] * g;
+    const double* src = &Bs->P[NV * cidx(Bs, e->is + gs[0] * (g - 1), e->js + gs[1] * (g - 1), e->ks + gs[2] * (g - 1))];
+    double* dst = &Bm->P[NV * cidx(Bm, Ig, Jg, Kg)];
+    for (int v = 0; v < NV; ++v) dst[v] = src[v];
+  }
+}
+static void ghost_symmetry(OBlk* B, int Im, int Jm, int Km, int Fm) {     /* :193-225 */
+  const int* g = guide[Fm - 1];
+  int dir, fi, fj, fk;
+  face_index(Fm, Im, Jm, Km, &dir, &fi, &fj, &fk);
+  const double* n = &B->fn[dir][3 * cidx(B, fi, fj, fk)];
+  const double* P = &B->P[NV * cidx(B, Im, Jm, Km)];
+  double* G = &B->P[NV * cidx(B, Im - g[0], Jm - g[1], Km - g[2])];
+  double vel[3] = {P[1], P[2], P[3]};
+  double udotn = vel[0] * n[0] + vel[1] * n[1] + vel[2] * n[2];
+  for (int c = 0; c < 3; ++c) vel[c] = vel[c] - 2.0 * udotn * n[c];
+  G[1] = vel[0]; G[2] = vel[1]; G[3] = vel[2]; G[0] = P[0]; G[4] = P[4];
+}
+static void ghost_zg(OBlk* B, int Im, int Jm, int Km, int Fm) {            /* :279-304 */
+  const int* g = guide[Fm - 1];
+  const double* P = &B->P[NV * cidx(B, Im, Jm, Km)];
+  for (int l = 1; l <= GC; ++l) {
+    double* G = &B->P[NV * cidx(B, Im - g[0] * l, Jm - g[1] * l, Km - g[2] * l)];
+    for (int v = 0; v < NV; ++v) G[v] = P[v];
+  }
+}
+static void ghost_extrapolate(OBlk* B, int Im, int Jm, int Km, int Fm) {   /* :363-402 */
+  const int* g = guide[Fm - 1];
+  int s0 = -g[0], s1 = -g[1], s2 = -g[2];     /* outward step */
+  for (int l = 1; l <= GC; ++l) {
+    double* G = &B->P[NV * cidx(B, Im + s0 * l, Jm + s1 * l, Km + s2 * l)];
+    const double* a = &B->P[NV * cidx(B, Im + s0 * (l - 1), Jm + s1 * (l - 1), Km + s2 * (l - 1))];
+    const double* b = &B->P[NV * cidx(B, Im + s0 * (l - 2), Jm + s1 * (l - 2), Km + s2 * (l - 2))];
+    const double* c = &B->P[NV * cidx(B, Im + s0 * (l - 3), Jm + s1 * (l - 3), Km + s2 * (l - 3))];
+    for (int v = 0; v < NV; ++v) G[v] = 3.0 * a[v] - 3.0 * b[v] + c[v];
+  }
+}
+int ob_fill_ghost(ob_domain* d) {
+  for (int64_t q = 0; q < d->nbc; ++q) {
+    const mose_bc_entry* e = &d->bc[q];
+    OBlk* B = &d->blk[e->b - 1];
+    switch (e->type) {
+      case 101: case 201: ghost_connection(d, e); break;
+      case 300: ghost_symmetry(B, e->i, e->j, e->k, e->f); break;
+      case 0: case 401: case 402: case 403: case 404: case 405: case 406: case 407: case 420:
+        ghost_zg(B, e->i, e->j, e->k, e->f); break;
+      default: ghost_extrapolate(B, e->i, e->j, e->k, e->f); break;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------ metrics ---------------------------------------- */
+/* Compute_Metric_Tensor, src/lib/numerics/space/Lib_Metrics.f90:250-376 */
+static double tvol(const double* vx, const double* vy, const double* vz, int i1, int i2, int i3, int i4) {
+  --i1; --i2; --i3; --i4;
+  return fabs(((vx[i2] - vx[i1]) * ((vy[i3] - vy[i1]) * (vz[i4] - vz[i1]) - (vy[i4] - vy[i1]) * (vz[i3] - vz[i1])) +
+               (vy[i2] - vy[i1]) * ((vx[i4] - vx[i1]) * (vz[i3] - vz[i1]) - (vx[i3] - vx[i1]) * (vz[i4] - vz[i1])) +
+               (vz[i2] - vz[i1]) * ((vx[i3] - vx[i1]) * (vy[i4] - vy[i1]) - (vx[i4] - vx[i1]) * (vy[i3] - vy[i1]))) / 6.0);
+}
+static int metric_tensor(const double* N[8], double* M, double* dl, double* vol) {
+  double A[3][3];
+  for (int c = 0; c < 3; ++c) {
+    A[0][c] = N[4][c] - N[0][c] + N[5][c] - N[1][c] + N[6][c] - N[2][c] + N[7][c] - N[3][c];
+    A[1][c] = N[2][c] - N[0][c] + N[3][c] - N[1][c] + N[6][c] - N[4][c] + N[7][c] - N[5][c];
+    A[2][c] = N[1][c] - N[0][c] + N[3][c] - N[2][c] + N[5][c] - N[4][c] + N[7][c] - N[6][c];
+  }
+  for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) A[r][c] *= 0.25;
+  double det = A[0][0] * A[1][1] * A[2][2] - A[0][0] * A[1][2] * A[2][1] - A[0][1] * A[1][0] * A[2][2] +
+               A[0][1] * A[1][2] * A[2][0] + A[0][2] * A[1][0] * A[2][1] - A[0][2] * A[1][1] * A[2][0];
+  if (fabs(det) == 0.0) return MOSE_E_METRIC;
+  double cf[3][3];
+  cf[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]);
+  cf[0][1] = -(A[1][0] * A[2][2] - A[1][2] * A[2][0]);
+  cf[0][2] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+  cf[1][0] = -(A[0][1] * A[2][2] - A[0][2] * A[2][1]);
+  cf[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]);
+  cf[1][2] = -(A[0][0] * A[2][1] - A[0][1] * A[2][0]);
+  cf[2][0] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]);
+  cf[2][1] = -(A[0][0] * A[1][2] - A[0][2] * A[1][0]);
+  cf[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]);
+  for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) M[3 * r + c] = cf[r][c] / det;
+  for (int h = 0; h < 3; ++h) dl[h] = sqrt(A[h][0] * A[h][0] + A[h][1] * A[h][1] + A[h][2] * A[h][2]);
+  /* vertex order of the 5-tetrahedra split: 1,5,3,7,2,6,4,8 */
+  static const int ord[8] = {0, 4, 2, 6, 1, 5, 3, 7};
+  double vx[8], vy[8], vz[8];
+  for (int q = 0; q < 8; ++q) { vx[q] = N[ord[q]][0]; vy[q] = N[ord[q]][1]; vz[q] = N[ord[q]][2]; }
+  *vol = tvol(vx, vy, vz, 1, 2, 3, 5) + tvol(vx, vy, vz, 2, 4, 3, 8) + tvol(vx, vy, vz, 5, 8, 6, 2) +
+         tvol(vx, vy, vz, 5, 7, 8, 3) + tvol(vx, vy, vz, 5, 8, 2, 3);
+  if (*vol <= 0.0) return MOSE_E_METRIC;
+  return 0;
+}
+
+/* Compute_Norm_Area, Lib_Metrics.f90:69-247 */
+static void cross_half(const double* d1, const double* d2, double* d3, double s) {
+  d3[0] = (d1[1] * d2[2] - d1[2] * d2[1]) * s;
+  d3[1] = (d1[2] * d2[0] - d1[0] * d2[2]) * s;
+  d3[2] = (d1[0] * d2[1] - d1[1] * d2[0]) * s;
+}
+static void sub3(const double* a, const double* b, double* c) { c[0] = a[0] - b[0]; c[1] = a[1] - b[1]; c[2] = a[2] - b[2]; }
+static double face_sign(const OBlk* B, int dir) {
+  double d1[3], d2[3], d3[3], c1[3], c2[3];
+  const double* (*N)(const OBlk*, int, int, int) = node_at;
+  if (dir == 0) { sub3(N(B,1,1,1), N(B,1,0,0), d1); sub3(N(B,1,0,1), N(B,1,1,0), d2); }
+  else if (dir == 1) { sub3(N(B,0,1,1), N(B,1,1,0), d1); sub3(N(B,1,1,1), N(B,0,1,0), d2); }
+  else { sub3(N(B,0,1,1), N(B,1,0,1), d1); sub3(N(B,0,0,1), N(B,1,1,1), d2); }
+  cross_half(d1, d2, d3, 1.0);
+  static const int hi[3][4][3] = {{{1,1,1},{1,0,1},{1,0,0},{1,1,0}}, {{1,1,1},{0,1,1},{0,1,0},{1,1,0}}, {{1,1,1},{0,1,1},{0,0,1},{1,0,1}}};
+  static const int lo[3][4][3] = {{{0,1,1},{0,0,1},{0,0,0},{0,1,0}}, {{1,0,1},{0,0,1},{0,0,0},{1,0,0}}, {{1,1,0},{0,1,0},{0,0,0},{1,0,0}}};
+  for (int c = 0; c < 3; ++c) {
+    c1[c] = 0.25 * (N(B, hi[dir][0][0], hi[dir][0][1], hi[dir][0][2])[c] + N(B, hi[dir][1][0], hi[dir][1][1], hi[dir][1][2])[c] +
+                    N(B, hi[dir][2][0], hi[dir][2][1], hi[dir][2][2])[c] + N(B, hi[dir][3][0], hi[dir][3][1], hi[dir][3][2])[c]);
+    c2[c] = 0.25 * (N(B, lo[dir][0][0], lo[dir][0][1], lo[dir][0][2])[c] + N(B, lo[dir][1][0], lo[dir][1][1], lo[dir][1][2])[c] +
+                    N(B, lo[dir][2][0], lo[dir][2][1], lo[dir][2][2])[c] + N(B, lo[dir][3][0], lo[dir][3][1], lo[dir][3][2])[c]);
+  }
+  double scal = (c1[0] - c2[0]) * d3[0] + (c1[1] - c2[1]) * d3[1] + (c1[2] - c2[2]) * d3[2];
+  return copysign(1.0, scal);
+}
+static void norm_area(OBlk* B) {
+  double sgn[3] = {face_sign(B, 0), face_sign(B, 1), face_sign(B, 2)};
+  for (int dir = 0; dir < 3; ++dir) {
+    int i0 = dir == 0 ? 0 : 1, j0 = dir == 1 ? 0 : 1, k0 = dir == 2 ? 0 : 1;
+    for (int k = k0; k <= B->nk; ++k)
+      for (int j = j0; j <= B->nj; ++j)
+        for (int i = i0; i <= B->ni; ++i) {
+          double d1[3], d2[3], d3[3];
+          if (dir == 0) { sub3(node_at(B, i, j, k), node_at(B, i, j - 1, k - 1), d1); sub3(node_at(B, i, j - 1, k), node_at(B, i, j, k - 1), d2); }
+          else if (dir == 1) { sub3(node_at(B, i - 1, j, k), node_at(B, i, j, k - 1), d1); sub3(node_at(B, i, j, k), node_at(B, i - 1, j, k - 1), d2); }
+          else { sub3(node_at(B, i - 1, j, k), node_at(B, i, j - 1, k), d1); sub3(node_at(B, i - 1, j - 1, k), node_at(B, i, j, k), d2); }
+          cross_half(d1, d2, d3, 0.5);
+          double A = sqrt(d3[0] * d3[0] + d3[1] * d3[1] + d3[2] * d3[2]);
+          int64_t f = cidx(B, i, j, k);
+          double* n = &B->fn[dir][3 * f];
+          if (A == 0.0) { n[0] = n[1] = n[2] = 0.0; }
+          else { n[0] = d3[0] / A * sgn[dir]; n[1] = d3[1] / A * sgn[dir]; n[2] = d3[2] / A * sgn[dir]; }
+          B->fa[dir][f] = A;
+        }
+  }
+}
+
+static int cell_metric(OBlk* B, int i, int j, int k, const double* N[8]) {
+  int64_t c = cidx(B, i, j, k);
+  return metric_tensor(N, &B->M[9 * c], &B->dl[3 * c], &B->vol[c]);
+}
+
+/* BC_Extrapolate_Metrics, Lib_Metrics.f90:535-797 (delthe = 0: no 2-D axisymmetry) */
+static int extrapolate_metrics(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm) {
+  int dir = (Fm - 1) / 2, hi = (Fm % 2 == 0);
+  int cubic;
+  if (dir == 0) cubic = 1;
+  else if (dir == 1) cubic = B->nj > 2;
+  else cubic = (d->ndir == 3);
+  /* the four corner columns of the face: the two tangential node offsets (0/1) */
+  double plane[4][5][3];   /* [corner][plane index 0..4 -> g=-2..2][xyz] */
+  int tdir[2], t = 0;
+  for (int e = 0; e < 3; ++e) if (e != dir) tdir[t++] = e;
+  int base[3] = {Im - 1, Jm - 1, Km - 1};   /* node of corner (0,0,0) of cell (Im,Jm,Km) */
+  for (int c = 0; c < 4; ++c) {
+    int off[3] = {0, 0, 0};
+    off[tdir[0]] = (c >> 1) & 1; off[tdir[1]] = c & 1;
+    int nknown = cubic ? GC + 1 : 2;
+    for (int g = 0; g < nknown; ++g) {
+      int q[3] = {base[0] + off[0], base[1] + off[1], base[2] + off[2]};
+      /* low side: node planes Im-1+g; high side: planes Im+(-g) */
+      q[dir] = hi ? (base[dir] + 1 - g) : (base[dir] + g);
+      const double* p = node_at(B, q[0], q[1], q[2]);
+      for (int x = 0; x < 3; ++x) plane[c][2 + (hi ? -g : g)][x] = p[x];
+    }
+    for (int g = 1; g <= GC; ++g) {
+      int s = hi ? 1 : -1;               /* outward */
+      int o = 2 + s * g;                 /* slot of the new plane */
+      for (int x = 0; x < 3; ++x) {
+        if (cubic) plane[c][o][x] = 3.0 * plane[c][o - s][x] - 3.0 * plane[c][o - 2 * s][x] + plane[c][o - 3 * s][x];
+        else plane[c][o][x] = 2.0 * plane[c][o - s][x] - plane[c][o - 2 * s][x];
+      }
+    }
+  }
+  for (int g = 1; g <= GC; ++g) {
+    int s = hi ? 1 : -1;
+    /* ghost cell g lies between planes (slot) lo and lo+1 */
+    int lo = hi ? 2 + g - 1 : 2 - g;
+    const double* N[8];
+    for (int n8 = 0; n8 < 8; ++n8) {
+      int a[3] = {(n8 >> 2) & 1, (n8 >> 1) & 1, n8 & 1};     /* corner bits (i, j, k) */
+      int c = (a[tdir[0]] << 1) | a[tdir[1]];
+      N[n8] = plane[c][lo + a[dir]];
+    }
+    int cell[3] = {Im, Jm, Km};
+    cell[dir] += s * g;
+    int e = cell_metric(B, cell[0], cell[1], cell[2], N);
+    if (e) return set_err(d, e, 0, cell[0], cell[1], cell[2]);
+  }
+  return 0;
+}
+/* BC_Symmetry_Metrics, Lib_Metrics.f90:800-858 */
+static void symmetry_metrics(OBlk* B, int Im, int Jm, int Km, int Fm) {
+  const int* gd = guide[Fm - 1];
+  for (int g = 1; g <= GC; ++g) {
+    int64_t dst = cidx(B, Im - gd[0] * g, Jm - gd[1] * g, Km - gd[2] * g);
+    int64_t src = cidx(B, Im + gd[0] * (g - 1), Jm + gd[1] * (g - 1), Km + gd[2] * (g - 1));
+    memcpy(&B->M[9 * dst], &B->M[9 * src], 9 * sizeof(double));
+    memcpy(&B->dl[3 * dst], &B->dl[3 * src], 3 * sizeof(double));
+    B->vol[dst] = B->vol[src];
+  }
+}
+/* BC_Connect_Metrics (Lib_Metrics.f90:860-1092) rebuilds the ghost cells from the donor
+ * block's nodes.  For node-matching connections those are the donor cells themselves, so
+ * this restatement copies the donor cell metrics (dl component of the connection
+ * direction mapped from the donor's direction; round-off level difference only). */
+static void connect_metrics(ob_domain* d, const mose_bc_entry* e) {
+  OBlk* Bm = &d->blk[e->b - 1];
+  OBlk* Bs = &d->blk[e->bs - 1];
+  const int* gm = guide[e->f - 1];
+  const int* gs = guide[e->fs - 1];
+  int dm = (e->f - 1) / 2, ds = (e->fs - 1) / 2;
+  for (int g = 1; g <= GC; ++g) {
+    int64_t dst = cidx(Bm, e->i - gm[0] * g, e->j - gm[1] * g, e->k - gm[2] * g);
+    int64_t src = cidx(Bs, e->is + gs[0] * (g - 1), e->js + gs[1] * (g - 1), e->ks + gs[2] * (g - 1));
+    memcpy(&Bm->M[9 * dst], &Bs->M[9 * src], 9 * sizeof(double));
+    memcpy(&Bm->dl[3 * dst], &Bs->dl[3 * src], 3 * sizeof(double));
+    if (dm != ds) { double t = Bm->dl[3 * dst + dm]; Bm->dl[3 * dst + dm] = Bs->dl[3 * src + ds]; Bm->dl[3 * dst + ds] = t; }
+    Bm->vol[dst] = Bs->vol[src];
+  }
+}
+
+/* Check_Mesh_Type (Lib_Metrics.f90:13-66): ndir from block 1 */
+static void check_mesh_type(ob_domain* d) {
+  OBlk* B = &d->blk[0];
+  if (B->nk > 1) d->ndir = 3;
+  else if (B->nk == 1 && B->nj == 1) d->ndir = 1;
+  else d->ndir = 2;
+}
+
+/* Setup_Metrics, src/lib/numerics/space/Mod_Metrics.f90:8-107 */
+int ob_setup(ob_domain* d) {
+  check_mesh_type(d);
+  for (int b = 0; b < d->nb; ++b) {
+    OBlk* B = &d->blk[b];
+    if (!B->node) return set_err(d, MOSE_E_ARG, b + 1, 0, 0, 0);
+    for (int k = 1; k <= B->nk; ++k)
+      for (int j = 1; j <= B->nj; ++j)
+        for (int i = 1; i <= B->ni; ++i) {
+          const double* N[8] = {node_at(B, i - 1, j - 1, k - 1), node_at(B, i - 1, j - 1, k), node_at(B, i - 1, j, k - 1),
+                                node_at(B, i - 1, j, k), node_at(B, i, j - 1, k - 1), node_at(B, i, j - 1, k),
+                                node_at(B, i, j, k - 1), node_at(B, i, j, k)};
+          int e = cell_metric(B, i, j, k, N);
+          if (e) return set_err(d, e, b + 1, i, j, k);
+        }
+    norm_area(B);
+  }
+  /* ghost metrics, entries in list order (Mod_Metrics.f90:74-105) */
+  for (int64_t q = 0; q < d->nbc; ++q) {
+    const mose_bc_entry* e = &d->bc[q];
+    OBlk* B = &d->blk[e->b - 1];
+    if (e->type == 101) connect_metrics(d, e);
+    else if (e->type == 300) symmetry_metrics(B, e->i, e->j, e->k, e->f);
+    else { int r = extrapolate_metrics(d, B, e->i, e->j, e->k, e->f); if (r) return r; }
+  }
+  return 0;
+}
+
+/* ------------------------------------ Compute_dt ------------------------------------
+ * src/lib/numerics/time/Mod_dt.f90:10-107 (inviscid), Set_Global_dt :110-130 */
+int ob_compute_dt(ob_domain* d, int32_t iter, double* dtglobal_out) {
+  double dtglobal = 1e5;   /* Mod_Explicit.f90:44 */
+  for (int b = 0; b < d->nb; ++b) {
+    OBlk* B = &d->blk[b];
+    for (int k = 1; k <= B->nk; ++k)
+      for (int j = 1; j <= B->nj; ++j)
+        for (int i = 1; i <= B->ni; ++i) {
+          int64_t c = cidx(B, i, j, k);
+          const double* P = &B->P[NV * c];
+          double Sound = f_ss(d, P[4], P[0]);
+          double dtmin = 1e8;
+          for (int dd = 0; dd < d->ndir; ++dd) {
+            const double* m = &B->M[9 * c + 3 * dd];
+            double nm = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+            double vs[3] = {m[0] / nm, m[1] / nm, m[2] / nm};
+            double vel_ = fabs(P[1] * vs[0] + P[2] * vs[1] + P[3] * vs[2]);
+            double lambda = vel_ + Sound;
+            double dt = B->dl[3 * c + dd] / lambda * d->cfg.cfl;
+            dtmin = fmin(dt, dtmin);
+          }
+          if (iter < d->cfg.rampa_cfl_iter) dtmin = dtmin * iter / d->cfg.rampa_cfl_iter;
+          B->dt[c] = dtmin;
+          dtglobal = fmin(dtmin, dtglobal);
+        }
+  }
+  d->dtglobal = dtglobal;
+  if (d->cfg.time_accurate)
+    for (int b = 0; b < d->nb; ++b) {
+      OBlk* B = &d->blk[b];
+      for (int64_t c = 0; c < B->npad; ++c) B->dt[c] = dtglobal;
+    }
+  if (dtglobal_out) *dtglobal_out = dtglobal;
+  return 0;
+}
+
+/* ------------------------------------ RK update -------------------------------------
+ * Newstate_Conservative / Compute_Cons_NoIRS (Lib_Newstate.f90:13-93,134-160), RK_stage
+ * with RKcoeff (Lib_RK.f90:8-27), Check_And_Fix_State (Lib_Newstate.f90:323-356) */
+static const double RKcoeff[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};
+static int newstate(ob_domain* d, int irk) {
+  double c = RKcoeff[d->cfg.n_rk - 1][irk - 1];
+  for (int b = 0; b < d->nb; ++b) {
+    OBlk* B = &d->blk[b];
+    for (int k = 1; k <= B->nk; ++k)
+      for (int j = 1; j <= B->nj; ++j)
+        for (int i = 1; i <= B->ni; ++i) {
+          int64_t q = cidx(B, i, j, k);
+          double* P = &B->P[NV * q];
+          double* R = &B->R[NV * q];
+          const double* PO = &B->PO[NV * q];
+          double cons[NV], consold[NV];
+          for (int v = 0; v < NV; ++v) R[v] = -R[v] / B->vol[q] * B->dt[q] * 1.0;
+          prim2cons_d(d, PO, consold);
+          prim2cons_d(d, P, cons);
+          for (int v = 0; v < NV; ++v) {
+            double res_ = (cons[v] - consold[v] + R[v]) * c;
+            cons[v] = consold[v] + res_;
+          }
+          cons2prim_d(d, cons, P);
+          int bad = 0;
+          for (int v = 0; v < NV; ++v) if (isnan(P[v])) bad = 1;
+          if (bad || P[4] < 0.0) return set_err(d, MOSE_E_STATE, b + 1, i, j, k);
+          if (P[0] < 1e-20) P[0] = 1e-20;
+        }
+  }
+  return 0;
+}
+
+/* Compute_Residual, src/lib/diagnostic/Mod_Diagnostic.f90:13-50 (+ sqrt, Mod_Explicit.f90:112) */
+static void compute_residual(ob_domain* d, double* norms) {
+  double tot[NV] = {0, 0, 0, 0, 0};
+  for (int b = 0; b < d->nb; ++b) {
+    OBlk* B = &d->blk[b];
+    for (int k = 1; k <= B->nk; ++k)
+      for (int j = 1; j <= B->nj; ++j)
+        for (int i = 1; i <= B->ni; ++i) {
+          int64_t q = cidx(B, i, j, k);
+          const double* n = &B->P[NV * q];
+          const double* o = &B->PO[NV * q];
+          for (int v = 0; v < NV; ++v) { double r = fabs(n[v] - o[v]); tot[v] += r * r; }
+        }
+  }
+  for (int v = 0; v < NV; ++v) norms[v] = sqrt(tot[v]);
+}
+
+/* Explicit_Step, src/lib/numerics/time/explicit/Mod_Explicit.f90:10-113 */
+int ob_step(ob_domain* d, double* dt_out, double* time_out, double* norms) {
+  if (d->err.code) return d->err.code;
+  d->iter += 1;
+  ob_compute_dt(d, d->iter, NULL);
+  if (d->cfg.time_accurate) d->time += d->dtglobal;
+  for (int b = 0; b < d->nb; ++b) {          /* Copy_State (interior), :149-170 */
+    OBlk* B = &d->blk[b];
+    for (int k = 1; k <= B->nk; ++k)
+      for (int j = 1; j <= B->nj; ++j)
+        for (int i = 1; i <= B->ni; ++i) {
+          int64_t q = cidx(B, i, j, k);
+          memcpy(&B->PO[NV * q], &B->P[NV * q], NV * sizeof(double));
+        }
+  }
+  for (int irk = 1; irk <= d->cfg.n_rk; ++irk) {
+    ob_fill_ghost(d);
+    for (int b = 0; b < d->nb; ++b) { int e = fluxes_blk(d, b); if (e) return e; }
+    int e = bc_fluxes(d); if (e) return e;
+    e = newstate(d, irk); if (e) return e;
+  }
+  double nr[NV];
+  compute_residual(d, nr);
+  if (norms) memcpy(norms, nr, sizeof nr);
+  if (dt_out) *dt_out = d->dtglobal;
+  if (time_out) *time_out = d->time;
+  return 0;
+}
+
+int ob_residual(ob_domain* d, int32_t b, double* R) {
+  if (b < 1 || b > d->nb) return set_err(d, MOSE_E_ARG, b, 0, 0, 0);
+  ob_fill_ghost(d);
+  for (int q = 0; q < d->nb; ++q) { int e = fluxes_blk(d, q); if (e) return e; }
+  int e = bc_fluxes(d); if (e) return e;
+  OBlk* B = &d->blk[b - 1];
+  memset(R, 0, sizeof(double) * NV * B->npad);
+  for (int k = 1; k <= B->nk; ++k)
+    for (int j = 1; j <= B->nj; ++j)
+      for (int i = 1; i <= B->ni; ++i) {
+        int64_t q = cidx(B, i, j, k);
+        memcpy(&R[NV * q], &B->R[NV * q], NV * sizeof(double));
+      }
+  return 0;
+}
+
+/* ------------------------------ construction / IO ------------------------------------ */
+ob_domain* ob_create(const mose_cfg* cfg, int32_t nblocks) {
+  if (!cfg || nblocks < 1) return NULL;
+  ob_domain* d = (ob_domain*)calloc(1, sizeof(ob_domain));
+  d->cfg = *cfg;
+  init_gas(d);
+  d->nb = nblocks;
+  d->blk = (OBlk*)calloc((size_t)nblocks, sizeof(OBlk));
+  return d;
+}
+int ob_add_block(ob_domain* d, int32_t b, const int32_t dims[3], const double* nodes) {
+  if (b < 1 || b > d->nb || dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return set_err(d, MOSE_E_ARG, b, 0, 0, 0);
+  OBlk* B = &d->blk[b - 1];
+  B->ni = dims[0]; B->nj = dims[1]; B->nk = dims[2];
+  B->pi = B->ni + 2 * GC; B->pj = B->nj + 2 * GC; B->pk = B->nk + 2 * GC;
+  B->npad = (int64_t)B->pi * B->pj * B->pk;
+  int64_t nn = (int64_t)(B->ni + 1) * (B->nj + 1) * (B->nk + 1);
+  B->node = (double*)malloc(sizeof(double) * 3 * nn);
+  memcpy(B->node, nodes, sizeof(double) * 3 * nn);
+  B->P = (double*)calloc((size_t)(NV * B->npad), sizeof(double));
+  B->PO = (double*)calloc((size_t)(NV * B->npad), sizeof(double));
+  B->R = (double*)calloc((size_t)(NV * B->npad), sizeof(double));
+  for (int dd = 0; dd < 3; ++dd) {
+    B->fn[dd] = (double*)calloc((size_t)(3 * B->npad), sizeof(double));
+    B->fa[dd] = (double*)calloc((size_t)B->npad, sizeof(double));
+  }
+  B->dl = (double*)calloc((size_t)(3 * B->npad), sizeof(double));
+  B->M = (double*)calloc((size_t)(9 * B->npad), sizeof(double));
+  B->vol = (double*)calloc((size_t)B->npad, sizeof(double));
+  B->dt = (double*)calloc((size_t)B->npad, sizeof(double));
+  return 0;
+}
+int ob_set_bc(ob_domain* d, const mose_bc_entry* entries, int64_t n) {
+  free(d->bc);
+  d->bc = (mose_bc_entry*)malloc(sizeof(mose_bc_entry) * (size_t)(n > 0 ? n : 1));
+  memcpy(d->bc, entries, sizeof(mose_bc_entry) * (size_t)n);
+  d->nbc = n;
+  return 0;
+}
+int ob_put_state(ob_domain* d, int32_t b, const double* P) {
+  if (b < 1 || b > d->nb) return set_err(d, MOSE_E_ARG, b, 0, 0, 0);
+  OBlk* B = &d->blk[b - 1];
+  memcpy(B->P, P, sizeof(double) * NV * B->npad);
+  return 0;
+}
+int ob_get_state(ob_domain* d, int32_t b, double* P) {
+  if (b < 1 || b > d->nb) return set_err(d, MOSE_E_ARG, b, 0, 0, 0);
+  OBlk* B = &d->blk[b - 1];
+  memcpy(P, B->P, sizeof(double) * NV * B->npad);
+  return 0;
+}
+int ob_get_time(ob_domain* d, double* time, int32_t* iter) { *time = d->time; *iter = d->iter; return 0; }
+int ob_set_time(ob_domain* d, double time, int32_t iter) { d->time = time; d->iter = iter; return 0; }
+int ob_get_metrics(ob_domain* d, int32_t b, double* fn, double* fa, double* dl, double* M, double* vol) {
+  OBlk* B = &d->blk[b - 1];
+  for (int dd = 0; dd < 3; ++dd) {
+    if (fn) memcpy(fn + 3 * dd * B->npad, B->fn[dd], sizeof(double) * 3 * B->npad);
+    if (fa) memcpy(fa + dd * B->npad, B->fa[dd], sizeof(double) * B->npad);
+  }
+  if (dl) memcpy(dl, B->dl, sizeof(double) * 3 * B->npad);
+  if (M) memcpy(M, B->M, sizeof(double) * 9 * B->npad);
+  if (vol) memcpy(vol, B->vol, sizeof(double) * B->npad);
+  return 0;
+}
+int ob_get_dtlocal(ob_domain* d, int32_t b, double* dt) {
+  OBlk* B = &d->blk[b - 1];
+  memcpy(dt, B->dt, sizeof(double) * B->npad);
+  return 0;
+}
+int ob_last_error(ob_domain* d, mose_err* e) { *e = d->err; return d->err.code; }
+int ob_ndir(ob_domain* d) { return d->ndir; }
+void ob_destroy(ob_domain* d) {
+  if (!d) return;
+  for (int b = 0; b < d->nb; ++b) {
+    OBlk* B = &d->blk[b];
+    free(B->node); free(B->P); free(B->PO); free(B->R);
+    for (int dd = 0; dd < 3; ++dd) { free(B->fn[dd]); free(B->fa[dd]); }
+    free(B->dl); free(B->M); free(B->vol); free(B->dt);
+  }
+  free(d->blk); free(d->bc); free(d);
+}
