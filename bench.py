@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit residual/update path on one MI355X.
+
+Workload (the headline configuration of this path): 3-D isentropic vortex on one periodic
+256^3 block over [0,10]^3, gamma 1.4, free stream (1,1,1,0,1), vortex strength 5,
+MUSCL van Leer + HLLC, SSP-RK3, CFL 0.8, FP64.  A step = one full explicit step
+(CFL dt + 3 RK stages with ghost fill, fluxes, update, residual norm) over all cells,
+with the state resident in HBM.  value = cell-updates/s = cells x steps / seconds.
+
+Also reported on the same line: the FP32-flux variant of the same workload, the L2 error
+against the advected exact vortex, the roofline of the dominant kernel (the fused
+flux + RK-stage kernel) measured with HIP events on the library's stream, and a CPU
+baseline (the C oracle, single thread, on a bounded sample of the same workload).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+STAGE_BYTES_PER_CELL = 3 * 5 * 8       # read U^(k-1), read U^n, write U^(k): 5 FP64 each
+STEP_BYTES_PER_CELL = 3 * STAGE_BYTES_PER_CELL + 5 * 8   # + the CFL pass reading U^n
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def l2_vs_exact(g, case, t):
+    from mose_amd import abi, cases
+    P = abi.interior(g.get_state(1))
+    ex = cases.vortex_exact(cases.cell_centres(case.blocks[0].nodes), t, L=case.meta["L"])
+    return float(np.sqrt(np.mean((P[..., 0] - ex[..., 0]) ** 2)))
+
+
+def run_gpu(case, steps, warmup, label):
+    from mose_amd.lib import GpuSolver
+    t0 = time.perf_counter()
+    g = GpuSolver(case)
+    log("[bench] %s: setup %.1f s, %d cells" % (label, time.perf_counter() - t0, case.ncells))
+    g.advance(warmup)
+    t0 = time.perf_counter()
+    t_sim, norms = g.advance(steps)     # synchronises at the end
+    dt = time.perf_counter() - t0
+    ms = g.time_kernels(nrep=10)
+    log("[bench] %s: %d steps %.3f s; kernel ms dt/ghost/stage/last = %s" % (label, steps, dt, ms))
+    return g, dt, t_sim, ms
+
+
+def cpu_baseline(n=64, steps=20):
+    """The C oracle (single thread, no fast-math) on the same workload at n^3."""
+    from mose_amd import cases
+    from oracle.oracle import OracleSolver
+    case = cases.vortex(n=n)
+    o = OracleSolver(case)
+    o.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step()
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": "vortex %d^3, %d steps after 1 warm-up step, C oracle (gcc -O2, 1 thread) on %s"
+                      % (n, steps, platform.processor() or platform.machine())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--riemann", default="HLLC")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this path runs on one GPU; multi-GPU block sharding is a design only "
+              "(see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    from mose_amd import abi, cases
+
+    n = args.n
+    ncell = n ** 3
+    case = cases.vortex(n=n, riemann=args.riemann, precision=abi.PREC_FP64)
+    g, secs, t_sim, ms = run_gpu(case, args.steps, args.warmup, "fp64")
+    cups = ncell * args.steps / secs
+    err64 = l2_vs_exact(g, case, t_sim)
+    P64 = abi.interior(g.get_state(1))
+    g.close()
+    del g
+
+    stage_ms = ms[2]
+    achieved = STAGE_BYTES_PER_CELL * ncell / (stage_ms * 1e-3) / 1e9
+    result = {
+        "metric": "cell-updates/s + % HBM roofline (FP64 vs FP32-flux); adjoint/primal time ratio",
+        "value": cups,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "isentropic vortex %d^3 periodic block, MUSCL van Leer + %s, SSP-RK3, "
+                               "CFL 0.8, FP64 state and flux" % (n, args.riemann),
+                   "cells": ncell, "riemann": args.riemann, "limiter": "vanleer"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_stage (fused face fluxes + RK stage), %d B/cell algorithmic"
+                               % STAGE_BYTES_PER_CELL,
+                     "kernel_ms": stage_ms},
+        "step_roofline_frac": STEP_BYTES_PER_CELL * cups / (HBM_PEAK_GBS * 1e9),
+        "kernel_ms": {"dt": ms[0], "ghost": ms[1], "stage": ms[2], "stage_last": ms[3]},
+        "l2_rho_vs_exact": err64,
+        "sim_time": t_sim,
+    }
+    if not args.no_fp32:
+        c32 = cases.vortex(n=n, riemann=args.riemann, precision=abi.PREC_FP32_FLUX)
+        g32, secs32, t32, ms32 = run_gpu(c32, args.steps, args.warmup, "fp32-flux")
+        P32 = abi.interior(g32.get_state(1))
+        g32.close()
+        cups32 = ncell * args.steps / secs32
+        result["fp32_flux"] = {
+            "value": cups32, "ms_per_step": 1e3 * secs32 / args.steps,
+            "speedup_vs_fp64": cups32 / cups,
+            "rel_l2_vs_fp64": float(np.linalg.norm(P32 - P64) / np.linalg.norm(P64)),
+            "stage_kernel_ms": ms32[2],
+            "stage_frac": STAGE_BYTES_PER_CELL * ncell / (ms32[2] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "step_roofline_frac": STEP_BYTES_PER_CELL * cups32 / (HBM_PEAK_GBS * 1e9),
+        }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -116,6 +116,13 @@ int  mose_residual(mose_ctx* ctx, int32_t b, double* R);
 int  mose_get_time(mose_ctx* ctx, double* time, int32_t* iter);
 int  mose_set_time(mose_ctx* ctx, double time, int32_t iter);
 
+/* Average device time in ms of one launch of each kernel of a step, measured with HIP events
+ * on the context's stream over nrep back-to-back launches on the current state:
+ * ms[0] CFL time step, ms[1] ghost fill, ms[2] intermediate RK stage, ms[3] last RK stage
+ * (with residual-norm partials).  The state U^n is not modified.  Profiling entry point with
+ * no counterpart in the reference (which has no per-kernel timers). */
+int  mose_time_kernels(mose_ctx* ctx, int32_t nrep, double ms[4]);
+
 /* Total interior cells over all blocks. */
 int64_t mose_num_cells(const mose_ctx* ctx);
 

@@ -1,0 +1,60 @@
+"""Build libmose_hip.so (HIP, gfx950) and the Fortran host driver in-tree.
+
+Usage: python -m mose_amd.build   (also called by __graft_entry__.build()).
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmose_hip.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FC = os.environ.get("MOSE_FC", "/opt/rocm/llvm/bin/amdflang")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # packed FP32 instructions are disabled for all device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def build_hip(extra=()):
+    src = os.path.join(CSRC, "mose_hip.hip")
+    tmp = LIB + ".tmp"
+    run([HIPCC] + HIP_FLAGS + list(extra) + ["-I", os.path.join(ROOT, "include"), "-o", tmp, src])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran():
+    """Fortran host driver (ISO_C_BINDING) linked against libmose_hip.so."""
+    fdir = os.path.join(HERE, "fortran")
+    if not os.path.exists(FC) or not os.path.exists(os.path.join(fdir, "mose_run.f90")):
+        return None
+    out = os.path.join(fdir, "mose_run")
+    run([FC, "-O2", "-o", out, "mose_hip_iface.f90", "mose_explicit.f90", "mose_run.f90",
+         "-L" + HERE, "-lmose_hip", "-Wl,-rpath," + HERE], cwd=fdir)
+    return out
+
+
+def build_oracle():
+    run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+def main():
+    build_hip()
+    build_fortran()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

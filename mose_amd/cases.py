@@ -65,26 +65,26 @@ def cell_centres(nodes):
 def padded_state(dims, interior_prim):
     """blk%P array with the interior set from (nk, nj, ni, 5) and ghosts filled by copy
     of the nearest interior cell (ghosts are refilled by the solver before use)."""
-    P = np.zeros(abi.padded_shape(dims))
     g = abi.GC
-    P[g:-g, g:-g, g:-g, :] = interior_prim
-    P[:] = np.pad(interior_prim, ((g, g), (g, g), (g, g), (0, 0)), mode="edge")
-    return P
+    P = np.pad(np.asarray(interior_prim, dtype=np.float64), ((g, g), (g, g), (g, g), (0, 0)), mode="edge")
+    assert P.shape == abi.padded_shape(dims)
+    return np.ascontiguousarray(P)
 
 
 # ------------------------------------------------------------------------- bc records
 def face_cells(dims, f):
-    """(i, j, k) 1-based boundary cells of face f (1/2 i-min/max, 3/4 j, 5/6 k), ordered
-    with the first tangential index fastest."""
+    """(i, j, k) 1-based boundary cells of face f (1/2 i-min/max, 3/4 j, 5/6 k) as an (n, 3)
+    int array, ordered with the first tangential index fastest."""
     ni, nj, nk = dims
-    if f in (1, 2):
-        i = 1 if f == 1 else ni
-        return [(i, j, k) for k in range(1, nk + 1) for j in range(1, nj + 1)]
-    if f in (3, 4):
-        j = 1 if f == 3 else nj
-        return [(i, j, k) for k in range(1, nk + 1) for i in range(1, ni + 1)]
-    k = 1 if f == 5 else nk
-    return [(i, j, k) for j in range(1, nj + 1) for i in range(1, ni + 1)]
+    d = (f - 1) // 2
+    n = [ni, nj, nk]
+    t = [a for a in range(3) if a != d]
+    A, B = np.meshgrid(np.arange(1, n[t[0]] + 1), np.arange(1, n[t[1]] + 1), indexing="xy")
+    out = np.empty((A.size, 3), dtype=np.int32)
+    out[:, d] = 1 if f % 2 == 1 else n[d]
+    out[:, t[0]] = A.ravel()
+    out[:, t[1]] = B.ravel()
+    return out
 
 
 def bc_records(block_faces):
@@ -94,7 +94,7 @@ def bc_records(block_faces):
     int BC type (0, 300, 400), a dict {"type": 405, mach, T, p, alpha, beta, rel_fac},
     or a dict {"type": 101|201, "bs": donor block, "fs": donor face} for an aligned
     1-to-1 connection (orientation d11,d12,d21,d22 = 1,0,0,1)."""
-    recs = []
+    parts = []
     dims_of = [bf["dims"] for bf in block_faces]
     for b, bf in enumerate(block_faces, start=1):
         dims = bf["dims"]
@@ -102,27 +102,26 @@ def bc_records(block_faces):
             spec = bf["faces"][f - 1]
             if not isinstance(spec, dict):
                 spec = {"type": int(spec)}
-            for (i, j, k) in face_cells(dims, f):
-                r = dict(b=b, i=i, j=j, k=k, f=f, type=spec["type"])
-                if spec["type"] in (101, 201):
-                    bs, fs = spec["bs"], spec["fs"]
-                    sd = dims_of[bs - 1]
-                    di = (f - 1) // 2
-                    src = [i, j, k]
-                    src[di] = 1 if fs % 2 == 1 else sd[di]
-                    if (fs - 1) // 2 != di:
-                        raise ValueError("only same-direction connections are generated")
-                    r.update(bs=bs, fs=fs, d11=1, d12=0, d21=0, d22=1)
-                    r["is"], r["js"], r["ks"] = src
-                elif spec["type"] == 405:
-                    for key in ("mach", "T", "p", "alpha", "beta", "rel_fac"):
-                        r[key] = spec[key]
-                recs.append(r)
-    arr = np.zeros(len(recs), dtype=abi.BC_DTYPE)
-    for n, r in enumerate(recs):
-        for key, val in r.items():
-            arr[n][key] = val
-    return arr
+            cells = face_cells(dims, f)
+            arr = np.zeros(len(cells), dtype=abi.BC_DTYPE)
+            arr["b"], arr["f"], arr["type"] = b, f, spec["type"]
+            arr["i"], arr["j"], arr["k"] = cells[:, 0], cells[:, 1], cells[:, 2]
+            if spec["type"] in (101, 201):
+                bs, fs = spec["bs"], spec["fs"]
+                sd = dims_of[bs - 1]
+                di = (f - 1) // 2
+                if (fs - 1) // 2 != di:
+                    raise ValueError("only same-direction connections are generated")
+                src = cells.copy()
+                src[:, di] = 1 if fs % 2 == 1 else sd[di]
+                arr["bs"], arr["fs"] = bs, fs
+                arr["is"], arr["js"], arr["ks"] = src[:, 0], src[:, 1], src[:, 2]
+                arr["d11"], arr["d22"] = 1, 1
+            elif spec["type"] == 405:
+                for key in ("mach", "T", "p", "alpha", "beta", "rel_fac"):
+                    arr[key] = spec[key]
+            parts.append(arr)
+    return np.concatenate(parts)
 
 
 def expected_bc_count(dims_list):
@@ -308,7 +307,7 @@ def split_block_case(case, axis=0, parts=2):
         sl[2 - axis] = slice(a, b + 1)
         nodes = np.ascontiguousarray(B.nodes[tuple(sl)])
         psl = [slice(None)] * 3
-        psl[2 - axis] = slice(a + g, b + g)
+        psl[2 - axis] = slice(a, b)
         prim = B.P[g:-g, g:-g, g:-g][tuple(psl)]
         faces = []
         for f in range(1, 7):

@@ -1,0 +1,311 @@
+// mose_geometry.hpp — host-side setup of the device layout (blocks, geometry lines, tiles,
+// ghost operations) from the reference's inputs: block nodes (blk%node) and the bc.txt
+// records.  Host code only; included by mose_hip.hip.
+//
+// Geometry.  The reference computes, per cell, a metric tensor, the cell lengths dl and the
+// volume from the 8 cell nodes (Compute_Metric_Tensor, src/lib/numerics/space/
+// Lib_Metrics.f90:250-376) and per face a unit normal and area from the cross product of
+// the face diagonals (Compute_Norm_Area, :69-247); ghost cells get metrics from donor
+// cells (type 101, BC_Connect_Metrics :860-1092), from a mirror (300, BC_Symmetry_Metrics
+// :800-858) or from extrapolated nodes (every other type incl. 201 periodic,
+// BC_Extrapolate_Metrics :535-797).  For a rectilinear block these reduce exactly to one
+// line of values per direction: dl(i) from the reference's own summation order on the two
+// node planes, the face area of an i-face = (y_j - y_{j-1}) (z_k - z_{k-1}) and the
+// normal = the axis unit vector.  This setup verifies rectilinearity bit-for-bit and
+// builds those lines; curvilinear blocks are rejected with MOSE_E_ARG.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+struct HostBlock {
+  int ni = 0, nj = 0, nk = 0;
+  std::vector<double> nodes;   // (3, 0:ni, 0:nj, 0:nk)
+  const double* node(int i, int j, int k) const {
+    return &nodes[3 * ((size_t)i + (size_t)(ni + 1) * ((size_t)j + (size_t)(nj + 1) * k))];
+  }
+};
+
+struct Layout {
+  std::vector<mose::BlockDev> blocks;
+  std::vector<mose::LineEnt> lines;
+  std::vector<mose::Tile> tiles;
+  std::vector<mose::GhostOp> ghost;
+  std::vector<double> facebc;
+  int64_t npool = 0;
+  int ndir = 3;
+};
+
+struct GeoError {
+  int code = 0, b = 0, i = 0, j = 0, k = 0;
+};
+
+namespace geo {
+
+static const int guide[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+// |A(d,d)| of Compute_Metric_Tensor on a rectilinear cell with node planes a (low), b (high):
+// A(d,:) = N(hi) - N(lo) + N(hi) - N(lo) + ... evaluated left to right, then x 0.25.
+static inline double ref_dl(double a, double b) {
+  double s = b - a;
+  s = s + b; s = s - a;
+  s = s + b; s = s - a;
+  s = s + b; s = s - a;
+  return std::fabs(0.25 * s);
+}
+
+static bool set_err(GeoError& e, int code, int b, int i = 0, int j = 0, int k = 0) {
+  e.code = code; e.b = b; e.i = i; e.j = j; e.k = k;
+  return false;
+}
+
+static int face_kind(int type) {
+  switch (type) {
+    case 101: case 201: return mose::F_CONN;
+    case 0: return mose::F_NONE;
+    case 400: return mose::F_EXTRAP;
+    case 300: return mose::F_SYM;
+    case 405: return mose::F_INFLOW;
+    default: return -1;
+  }
+}
+
+}  // namespace geo
+
+static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mose_bc_entry>& bc, int TXc,
+                         int TYc, Layout& L, GeoError& err) {
+  using namespace mose;
+  const int nb = (int)hb.size();
+  if (nb < 1) return geo::set_err(err, MOSE_E_ARG, 0);
+  // ndir from block 1 (Check_Mesh_Type, Lib_Metrics.f90:13-66)
+  L.ndir = hb[0].nk > 1 ? 3 : ((hb[0].nj == 1) ? 1 : 2);
+
+  // ---- bc records: completeness and one type per block face
+  std::vector<int> ftype(6 * nb, -9999), fentry(6 * nb, -1);
+  std::vector<int64_t> fcount(6 * nb, 0);
+  int64_t expected = 0;
+  for (const HostBlock& B : hb)
+    expected += 2 * ((int64_t)B.nj * B.nk + (int64_t)B.ni * B.nk + (int64_t)B.ni * B.nj);
+  if ((int64_t)bc.size() != expected) return geo::set_err(err, MOSE_E_BC, 0);
+  for (size_t q = 0; q < bc.size(); ++q) {
+    const mose_bc_entry& e = bc[q];
+    if (e.b < 1 || e.b > nb || e.f < 1 || e.f > 6) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
+    const HostBlock& B = hb[e.b - 1];
+    const int n[3] = {B.ni, B.nj, B.nk};
+    const int ijk[3] = {e.i, e.j, e.k};
+    const int d = (e.f - 1) / 2;
+    for (int a = 0; a < 3; ++a)
+      if (ijk[a] < 1 || ijk[a] > n[a]) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
+    if (ijk[d] != ((e.f % 2) ? 1 : n[d])) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
+    const int slot = 6 * (e.b - 1) + (e.f - 1);
+    if (geo::face_kind(e.type) < 0) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
+    if (ftype[slot] == -9999) { ftype[slot] = e.type; fentry[slot] = (int)q; }
+    else if (ftype[slot] != e.type) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
+    if (e.type == 101 || e.type == 201) {
+      if (e.bs < 1 || e.bs > nb || e.fs < 1 || e.fs > 6) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
+      const HostBlock& S = hb[e.bs - 1];
+      const int ns[3] = {S.ni, S.nj, S.nk};
+      const int sijk[3] = {e.is, e.js, e.ks};
+      const int ds = (e.fs - 1) / 2;
+      for (int a = 0; a < 3; ++a)
+        if (sijk[a] < 1 || sijk[a] > ns[a]) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
+      if (ns[ds] < 2) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
+    }
+    fcount[slot]++;
+  }
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock& B = hb[b];
+    const int64_t fc[3] = {(int64_t)B.nj * B.nk, (int64_t)B.ni * B.nk, (int64_t)B.ni * B.nj};
+    for (int f = 0; f < 6; ++f)
+      if (fcount[6 * b + f] != fc[f / 2]) return geo::set_err(err, MOSE_E_BC, b + 1);
+  }
+
+  // ---- pool layout
+  std::vector<int64_t> off(nb);
+  int64_t pool = 0;
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock& B = hb[b];
+    off[b] = pool;
+    pool += (int64_t)(B.ni + 4) * (B.nj + 4) * (B.nk + 4);
+    pool = (pool + 63) / 64 * 64;
+  }
+  L.npool = pool;
+
+  // ---- geometry lines: rectilinearity and interior values
+  std::vector<std::vector<double>> coord(3 * nb);
+  std::vector<int> lineoff(3 * nb);
+  L.lines.clear();
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock& B = hb[b];
+    const int n[3] = {B.ni, B.nj, B.nk};
+    for (int d = 0; d < 3; ++d) {
+      coord[3 * b + d].resize(n[d] + 1);
+      for (int q = 0; q <= n[d]; ++q) {
+        const int ii = d == 0 ? q : 0, jj = d == 1 ? q : 0, kk = d == 2 ? q : 0;
+        coord[3 * b + d][q] = B.node(ii, jj, kk)[d];
+      }
+      for (int q = 1; q <= n[d]; ++q)
+        if (!(coord[3 * b + d][q] > coord[3 * b + d][q - 1])) return geo::set_err(err, MOSE_E_ARG, b + 1);
+    }
+    for (int k = 0; k <= B.nk; ++k)
+      for (int j = 0; j <= B.nj; ++j)
+        for (int i = 0; i <= B.ni; ++i) {
+          const double* p = B.node(i, j, k);
+          if (p[0] != coord[3 * b][i] || p[1] != coord[3 * b + 1][j] || p[2] != coord[3 * b + 2][k])
+            return geo::set_err(err, MOSE_E_ARG, b + 1, i, j, k);   // curvilinear: not supported
+        }
+    for (int d = 0; d < 3; ++d) {
+      lineoff[3 * b + d] = (int)L.lines.size();
+      for (int q = -1; q <= n[d] + 2; ++q) {
+        LineEnt e{};
+        if (q >= 1 && q <= n[d]) {
+          const double a = coord[3 * b + d][q - 1], c = coord[3 * b + d][q];
+          e.dl = geo::ref_dl(a, c);
+          e.delta = c - a;
+        }
+        L.lines.push_back(e);
+      }
+    }
+  }
+  auto LE = [&](int b, int d, int q) -> LineEnt& { return L.lines[lineoff[3 * b + d] + q + 1]; };
+
+  // ---- ghost lengths per block face
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock& B = hb[b];
+    const int n[3] = {B.ni, B.nj, B.nk};
+    for (int f = 0; f < 6; ++f) {
+      const int d = f / 2;
+      const bool hi = (f % 2) == 1;
+      const int type = ftype[6 * b + f];
+      const mose_bc_entry& e0 = bc[fentry[6 * b + f]];
+      const std::vector<double>& c = coord[3 * b + d];
+      const int nd = n[d];
+      if (type == 101) {
+        const int ds = (e0.fs - 1) / 2;
+        const int sijk[3] = {e0.is, e0.js, e0.ks};
+        for (int g = 1; g <= 2; ++g) {
+          const int sidx = sijk[ds] + geo::guide[e0.fs - 1][ds] * (g - 1);
+          LE(b, d, hi ? nd + g : 1 - g).dl = LE(e0.bs - 1, ds, sidx).dl;
+        }
+      } else if (type == 300) {
+        for (int g = 1; g <= 2; ++g) {
+          if (hi) LE(b, d, nd + g).dl = LE(b, d, nd + 1 - g).dl;
+          else LE(b, d, 1 - g).dl = LE(b, d, g).dl;
+        }
+      } else {
+        bool cubic = (d == 0) ? true : (d == 1 ? (B.nj > 2) : (L.ndir == 3));
+        if (nd < 2) cubic = false;
+        if (nd < 1) return geo::set_err(err, MOSE_E_ARG, b + 1);
+        double e1 = 0.0, e2 = 0.0;
+        if (!hi) {
+          if (cubic) { e1 = 3.0 * c[0] - 3.0 * c[1] + c[2]; e2 = 3.0 * e1 - 3.0 * c[0] + c[1]; }
+          else { e1 = 2.0 * c[0] - c[1]; e2 = 2.0 * e1 - c[0]; }
+          LE(b, d, 0).dl = geo::ref_dl(e1, c[0]);
+          LE(b, d, -1).dl = geo::ref_dl(e2, e1);
+        } else {
+          if (cubic) { e1 = 3.0 * c[nd] - 3.0 * c[nd - 1] + c[nd - 2]; e2 = 3.0 * e1 - 3.0 * c[nd] + c[nd - 1]; }
+          else { e1 = 2.0 * c[nd] - c[nd - 1]; e2 = 2.0 * e1 - c[nd]; }
+          LE(b, d, nd + 1).dl = geo::ref_dl(c[nd], e1);
+          LE(b, d, nd + 2).dl = geo::ref_dl(e1, e2);
+        }
+      }
+    }
+  }
+  // ---- face spacings (face between q-1 and q), Lib_Convective.f90:30-34
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock& B = hb[b];
+    const int n[3] = {B.ni, B.nj, B.nk};
+    for (int d = 0; d < 3; ++d)
+      for (int q = 1; q <= n[d] + 1; ++q) {
+        LineEnt& e = LE(b, d, q);
+        const double dm2 = LE(b, d, q - 2).dl, dm1 = LE(b, d, q - 1).dl, d0 = LE(b, d, q).dl,
+                     dp1 = LE(b, d, q + 1).dl;
+        e.rl0 = 1.0 / (0.5 * (dm2 + dm1));
+        e.rl1 = 1.0 / (0.5 * (dm1 + d0));
+        e.rl2 = 1.0 / (0.5 * (d0 + dp1));
+        e.lm = 0.5 * dm1;
+        e.lp = 0.5 * d0;
+      }
+  }
+
+  // ---- block descriptors
+  L.blocks.resize(nb);
+  L.facebc.clear();
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock& B = hb[b];
+    BlockDev& D = L.blocks[b];
+    D.off = off[b];
+    D.ni = B.ni; D.nj = B.nj; D.nk = B.nk;
+    D.sy = B.ni + 4;
+    D.sz = (int64_t)(B.ni + 4) * (B.nj + 4);
+    for (int d = 0; d < 3; ++d) D.line[d] = lineoff[3 * b + d];
+    for (int f = 0; f < 6; ++f) {
+      D.bct[f] = geo::face_kind(ftype[6 * b + f]);
+      D.bcd[f] = -1;
+      if (ftype[6 * b + f] == 405) {
+        const mose_bc_entry& e = bc[fentry[6 * b + f]];
+        D.bcd[f] = (int)(L.facebc.size() / 6);
+        const double v[6] = {e.mach, e.T, e.p, e.alpha, e.beta, e.rel_fac};
+        L.facebc.insert(L.facebc.end(), v, v + 6);
+      }
+    }
+  }
+
+  // ---- tiles: TX x TY column patches, k chunks
+  int64_t ncell = 0;
+  for (const HostBlock& B : hb) ncell += (int64_t)B.ni * B.nj * B.nk;
+  int kc = 32;
+  if (const char* s = getenv("MOSE_KCHUNK")) kc = std::max(1, atoi(s));
+  L.tiles.clear();
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock& B = hb[b];
+    for (int k0 = 1; k0 <= B.nk; k0 += kc)
+      for (int j0 = 1; j0 <= B.nj; j0 += TYc)
+        for (int i0 = 1; i0 <= B.ni; i0 += TXc) {
+          Tile t{};
+          t.b = b; t.i0 = i0; t.j0 = j0; t.k0 = k0; t.k1 = std::min(k0 + kc, B.nk + 1);
+          L.tiles.push_back(t);
+        }
+  }
+
+  // ---- ghost operations, one per bc record (Fill_Ghost_Cell)
+  auto pidx = [&](int b, int i, int j, int k) -> int64_t {
+    const BlockDev& D = L.blocks[b];
+    return D.off + (int64_t)(i + 1) + (int64_t)D.sy * (j + 1) + D.sz * (int64_t)(k + 1);
+  };
+  auto stride = [&](int b, int f) -> int64_t {
+    const BlockDev& D = L.blocks[b];
+    const int* g = geo::guide[f - 1];
+    return (int64_t)g[0] + (int64_t)g[1] * D.sy + (int64_t)g[2] * D.sz;
+  };
+  L.ghost.clear();
+  L.ghost.reserve(bc.size());
+  for (const mose_bc_entry& e : bc) {
+    GhostOp op{};
+    op.b = pidx(e.b - 1, e.i, e.j, e.k);
+    op.s = stride(e.b - 1, e.f);
+    switch (e.type) {
+      case 101: case 201:
+        op.type = G_COPY;
+        op.src1 = pidx(e.bs - 1, e.is, e.js, e.ks);
+        op.src2 = op.src1 + stride(e.bs - 1, e.fs);
+        break;
+      case 300: {
+        op.type = G_SYM;
+        const int d = (e.f - 1) / 2;
+        op.n[d] = 1.0;
+        break;
+      }
+      case 400: op.type = G_EXTRAP; break;
+      default: op.type = G_ZG; break;
+    }
+    L.ghost.push_back(op);
+  }
+  (void)ncell;
+  return true;
+}

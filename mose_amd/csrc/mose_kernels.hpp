@@ -1,0 +1,78 @@
+// mose_kernels.hpp — device data structures and kernels of the explicit step (gfx950).
+//
+// Memory layout in HBM
+//   * Cell pool: the padded cells (gc = 2 ghost layers) of every block concatenated,
+//     i fastest.  Conserved state is SoA over the pool: U[v * npool + cell], v = rho, rho u,
+//     rho v, rho w, rho E, FP64.  Three state buffers: U^n, and two stage buffers.
+//   * Geometry of rectilinear blocks: per block and direction a line of `LineEnt`
+//     (cell length dl as the reference computes it, node spacing, reconstruction spacings
+//     of the face on the cell's low side).  Normals are the axis unit vectors.
+//   * Work: a list of tiles (block, i0, j0, k-range).  Each workgroup of TX x TY threads
+//     owns a TX x TY column patch and marches along k; the current k-plane's primitive
+//     state with a 2-cell i/j halo is staged in LDS, the k-direction stencil of each column
+//     lives in registers.  Every face flux is evaluated once.
+#pragma once
+#include "mose_device.hpp"
+
+namespace mose {
+
+constexpr int TX = 32;
+constexpr int TY = 8;
+constexpr int NT = TX * TY;
+constexpr int GCL = 2;
+
+// Ghost-op kinds.
+enum { G_COPY = 0, G_ZG = 1, G_EXTRAP = 2, G_SYM = 3 };
+// Face BC kinds used inside the stage kernel (bctype of a block face).
+enum { F_CONN = 0, F_NONE = 1, F_EXTRAP = 2, F_SYM = 3, F_INFLOW = 4 };
+// Scalars kept on the device.
+enum { S_DT = 0, S_TIME = 1, S_NORM = 2, S_COUNT = 8 };
+
+struct LineEnt {
+  double dl;      // cell length along the line (Compute_Metric_Tensor dl)
+  double delta;   // node spacing x_i - x_{i-1} (face areas / volume)
+  double rl0, rl1, rl2, lm, lp;   // spacing of the face between cells i-1 and i
+  double pad;
+};
+
+struct BlockDev {
+  int64_t off;       // pool index of padded cell (-1,-1,-1)
+  int64_t sz;        // plane stride (ni+4)(nj+4)
+  int ni, nj, nk, sy;
+  int line[3];       // LineEnt index of padded position 0 (cell -1) per direction
+  int bct[6];        // F_* per face
+  int bcd[6];        // index of the 6-double data record (inflow), or -1
+};
+
+struct Tile {
+  int b, i0, j0, k0, k1, pad0, pad1, pad2;
+};
+
+struct GhostOp {
+  int64_t b;         // boundary cell (pool index)
+  int64_t s;         // inward pool stride
+  int64_t src1, src2;
+  int type, pad;
+  double n[3];
+};
+
+struct StageArgs {
+  const double* __restrict__ Uin;
+  const double* __restrict__ Un;
+  double* __restrict__ Uout;
+  double* __restrict__ Rout;
+  const Tile* __restrict__ tiles;
+  const BlockDev* __restrict__ blocks;
+  const LineEnt* __restrict__ lines;
+  const double* __restrict__ facebc;
+  const double* __restrict__ scal;
+  const double* __restrict__ dtloc;
+  double* __restrict__ partials;
+  int* err;
+  int64_t npool;
+  int ntiles;
+  int local_dt;
+  double rkc, gam, R;
+};
+
+}  // namespace mose

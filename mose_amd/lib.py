@@ -1,0 +1,160 @@
+"""ctypes binding of libmose_hip.so (the C ABI of include/mose_hip.h).
+
+`GpuSolver(case)` drives the HIP path for one Case with the same method names as the CPU
+oracle's driver, so parity tests read the same on both sides.  The library must be built
+in-tree (python -m mose_amd.build); there is no CPU fallback: every method calls the HIP
+library and raises when it is missing or returns an error.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmose_hip.so")
+_LIB = None
+
+EXPORTS = ["mose_create", "mose_add_block", "mose_set_bc", "mose_finalize", "mose_put_state",
+           "mose_get_state", "mose_compute_dt", "mose_step", "mose_advance", "mose_residual",
+           "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_num_cells", "mose_last_error", "mose_strerror",
+           "mose_destroy"]
+
+
+def load():
+    """Load libmose_hip.so and declare the ABI; raises if the library was not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libmose_hip.so is not built: run `python -m mose_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    v = C.c_void_p
+    L.mose_create.argtypes = [P(abi.MoseCfg), P(v)]
+    L.mose_add_block.argtypes = [v, C.c_int32, P(C.c_int32), P(C.c_double)]
+    L.mose_set_bc.argtypes = [v, C.c_void_p, C.c_int64]
+    L.mose_finalize.argtypes = [v]
+    L.mose_put_state.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_get_state.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_compute_dt.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_step.argtypes = [v, P(C.c_double), P(C.c_double), P(C.c_double)]
+    L.mose_advance.argtypes = [v, C.c_int32, P(C.c_double), P(C.c_double)]
+    L.mose_residual.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_get_time.argtypes = [v, P(C.c_double), P(C.c_int32)]
+    L.mose_set_time.argtypes = [v, C.c_double, C.c_int32]
+    L.mose_time_kernels.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_num_cells.argtypes = [v]
+    L.mose_num_cells.restype = C.c_int64
+    L.mose_last_error.argtypes = [v, P(abi.MoseErr)]
+    L.mose_strerror.argtypes = [C.c_int32]
+    L.mose_strerror.restype = C.c_char_p
+    L.mose_destroy.argtypes = [v]
+    L.mose_destroy.restype = None
+    _LIB = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class MoseError(RuntimeError):
+    def __init__(self, code, err=None):
+        self.code = code
+        self.err = err
+        msg = load().mose_strerror(code).decode()
+        if err is not None and err.block:
+            msg += " at b, i, j, k: %d %d %d %d" % (err.block, err.i, err.j, err.k)
+        super().__init__("mose error %d: %s" % (code, msg))
+
+
+class GpuSolver:
+    """One HIP context (device buffers, graph) for a Case."""
+
+    def __init__(self, case, cfg=None):
+        L = load()
+        self.case = case
+        self.cfg = cfg if cfg is not None else case.cfg
+        self.h = C.c_void_p()
+        rc = L.mose_create(C.byref(self.cfg), C.byref(self.h))
+        if rc:
+            self.h = None
+            raise MoseError(rc)
+        self.dims = [tuple(int(x) for x in b.dims) for b in case.blocks]
+        for q, blk in enumerate(case.blocks, start=1):
+            dims = (C.c_int32 * 3)(*blk.dims)
+            nodes = np.ascontiguousarray(blk.nodes, dtype=np.float64)
+            self._check(L.mose_add_block(self.h, q, dims, _dp(nodes)))
+        bc = np.ascontiguousarray(case.bc)
+        self._check(L.mose_set_bc(self.h, bc.ctypes.data, len(bc)))
+        self._check(L.mose_finalize(self.h))
+        for q, blk in enumerate(case.blocks, start=1):
+            self.put_state(q, blk.P)
+
+    def _check(self, rc):
+        if rc:
+            e = abi.MoseErr()
+            load().mose_last_error(self.h, C.byref(e))
+            raise MoseError(rc, e)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().mose_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def put_state(self, b, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        assert P.shape == abi.padded_shape(self.dims[b - 1])
+        self._check(load().mose_put_state(self.h, b, _dp(P)))
+
+    def get_state(self, b):
+        P = np.zeros(abi.padded_shape(self.dims[b - 1]))
+        self._check(load().mose_get_state(self.h, b, _dp(P)))
+        return P
+
+    def residual(self, b):
+        R = np.zeros(abi.padded_shape(self.dims[b - 1]))
+        self._check(load().mose_residual(self.h, b, _dp(R)))
+        return R
+
+    def residuals(self):
+        return [self.residual(b) for b in range(1, len(self.dims) + 1)]
+
+    def compute_dt(self, it=1):
+        dt = C.c_double()
+        self._check(load().mose_compute_dt(self.h, it, C.byref(dt)))
+        return dt.value
+
+    def step(self):
+        dt, t = C.c_double(), C.c_double()
+        norms = np.zeros(5)
+        self._check(load().mose_step(self.h, C.byref(dt), C.byref(t), _dp(norms)))
+        return dt.value, t.value, norms
+
+    def advance(self, nsteps):
+        t = C.c_double()
+        norms = np.zeros(5)
+        self._check(load().mose_advance(self.h, nsteps, C.byref(t), _dp(norms)))
+        return t.value, norms
+
+    def time(self):
+        t, it = C.c_double(), C.c_int32()
+        self._check(load().mose_get_time(self.h, C.byref(t), C.byref(it)))
+        return t.value, it.value
+
+    def time_kernels(self, nrep=20):
+        """Average ms per launch: [CFL dt, ghost fill, RK stage, last RK stage]."""
+        ms = np.zeros(4)
+        self._check(load().mose_time_kernels(self.h, nrep, _dp(ms)))
+        return ms
+
+    def num_cells(self):
+        return load().mose_num_cells(self.h)

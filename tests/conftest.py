@@ -1,0 +1,21 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+
+
+def pytest_collection_modifyitems(config, items):
+    if os.path.exists("/dev/kfd"):
+        return
+    skip = pytest.mark.skip(reason="no /dev/kfd: no AMD GPU in this machine")
+    for it in items:
+        if "gpu" in it.keywords:
+            it.add_marker(skip)

@@ -1,0 +1,181 @@
+"""GPU parity of the HIP path (libmose_hip.so, through the C ABI) against the CPU oracle.
+
+Bar: FP64 residuals within 1e-12 relative L2 of the oracle's; FP64 multi-step states within
+1e-10; the FP32-flux variant within 1e-5 of the FP64 path; Sod within the reference's 1 %
+probe rule against its committed exact solution.
+"""
+import numpy as np
+import pytest
+
+from mose_amd import abi, cases
+from mose_amd.lib import GpuSolver, MoseError
+from oracle.oracle import OracleError, OracleSolver
+from oracle import exact_riemann
+
+from helpers import flux_scale, interior, perturb, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+LIMS = ["minmod", "vanleer", "vanalbada"]
+RSS = ["HLLC", "AUSM+"]
+
+
+def _pair(case):
+    return GpuSolver(case), OracleSolver(case)
+
+
+def _residual_check(case, tol=1e-12):
+    g, o = _pair(case)
+    for b in range(1, len(case.blocks) + 1):
+        Rg, Ro = g.residual(b), o.residual(b)
+        err = rel_l2(Rg, Ro)
+        scale = flux_scale(case.blocks[b - 1].P)
+        assert err < tol, "block %d: rel L2 %.3e" % (b, err)
+        assert np.max(np.abs(interior(Rg) - interior(Ro))) < 1e-13 * scale
+    g.close()
+
+
+def box_case(dims, faces, limiter="vanleer", riemann="HLLC", lo=(0.0, 0.0, 0.0), hi=(1.0, 0.7, 0.4),
+             mach=1.5, seed=3):
+    """Rectilinear box with arbitrary face types and a perturbed supersonic state."""
+    nodes = cases.cartesian_nodes(dims, lo, hi)
+    prim = np.zeros(tuple(dims[::-1]) + (5,))
+    prim[..., 0] = 1.0
+    prim[..., 1] = mach
+    prim[..., 4] = 1.0 / 1.4
+    blk = cases.Block(tuple(dims), nodes, cases.padded_state(dims, prim))
+    bc = cases.bc_records([{"dims": tuple(dims), "faces": faces}])
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.5, limiter, riemann, 3, True)
+    return perturb(cases.Case("box", cfg, [blk], bc), amp=0.03, seed=seed)
+
+
+@pytest.mark.parametrize("lim", LIMS)
+@pytest.mark.parametrize("rs", RSS)
+def test_residual_periodic_3d(lim, rs):
+    case = perturb(cases.vortex(dims=(20, 12, 9), limiter=lim, riemann=rs), seed=1)
+    _residual_check(case)
+
+
+@pytest.mark.parametrize("lim", LIMS)
+def test_residual_sod_1d(lim):
+    case = perturb(cases.sod(n=64, limiter=lim), seed=2)
+    _residual_check(case)
+
+
+@pytest.mark.parametrize("rs", RSS)
+def test_residual_symmetry_inflow_extrapolation(rs):
+    inflow = {"type": 405, "mach": 1.8, "T": 1.05, "p": 0.7, "alpha": 0.1, "beta": -0.05, "rel_fac": 0.8}
+    case = box_case((18, 11, 7), [inflow, 400, 300, 300, 300, 0], riemann=rs)
+    _residual_check(case)
+
+
+def test_residual_inflow_falls_back_to_symmetry():
+    # flow leaving through the 405 face -> symmetry flux (Mod_BC_Fluxes.f90:159-168)
+    inflow = {"type": 405, "mach": 2.0, "T": 1.0, "p": 0.7, "alpha": 0.0, "beta": 0.0, "rel_fac": 1.0}
+    case = box_case((12, 9, 5), [400, inflow, 300, 400, 0, 300], mach=1.2)
+    _residual_check(case)
+
+
+@pytest.mark.parametrize("rs", RSS)
+def test_residual_multiblock_connections(rs):
+    case = cases.multiblock_chain(nblocks=6, seed=5, lo=6, hi=14, riemann=rs, precision=abi.PREC_FP64)
+    _residual_check(perturb(case, seed=4))
+
+
+def test_residual_two_dimensional_block():
+    case = box_case((24, 16, 1), [400, 400, 300, 300, 0, 0], limiter="vanalbada")
+    _residual_check(case)
+
+
+def test_dt_parity():
+    case = perturb(cases.vortex(dims=(16, 12, 10)), seed=6)
+    g, o = _pair(case)
+    dg, do = g.compute_dt(1), o.compute_dt(1)
+    assert abs(dg - do) <= 1e-15 * do
+
+
+@pytest.mark.parametrize("rs", RSS)
+def test_steps_periodic(rs):
+    case = cases.vortex(dims=(16, 16, 8), riemann=rs, limiter="vanleer")
+    g, o = _pair(case)
+    for _ in range(8):
+        dtg, tg, ng = g.step()
+        dto, to, no = o.step()
+        assert abs(dtg - dto) <= 1e-13 * dto
+        assert abs(tg - to) <= 1e-13 * to
+        # norms of a variable that is constant up to round-off (w) are compared absolutely
+        assert np.allclose(ng, no, rtol=1e-9, atol=1e-12 * np.max(no))
+    err = rel_l2(g.get_state(1), o.get_state(1))
+    assert err < 1e-12, err
+
+
+def test_steps_multiblock():
+    case = cases.multiblock_chain(nblocks=5, seed=9, lo=6, hi=12, precision=abi.PREC_FP64)
+    g, o = _pair(case)
+    for _ in range(5):
+        g.step()
+        o.step()
+    for b in range(1, 6):
+        assert rel_l2(g.get_state(b), o.get_state(b)) < 1e-12
+
+
+def test_advance_equals_steps():
+    case = cases.vortex(dims=(16, 8, 8))
+    g1, g2 = GpuSolver(case), GpuSolver(case)
+    for _ in range(6):
+        g1.step()
+    g2.advance(6)
+    assert np.array_equal(g1.get_state(1), g2.get_state(1))
+    assert g1.time() == g2.time()
+
+
+def test_sod_exact_solution():
+    case = cases.sod(n=1000)
+    g = GpuSolver(case)
+    t = 0.0
+    while t < 0.2:
+        _, t, _ = g.step()
+    P = interior(g.get_state(1))[0, 0]
+    xc = cases.cell_centres(case.blocks[0].nodes)[0, 0, :, 0]
+    table = np.loadtxt(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "sod79_exact.txt"))
+    sie = P[:, 4] / (0.4 * P[:, 0])
+    fails = exact_riemann.probe_check(xc, P[:, 0], P[:, 1], P[:, 4], sie, table, threshold=1.0)
+    assert not fails, fails
+    # same run on the oracle
+    o = OracleSolver(case)
+    to = 0.0
+    while to < 0.2:
+        _, to, _ = o.step()
+    assert abs(to - t) < 1e-12
+    assert rel_l2(g.get_state(1), o.get_state(1)) < 1e-10
+
+
+def test_fp32_flux_against_fp64():
+    c64 = cases.vortex(dims=(32, 32, 8), precision=abi.PREC_FP64)
+    c32 = cases.vortex(dims=(32, 32, 8), precision=abi.PREC_FP32_FLUX)
+    g64, g32 = GpuSolver(c64), GpuSolver(c32)
+    g64.advance(20)
+    g32.advance(20)
+    err = rel_l2(g32.get_state(1), g64.get_state(1))
+    assert err < 1e-5, err
+
+
+def test_error_nonphysical_state_reported():
+    case = cases.vortex(dims=(12, 10, 8))
+    case.blocks[0].P[2 + 3, 2 + 4, 2 + 5, 4] = -1.0      # p < 0 at (i,j,k) = (6,5,4)
+    g = GpuSolver(case)
+    with pytest.raises(MoseError) as ei:
+        g.step()
+    assert ei.value.code in (abi.E_INTERFACE, abi.E_STATE)
+    e = ei.value.err
+    assert abs(e.i - 6) <= 2 and abs(e.j - 5) <= 2 and abs(e.k - 4) <= 2
+    with pytest.raises(OracleError):
+        OracleSolver(case).step()
+
+
+def test_bad_bc_list_rejected():
+    case = cases.vortex(dims=(8, 8, 8))
+    case.bc = case.bc[:-1]
+    with pytest.raises(MoseError) as ei:
+        GpuSolver(case)
+    assert ei.value.code == abi.E_BC

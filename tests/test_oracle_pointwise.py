@@ -1,0 +1,183 @@
+"""Pins of the oracle's point kernels (limiters, reconstruction, Riemann fluxes, thermo)
+against independent definitions, invariants and special cases — never by re-typing the
+oracle's own formula.  CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+
+from mose_amd import abi
+from oracle import oracle as O
+
+CFG = abi.make_cfg()
+G = 1.4
+
+
+def minmod_def(x, y):
+    """Textbook minmod: 0 for opposite signs, else the argument of smaller magnitude."""
+    if x * y < 0:
+        return 0.0
+    return x if abs(x) <= abs(y) else y
+
+
+def vanleer_def(x, y):
+    """Harmonic-mean form of van Leer's limited slope."""
+    return 2 * x * y / (x + y) if x * y > 0 else 0.0
+
+
+def vanalbada_def(x, y):
+    return x * y * (x + y) / (x * x + y * y) if x * x + y * y > 0 else 0.0
+
+
+VALS = [-3.0, -1.0, -0.25, 0.0, 0.25, 1.0, 2.0, 7.5]
+
+
+@pytest.mark.parametrize("x,y", list(itertools.product(VALS, VALS)))
+def test_limiters_against_definitions(x, y):
+    assert O.limiter(abi.LIM_MINMOD, x, y) == minmod_def(x, y)
+    # the reference regularises r = y/(x + 1e-30): agrees with the closed forms to round-off
+    assert O.limiter(abi.LIM_VANLEER, x, y) == pytest.approx(vanleer_def(x, y), rel=1e-14, abs=1e-28)
+    assert O.limiter(abi.LIM_VANALBADA, x, y) == pytest.approx(vanalbada_def(x, y), rel=1e-14, abs=1e-28)
+    assert O.limiter(abi.LIM_FIRST_ORDER, x, y) == 0.0
+
+
+def test_limiter_invariants():
+    rng = np.random.default_rng(0)
+    for x, y in rng.normal(size=(500, 2)):
+        for k in (abi.LIM_MINMOD, abi.LIM_VANLEER):
+            # symmetric and degree-1 homogeneous; equal slopes are kept
+            assert O.limiter(k, x, y) == pytest.approx(O.limiter(k, y, x), rel=1e-13, abs=1e-15)
+            assert O.limiter(k, 3 * x, 3 * y) == pytest.approx(3 * O.limiter(k, x, y), rel=1e-13, abs=1e-15)
+            assert O.limiter(k, x, x) == pytest.approx(x, rel=1e-14)
+        # TVD bound of the minmod/van Leer slopes: same sign as x, magnitude <= 2 min(|x|,|y|)
+        for k in (abi.LIM_MINMOD, abi.LIM_VANLEER):
+            s = O.limiter(k, x, y)
+            if x * y <= 0:
+                assert s == 0.0
+            else:
+                assert np.sign(s) == np.sign(x) and abs(s) <= 2 * min(abs(x), abs(y)) + 1e-15
+
+
+@pytest.mark.parametrize("lim", [abi.LIM_MINMOD, abi.LIM_VANLEER, abi.LIM_VANALBADA])
+def test_reconstruction_exact_for_linear_data(lim):
+    # non-uniform cells: lengths dl; linear field q(x) = a + b x sampled at the centres
+    dl = np.array([0.7, 1.3, 0.9, 1.6])
+    xc = np.cumsum(dl) - 0.5 * dl
+    a, b = 0.3, -1.7
+    q = a + b * xc
+    l0, l1, l2 = 0.5 * (dl[0] + dl[1]), 0.5 * (dl[1] + dl[2]), 0.5 * (dl[2] + dl[3])
+    ql, qr = O.reconstruct(lim, q, l0, l1, l2, 0.5 * dl[1], 0.5 * dl[2])
+    xf = np.sum(dl[:2])
+    assert ql == pytest.approx(a + b * xf, rel=1e-14)
+    assert qr == pytest.approx(a + b * xf, rel=1e-14)
+
+
+def test_reconstruction_extremum_is_first_order():
+    q = np.array([0.0, 1.0, 0.0, 1.0])
+    for lim in (abi.LIM_MINMOD, abi.LIM_VANLEER):
+        ql, qr = O.reconstruct(lim, q, 1, 1, 1, 0.5, 0.5)
+        assert (ql, qr) == (1.0, 0.0)
+
+
+def physical_flux(P, n):
+    rho, u, v, w, p = P
+    un = u * n[0] + v * n[1] + w * n[2]
+    E = p / (G - 1) + 0.5 * rho * (u * u + v * v + w * w)
+    return np.array([rho * un, rho * un * u + p * n[0], rho * un * v + p * n[1], rho * un * w + p * n[2],
+                     (E + p) * un])
+
+
+def rand_state(rng, mach=None):
+    rho = rng.uniform(0.2, 3.0)
+    p = rng.uniform(0.1, 4.0)
+    a = np.sqrt(G * p / rho)
+    vel = rng.normal(size=3)
+    if mach is not None:
+        vel = vel / np.linalg.norm(vel) * mach * a
+    return np.array([rho, *vel, p])
+
+
+def rand_normal(rng):
+    n = rng.normal(size=3)
+    return n / np.linalg.norm(n)
+
+
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP])
+def test_riemann_consistency(rs):
+    cfg = abi.make_cfg(riemann=rs)
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        P = rand_state(rng, mach=rng.uniform(0, 3))
+        n = rand_normal(rng)
+        F = O.riemann(cfg, P, P, n)
+        assert np.allclose(F, physical_flux(P, n), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP])
+def test_riemann_supersonic_upwinding(rs):
+    cfg = abi.make_cfg(riemann=rs)
+    rng = np.random.default_rng(2)
+    n = np.array([1.0, 0.0, 0.0])
+    for _ in range(100):
+        L, R = rand_state(rng), rand_state(rng)
+        aL, aR = np.sqrt(G * L[4] / L[0]), np.sqrt(G * R[4] / R[0])
+        L[1] = 1.3 * (aL + aR) + abs(L[1]); R[1] = 1.3 * (aL + aR) + abs(R[1])
+        F = O.riemann(cfg, L, R, n)
+        assert np.allclose(F, physical_flux(L, n), rtol=1e-12, atol=1e-12)
+        L[1], R[1] = -L[1], -R[1]
+        F = O.riemann(cfg, L, R, n)
+        assert np.allclose(F, physical_flux(R, n), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP])
+def test_riemann_stationary_contact(rs):
+    cfg = abi.make_cfg(riemann=rs)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        p = rng.uniform(0.2, 2)
+        n = rand_normal(rng)
+        t = np.cross(n, rand_normal(rng))        # tangential velocity, equal on both sides
+        L = np.array([rng.uniform(0.1, 3), *t, p])
+        R = np.array([rng.uniform(0.1, 3), *t, p])
+        F = O.riemann(cfg, L, R, n)
+        assert np.allclose(F, [0, p * n[0], p * n[1], p * n[2], 0], atol=1e-13)
+
+
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP])
+def test_riemann_mirror_symmetry(rs):
+    """F(L, R, n) = -F(R, L, -n) componentwise (relabelling the two sides)."""
+    cfg = abi.make_cfg(riemann=rs)
+    rng = np.random.default_rng(4)
+    for _ in range(200):
+        L, R, n = rand_state(rng, rng.uniform(0, 2)), rand_state(rng, rng.uniform(0, 2)), rand_normal(rng)
+        F1 = O.riemann(cfg, L, R, n)
+        F2 = O.riemann(cfg, R, L, -n)
+        assert np.allclose(F1, -F2, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP])
+def test_riemann_moving_contact_is_upwinded(rs):
+    """An isolated contact moving with velocity u (equal u and p, different rho) is resolved
+    exactly: the flux is the physical flux of the upwind side."""
+    cfg = abi.make_cfg(riemann=rs)
+    rng = np.random.default_rng(6)
+    for _ in range(50):
+        n = rand_normal(rng)
+        un = rng.uniform(0.05, 0.6) * rng.choice([-1, 1])
+        t = np.cross(n, rand_normal(rng))
+        vel = un * n + t
+        p = rng.uniform(0.5, 2.0)
+        L = np.array([rng.uniform(0.5, 2.0), *vel, p])
+        R = np.array([rng.uniform(0.5, 2.0), *vel, p])
+        F = O.riemann(cfg, L, R, n)
+        up = L if un > 0 else R
+        assert np.allclose(F, physical_flux(up, n), rtol=1e-12, atol=1e-13)
+
+
+def test_prim_cons_round_trip():
+    rng = np.random.default_rng(5)
+    for _ in range(100):
+        P = rand_state(rng, rng.uniform(0, 5))
+        U = O.prim2cons(CFG, P)
+        assert U[4] == pytest.approx(P[4] / 0.4 + 0.5 * P[0] * np.sum(P[1:4] ** 2), rel=1e-14)
+        assert np.allclose(O.cons2prim(CFG, U), P, rtol=1e-13, atol=1e-14)
