@@ -18,6 +18,9 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP32 division and square root of the FP32-flux kernels use the hardware reciprocal /
+    # square-root instructions (~1 ulp); FP64 arithmetic stays IEEE correctly rounded
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 

@@ -27,8 +27,20 @@ struct GasT {
 };
 
 template <typename T> __device__ __forceinline__ T fabs_(T x) { return x < T(0) ? -x : x; }
+// FP64: IEEE correctly rounded.  FP32 (flux variant only): the hardware square root and
+// reciprocal (v_sqrt_f32, v_rcp_f32, ~1 ulp), without denormal rescaling sequences.
 __device__ __forceinline__ double sqrt_(double x) { return __builtin_sqrt(x); }
-__device__ __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+// FP64 reciprocal: hardware estimate refined by two Newton steps (within 1 ulp of 1/x for
+// normal x; the results differ from IEEE division by at most 1 ulp).
+__device__ __forceinline__ double rcp_(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
 template <typename T> __device__ __forceinline__ T min_(T a, T b) { return a < b ? a : b; }
 template <typename T> __device__ __forceinline__ T max_(T a, T b) { return a > b ? a : b; }
 
@@ -49,10 +61,10 @@ __device__ __forceinline__ T limiter(T x, T y) {
   } else if constexpr (LIM == LIM_VANLEER) {
     T xy = x * y;
     T s = x + y;
-    return (xy > T(0)) ? (T(2) * xy) / s : T(0);
+    return (xy > T(0)) ? (T(2) * xy) * rcp_(s) : T(0);
   } else if constexpr (LIM == LIM_VANALBADA) {
     T q = x * x + y * y;
-    return (q > T(0)) ? (x * y * (x + y)) / q : T(0);
+    return (q > T(0)) ? (x * y * (x + y)) * rcp_(q) : T(0);
   } else {
     return T(0);
   }
@@ -88,7 +100,7 @@ template <typename T>
 __device__ __forceinline__ void hllc(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F) {
   const T rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   const T rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
-  const T irl = T(1) / rl, irr = T(1) / rr;
+  const T irl = rcp_(rl), irr = rcp_(rr);
   const T al = sqrt_(g.gam * pl * irl), ar = sqrt_(g.gam * pr * irr);
   const T unl = ul * nx + vl * ny + wl * nz;
   const T unr = ur * nx + vr * ny + wr * nz;
@@ -98,7 +110,7 @@ __device__ __forceinline__ void hllc(const T* L, const T* R, T nx, T ny, T nz, c
   const T h0r = kgl * pr * irr + T(0.5) * (ur * ur + vr * vr + wr * wr);
   // Roe averages (roe_averages)
   const T srl = sqrt_(rl), srr = sqrt_(rr);
-  const T isr = T(1) / (srl + srr);
+  const T isr = rcp_(srl + srr);
   const T u_roe = (srr * ur + srl * ul) * isr;
   const T v_roe = (srr * vr + srl * vl) * isr;
   const T w_roe = (srr * wr + srl * wl) * isr;
@@ -110,7 +122,7 @@ __device__ __forceinline__ void hllc(const T* L, const T* R, T nx, T ny, T nz, c
   const T S1 = min_(T(0), min_(unl - al, un_roe - a_roe));
   const T S4 = max_(T(0), max_(unr + ar, un_roe + a_roe));
   const T dl = rl * (S1 - unl), dr = rr * (S4 - unr);
-  const T Sstar = (pr - pl + dl * unl - dr * unr) / (dl - dr);
+  const T Sstar = (pr - pl + dl * unl - dr * unr) * rcp_(dl - dr);
   const T pstar = rl * (unl - S1) * (unl - Sstar) + pl;
   // Side selection: S1>=0 -> F_L ; S*>=0 -> F_L + S1(U*L - U_L); S4>0 -> F_R + S4(U*R - U_R); else F_R.
   const bool left = (S1 >= T(0)) || (Sstar >= T(0));
@@ -121,7 +133,7 @@ __device__ __forceinline__ void hllc(const T* L, const T* R, T nx, T ny, T nz, c
   const T Fr = r_ * un_;
   T f0 = Fr, f1 = Fr * u_ + p_ * nx, f2 = Fr * v_ + p_ * ny, f3 = Fr * w_ + p_ * nz, f4 = Fr * h0_;
   if (star) {
-    const T inv = T(1) / (S - Sstar);
+    const T inv = rcp_(S - Sstar);
     const T sd = (S - un_) * r_;
     const T dp = pstar - p_;
     const T U1S = sd * inv;
@@ -143,7 +155,7 @@ __device__ __forceinline__ void ausmp(const T* L, const T* R, T nx, T ny, T nz, 
   const T beta = T(0.125), alpha = T(0.1875);
   const T rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   const T rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
-  const T irl = T(1) / rl, irr = T(1) / rr;
+  const T irl = rcp_(rl), irr = rcp_(rr);
   const T al2 = g.gam * pl * irl, ar2 = g.gam * pr * irr;
   const T unl = ul * nx + vl * ny + wl * nz;
   const T unr = ur * nx + vr * ny + wr * nz;
@@ -151,7 +163,7 @@ __device__ __forceinline__ void ausmp(const T* L, const T* R, T nx, T ny, T nz, 
   const T H_L = kgl * pl * irl + T(0.5) * (ul * ul + vl * vl + wl * wl);
   const T H_R = kgl * pr * irr + T(0.5) * (ur * ur + vr * vr + wr * wr);
   const T a_half = sqrt_(sqrt_(al2 * ar2));            // sqrt(aL aR), Lib_Riemann_AUSM.f90:196
-  const T ia = T(1) / a_half;
+  const T ia = rcp_(a_half);
   const T ML = unl * ia, MR = unr * ia;
   T MplusL, pplusL, MminusR, pminusR;
   if (fabs_(ML) < T(1)) {
@@ -248,7 +260,8 @@ __device__ __forceinline__ void symmetry_flux(const T* Pb, const T* Pi, T bdl, T
 
 // Supersonic inflow face (BC_Inlet_Supersonic_Static + Supersonic_Inflow,
 // Lib_BC_Fluxes_Inflow.f90:244-382).  Returns false when the boundary-cell flow leaves the
-// domain (the caller then uses the symmetry flux).  bc = {mach, T, p, alpha, beta, rel_fac}.
+// domain (the caller then uses the symmetry flux).  bc = {mach, T, p, cos(alpha) cos(beta),
+// sin(alpha) cos(beta), sin(beta), rel_fac} (inflow direction precomputed on the host).
 template <int LIM, int RS, typename T>
 __device__ __forceinline__ bool inflow405_flux(const T* Pb, const T* Pi, bool hi, T nx, T ny, T nz,
                                                const double* bc, const GasT<T>& g, T* F) {
@@ -257,13 +270,12 @@ __device__ __forceinline__ bool inflow405_flux(const T* Pb, const T* Pi, bool hi
   if (sgn * un >= T(0)) return false;
   const T flow_a = sqrt_(g.gam * Pb[4] / Pb[0]);
   const T flow_M = sqrt_(Pb[1] * Pb[1] + Pb[2] * Pb[2] + Pb[3] * Pb[3]) / flow_a;
-  const T rel = T(bc[5]);
+  const T rel = T(bc[6]);
   const T M = rel * T(bc[0]) + (T(1) - rel) * flow_M;
   const T p = rel * T(bc[2]) + (T(1) - rel) * Pb[4];
   const T rho = p / (g.R * T(bc[1]));
   const T a = sqrt_(g.gam * p / rho);
-  const T ca = T(cos(bc[3])), sa = T(sin(bc[3])), cb = T(cos(bc[4])), sb = T(sin(bc[4]));
-  T Sup[5] = {rho, M * a * ca * cb, M * a * sa * cb, M * a * sb, p};
+  T Sup[5] = {rho, M * a * T(bc[3]), M * a * T(bc[4]), M * a * T(bc[5]), p};
   T Fc[5], L[5], R[5];
   const T half = hi ? T(0.5) : T(-0.5);
 #pragma unroll
@@ -282,7 +294,7 @@ __device__ __forceinline__ bool inflow405_flux(const T* Pb, const T* Pi, bool hi
 // Conservative <-> primitive for the calorically perfect gas; always FP64 (the pressure
 // recovery p = (gamma-1)(E - |m|^2/(2 rho)) stays FP64 near vacuum).
 __device__ __forceinline__ void cons2prim(const double* U, double gm1, double* P) {
-  const double ir = 1.0 / U[0];
+  const double ir = rcp_(U[0]);
   const double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
   P[0] = U[0]; P[1] = u; P[2] = v; P[3] = w;
   P[4] = gm1 * (U[4] - 0.5 * (U[1] * u + U[2] * v + U[3] * w));

@@ -36,7 +36,9 @@ struct Layout {
   std::vector<mose::LineEnt> lines;
   std::vector<mose::Tile> tiles;
   std::vector<mose::GhostOp> ghost;
+  std::vector<mose::BcFace> bcfaces;
   std::vector<double> facebc;
+  int64_t nbcslots = 0;
   int64_t npool = 0;
   int ndir = 3;
 };
@@ -216,21 +218,21 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
       }
     }
   }
-  // ---- face spacings (face between q-1 and q), Lib_Convective.f90:30-34
+  // ---- spacings (entry q: cell q and the face q-1/2 between cells q-1 and q),
+  //      Lib_Convective.f90:30-34: lp = dl(q)/2, rl1 = 1 / ((dl(q-1) + dl(q))/2)
   for (int b = 0; b < nb; ++b) {
     const HostBlock& B = hb[b];
     const int n[3] = {B.ni, B.nj, B.nk};
-    for (int d = 0; d < 3; ++d)
-      for (int q = 1; q <= n[d] + 1; ++q) {
+    for (int d = 0; d < 3; ++d) {
+      for (int q = -1; q <= n[d] + 2; ++q) LE(b, d, q).lp = 0.5 * LE(b, d, q).dl;
+      for (int q = 0; q <= n[d] + 2; ++q) {
         LineEnt& e = LE(b, d, q);
-        const double dm2 = LE(b, d, q - 2).dl, dm1 = LE(b, d, q - 1).dl, d0 = LE(b, d, q).dl,
-                     dp1 = LE(b, d, q + 1).dl;
-        e.rl0 = 1.0 / (0.5 * (dm2 + dm1));
-        e.rl1 = 1.0 / (0.5 * (dm1 + d0));
-        e.rl2 = 1.0 / (0.5 * (d0 + dp1));
-        e.lm = 0.5 * dm1;
-        e.lp = 0.5 * d0;
+        e.rl1 = 1.0 / (0.5 * (LE(b, d, q - 1).dl + e.dl));
+        e.lm = 0.5 * LE(b, d, q - 1).dl;
+        if (q >= 1) e.rl0 = 1.0 / (0.5 * (LE(b, d, q - 2).dl + LE(b, d, q - 1).dl));
+        if (q <= n[d] + 1) e.rl2 = 1.0 / (0.5 * (e.dl + LE(b, d, q + 1).dl));
       }
+    }
   }
 
   // ---- block descriptors
@@ -247,11 +249,14 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
     for (int f = 0; f < 6; ++f) {
       D.bct[f] = geo::face_kind(ftype[6 * b + f]);
       D.bcd[f] = -1;
+      D.bcs[f] = -1;
       if (ftype[6 * b + f] == 405) {
         const mose_bc_entry& e = bc[fentry[6 * b + f]];
-        D.bcd[f] = (int)(L.facebc.size() / 6);
-        const double v[6] = {e.mach, e.T, e.p, e.alpha, e.beta, e.rel_fac};
-        L.facebc.insert(L.facebc.end(), v, v + 6);
+        D.bcd[f] = (int)(L.facebc.size() / 8);
+        // Sup velocity direction of Supersonic_Inflow (Lib_BC_Fluxes_Inflow.f90:336-338)
+        const double v[8] = {e.mach, e.T, e.p, std::cos(e.alpha) * std::cos(e.beta),
+                             std::sin(e.alpha) * std::cos(e.beta), std::sin(e.beta), e.rel_fac, 0.0};
+        L.facebc.insert(L.facebc.end(), v, v + 8);
       }
     }
   }
@@ -306,6 +311,39 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
     }
     L.ghost.push_back(op);
   }
+  // ---- boundary faces of non-connection type: one flux slot per face cell, tangential
+  //      indices in cyclic order (d+1, d+2), first one fastest
+  L.bcfaces.clear();
+  int64_t slot = 0;
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock& B = hb[b];
+    const int n[3] = {B.ni, B.nj, B.nk};
+    BlockDev& D = L.blocks[b];
+    for (int f = 0; f < 6; ++f) {
+      if (D.bct[f] == F_CONN) continue;
+      const int d = f / 2, d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+      const bool hi = f & 1;
+      D.bcs[f] = (int)slot;
+      for (int t2 = 1; t2 <= n[d2]; ++t2)
+        for (int t1 = 1; t1 <= n[d1]; ++t1) {
+          int c3[3];
+          c3[d] = hi ? n[d] : 1;
+          c3[d1] = t1;
+          c3[d2] = t2;
+          int m3[3] = {c3[0], c3[1], c3[2]};
+          m3[d] = hi ? n[d] - 1 : 2;
+          BcFace e{};
+          e.cb = pidx(b, c3[0], c3[1], c3[2]);
+          e.cn = pidx(b, m3[0], m3[1], m3[2]);
+          e.b = b; e.f = f; e.slot = (int)(slot + (t1 - 1) + (int64_t)n[d1] * (t2 - 1));
+          e.bdl = 0.5 * LE(b, d, c3[d]).dl;
+          e.idl = 0.5 * LE(b, d, m3[d]).dl;
+          L.bcfaces.push_back(e);
+        }
+      slot += (int64_t)n[d1] * n[d2];
+    }
+  }
+  L.nbcslots = slot;
   (void)ncell;
   return true;
 }

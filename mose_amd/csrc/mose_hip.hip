@@ -2,15 +2,17 @@
 //
 // One explicit step (Explicit_Step, src/lib/numerics/time/explicit/Mod_Explicit.f90:10-146)
 // is this launch sequence on one HIP stream, captured once into a hipGraph:
-//   k_begin    iter += 1, dt accumulator = 1e5                     (Mod_Explicit.f90:36,44)
-//   k_dt       CFL time step, wave-shuffle min + one atomic per wave (Mod_dt.f90:10-107)
-//   k_dtfin    global dt, time += dt when time-accurate             (Mod_Explicit.f90:46-49)
-//   n_rk x { k_ghost  ghost layers of every block face             (Lib_Ghost.f90:11-137)
-//            k_stage  all face fluxes (interior + boundary faces, Mod_Fluxes.f90:46-209 and
-//                     Mod_BC_Fluxes.f90:10-218) and the SSP-RK stage update
-//                     (Lib_Newstate.f90:134-160) fused; the last stage also produces the
-//                     residual-norm partial sums (Mod_Diagnostic.f90:13-50) }
+//   k_begin    iter += 1; dt = CFL time step of the current state; time += dt when
+//              time-accurate; reset the dt accumulator to 1e5     (Mod_Explicit.f90:36-49)
+//   n_rk x { k_ghost   ghost layers of every block face            (Lib_Ghost.f90:11-137)
+//            k_bcface  fluxes of the non-connection boundary faces (Mod_BC_Fluxes.f90:10-218)
+//            k_stage   all other face fluxes (Mod_Fluxes.f90:46-209) and the SSP-RK stage
+//                      update (Lib_Newstate.f90:134-160) fused; the last stage also
+//                      produces the residual-norm partial sums (Mod_Diagnostic.f90:13-50)
+//                      and, from the state it writes, the CFL time step of the next step
+//                      (Compute_dt, Mod_dt.f90:10-107: wave-shuffle min, one atomic per wave) }
 //   k_normfin  fixed-order sum of the partials, sqrt                (Mod_Explicit.f90:111-112)
+// k_dt computes the CFL time step directly when the state did not come from a step.
 // Copy_State (PO = P) needs no pass: U^n stays in its own buffer while the stages run.
 #include <hip/hip_runtime.h>
 
@@ -33,265 +35,7 @@ __device__ __forceinline__ void set_error(int* err, int code, int b, int i, int 
   }
 }
 
-template <typename FT>
-__device__ __forceinline__ void load_prim(const double* U, int64_t N, int64_t c, double gm1, FT* P) {
-  double u[5], p[5];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) u[v] = U[v * N + c];
-  cons2prim(u, gm1, p);
-#pragma unroll
-  for (int v = 0; v < 5; ++v) P[v] = (FT)p[v];
-}
-
-template <typename FT>
-__device__ __forceinline__ FaceSpacing<FT> spacing_of(const LineEnt& e) {
-  return FaceSpacing<FT>{FT(e.rl0), FT(e.rl1), FT(e.rl2), FT(e.lm), FT(e.lp)};
-}
-
-// Flux (times area) through the face between stencil cells q1 and q2, i.e. between cells
-// idx-1 and idx along direction DIR.  bct = F_CONN for interior and connection faces,
-// otherwise the boundary treatment of the block face (hi: max face).
-template <int LIM, int RS, int DIR, typename FT>
-__device__ __forceinline__ bool face_any(int bct, bool hi, const double* bcd, const FT* q0,
-                                         const FT* q1, const FT* q2, const FT* q3,
-                                         const LineEnt* L, int idx, double area,
-                                         const GasT<FT>& g, double* F) {
-  constexpr FT nx = DIR == 0 ? FT(1) : FT(0);
-  constexpr FT ny = DIR == 1 ? FT(1) : FT(0);
-  constexpr FT nz = DIR == 2 ? FT(1) : FT(0);
-  FT f[5];
-  bool ok = true;
-  if (bct == F_CONN) {
-    ok = face_flux<LIM, RS>(q0, q1, q2, q3, spacing_of<FT>(L[idx + 1]), nx, ny, nz, g, f);
-  } else if (bct == F_NONE) {
-#pragma unroll
-    for (int v = 0; v < 5; ++v) f[v] = FT(0);
-  } else {
-    const FT* Pb = hi ? q1 : q2;
-    const FT* Pi = hi ? q0 : q3;
-    if (bct == F_EXTRAP) {
-      physical_flux(Pb, nx, ny, nz, g, f);
-    } else {
-      bool done = false;
-      if (bct == F_INFLOW) done = inflow405_flux<LIM, RS>(Pb, Pi, hi, nx, ny, nz, bcd, g, f);
-      if (!done) {
-        const LineEnt& e = L[idx + 1];
-        const FT bdl = hi ? FT(e.lm) : FT(e.lp);
-        const FT idl = FT(0.5 * (hi ? L[idx - 1].dl : L[idx + 2].dl));
-        symmetry_flux<LIM, RS>(Pb, Pi, bdl, idl, hi, nx, ny, nz, g, f);
-      }
-    }
-  }
-#pragma unroll
-  for (int v = 0; v < 5; ++v) F[v] = double(f[v]) * area;
-  return ok;
-}
-
-// MODE 0: stage update; 1: last stage update + residual-norm partials; 2: residual only.
-template <int LIM, int RS, typename FT, int MODE>
-__global__ void __launch_bounds__(NT) k_stage(StageArgs a) {
-  __shared__ FT sP[5][TY + 4][TX + 4];
-  __shared__ double sFi[5][TY][TX + 1];
-  __shared__ double sFj[5][TY + 1][TX];
-  __shared__ double sRed[NT / 64][5];
-
-  const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
-  const int lane = tid & 63, wave = tid >> 6;
-  const double gm1 = a.gam - 1.0;
-  const GasT<FT> g{FT(a.gam), FT(gm1), FT(1.0 / gm1), FT(a.R), FT(a.gam * a.R / gm1)};
-  const int64_t N = a.npool;
-  const double* Uin = a.Uin;
-
-  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-    const Tile T = a.tiles[t];
-    const BlockDev B = a.blocks[T.b];
-    const int i = T.i0 + tx, j = T.j0 + ty;
-    const bool colmem = (i <= B.ni + 2) && (j <= B.nj + 2);
-    const bool active = (i <= B.ni) && (j <= B.nj);
-    const int64_t cbase = B.off + (int64_t)(i + 1) + (int64_t)B.sy * (j + 1);
-    const LineEnt* Lx = a.lines + B.line[0];
-    const LineEnt* Ly = a.lines + B.line[1];
-    const LineEnt* Lz = a.lines + B.line[2];
-    const int k0 = T.k0, k1 = T.k1;
-
-    FT q[4][5];
-    if (colmem) {
-#pragma unroll
-      for (int s = 0; s < 4; ++s) load_prim(Uin, N, cbase + B.sz * (int64_t)(k0 - 1 + s), gm1, q[s]);
-    }
-    double Fkp[5], Fkn[5], nsum[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) { Fkp[v] = 0.0; nsum[v] = 0.0; }
-    const double Az = active ? Lx[i + 1].delta * Ly[j + 1].delta : 0.0;
-    if (active) {     // face k0 - 1/2
-      const bool bnd = (k0 == 1);
-      const int bct = bnd ? B.bct[4] : F_CONN;
-      const double* bcd = (bnd && B.bcd[4] >= 0) ? a.facebc + 6 * B.bcd[4] : nullptr;
-      if (!face_any<LIM, RS, 2>(bct, false, bcd, q[0], q[1], q[2], q[3], Lz, k0, Az, g, Fkp))
-        set_error(a.err, MOSE_E_INTERFACE, T.b + 1, i, j, k0 - 1);
-    }
-
-    for (int k = k0; k < k1; ++k) {
-      if (colmem) {
-#pragma unroll
-        for (int v = 0; v < 5; ++v) { q[0][v] = q[1][v]; q[1][v] = q[2][v]; q[2][v] = q[3][v]; }
-        load_prim(Uin, N, cbase + B.sz * (int64_t)(k + 3), gm1, q[3]);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) sP[v][ty + 2][tx + 2] = q[1][v];
-      }
-      // 2-cell i/j halo of plane k
-      for (int h = tid; h < 4 * (TX + TY); h += NT) {
-        int hi_, hj;
-        if (h < 2 * TY) { hj = h >> 1; hi_ = -2 + (h & 1); }
-        else if (h < 4 * TY) { const int h2 = h - 2 * TY; hj = h2 >> 1; hi_ = TX + (h2 & 1); }
-        else if (h < 4 * TY + 2 * TX) { const int h2 = h - 4 * TY; hi_ = h2 % TX; hj = -2 + h2 / TX; }
-        else { const int h2 = h - 4 * TY - 2 * TX; hi_ = h2 % TX; hj = TY + h2 / TX; }
-        const int gi = T.i0 + hi_, gj = T.j0 + hj;
-        if (gi <= B.ni + 2 && gj <= B.nj + 2) {
-          FT tmp[5];
-          load_prim(Uin, N, B.off + (int64_t)(gi + 1) + (int64_t)B.sy * (gj + 1) + B.sz * (int64_t)(k + 1), gm1, tmp);
-#pragma unroll
-          for (int v = 0; v < 5; ++v) sP[v][hj + 2][hi_ + 2] = tmp[v];
-        }
-      }
-      __syncthreads();
-
-      if (active) {   // face k + 1/2
-        const bool bnd = (k == B.nk);
-        const int bct = bnd ? B.bct[5] : F_CONN;
-        const double* bcd = (bnd && B.bcd[5] >= 0) ? a.facebc + 6 * B.bcd[5] : nullptr;
-        if (!face_any<LIM, RS, 2>(bct, bnd, bcd, q[0], q[1], q[2], q[3], Lz, k + 1, Az, g, Fkn))
-          set_error(a.err, MOSE_E_INTERFACE, T.b + 1, i, j, k);
-      }
-      // i face on the low side of (i, j)
-      if (j <= B.nj && i <= B.ni + 1) {
-        const int f = (i == 1) ? 0 : ((i == B.ni + 1) ? 1 : -1);
-        const int bct = f >= 0 ? B.bct[f] : F_CONN;
-        const double* bcd = (f >= 0 && B.bcd[f] >= 0) ? a.facebc + 6 * B.bcd[f] : nullptr;
-        FT s[4][5];
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-          for (int v = 0; v < 5; ++v) s[m][v] = sP[v][ty + 2][tx + m];
-        double F[5];
-        const double A = Ly[j + 1].delta * Lz[k + 1].delta;
-        if (!face_any<LIM, RS, 0>(bct, f == 1, bcd, s[0], s[1], s[2], s[3], Lx, i, A, g, F))
-          set_error(a.err, MOSE_E_INTERFACE, T.b + 1, i - 1, j, k);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) sFi[v][ty][tx] = F[v];
-      }
-      if (wave == 0 && lane < TY) {   // i face on the tile's high edge, row `lane`
-        const int jr = T.j0 + lane, ie = T.i0 + TX;
-        if (jr <= B.nj && ie <= B.ni + 1) {
-          const int f = (ie == B.ni + 1) ? 1 : -1;
-          const int bct = f >= 0 ? B.bct[f] : F_CONN;
-          const double* bcd = (f >= 0 && B.bcd[f] >= 0) ? a.facebc + 6 * B.bcd[f] : nullptr;
-          FT s[4][5];
-#pragma unroll
-          for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int v = 0; v < 5; ++v) s[m][v] = sP[v][lane + 2][TX + m];
-          double F[5];
-          const double A = Ly[jr + 1].delta * Lz[k + 1].delta;
-          if (!face_any<LIM, RS, 0>(bct, f == 1, bcd, s[0], s[1], s[2], s[3], Lx, ie, A, g, F))
-            set_error(a.err, MOSE_E_INTERFACE, T.b + 1, ie - 1, jr, k);
-#pragma unroll
-          for (int v = 0; v < 5; ++v) sFi[v][lane][TX] = F[v];
-        }
-      }
-      // j face on the low side of (i, j)
-      if (i <= B.ni && j <= B.nj + 1) {
-        const int f = (j == 1) ? 2 : ((j == B.nj + 1) ? 3 : -1);
-        const int bct = f >= 0 ? B.bct[f] : F_CONN;
-        const double* bcd = (f >= 0 && B.bcd[f] >= 0) ? a.facebc + 6 * B.bcd[f] : nullptr;
-        FT s[4][5];
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-          for (int v = 0; v < 5; ++v) s[m][v] = sP[v][ty + m][tx + 2];
-        double F[5];
-        const double A = Lx[i + 1].delta * Lz[k + 1].delta;
-        if (!face_any<LIM, RS, 1>(bct, f == 3, bcd, s[0], s[1], s[2], s[3], Ly, j, A, g, F))
-          set_error(a.err, MOSE_E_INTERFACE, T.b + 1, i, j - 1, k);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) sFj[v][ty][tx] = F[v];
-      }
-      if (wave == 1 && lane < TX) {   // j face on the tile's high edge, column `lane`
-        const int ic = T.i0 + lane, je = T.j0 + TY;
-        if (ic <= B.ni && je <= B.nj + 1) {
-          const int f = (je == B.nj + 1) ? 3 : -1;
-          const int bct = f >= 0 ? B.bct[f] : F_CONN;
-          const double* bcd = (f >= 0 && B.bcd[f] >= 0) ? a.facebc + 6 * B.bcd[f] : nullptr;
-          FT s[4][5];
-#pragma unroll
-          for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int v = 0; v < 5; ++v) s[m][v] = sP[v][TY + m][lane + 2];
-          double F[5];
-          const double A = Lx[ic + 1].delta * Lz[k + 1].delta;
-          if (!face_any<LIM, RS, 1>(bct, f == 3, bcd, s[0], s[1], s[2], s[3], Ly, je, A, g, F))
-            set_error(a.err, MOSE_E_INTERFACE, T.b + 1, ic, je - 1, k);
-#pragma unroll
-          for (int v = 0; v < 5; ++v) sFj[v][TY][lane] = F[v];
-        }
-      }
-      __syncthreads();
-
-      if (active) {
-        double R[5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v)
-          R[v] = (sFi[v][ty][tx + 1] - sFi[v][ty][tx]) + (sFj[v][ty + 1][tx] - sFj[v][ty][tx]) + (Fkn[v] - Fkp[v]);
-        const int64_t c = cbase + B.sz * (int64_t)(k + 1);
-        if constexpr (MODE == 2) {
-#pragma unroll
-          for (int v = 0; v < 5; ++v) a.Rout[v * N + c] = R[v];
-        } else {
-          double un[5], ui[5], unew[5], P[5];
-#pragma unroll
-          for (int v = 0; v < 5; ++v) { un[v] = a.Un[v * N + c]; ui[v] = Uin[v * N + c]; }
-          const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
-          const double vol = Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta;
-          const double rv = dt / vol;
-#pragma unroll
-          for (int v = 0; v < 5; ++v) unew[v] = un[v] + a.rkc * (ui[v] - un[v] - R[v] * rv);
-          cons2prim(unew, gm1, P);
-          bool bad = !(P[4] >= 0.0);
-#pragma unroll
-          for (int v = 0; v < 5; ++v) bad = bad || (P[v] != P[v]);
-          if (bad) set_error(a.err, MOSE_E_STATE, T.b + 1, i, j, k);
-          if (P[0] < 1e-20) { P[0] = 1e-20; prim2cons(P, 1.0 / gm1, unew); }
-          if constexpr (MODE == 1) {
-            double pn[5];
-            cons2prim(un, gm1, pn);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) { const double d = P[v] - pn[v]; nsum[v] += d * d; }
-          }
-#pragma unroll
-          for (int v = 0; v < 5; ++v) a.Uout[v * N + c] = unew[v];
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < 5; ++v) Fkp[v] = Fkn[v];
-    }
-
-    if constexpr (MODE == 1) {
-#pragma unroll
-      for (int v = 0; v < 5; ++v) {
-        double x = nsum[v];
-        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-        if (lane == 0) sRed[wave][v] = x;
-      }
-      __syncthreads();
-      if (tid < 5) {
-        double s = 0.0;
-        for (int w = 0; w < NT / 64; ++w) s += sRed[w][tid];
-        a.partials[5 * (int64_t)t + tid] = s;
-      }
-      __syncthreads();
-    }
-  }
-}
+#include "mose_stage.hpp"
 
 struct DtArgs {
   const double* U;
@@ -303,15 +47,17 @@ struct DtArgs {
   const int* iter;
   int64_t npool;
   int ntiles, ndir, rampa, local_dt;
+  int tx, ty;        // patch shape of the tiles
+  int iter_add;      // the ramp uses *iter + iter_add
   double cfl, gam;
 };
 
 // CFL time step of every interior cell (Compute_Dt, Mod_dt.f90:61-85 with rectilinear
 // metrics: versor = axis, dl = cell length) and its global minimum.
-__global__ void __launch_bounds__(NT) k_dt(DtArgs a) {
-  const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+__global__ void __launch_bounds__(256) k_dt(DtArgs a) {
+  const int tid = threadIdx.x, tx = tid % a.tx, ty = tid / a.tx;
   const double gm1 = a.gam - 1.0;
-  const int it = *a.iter;
+  const int it = *a.iter + a.iter_add;
   double mymin = 1e300;
   for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
     const Tile T = a.tiles[t];
@@ -343,15 +89,19 @@ __global__ void __launch_bounds__(NT) k_dt(DtArgs a) {
     atomicMin(a.dtmin, (unsigned long long)__double_as_longlong(mymin));
 }
 
-__global__ void k_begin(unsigned long long* dtmin, int* iter) {
+// Start of a step: iter += 1, take the CFL time step computed from the current state (by
+// the previous step's last stage or by k_dt), time += dt when time-accurate, and reset the
+// accumulator (dtglobal = 1e5, Mod_Explicit.f90:36-49) for this step's last stage.
+__global__ void k_begin(unsigned long long* dtmin, int* iter, double* scal, int time_accurate) {
   *iter += 1;
-  *dtmin = (unsigned long long)__double_as_longlong(1e5);
-}
-
-__global__ void k_dtfin(const unsigned long long* dtmin, double* scal, int time_accurate) {
   const double dt = __longlong_as_double((long long)*dtmin);
   scal[S_DT] = dt;
   if (time_accurate) scal[S_TIME] += dt;
+  *dtmin = (unsigned long long)__double_as_longlong(1e5);
+}
+
+__global__ void k_dtreset(unsigned long long* dtmin) {
+  *dtmin = (unsigned long long)__double_as_longlong(1e5);
 }
 
 __global__ void __launch_bounds__(256) k_normfin(const double* partials, int ntiles, double* scal) {
@@ -479,34 +229,60 @@ __global__ void k_copy(const double* __restrict__ s, double* __restrict__ d, int
 namespace {
 
 typedef void (*StageFn)(StageArgs);
+typedef void (*BcFn)(StageArgs, const BcFace*, int);
 
-template <int LIM, int RS, typename FT>
-void stage_table(StageFn* t) {
-  t[0] = k_stage<LIM, RS, FT, 0>;
-  t[1] = k_stage<LIM, RS, FT, 1>;
-  t[2] = k_stage<LIM, RS, FT, 2>;
+// Patch shapes (TX x TY threads) of the stage kernel per precision; shape 0 is the default,
+// the others are compiled for the van Leer limiter and selected by the tile tuner.
+constexpr int NSHAPE = 3;
+constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 8}, {8, 16}}, {{32, 8}, {16, 16}, {16, 8}}};
+
+template <int LIM, int RS, typename FT, int TXc, int TYc>
+void stage_modes(StageFn* t) {
+  t[0] = k_stage<LIM, RS, FT, 0, TXc, TYc>;
+  t[1] = k_stage<LIM, RS, FT, 1, TXc, TYc>;
+  t[2] = k_stage<LIM, RS, FT, 2, TXc, TYc>;
 }
 
-// kernel table [prec][lim][rs][mode]
+template <int LIM, typename FT>
+void bc_table(BcFn* t) {
+  t[0] = k_bcface<LIM, RS_HLLC, FT>;
+  t[1] = k_bcface<LIM, RS_AUSMP, FT>;
+}
+
+// kernel table [prec][lim][rs][shape][mode]; null where a shape is not compiled
 struct KTable {
-  StageFn f[2][4][2][3];
+  StageFn f[2][4][2][NSHAPE][3] = {};
+  BcFn bc[2][4][2] = {};
+  template <int LIM, int RS>
+  void all_shapes() {
+    stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1]>(f[0][LIM][RS][0]);
+    stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1]>(f[1][LIM][RS][0]);
+  }
   KTable() {
-    stage_table<LIM_MINMOD, RS_HLLC, double>(f[0][0][0]);
-    stage_table<LIM_MINMOD, RS_AUSMP, double>(f[0][0][1]);
-    stage_table<LIM_VANLEER, RS_HLLC, double>(f[0][1][0]);
-    stage_table<LIM_VANLEER, RS_AUSMP, double>(f[0][1][1]);
-    stage_table<LIM_VANALBADA, RS_HLLC, double>(f[0][2][0]);
-    stage_table<LIM_VANALBADA, RS_AUSMP, double>(f[0][2][1]);
-    stage_table<LIM_FIRST, RS_HLLC, double>(f[0][3][0]);
-    stage_table<LIM_FIRST, RS_AUSMP, double>(f[0][3][1]);
-    stage_table<LIM_MINMOD, RS_HLLC, float>(f[1][0][0]);
-    stage_table<LIM_MINMOD, RS_AUSMP, float>(f[1][0][1]);
-    stage_table<LIM_VANLEER, RS_HLLC, float>(f[1][1][0]);
-    stage_table<LIM_VANLEER, RS_AUSMP, float>(f[1][1][1]);
-    stage_table<LIM_VANALBADA, RS_HLLC, float>(f[1][2][0]);
-    stage_table<LIM_VANALBADA, RS_AUSMP, float>(f[1][2][1]);
-    stage_table<LIM_FIRST, RS_HLLC, float>(f[1][3][0]);
-    stage_table<LIM_FIRST, RS_AUSMP, float>(f[1][3][1]);
+    bc_table<LIM_MINMOD, double>(bc[0][0]);
+    bc_table<LIM_VANLEER, double>(bc[0][1]);
+    bc_table<LIM_VANALBADA, double>(bc[0][2]);
+    bc_table<LIM_FIRST, double>(bc[0][3]);
+    bc_table<LIM_MINMOD, float>(bc[1][0]);
+    bc_table<LIM_VANLEER, float>(bc[1][1]);
+    bc_table<LIM_VANALBADA, float>(bc[1][2]);
+    bc_table<LIM_FIRST, float>(bc[1][3]);
+    all_shapes<LIM_MINMOD, RS_HLLC>();
+    all_shapes<LIM_MINMOD, RS_AUSMP>();
+    all_shapes<LIM_VANLEER, RS_HLLC>();
+    all_shapes<LIM_VANLEER, RS_AUSMP>();
+    all_shapes<LIM_VANALBADA, RS_HLLC>();
+    all_shapes<LIM_VANALBADA, RS_AUSMP>();
+    all_shapes<LIM_FIRST, RS_HLLC>();
+    all_shapes<LIM_FIRST, RS_AUSMP>();
+    stage_modes<LIM_VANLEER, RS_HLLC, double, kShape[0][1][0], kShape[0][1][1]>(f[0][1][0][1]);
+    stage_modes<LIM_VANLEER, RS_HLLC, double, kShape[0][2][0], kShape[0][2][1]>(f[0][1][0][2]);
+    stage_modes<LIM_VANLEER, RS_AUSMP, double, kShape[0][1][0], kShape[0][1][1]>(f[0][1][1][1]);
+    stage_modes<LIM_VANLEER, RS_AUSMP, double, kShape[0][2][0], kShape[0][2][1]>(f[0][1][1][2]);
+    stage_modes<LIM_VANLEER, RS_HLLC, float, kShape[1][1][0], kShape[1][1][1]>(f[1][1][0][1]);
+    stage_modes<LIM_VANLEER, RS_HLLC, float, kShape[1][2][0], kShape[1][2][1]>(f[1][1][0][2]);
+    stage_modes<LIM_VANLEER, RS_AUSMP, float, kShape[1][1][0], kShape[1][1][1]>(f[1][1][1][1]);
+    stage_modes<LIM_VANLEER, RS_AUSMP, float, kShape[1][2][0], kShape[1][2][1]>(f[1][1][1][2]);
   }
 };
 const KTable& ktable() {
@@ -539,6 +315,8 @@ struct mose_ctx {
   LineEnt* d_lines = nullptr;
   Tile* d_tiles = nullptr;
   GhostOp* d_ghost = nullptr;
+  BcFace* d_bcfaces = nullptr;
+  double* d_bcbuf = nullptr;
   double* d_facebc = nullptr;
   double* d_scal = nullptr;
   double* d_partials = nullptr;
@@ -548,7 +326,10 @@ struct mose_ctx {
   int* d_iter = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  bool dt_primed = false;   // d_dtmin holds the CFL step of the current state
   int grid_stage = 0;
+  int shape = 0;       // index into kShape of the stage kernel in use
+  int tx = 16, ty = 8;
 };
 
 static int fail(mose_ctx* c, int code, int b = 0, int i = 0, int j = 0, int k = 0) {
@@ -579,6 +360,9 @@ static StageArgs stage_args(mose_ctx* c, const double* Uin, const double* Un, do
   a.Uin = Uin; a.Un = Un; a.Uout = Uout; a.Rout = c->Rbuf;
   a.tiles = c->d_tiles; a.blocks = c->d_blocks; a.lines = c->d_lines; a.facebc = c->d_facebc;
   a.scal = c->d_scal; a.dtloc = c->d_dtloc; a.partials = c->d_partials; a.err = c->d_err;
+  a.bcbuf = c->d_bcbuf;
+  a.dtmin = c->d_dtmin; a.dtnext = c->d_dtloc; a.iter = c->d_iter;
+  a.ndir = c->ndir; a.rampa = c->cfg.rampa_cfl_iter; a.cfl = c->cfg.cfl;
   a.npool = c->npool; a.ntiles = (int)c->lay.tiles.size(); a.local_dt = c->cfg.time_accurate ? 0 : 1;
   a.rkc = rkc; a.gam = c->cfg.gamma; a.R = c->cfg.Rgas;
   return a;
@@ -586,31 +370,57 @@ static StageArgs stage_args(mose_ctx* c, const double* Uin, const double* Un, do
 
 static void launch_stage(mose_ctx* c, StageFn fn, StageArgs a) {
   void* args[] = {&a};
-  (void)hipLaunchKernel((const void*)fn, dim3(c->grid_stage), dim3(NT), args, 0, c->stream);
+  (void)hipLaunchKernel((const void*)fn, dim3(c->grid_stage), dim3(c->tx * c->ty), args, 0, c->stream);
+}
+
+// Boundary-face fluxes of the stage input (the BC_Fluxes pass), before the stage kernel.
+static void enqueue_bcfaces(mose_ctx* c, const StageArgs& a) {
+  const int n = (int)c->lay.bcfaces.size();
+  if (n == 0) return;
+  const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
+  BcFn fn = ktable().bc[p][c->cfg.limiter][c->cfg.riemann];
+  StageArgs aa = a;
+  const BcFace* faces = c->d_bcfaces;
+  void* args[] = {&aa, &faces, (void*)&n};
+  (void)hipLaunchKernel((const void*)fn, dim3(launch_blocks(n)), dim3(256), args, 0, c->stream);
 }
 
 static StageFn stage_fn(mose_ctx* c, int mode) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
-  return ktable().f[p][c->cfg.limiter][c->cfg.riemann][mode];
+  return ktable().f[p][c->cfg.limiter][c->cfg.riemann][c->shape][mode];
 }
 
-// One explicit step enqueued on the context stream (no host synchronisation).
-static int enqueue_step(mose_ctx* c) {
-  const int nt = (int)c->lay.tiles.size();
-  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, c->stream, c->d_dtmin, c->d_iter);
+static DtArgs dt_args(mose_ctx* c, int iter_add) {
   DtArgs da{};
   da.U = c->U[0]; da.tiles = c->d_tiles; da.blocks = c->d_blocks; da.lines = c->d_lines;
   da.dtmin = c->d_dtmin; da.dtloc = c->d_dtloc; da.iter = c->d_iter; da.npool = c->npool;
-  da.ntiles = nt; da.ndir = c->ndir; da.rampa = c->cfg.rampa_cfl_iter;
+  da.ntiles = (int)c->lay.tiles.size(); da.ndir = c->ndir; da.rampa = c->cfg.rampa_cfl_iter;
   da.local_dt = c->cfg.time_accurate ? 0 : 1; da.cfl = c->cfg.cfl; da.gam = c->cfg.gamma;
-  hipLaunchKernelGGL(k_dt, dim3(nt), dim3(NT), 0, c->stream, da);
-  hipLaunchKernelGGL(k_dtfin, dim3(1), dim3(1), 0, c->stream, c->d_dtmin, c->d_scal, c->cfg.time_accurate);
+  da.tx = c->tx; da.ty = c->ty; da.iter_add = iter_add;
+  return da;
+}
+
+// CFL time step of the current state for the next step (when the state did not come from a
+// step: after mose_put_state, mose_set_time or mose_compute_dt).
+static void enqueue_prime_dt(mose_ctx* c) {
+  hipLaunchKernelGGL(k_dtreset, dim3(1), dim3(1), 0, c->stream, c->d_dtmin);
+  DtArgs da = dt_args(c, 1);
+  hipLaunchKernelGGL(k_dt, dim3(da.ntiles), dim3(c->tx * c->ty), 0, c->stream, da);
+}
+
+// One explicit step enqueued on the context stream (no host synchronisation).  The CFL
+// time step of the next step is produced by the last RK stage from the state it writes.
+static int enqueue_step(mose_ctx* c) {
+  const int nt = (int)c->lay.tiles.size();
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, c->stream, c->d_dtmin, c->d_iter, c->d_scal,
+                     c->cfg.time_accurate);
   const int nrk = c->cfg.n_rk;
   for (int s = 1; s <= nrk; ++s) {
     double* in = (s == 1) ? c->U[0] : c->U[s - 1];
     double* out = (s == nrk) ? (nrk == 1 ? c->U[1] : c->U[0]) : c->U[s];
     enqueue_ghost(c, in);
     StageArgs a = stage_args(c, in, c->U[0], out, kRK[nrk - 1][s - 1]);
+    enqueue_bcfaces(c, a);
     launch_stage(c, stage_fn(c, s == nrk ? 1 : 0), a);
   }
   if (nrk == 1) {
@@ -704,8 +514,17 @@ int mose_finalize(mose_ctx* c) {
     if (it == c->hb.end()) return fail(c, MOSE_E_ARG, b);
     blocks.push_back(it->second);
   }
+  const int prec = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
+  c->shape = 0;
+  if (const char* sh = getenv("MOSE_TILE_SHAPE")) {
+    const int want = atoi(sh);
+    if (want >= 0 && want < NSHAPE && ktable().f[prec][c->cfg.limiter][c->cfg.riemann][want][0]) c->shape = want;
+  }
+  c->tx = kShape[prec][c->shape][0];
+  c->ty = kShape[prec][c->shape][1];
   GeoError ge;
-  if (!build_layout(blocks, c->bc, TX, TY, c->lay, ge)) return fail(c, ge.code, ge.b, ge.i, ge.j, ge.k);
+  if (!build_layout(blocks, c->bc, c->tx, c->ty, c->lay, ge)) return fail(c, ge.code, ge.b, ge.i, ge.j, ge.k);
+  if (5 * c->lay.npool >= (int64_t)1 << 31) return fail(c, MOSE_E_ARG);   // 32-bit element offsets
   c->npool = c->lay.npool;
   c->ndir = c->lay.ndir;
   // device buffers
@@ -719,6 +538,9 @@ int mose_finalize(mose_ctx* c) {
   if (upload(c, &c->d_tiles, c->lay.tiles)) return c->err.code;
   if (upload(c, &c->d_ghost, c->lay.ghost)) return c->err.code;
   if (upload(c, &c->d_facebc, c->lay.facebc)) return c->err.code;
+  if (upload(c, &c->d_bcfaces, c->lay.bcfaces)) return c->err.code;
+  HIPCHK(c, hipMalloc((void**)&c->d_bcbuf, 5 * std::max<int64_t>(c->lay.nbcslots, 1) * sizeof(double)));
+  HIPCHK(c, hipMemset(c->d_bcbuf, 0, 5 * std::max<int64_t>(c->lay.nbcslots, 1) * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_scal, S_COUNT * sizeof(double)));
   HIPCHK(c, hipMemset(c->d_scal, 0, S_COUNT * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_partials, 5 * std::max<size_t>(c->lay.tiles.size(), 1) * sizeof(double)));
@@ -745,6 +567,7 @@ int mose_put_state(mose_ctx* c, int32_t b, const double* P) {
   hipLaunchKernelGGL(k_put, dim3(launch_blocks(ncell)), dim3(256), 0, c->stream, c->stage, c->U[0], c->npool,
                      B.off, ncell, c->cfg.gamma);
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->dt_primed = false;
   return 0;
 }
 
@@ -762,19 +585,15 @@ int mose_get_state(mose_ctx* c, int32_t b, double* P) {
 
 int mose_compute_dt(mose_ctx* c, int32_t iter, double* dtglobal) {
   if (!c || !c->finalized) return fail(c, MOSE_E_ARG);
-  HIPCHK(c, hipMemcpy(c->d_iter, &iter, sizeof(int), hipMemcpyHostToDevice));
-  const unsigned long long init = 0x40F86A0000000000ull;   // bit pattern of 1e5
-  HIPCHK(c, hipMemcpy(c->d_dtmin, &init, sizeof init, hipMemcpyHostToDevice));
-  DtArgs da{};
-  const int nt = (int)c->lay.tiles.size();
-  da.U = c->U[0]; da.tiles = c->d_tiles; da.blocks = c->d_blocks; da.lines = c->d_lines;
-  da.dtmin = c->d_dtmin; da.dtloc = c->d_dtloc; da.iter = c->d_iter; da.npool = c->npool;
-  da.ntiles = nt; da.ndir = c->ndir; da.rampa = c->cfg.rampa_cfl_iter;
-  da.local_dt = c->cfg.time_accurate ? 0 : 1; da.cfl = c->cfg.cfl; da.gam = c->cfg.gamma;
-  hipLaunchKernelGGL(k_dt, dim3(nt), dim3(NT), 0, c->stream, da);
+  int cur = 0;
+  HIPCHK(c, hipMemcpy(&cur, c->d_iter, sizeof cur, hipMemcpyDeviceToHost));
+  hipLaunchKernelGGL(k_dtreset, dim3(1), dim3(1), 0, c->stream, c->d_dtmin);
+  DtArgs da = dt_args(c, iter - cur);
+  hipLaunchKernelGGL(k_dt, dim3(da.ntiles), dim3(c->tx * c->ty), 0, c->stream, da);
   unsigned long long bits = 0;
   HIPCHK(c, hipMemcpyAsync(&bits, c->d_dtmin, sizeof bits, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->dt_primed = false;
   double dt;
   memcpy(&dt, &bits, sizeof dt);
   if (dtglobal) *dtglobal = dt;
@@ -785,6 +604,10 @@ int mose_advance(mose_ctx* c, int32_t nsteps, double* time_out, double* norms) {
   if (!c || !c->finalized || nsteps < 0) return fail(c, MOSE_E_ARG);
   if (c->err.code) return c->err.code;
   if (ensure_graph(c)) return c->err.code;
+  if (!c->dt_primed && nsteps > 0) {
+    enqueue_prime_dt(c);
+    c->dt_primed = true;
+  }
   for (int s = 0; s < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
   int rc = sync_and_check(c);
   double sc[S_COUNT];
@@ -810,6 +633,7 @@ int mose_residual(mose_ctx* c, int32_t b, double* R) {
   hipLaunchKernelGGL(k_fill, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->Rbuf, n5, 0.0);
   enqueue_ghost(c, c->U[0]);
   StageArgs a = stage_args(c, c->U[0], c->U[0], c->U[1], 0.0);
+  enqueue_bcfaces(c, a);
   launch_stage(c, stage_fn(c, 2), a);
   const BlockDev& B = c->lay.blocks[b - 1];
   const int64_t ncell = B.sz * (B.nk + 4);
@@ -835,6 +659,7 @@ int mose_set_time(mose_ctx* c, double time, int32_t iter) {
   if (!c || !c->finalized) return fail(c, MOSE_E_ARG);
   HIPCHK(c, hipMemcpy(c->d_scal + S_TIME, &time, sizeof time, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_iter, &iter, sizeof iter, hipMemcpyHostToDevice));
+  c->dt_primed = false;
   return 0;
 }
 
@@ -844,19 +669,19 @@ int mose_time_kernels(mose_ctx* c, int32_t nrep, double ms[4]) {
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
   const int nt = (int)c->lay.tiles.size();
-  DtArgs da{};
-  da.U = c->U[0]; da.tiles = c->d_tiles; da.blocks = c->d_blocks; da.lines = c->d_lines;
-  da.dtmin = c->d_dtmin; da.dtloc = c->d_dtloc; da.iter = c->d_iter; da.npool = c->npool;
-  da.ntiles = nt; da.ndir = c->ndir; da.rampa = c->cfg.rampa_cfl_iter;
-  da.local_dt = c->cfg.time_accurate ? 0 : 1; da.cfl = c->cfg.cfl; da.gam = c->cfg.gamma;
+  DtArgs da = dt_args(c, 1);
   for (int which = 0; which < 4; ++which) {
     for (int pass = 0; pass < 2; ++pass) {      // pass 0: warm-up launch, pass 1: timed
       const int reps = pass == 0 ? 1 : nrep;
       HIPCHK(c, hipEventRecord(e0, c->stream));
       for (int r = 0; r < reps; ++r) {
-        if (which == 0) hipLaunchKernelGGL(k_dt, dim3(nt), dim3(NT), 0, c->stream, da);
+        if (which == 0) hipLaunchKernelGGL(k_dt, dim3(nt), dim3(c->tx * c->ty), 0, c->stream, da);
         else if (which == 1) enqueue_ghost(c, c->U[0]);
-        else launch_stage(c, stage_fn(c, which == 2 ? 0 : 1), stage_args(c, c->U[0], c->U[0], c->U[1], 0.25));
+        else {
+          StageArgs sa = stage_args(c, c->U[0], c->U[0], c->U[1], 0.25);
+          if (which == 2) enqueue_bcfaces(c, sa);
+          launch_stage(c, stage_fn(c, which == 2 ? 0 : 1), sa);
+        }
       }
       HIPCHK(c, hipEventRecord(e1, c->stream));
       HIPCHK(c, hipEventSynchronize(e1));
@@ -867,6 +692,7 @@ int mose_time_kernels(mose_ctx* c, int32_t nrep, double ms[4]) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  c->dt_primed = false;
   return sync_and_check(c);
 }
 
@@ -903,7 +729,7 @@ void mose_destroy(mose_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_tiles,
-                  c->d_ghost, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
+                  c->d_ghost, c->d_bcfaces, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
                   c->d_iter};
   for (void* p : bufs)
     if (p) (void)hipFree(p);

@@ -16,10 +16,6 @@
 
 namespace mose {
 
-constexpr int TX = 32;
-constexpr int TY = 8;
-constexpr int NT = TX * TY;
-constexpr int GCL = 2;
 
 // Ghost-op kinds.
 enum { G_COPY = 0, G_ZG = 1, G_EXTRAP = 2, G_SYM = 3 };
@@ -41,7 +37,16 @@ struct BlockDev {
   int ni, nj, nk, sy;
   int line[3];       // LineEnt index of padded position 0 (cell -1) per direction
   int bct[6];        // F_* per face
-  int bcd[6];        // index of the 6-double data record (inflow), or -1
+  int bcd[6];        // index of the 8-double data record (inflow), or -1
+  int bcs[6];        // first slot of the face in the boundary-flux buffer, or -1
+};
+
+// One boundary face cell of a non-connection face (flux computed by k_bcface).
+struct BcFace {
+  int64_t cb, cn;    // pool index of the boundary cell and of its interior neighbour
+  int b, f;          // block (0-based), face (0..5)
+  int slot, pad;     // output slot in the boundary-flux buffer
+  double bdl, idl;   // half lengths of the two cells along the face normal
 };
 
 struct Tile {
@@ -66,13 +71,18 @@ struct StageArgs {
   const LineEnt* __restrict__ lines;
   const double* __restrict__ facebc;
   const double* __restrict__ scal;
-  const double* __restrict__ dtloc;
+  const double* dtloc;          // local time stepping: dt of each cell (same array as dtnext)
   double* __restrict__ partials;
+  double* __restrict__ bcbuf;
   int* err;
+  unsigned long long* dtmin;   // last stage: CFL time step of the next step (min as bits)
+  double* dtnext;              // last stage, local time stepping: dtlocal of the next step
+  const int* iter;
   int64_t npool;
   int ntiles;
   int local_dt;
-  double rkc, gam, R;
+  int ndir, rampa;
+  double rkc, gam, R, cfl;
 };
 
 }  // namespace mose

@@ -1,0 +1,369 @@
+// mose_stage.hpp — the fused flux + SSP-RK stage kernel and the boundary-face kernel
+// (gfx950), included by mose_hip.hip.
+//
+// Stage kernel.  A workgroup of TXc x TYc threads owns a TXc x TYc patch of (i, j) columns
+// of one block and marches along k over a chunk [k0, k1].  Per k-plane:
+//   * the plane's primitive state with a 2-cell i/j halo is staged in LDS; the k-direction
+//     neighbours of each column live in registers, prefetched one plane ahead;
+//   * for each direction in turn (i, j, k): every cell computes its limited slope once,
+//     sigma = phi(s+, s-), and its two face states P -/+ sigma dl/2 (the reference's per-face
+//     reconstruction, Lib_Reconstruction.f90:24-28, evaluated once per cell: the right state
+//     of face i-1/2 and the left state of face i+1/2 come from the same limiter call); the
+//     face states go to LDS, one Riemann solve per face follows, and each cell adds its two
+//     face fluxes (FP64);
+//   * faces on a block boundary of non-connection type take the flux computed by
+//     k_bcface (the BC_Fluxes pass);
+//   * the k faces lag by one plane so every face is solved exactly once; the RK stage
+//     update (Lib_Newstate.f90:134-160, Lib_RK.f90:15-27) of plane k-1 runs at plane k.
+#pragma once
+
+namespace mose {
+
+// Limited slope of one cell along one axis and its two face states.
+// rlm / rlp: reciprocal centre distances to the low / high neighbour, hdl = dl/2.
+template <int LIM, typename FT>
+__device__ __forceinline__ void cell_states(const FT* Pm, const FT* P0, const FT* Pp, FT rlm, FT rlp,
+                                            FT hdl, FT* Sm, FT* Sp) {
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const FT sm = (P0[v] - Pm[v]) * rlm;
+    const FT sp = (Pp[v] - P0[v]) * rlp;
+    const FT sg = limiter<LIM>(sp, sm) * hdl;
+    Sm[v] = P0[v] - sg;
+    Sp[v] = P0[v] + sg;
+  }
+}
+
+template <typename FT>
+__device__ __forceinline__ void prim_at(const double* __restrict__ U, unsigned N, unsigned c, double gm1, FT* P) {
+  double u[5], p[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) u[v] = U[v * N + c];
+  cons2prim(u, gm1, p);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) P[v] = (FT)p[v];
+}
+
+// Boundary faces of non-connection type (BC_Fluxes, Mod_BC_Fluxes.f90:10-218): one thread
+// per boundary face cell; the flux per unit area goes to bcbuf.
+template <int LIM, int RS, typename FT>
+__global__ void __launch_bounds__(256) k_bcface(StageArgs a, const BcFace* __restrict__ faces, int nfaces) {
+  const double gm1 = a.gam - 1.0;
+  const GasT<FT> g{FT(a.gam), FT(gm1), FT(1.0 / gm1), FT(a.R), FT(a.gam * a.R / gm1)};
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nfaces; q += gridDim.x * blockDim.x) {
+    const BcFace e = faces[q];
+    const BlockDev* Bp = a.blocks + e.b;
+    const int d = e.f >> 1;
+    const bool hi = e.f & 1;
+    const int bct = Bp->bct[e.f];
+    FT F[5] = {FT(0), FT(0), FT(0), FT(0), FT(0)};
+    if (bct != F_NONE) {
+      FT Pb[5], Pi[5];
+      prim_at(a.Uin, (unsigned)a.npool, (unsigned)e.cb, gm1, Pb);
+      prim_at(a.Uin, (unsigned)a.npool, (unsigned)e.cn, gm1, Pi);
+      const FT nx = d == 0 ? FT(1) : FT(0), ny = d == 1 ? FT(1) : FT(0), nz = d == 2 ? FT(1) : FT(0);
+      if (bct == F_EXTRAP) {
+        physical_flux(Pb, nx, ny, nz, g, F);
+      } else {
+        bool done = false;
+        if (bct == F_INFLOW) done = inflow405_flux<LIM, RS>(Pb, Pi, hi, nx, ny, nz, a.facebc + 8 * Bp->bcd[e.f], g, F);
+        if (!done) symmetry_flux<LIM, RS>(Pb, Pi, FT(e.bdl), FT(e.idl), hi, nx, ny, nz, g, F);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) a.bcbuf[5 * (int64_t)e.slot + v] = (double)F[v];
+  }
+}
+
+// MODE 0: stage update; 1: last stage (update + residual-norm partials); 2: residual only.
+template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc>
+__global__ void __launch_bounds__(TXc* TYc) k_stage(StageArgs a) {
+  constexpr int NTc = TXc * TYc;
+  constexpr int NI = (TXc + 1) * TYc;            // i faces of a plane patch
+  constexpr int NJ = TXc * (TYc + 1);            // j faces
+  constexpr int NK = TXc * TYc;                  // k faces (one per column)
+  constexpr int KOFF = NI + NJ;
+  constexpr int NF = NI + NJ + NK;
+  constexpr int NH = 4 * (TXc + TYc);            // halo cells of a plane
+  constexpr int NX = 2 * (TXc + TYc);            // cells just outside the patch needing a slope
+  static_assert(NH <= NTc && NX <= NTc, "one halo / edge cell per thread at most");
+  __shared__ FT sP[5][TYc + 4][TXc + 4];
+  __shared__ FT sL[5][NF];
+  __shared__ FT sR[5][NF];
+  // residual-norm sums: LDS for the FP32-flux kernels (register-bound), registers for FP64
+  // (LDS-bound); reduced in a fixed order either way
+  constexpr bool NLDS = MODE == 1 && sizeof(FT) == 4;
+  __shared__ double sN[5][NLDS ? NTc : 1];
+  __shared__ double sW[(NTc + 63) / 64][5];
+
+  const int tid = threadIdx.x, tx = tid % TXc, ty = tid / TXc;
+  const double gm1 = a.gam - 1.0;
+  const GasT<FT> g{FT(a.gam), FT(gm1), FT(1.0 / gm1), FT(a.R), FT(a.gam * a.R / gm1)};
+  const unsigned N = (unsigned)a.npool;
+  const double* __restrict__ Uin = a.Uin;
+
+  // the halo cell this thread stages (LDS coordinates, -2..TX+1 / -2..TY+1)
+  int hx = 0, hy = 0;
+  if (tid < 2 * TYc) { hy = tid >> 1; hx = -2 + (tid & 1); }
+  else if (tid < 4 * TYc) { const int h = tid - 2 * TYc; hy = h >> 1; hx = TXc + (h & 1); }
+  else if (tid < 4 * TYc + 2 * TXc) { const int h = tid - 4 * TYc; hx = h % TXc; hy = -2 + h / TXc; }
+  else if (tid < NH) { const int h = tid - 4 * TYc - 2 * TXc; hx = h % TXc; hy = TYc + h / TXc; }
+  // the cell just outside the patch whose slope this thread computes (direction xd)
+  int xd = 0, xlx = 0, xly = 0;
+  bool xplus = false;
+  if (tid < TYc) { xd = 0; xly = tid; xlx = -1; xplus = true; }
+  else if (tid < 2 * TYc) { xd = 0; xly = tid - TYc; xlx = TXc; }
+  else if (tid < 2 * TYc + TXc) { xd = 1; xlx = tid - 2 * TYc; xly = -1; xplus = true; }
+  else if (tid < NX) { xd = 1; xlx = tid - 2 * TYc - TXc; xly = TYc; }
+
+  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const Tile T = a.tiles[t];
+    const BlockDev* __restrict__ Bp = a.blocks + T.b;
+    const int ni = Bp->ni, nj = Bp->nj, nk = Bp->nk, sy_ = Bp->sy;
+    const unsigned off = (unsigned)Bp->off, sz = (unsigned)Bp->sz;
+    const int i = T.i0 + tx, j = T.j0 + ty;
+    const bool colmem = (i <= ni + 2) && (j <= nj + 2);
+    const bool active = (i <= ni) && (j <= nj);
+    const unsigned cbase = off + (unsigned)(i + 1) + (unsigned)sy_ * (unsigned)(j + 1);
+    const LineEnt* Lx = a.lines + Bp->line[0];
+    const LineEnt* Ly = a.lines + Bp->line[1];
+    const LineEnt* Lz = a.lines + Bp->line[2];
+    const int k0 = T.k0, k1 = T.k1;
+    const int hgi = T.i0 + hx, hgj = T.j0 + hy;
+    const bool hval = tid < NH && hgi <= ni + 2 && hgj <= nj + 2;
+    const unsigned hbase = off + (unsigned)(hgi + 1) + (unsigned)sy_ * (unsigned)(hgj + 1);
+    const int xci = T.i0 + xlx, xcj = T.j0 + xly;
+    const bool xval = tid < NX && (xd == 0 ? (xcj <= nj && xci <= ni + 1) : (xci <= ni && xcj <= nj + 1));
+
+    // column queue P(k), P(k+1), P(k+2); face state P+(k); flux of face k-1/2;
+    // prefetched U(k+3) of the column and U of the next plane's halo cell
+    FT Q0[5], Q1[5], Q2[5], Skp[5], Fkm[5];
+    double Un3[5], Uh[5], Racc[5], nsum[5];
+    double dtmin_t = 1e300;     // last stage: min CFL time step of the new state
+#pragma unroll
+    for (int v = 0; v < 5; ++v) { Fkm[v] = FT(0); Skp[v] = FT(0); Un3[v] = 0.0; Uh[v] = 0.0; nsum[v] = 0.0; }
+    if constexpr (NLDS) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) sN[v][tid] = 0.0;
+    }
+    if (colmem) {
+      FT Qm[5], Sm_[5];
+      prim_at(Uin, N, cbase + sz * (unsigned)(k0 - 1), gm1, Qm);     // P(k0-2)
+      prim_at(Uin, N, cbase + sz * (unsigned)(k0), gm1, Q0);         // P(k0-1)
+      prim_at(Uin, N, cbase + sz * (unsigned)(k0 + 1), gm1, Q1);     // P(k0)
+      prim_at(Uin, N, cbase + sz * (unsigned)(k0 + 2), gm1, Q2);     // P(k0+1)
+      cell_states<LIM>(Qm, Q0, Q1, FT(Lz[k0].rl1), FT(Lz[k0 + 1].rl1), FT(Lz[k0].lp), Sm_, Skp);   // P+(k0-1)
+    }
+    if (hval) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Uh[v] = Uin[v * N + hbase + sz * (unsigned)(k0 + 1)];
+    }
+
+    // k = k0-1 only produces the face k0-1/2; planes k0..k1-1 are updated
+    for (int k = k0 - 1; k < k1; ++k) {
+      const bool plane = k >= k0;
+      if (plane) {       // stage plane k
+        if (colmem) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) sP[v][ty + 2][tx + 2] = Q0[v];
+        }
+        if (hval) {
+          double p[5];
+          cons2prim(Uh, gm1, p);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) sP[v][hy + 2][hx + 2] = (FT)p[v];
+        }
+      }
+      // prefetch the next plane of the column and of the halo
+      if (colmem && k + 1 < k1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Un3[v] = Uin[v * N + cbase + sz * (unsigned)(k + 4)];
+      }
+      if (hval && plane && k + 1 < k1) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Uh[v] = Uin[v * N + hbase + sz * (unsigned)(k + 2)];
+      }
+      __syncthreads();
+
+      // ---- face states: k face k+1/2 of the own column (registers), then i and j from LDS
+      if (active) {
+        FT Sm_[5], Sp_[5];
+        cell_states<LIM>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), Sm_, Sp_);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) { sL[v][KOFF + tid] = Skp[v]; sR[v][KOFF + tid] = Sm_[v]; Skp[v] = Sp_[v]; }
+      }
+      if (plane) {
+        const int npass = tid < NX ? 3 : 2;
+#pragma unroll 1
+        for (int pass = 0; pass < npass; ++pass) {
+          int d, lx, ly;
+          bool ok, wl = true, wr = true;
+          if (pass < 2) {
+            d = pass; lx = tx; ly = ty;
+            ok = d == 0 ? (j <= nj && i <= ni + 1) : (i <= ni && j <= nj + 1);
+          } else {
+            d = xd; lx = xlx; ly = xly; ok = xval; wl = xplus; wr = !xplus;
+          }
+          if (ok) {
+            const int sx = d == 0 ? 1 : 0, sy = d == 0 ? 0 : 1;
+            FT m[5], c[5], p[5], Sm_[5], Sp_[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+              m[v] = sP[v][ly + 2 - sy][lx + 2 - sx];
+              c[v] = sP[v][ly + 2][lx + 2];
+              p[v] = sP[v][ly + 2 + sy][lx + 2 + sx];
+            }
+            const LineEnt* L = d == 0 ? Lx : Ly;
+            const int cc = d == 0 ? T.i0 + lx : T.j0 + ly;
+            cell_states<LIM>(m, c, p, FT(L[cc + 1].rl1), FT(L[cc + 2].rl1), FT(L[cc + 1].lp), Sm_, Sp_);
+            // slots: i face (s, r) -> r (TX+1) + s ; j face (c, s) -> NI + s TX + c
+            const int flo = d == 0 ? ly * (TXc + 1) + lx : NI + ly * TXc + lx;
+            const int fhi = d == 0 ? flo + 1 : flo + TXc;
+            if (wl) {
+#pragma unroll
+              for (int v = 0; v < 5; ++v) sL[v][fhi] = Sp_[v];
+            }
+            if (wr) {
+#pragma unroll
+              for (int v = 0; v < 5; ++v) sR[v][flo] = Sm_[v];
+            }
+          }
+        }
+      }
+      __syncthreads();
+
+      // ---- one Riemann problem per face of the plane
+      for (int f = (plane ? 0 : KOFF) + tid; f < NF; f += NTc) {
+        int d, gi, gj, idx, nd;
+        bool valid;
+        if (f < NI) { d = 0; gi = T.i0 + f % (TXc + 1); gj = T.j0 + f / (TXc + 1); idx = gi; nd = ni;
+                      valid = gj <= nj && gi <= ni + 1; }
+        else if (f < KOFF) { const int h = f - NI; d = 1; gi = T.i0 + h % TXc; gj = T.j0 + h / TXc; idx = gj; nd = nj;
+                             valid = gi <= ni && gj <= nj + 1; }
+        else { const int h = f - KOFF; d = 2; gi = T.i0 + h % TXc; gj = T.j0 + h / TXc; idx = k + 1; nd = nk;
+               valid = gi <= ni && gj <= nj; }
+        if (!valid) continue;
+        const int bf = idx == 1 ? 2 * d : (idx == nd + 1 ? 2 * d + 1 : -1);
+        FT F[5];
+        if (bf >= 0 && Bp->bct[bf] != F_CONN) {
+          // tangential position on the face, cyclic order (d+1, d+2)
+          const int kk = d == 2 ? k + 1 : k;
+          const int t1 = d == 0 ? gj : (d == 1 ? kk : gi);
+          const int t2 = d == 0 ? kk : (d == 1 ? gi : gj);
+          const int n1 = d == 0 ? nj : (d == 1 ? nk : ni);
+          const unsigned s = (unsigned)Bp->bcs[bf] + (unsigned)(t1 - 1) + (unsigned)n1 * (unsigned)(t2 - 1);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) F[v] = (FT)a.bcbuf[5 * s + v];
+        } else {
+          FT Lq[5], Rq[5];
+#pragma unroll
+          for (int v = 0; v < 5; ++v) { Lq[v] = sL[v][f]; Rq[v] = sR[v][f]; }
+          if (!(gas_ok(Lq) && gas_ok(Rq)))
+            set_error(a.err, MOSE_E_INTERFACE, T.b + 1, d == 0 ? gi - 1 : gi, d == 1 ? gj - 1 : gj, k);
+          riemann<RS>(Lq, Rq, d == 0 ? FT(1) : FT(0), d == 1 ? FT(1) : FT(0), d == 2 ? FT(1) : FT(0), g, F);
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) sL[v][f] = F[v];
+      }
+      __syncthreads();
+
+      // ---- residual of the own cell and the RK update of plane k
+      if (active) {
+        const double Az = Lx[i + 1].delta * Ly[j + 1].delta;
+        FT Fk[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) { Fk[v] = sL[v][KOFF + tid]; Racc[v] = Az * ((double)Fk[v] - (double)Fkm[v]); Fkm[v] = Fk[v]; }
+        if (plane) {
+          const int fi = ty * (TXc + 1) + tx, fj = NI + ty * TXc + tx;
+          const double Ax = Ly[j + 1].delta * Lz[k + 1].delta;
+          const double Ay = Lx[i + 1].delta * Lz[k + 1].delta;
+#pragma unroll
+          for (int v = 0; v < 5; ++v)
+            Racc[v] += Ax * ((double)sL[v][fi + 1] - (double)sL[v][fi]) + Ay * ((double)sL[v][fj + TXc] - (double)sL[v][fj]);
+          const unsigned c = cbase + sz * (unsigned)(k + 1);
+          if constexpr (MODE == 2) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) a.Rout[v * N + c] = Racc[v];
+          } else {
+            double un[5], ui[5], unew[5], P[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) { un[v] = a.Un[v * N + c]; ui[v] = Uin[v * N + c]; }
+            const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
+            const double vol = Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta;
+            const double rv = dt / vol;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) unew[v] = un[v] + a.rkc * (ui[v] - un[v] - Racc[v] * rv);
+            cons2prim(unew, gm1, P);
+            bool bad = !(P[4] >= 0.0);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) bad = bad || (P[v] != P[v]);
+            if (bad) set_error(a.err, MOSE_E_STATE, T.b + 1, i, j, k);
+            if (P[0] < 1e-20) { P[0] = 1e-20; prim2cons(P, 1.0 / gm1, unew); }
+            if constexpr (MODE == 1) {
+              // CFL time step of the new state for the next step (Compute_dt, Mod_dt.f90:61-85)
+              const double snd = sqrt(a.gam * P[4] / P[0]);
+              double dtc = 1e8;
+              dtc = fmin(Lx[i + 1].dl / (fabs(P[1]) + snd) * a.cfl, dtc);
+              if (a.ndir > 1) dtc = fmin(Ly[j + 1].dl / (fabs(P[2]) + snd) * a.cfl, dtc);
+              if (a.ndir > 2) dtc = fmin(Lz[k + 1].dl / (fabs(P[3]) + snd) * a.cfl, dtc);
+              const int itn = *a.iter + 1;
+              if (itn < a.rampa) dtc = dtc * itn / a.rampa;
+              if (a.local_dt) a.dtnext[c] = dtc;
+              dtmin_t = fmin(dtmin_t, dtc);
+              double pn[5];
+              cons2prim(un, gm1, pn);
+#pragma unroll
+              for (int v = 0; v < 5; ++v) {
+                const double dd = P[v] - pn[v];
+                if constexpr (NLDS) sN[v][tid] += dd * dd;
+                else nsum[v] += dd * dd;
+              }
+            }
+#pragma unroll
+            for (int v = 0; v < 5; ++v) a.Uout[v * N + c] = unew[v];
+          }
+        }
+      }
+      // advance the column queue
+      if (colmem && k + 1 < k1) {
+        double p[5];
+        cons2prim(Un3, gm1, p);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) { Q0[v] = Q1[v]; Q1[v] = Q2[v]; Q2[v] = (FT)p[v]; }
+      }
+    }
+    __syncthreads();
+
+    if constexpr (MODE == 1) {   // fixed-order reductions over the workgroup: deterministic
+      double tot = 0.0;
+      if constexpr (NLDS) {
+        for (int w = NTc / 2; w > 0; w >>= 1) {
+          if (tid < w) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sN[v][tid] += sN[v][tid + w];
+          }
+          __syncthreads();
+        }
+        if (tid < 5) tot = sN[tid][0];
+      } else {
+        const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+          double x = nsum[v];
+          for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+          if (lane == 0) sW[wave][v] = x;
+        }
+        __syncthreads();
+        if (tid < 5)
+          for (int w = 0; w < (NTc + 63) / 64; ++w) tot += sW[w][tid];
+      }
+      if (tid < 5) a.partials[5 * (int64_t)t + tid] = tot;
+      for (int o = 32; o > 0; o >>= 1) dtmin_t = fmin(dtmin_t, __shfl_xor(dtmin_t, o, 64));
+      if ((tid & 63) == 0 && dtmin_t < 1e300)
+        atomicMin(a.dtmin, (unsigned long long)__double_as_longlong(dtmin_t));
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace mose

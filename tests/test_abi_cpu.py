@@ -1,0 +1,63 @@
+"""CPU checks of the drop-in boundary: the C-ABI library loads, exports every entry point
+declared in include/mose_hip.h, its ctypes structs match the header layout, and without a GPU
+it fails loudly (MOSE_E_NOGPU) instead of falling back to CPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from mose_amd import abi, cases, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mose_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mose_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = lib.load()
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(lib.EXPORTS)
+
+
+def test_struct_layouts_match_header():
+    # mose_bc_entry: 16 int32 + 6 double = 112 bytes; mose_cfg: 3 double + 6 int32 + 4 int32 = 64
+    assert C.sizeof(abi.MoseBcEntry) == 112
+    assert C.sizeof(abi.MoseCfg) == 64
+    assert C.sizeof(abi.MoseErr) == 20
+    assert abi.BC_DTYPE.itemsize == 112
+
+
+def test_strerror_messages_match_reference_stops():
+    L = lib.load()
+    assert L.mose_strerror(abi.E_STATE).decode() == "NaN or p<0 detected"
+    assert "interface" in L.mose_strerror(abi.E_INTERFACE).decode()
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-GPU error path")
+def test_no_gpu_is_an_error_not_a_fallback():
+    case = cases.sod(n=16)
+    with pytest.raises(lib.MoseError) as ei:
+        lib.GpuSolver(case)
+    assert ei.value.code == abi.E_NOGPU
+
+
+def test_bc_record_count_and_order_follow_read_bcfile():
+    c = cases.multiblock_chain(nblocks=6, seed=5, lo=6, hi=14)
+    assert len(c.bc) == cases.expected_bc_count([b.dims for b in c.blocks])
+    key = c.bc["b"].astype(np.int64) * 10 + c.bc["f"]
+    assert np.all(np.diff(key) >= 0)          # sorted by block, then face
+    conn = c.bc[c.bc["type"] == 101]
+    # every connection has its reverse record on the donor side
+    fwd = {(r["b"], r["i"], r["j"], r["k"], r["f"]): (r["bs"], r["is"], r["js"], r["ks"], r["fs"]) for r in conn}
+    for src, dst in fwd.items():
+        assert fwd[dst] == src
