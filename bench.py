@@ -7,10 +7,17 @@ MUSCL van Leer + HLLC, SSP-RK3, CFL 0.8, FP64.  A step = one full explicit step
 (CFL dt + 3 RK stages with ghost fill, fluxes, update, residual norm) over all cells,
 with the state resident in HBM.  value = cell-updates/s = cells x steps / seconds.
 
+Tile shapes are picked by the library's autotuner before the warm-up steps.
+
 Also reported on the same line: the FP32-flux variant of the same workload, the L2 error
 against the advected exact vortex, the roofline of the dominant kernel (the fused
-flux + RK-stage kernel) measured with HIP events on the library's stream, and a CPU
-baseline (the C oracle, single thread, on a bounded sample of the same workload).
+flux + RK-stage kernel) measured with HIP events on the library's stream, the 64-block
+batched case (seeded block sizes in [16,128], FP32 flux vs FP64), and a CPU baseline
+(the C oracle, single thread, on a bounded sample of the same workload).
+
+roofline.traffic is the HBM bytes per launch of the stage kernel read from
+profiles/k_stage_vortex256_traffic.txt (two rocprofv3 --pmc runs, FETCH_SIZE and
+WRITE_SIZE, scaled against a device copy of known size; the file gives the commands).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
 """
@@ -31,6 +38,22 @@ STAGE_BYTES_PER_CELL = 3 * 5 * 8       # read U^(k-1), read U^n, write U^(k): 5 
 STEP_BYTES_PER_CELL = 3 * STAGE_BYTES_PER_CELL + 5 * 8   # + the CFL pass reading U^n
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stage_vortex256_traffic.txt")
+
+
+def measured_traffic(prec):
+    """HBM bytes per stage-kernel launch from the committed rocprofv3 summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == prec:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -47,6 +70,9 @@ def run_gpu(case, steps, warmup, label):
     t0 = time.perf_counter()
     g = GpuSolver(case)
     log("[bench] %s: setup %.1f s, %d cells" % (label, time.perf_counter() - t0, case.ncells))
+    tiling, tune_ms = g.autotune(3)
+    log("[bench] %s: autotune %s (%.3f ms for 3 stages)" % (label, tiling, tune_ms))
+    g.tiling = tiling
     g.advance(warmup)
     t0 = time.perf_counter()
     t_sim, norms = g.advance(steps)     # synchronises at the end
@@ -54,6 +80,33 @@ def run_gpu(case, steps, warmup, label):
     ms = g.time_kernels(nrep=10)
     log("[bench] %s: %d steps %.3f s; kernel ms dt/ghost/stage/last = %s" % (label, steps, dt, ms))
     return g, dt, t_sim, ms
+
+
+def batched(steps, warmup):
+    """64 seeded blocks (dims uniform in [16,128], seed 1234) joined by 1-to-1 connections,
+    Mach 0.8 free stream + 5 % Gaussian bumps, van Leer + HLLC, CFL 0.5: FP32 flux vs FP64,
+    all blocks in one stage launch."""
+    from mose_amd import abi, cases
+    out = {}
+    states = {}
+    for prec, name in ((abi.PREC_FP64, "fp64"), (abi.PREC_FP32_FLUX, "fp32_flux")):
+        t0 = time.perf_counter()
+        case = cases.multiblock_chain(precision=prec)
+        log("[bench] batched %s: case built in %.1f s" % (name, time.perf_counter() - t0))
+        g, secs, t_sim, ms = run_gpu(case, steps, warmup, "batched " + name)
+        states[name] = np.concatenate([abi.interior(g.get_state(b + 1)).ravel()
+                                       for b in range(len(case.blocks))])
+        out[name] = {"value": case.ncells * steps / secs, "ms_per_step": 1e3 * secs / steps,
+                     "stage_kernel_ms": ms[2], "tiling": g.tiling,
+                     "stage_frac": STAGE_BYTES_PER_CELL * case.ncells / (ms[2] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        out["cells"] = case.ncells
+        g.close()
+    out["blocks"] = 64
+    out["steps"] = steps
+    out["speedup_fp32_vs_fp64"] = out["fp32_flux"]["value"] / out["fp64"]["value"]
+    out["rel_l2_fp32_vs_fp64"] = float(np.linalg.norm(states["fp32_flux"] - states["fp64"])
+                                       / np.linalg.norm(states["fp64"]))
+    return out
 
 
 def cpu_baseline(n=64, steps=20):
@@ -81,10 +134,12 @@ def main():
     ap.add_argument("--riemann", default="HLLC")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this path runs on one GPU; multi-GPU block sharding is a design only "
-              "(see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: this path runs on one GPU; sharding blocks over several GPUs is not "
+              "built. Run with --gpus 1.", file=sys.stderr)
         return 2
 
     from mose_amd import abi, cases
@@ -96,6 +151,7 @@ def main():
     cups = ncell * args.steps / secs
     err64 = l2_vs_exact(g, case, t_sim)
     P64 = abi.interior(g.get_state(1))
+    tiling64 = g.tiling
     g.close()
     del g
 
@@ -116,9 +172,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "isentropic vortex %d^3 periodic block, MUSCL van Leer + %s, SSP-RK3, "
                                "CFL 0.8, FP64 state and flux" % (n, args.riemann),
-                   "cells": ncell, "riemann": args.riemann, "limiter": "vanleer"},
+                   "cells": ncell, "riemann": args.riemann, "limiter": "vanleer", "tiling": tiling64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("fp64"),
                      "kernel": "k_stage (fused face fluxes + RK stage), %d B/cell algorithmic"
                                % STAGE_BYTES_PER_CELL,
                      "kernel_ms": stage_ms},
@@ -138,9 +194,13 @@ def main():
             "speedup_vs_fp64": cups32 / cups,
             "rel_l2_vs_fp64": float(np.linalg.norm(P32 - P64) / np.linalg.norm(P64)),
             "stage_kernel_ms": ms32[2],
+            "tiling": g32.tiling,
+            "traffic": measured_traffic("fp32"),
             "stage_frac": STAGE_BYTES_PER_CELL * ncell / (ms32[2] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "step_roofline_frac": STEP_BYTES_PER_CELL * cups32 / (HBM_PEAK_GBS * 1e9),
         }
+    if not args.no_batched:
+        result["batched_64_blocks"] = batched(args.batched_steps, min(args.warmup, 2))
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)

@@ -123,6 +123,14 @@ int  mose_set_time(mose_ctx* ctx, double time, int32_t iter);
  * no counterpart in the reference (which has no per-kernel timers). */
 int  mose_time_kernels(mose_ctx* ctx, int32_t nrep, double ms[4]);
 
+/* Autotuner: times the three stage launches of one step (on the current state, U^n not
+ * modified) for every compiled tile shape, k-chunk of 8/16/32/64 planes and both tile orders
+ * (linear, XCD-aware), keeps the fastest and re-captures the step with it.  choice (may be
+ * NULL) receives {TX*1000+TY, k-chunk, xcd order}, best_ms the stage time of the winner
+ * summed over the three launches.  No counterpart in the reference (its loops have fixed
+ * OpenMP schedules). */
+int  mose_autotune(mose_ctx* ctx, int32_t nrep, int32_t choice[3], double* best_ms);
+
 /* Total interior cells over all blocks. */
 int64_t mose_num_cells(const mose_ctx* ctx);
 

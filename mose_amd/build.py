@@ -45,7 +45,7 @@ def build_fortran():
         return None
     out = os.path.join(fdir, "mose_run")
     run([FC, "-O2", "-o", out, "mose_hip_iface.f90", "mose_explicit.f90", "mose_run.f90",
-         "-L" + HERE, "-lmose_hip", "-Wl,-rpath," + HERE], cwd=fdir)
+         "-L" + HERE, "-lmose_hip", "-Wl,-rpath,$ORIGIN/.."], cwd=fdir)
     return out
 
 

@@ -79,8 +79,42 @@ static int face_kind(int type) {
 
 }  // namespace geo
 
-static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mose_bc_entry>& bc, int TXc,
-                         int TYc, Layout& L, GeoError& err) {
+// Work list of the stage kernels: TX x TY column patches of every block, each marching a
+// chunk of kc k-planes.  Spatial order is block, k chunk, j, i.  With xcd != 0 the list is
+// permuted so that workgroup w (dispatched to accelerator complex w mod 8) gets the
+// (w / 8)-th tile of the x-th eighth of the spatial order, x = w mod 8: each XCD then
+// sweeps one contiguous region, and the i/j halos a tile shares with its neighbours are
+// read through the same L2.
+static void build_tiles(Layout& L, int TXc, int TYc, int kc, int xcd) {
+  using namespace mose;
+  std::vector<Tile> S;
+  for (int b = 0; b < (int)L.blocks.size(); ++b) {
+    const BlockDev& B = L.blocks[b];
+    for (int k0 = 1; k0 <= B.nk; k0 += kc)
+      for (int j0 = 1; j0 <= B.nj; j0 += TYc)
+        for (int i0 = 1; i0 <= B.ni; i0 += TXc) {
+          Tile t{};
+          t.b = b; t.i0 = i0; t.j0 = j0; t.k0 = k0; t.k1 = std::min(k0 + kc, B.nk + 1);
+          S.push_back(t);
+        }
+  }
+  if (!xcd) {
+    L.tiles.swap(S);
+    return;
+  }
+  const int64_t n = (int64_t)S.size(), q = n / 8, r = n % 8;
+  L.tiles.assign(n, Tile{});
+  // XCD x owns S[start(x), start(x) + q + (x < r)); position slot*8 + x holds its slot-th tile.
+  // Only the last row (slot = q) is partial, for x < r, so positions 0..n-1 are all filled.
+  for (int64_t w = 0; w < n; ++w) {
+    const int64_t x = w % 8, slot = w / 8;
+    const int64_t start = x * q + std::min<int64_t>(x, r);
+    L.tiles[w] = S[start + slot];
+  }
+}
+
+static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mose_bc_entry>& bc, Layout& L,
+                         GeoError& err) {
   using namespace mose;
   const int nb = (int)hb.size();
   if (nb < 1) return geo::set_err(err, MOSE_E_ARG, 0);
@@ -261,23 +295,6 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
     }
   }
 
-  // ---- tiles: TX x TY column patches, k chunks
-  int64_t ncell = 0;
-  for (const HostBlock& B : hb) ncell += (int64_t)B.ni * B.nj * B.nk;
-  int kc = 32;
-  if (const char* s = getenv("MOSE_KCHUNK")) kc = std::max(1, atoi(s));
-  L.tiles.clear();
-  for (int b = 0; b < nb; ++b) {
-    const HostBlock& B = hb[b];
-    for (int k0 = 1; k0 <= B.nk; k0 += kc)
-      for (int j0 = 1; j0 <= B.nj; j0 += TYc)
-        for (int i0 = 1; i0 <= B.ni; i0 += TXc) {
-          Tile t{};
-          t.b = b; t.i0 = i0; t.j0 = j0; t.k0 = k0; t.k1 = std::min(k0 + kc, B.nk + 1);
-          L.tiles.push_back(t);
-        }
-  }
-
   // ---- ghost operations, one per bc record (Fill_Ghost_Cell)
   auto pidx = [&](int b, int i, int j, int k) -> int64_t {
     const BlockDev& D = L.blocks[b];
@@ -344,6 +361,5 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
     }
   }
   L.nbcslots = slot;
-  (void)ncell;
   return true;
 }

@@ -330,6 +330,8 @@ struct mose_ctx {
   int grid_stage = 0;
   int shape = 0;       // index into kShape of the stage kernel in use
   int tx = 16, ty = 8;
+  int kchunk = 32;     // k-planes per tile
+  int xcd = 1;         // XCD-aware tile order
 };
 
 static int fail(mose_ctx* c, int code, int b = 0, int i = 0, int j = 0, int k = 0) {
@@ -467,6 +469,40 @@ static int ensure_stage(mose_ctx* c, int64_t n) {
   return 0;
 }
 
+static bool shape_compiled(const mose_ctx* c, int shape) {
+  const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
+  return shape >= 0 && shape < NSHAPE && ktable().f[p][c->cfg.limiter][c->cfg.riemann][shape][0] != nullptr;
+}
+
+// Rebuild the work list for another tile shape / k chunk / order (buffers sized to the
+// new tile count); the captured step is discarded and re-captured on next use.
+static int retile(mose_ctx* c, int shape, int kchunk, int xcd) {
+  const int prec = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->shape = shape;
+  c->tx = kShape[prec][shape][0];
+  c->ty = kShape[prec][shape][1];
+  c->kchunk = kchunk;
+  c->xcd = xcd;
+  const size_t old = c->lay.tiles.size();
+  build_tiles(c->lay, c->tx, c->ty, kchunk, xcd);
+  const size_t nt = c->lay.tiles.size();
+  if (c->d_tiles) HIPCHK(c, hipFree(c->d_tiles));
+  c->d_tiles = nullptr;
+  if (upload(c, &c->d_tiles, c->lay.tiles)) return c->err.code;
+  if (!c->d_partials || nt > old) {
+    if (c->d_partials) HIPCHK(c, hipFree(c->d_partials));
+    HIPCHK(c, hipMalloc((void**)&c->d_partials, 5 * std::max<size_t>(nt, 1) * sizeof(double)));
+  }
+  c->grid_stage = (int)nt;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->dt_primed = false;
+  return 0;
+}
+
 // ============================================================================= C ABI
 extern "C" {
 
@@ -516,14 +552,10 @@ int mose_finalize(mose_ctx* c) {
   }
   const int prec = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
   c->shape = 0;
-  if (const char* sh = getenv("MOSE_TILE_SHAPE")) {
-    const int want = atoi(sh);
-    if (want >= 0 && want < NSHAPE && ktable().f[prec][c->cfg.limiter][c->cfg.riemann][want][0]) c->shape = want;
-  }
-  c->tx = kShape[prec][c->shape][0];
-  c->ty = kShape[prec][c->shape][1];
+  c->tx = kShape[prec][0][0];
+  c->ty = kShape[prec][0][1];
   GeoError ge;
-  if (!build_layout(blocks, c->bc, c->tx, c->ty, c->lay, ge)) return fail(c, ge.code, ge.b, ge.i, ge.j, ge.k);
+  if (!build_layout(blocks, c->bc, c->lay, ge)) return fail(c, ge.code, ge.b, ge.i, ge.j, ge.k);
   if (5 * c->lay.npool >= (int64_t)1 << 31) return fail(c, MOSE_E_ARG);   // 32-bit element offsets
   c->npool = c->lay.npool;
   c->ndir = c->lay.ndir;
@@ -535,7 +567,6 @@ int mose_finalize(mose_ctx* c) {
   HIPCHK(c, hipMalloc((void**)&c->Rbuf, 5 * c->npool * sizeof(double)));
   if (upload(c, &c->d_blocks, c->lay.blocks)) return c->err.code;
   if (upload(c, &c->d_lines, c->lay.lines)) return c->err.code;
-  if (upload(c, &c->d_tiles, c->lay.tiles)) return c->err.code;
   if (upload(c, &c->d_ghost, c->lay.ghost)) return c->err.code;
   if (upload(c, &c->d_facebc, c->lay.facebc)) return c->err.code;
   if (upload(c, &c->d_bcfaces, c->lay.bcfaces)) return c->err.code;
@@ -543,7 +574,6 @@ int mose_finalize(mose_ctx* c) {
   HIPCHK(c, hipMemset(c->d_bcbuf, 0, 5 * std::max<int64_t>(c->lay.nbcslots, 1) * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_scal, S_COUNT * sizeof(double)));
   HIPCHK(c, hipMemset(c->d_scal, 0, S_COUNT * sizeof(double)));
-  HIPCHK(c, hipMalloc((void**)&c->d_partials, 5 * std::max<size_t>(c->lay.tiles.size(), 1) * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_dtmin, sizeof(unsigned long long)));
   HIPCHK(c, hipMalloc((void**)&c->d_err, 8 * sizeof(int)));
   HIPCHK(c, hipMemset(c->d_err, 0, 8 * sizeof(int)));
@@ -553,7 +583,13 @@ int mose_finalize(mose_ctx* c) {
     HIPCHK(c, hipMalloc((void**)&c->d_dtloc, c->npool * sizeof(double)));
     HIPCHK(c, hipMemset(c->d_dtloc, 0, c->npool * sizeof(double)));
   }
-  c->grid_stage = (int)c->lay.tiles.size();
+  // tiling: default shape 0, 32-plane chunks, XCD-aware order; MOSE_TILE_SHAPE,
+  // MOSE_KCHUNK and MOSE_TILE_XCD override (mose_autotune picks by measurement)
+  int shape = 0, kc = 32, xcd = 1;
+  if (const char* e = getenv("MOSE_TILE_SHAPE")) shape = shape_compiled(c, atoi(e)) ? atoi(e) : 0;
+  if (const char* e = getenv("MOSE_KCHUNK")) kc = std::max(1, atoi(e));
+  if (const char* e = getenv("MOSE_TILE_XCD")) xcd = atoi(e) ? 1 : 0;
+  if (retile(c, shape, kc, xcd)) return c->err.code;
   c->finalized = true;
   return 0;
 }
@@ -693,6 +729,53 @@ int mose_time_kernels(mose_ctx* c, int32_t nrep, double ms[4]) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->dt_primed = false;
+  return sync_and_check(c);
+}
+
+int mose_autotune(mose_ctx* c, int32_t nrep, int32_t choice[3], double* best_ms) {
+  if (!c || !c->finalized || nrep < 1) return fail(c, MOSE_E_ARG);
+  if (c->err.code) return c->err.code;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  const int kcs[4] = {8, 16, 32, 64};
+  double best = 1e300;
+  int bs = c->shape, bk = c->kchunk, bx = c->xcd;
+  for (int shape = 0; shape < NSHAPE; ++shape) {
+    if (!shape_compiled(c, shape)) continue;
+    for (int kc : kcs)
+      for (int xcd = 0; xcd < 2; ++xcd) {
+        if (retile(c, shape, kc, xcd)) return c->err.code;
+        double t = 0.0;
+        for (int pass = 0; pass < 2; ++pass) {   // pass 0 warms up
+          const int reps = pass == 0 ? 1 : nrep;
+          HIPCHK(c, hipEventRecord(e0, c->stream));
+          for (int r = 0; r < reps; ++r)
+            for (int m = 0; m < 3; ++m) {     // the stage sequence of one RK3 step
+              StageArgs sa = stage_args(c, c->U[0], c->U[0], c->U[1], 0.25);
+              launch_stage(c, stage_fn(c, m == 2 ? 1 : 0), sa);
+            }
+          HIPCHK(c, hipEventRecord(e1, c->stream));
+          HIPCHK(c, hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+          t = (double)ms / reps;
+        }
+        if (t < best) {
+          best = t;
+          bs = shape; bk = kc; bx = xcd;
+        }
+      }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (retile(c, bs, bk, bx)) return c->err.code;
+  if (choice) {
+    choice[0] = c->tx * 1000 + c->ty;
+    choice[1] = bk;
+    choice[2] = bx;
+  }
+  if (best_ms) *best_ms = best;
   return sync_and_check(c);
 }
 

@@ -77,7 +77,9 @@ __global__ void __launch_bounds__(256) k_bcface(StageArgs a, const BcFace* __res
 
 // MODE 0: stage update; 1: last stage (update + residual-norm partials); 2: residual only.
 template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc>
-__global__ void __launch_bounds__(TXc* TYc) k_stage(StageArgs a) {
+__global__ void __launch_bounds__(TXc* TYc)
+    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 && TXc * TYc == 256 ? 3 : 1)))
+    k_stage(StageArgs a) {
   constexpr int NTc = TXc * TYc;
   constexpr int NI = (TXc + 1) * TYc;            // i faces of a plane patch
   constexpr int NJ = TXc * (TYc + 1);            // j faces
@@ -94,7 +96,6 @@ __global__ void __launch_bounds__(TXc* TYc) k_stage(StageArgs a) {
   // (LDS-bound); reduced in a fixed order either way
   constexpr bool NLDS = MODE == 1 && sizeof(FT) == 4;
   __shared__ double sN[5][NLDS ? NTc : 1];
-  __shared__ double sW[(NTc + 63) / 64][5];
 
   const int tid = threadIdx.x, tx = tid % TXc, ty = tid / TXc;
   const double gm1 = a.gam - 1.0;
@@ -347,15 +348,16 @@ __global__ void __launch_bounds__(TXc* TYc) k_stage(StageArgs a) {
         if (tid < 5) tot = sN[tid][0];
       } else {
         const int lane = tid & 63, wave = tid >> 6;
+        double* sW = reinterpret_cast<double*>(&sL[0][0]);     // face states are dead here
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
           double x = nsum[v];
           for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
-          if (lane == 0) sW[wave][v] = x;
+          if (lane == 0) sW[5 * wave + v] = x;
         }
         __syncthreads();
         if (tid < 5)
-          for (int w = 0; w < (NTc + 63) / 64; ++w) tot += sW[w][tid];
+          for (int w = 0; w < (NTc + 63) / 64; ++w) tot += sW[5 * w + tid];
       }
       if (tid < 5) a.partials[5 * (int64_t)t + tid] = tot;
       for (int o = 32; o > 0; o >>= 1) dtmin_t = fmin(dtmin_t, __shfl_xor(dtmin_t, o, 64));

@@ -7,6 +7,7 @@ HIP events (mose_time_kernels), then runs a few full steps.  Used under rocprofv
     rocprofv3 --pmc FETCH_SIZE -d <dir> -- python3 -m mose_amd.kernelbench --n 128 --reps 3
 
 Options: --prec fp64|fp32, --riemann HLLC|AUSM+, --limiter vanleer|minmod|vanalbada,
+--autotune picks the tile shape / k chunk / tile order by timing first,
 --copy also runs a plain device-to-device copy of 5 n^3 doubles (hipMemcpy) so the HBM
 byte counters (FETCH_SIZE / WRITE_SIZE) can be calibrated against a kernel of known size.
 """
@@ -25,6 +26,7 @@ def main():
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--copy", action="store_true", help="also time a plain copy kernel of 5 n^3 doubles")
+    ap.add_argument("--autotune", action="store_true", help="run the tile autotuner first")
     args = ap.parse_args()
     from mose_amd import abi, cases
     from mose_amd.lib import GpuSolver
@@ -32,6 +34,10 @@ def main():
     case = cases.vortex(n=args.n, riemann=args.riemann, limiter=args.limiter, precision=prec)
     g = GpuSolver(case)
     print("[kernelbench] setup done: %d cells" % case.ncells, flush=True)
+    tuned = None
+    if args.autotune:
+        tuned = g.autotune(3)
+        print("[kernelbench] autotune: %s" % (tuned,), flush=True)
     ms = g.time_kernels(args.reps)
     t0 = time.perf_counter()
     g.advance(args.steps)
@@ -39,7 +45,8 @@ def main():
     out = {"n": args.n, "prec": args.prec, "riemann": args.riemann, "limiter": args.limiter,
            "ms_dt": ms[0], "ms_ghost": ms[1], "ms_stage": ms[2], "ms_stage_last": ms[3],
            "steps": args.steps, "s_steps": dt,
-           "stage_GBs_algorithmic": 120.0 * case.ncells / (ms[2] * 1e-3) / 1e9}
+           "stage_GBs_algorithmic": 120.0 * case.ncells / (ms[2] * 1e-3) / 1e9,
+           "tiling": tuned[0] if tuned else None}
     if args.copy:
         import ctypes as C
         hip = C.CDLL("libamdhip64.so")

@@ -18,7 +18,7 @@ _LIB = None
 
 EXPORTS = ["mose_create", "mose_add_block", "mose_set_bc", "mose_finalize", "mose_put_state",
            "mose_get_state", "mose_compute_dt", "mose_step", "mose_advance", "mose_residual",
-           "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_num_cells", "mose_last_error", "mose_strerror",
+           "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_num_cells", "mose_last_error", "mose_strerror",
            "mose_destroy"]
 
 
@@ -45,6 +45,7 @@ def load():
     L.mose_get_time.argtypes = [v, P(C.c_double), P(C.c_int32)]
     L.mose_set_time.argtypes = [v, C.c_double, C.c_int32]
     L.mose_time_kernels.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_autotune.argtypes = [v, C.c_int32, P(C.c_int32), P(C.c_double)]
     L.mose_num_cells.argtypes = [v]
     L.mose_num_cells.restype = C.c_int64
     L.mose_last_error.argtypes = [v, P(abi.MoseErr)]
@@ -155,6 +156,15 @@ class GpuSolver:
         ms = np.zeros(4)
         self._check(load().mose_time_kernels(self.h, nrep, _dp(ms)))
         return ms
+
+    def autotune(self, nrep=3):
+        """Pick the fastest tile shape / k chunk / tile order by timing; returns
+        ({"tile": (TX, TY), "kchunk": kc, "xcd_order": bool}, ms of the three stage launches)."""
+        ch = np.zeros(3, dtype=np.int32)
+        ms = np.zeros(1)
+        self._check(load().mose_autotune(self.h, nrep, ch.ctypes.data_as(C.POINTER(C.c_int32)), _dp(ms)))
+        return {"tile": (int(ch[0]) // 1000, int(ch[0]) % 1000), "kchunk": int(ch[1]),
+                "xcd_order": bool(ch[2])}, float(ms[0])
 
     def num_cells(self):
         return load().mose_num_cells(self.h)
