@@ -82,6 +82,29 @@ def run_gpu(case, steps, warmup, label):
     return g, dt, t_sim, ms
 
 
+def adjoint_ratio(g, case, nrep=5):
+    """Device time of one discrete-adjoint evaluation (mose_residual_vjp) over one residual
+    evaluation (mose_residual) on the current state, seeded with Rbar = 1 on every cell."""
+    Rbars = [np.pad(np.ones(tuple(b.dims[::-1]) + (5,)), ((2, 2), (2, 2), (2, 2), (0, 0))) for b in case.blocks]
+    g.residual_vjp(Rbars)
+    res_ms, vjp_ms = g.time_residual_vjp(nrep)
+    return {"residual_ms": res_ms, "vjp_ms": vjp_ms, "ratio": vjp_ms / res_ms}
+
+
+def bump_adjoint():
+    """Adjoint/primal ratio on the 512 x 256 body-fitted bump channel (van Albada + HLLC,
+    405 inflow, 400 outflow, 300 walls, local time stepping), after 20 steps."""
+    from mose_amd import cases
+    from mose_amd.lib import GpuSolver
+    case = cases.bump_channel()
+    g = GpuSolver(case)
+    g.advance(20)
+    out = adjoint_ratio(g, case, nrep=20)
+    out["cells"] = case.ncells
+    g.close()
+    return out
+
+
 def batched(steps, warmup):
     """64 seeded blocks (dims uniform in [16,128], seed 1234) joined by 1-to-1 connections,
     Mach 0.8 free stream + 5 % Gaussian bumps, van Leer + HLLC, CFL 0.5: FP32 flux vs FP64,
@@ -136,6 +159,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; sharding blocks over several GPUs is not "
@@ -152,6 +176,7 @@ def main():
     err64 = l2_vs_exact(g, case, t_sim)
     P64 = abi.interior(g.get_state(1))
     tiling64 = g.tiling
+    adj_vortex = adjoint_ratio(g, case) if not args.no_adjoint else None
     g.close()
     del g
 
@@ -199,6 +224,9 @@ def main():
             "stage_frac": STAGE_BYTES_PER_CELL * ncell / (ms32[2] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "step_roofline_frac": STEP_BYTES_PER_CELL * cups32 / (HBM_PEAK_GBS * 1e9),
         }
+    if adj_vortex is not None:
+        result["adjoint"] = {"vortex": adj_vortex, "bump_512x256": bump_adjoint(),
+                             "note": "device ms of one mose_residual_vjp / one mose_residual, FP64"}
     if not args.no_batched:
         result["batched_64_blocks"] = batched(args.batched_steps, min(args.warmup, 2))
     if not args.no_cpu_baseline:

@@ -112,6 +112,21 @@ int  mose_advance(mose_ctx* ctx, int32_t nsteps, double* time_out, double* norms
  * This is the value the reference holds in blk%R right before RK_Newstate. */
 int  mose_residual(mose_ctx* ctx, int32_t b, double* R);
 
+/* Discrete adjoint of mose_residual with respect to the primitive state (no counterpart in
+ * the reference, which has no adjoint; it is the backward of the residual for gradient
+ * checks and adjoint-based sensitivities).  Usage: mose_put_residual_bar for every block
+ * (Rbar in the blk%R layout; ghost entries ignored), mose_residual_vjp, then
+ * mose_get_state_bar for every block: Pbar = (dR/dP)^T Rbar in the blk%P layout (ghost
+ * entries 0), R being the residual of the current state including its ghost fill.
+ * Limitation: ghost cells filled from other ghost cells (extrapolation across a block
+ * thinner than three cells) are treated as constants. */
+int  mose_put_residual_bar(mose_ctx* ctx, int32_t b, const double* Rbar);
+int  mose_residual_vjp(mose_ctx* ctx);
+int  mose_get_state_bar(mose_ctx* ctx, int32_t b, double* Pbar);
+/* Average device ms of one residual evaluation (ms[0]) and one adjoint evaluation (ms[1])
+ * over nrep back-to-back runs on the current state (profiling). */
+int  mose_time_residual_vjp(mose_ctx* ctx, int32_t nrep, double ms[2]);
+
 /* Simulation clock and iteration counter (domain%time, domain%iter). */
 int  mose_get_time(mose_ctx* ctx, double* time, int32_t* iter);
 int  mose_set_time(mose_ctx* ctx, double time, int32_t iter);

@@ -1,0 +1,768 @@
+// mose_adjoint.hpp — reverse-mode (discrete adjoint) of the face-flux math, FP64, gfx950.
+//
+// Each *_adj routine recomputes its forward pass (branch decisions included: limiter
+// regimes, min/max of the wave speeds, the HLLC side, AUSM+ Mach regimes — nothing is
+// stored by the primal) and then propagates the output adjoint back to its inputs,
+// accumulating (+=) into the input-adjoint arrays.  Forward formulas: mose_device.hpp.
+#pragma once
+
+namespace mose {
+
+// d phi / d(x, y) of limiter<LIM>(x, y), times gb, added to xb / yb.
+template <int LIM>
+__device__ __forceinline__ void limiter_adj(double x, double y, double gb, double& xb, double& yb) {
+  if constexpr (LIM == LIM_MINMOD) {
+    if (x * y < 0.0) return;
+    if (fabs(x) > fabs(y)) yb += gb;
+    else xb += gb;
+  } else if constexpr (LIM == LIM_VANLEER) {
+    const double xy = x * y;
+    if (!(xy > 0.0)) return;
+    const double s = x + y, is2 = 1.0 / (s * s);
+    xb += gb * 2.0 * y * y * is2;     // d(2xy/(x+y))/dx = 2 y^2/(x+y)^2
+    yb += gb * 2.0 * x * x * is2;
+  } else if constexpr (LIM == LIM_VANALBADA) {
+    const double q = x * x + y * y;
+    if (!(q > 0.0)) return;
+    const double n = x * y * (x + y), iq = 1.0 / q;
+    // f = n / q, n = x^2 y + x y^2
+    const double nx = 2.0 * x * y + y * y, ny = x * x + 2.0 * x * y;
+    xb += gb * (nx * q - n * 2.0 * x) * iq * iq;
+    yb += gb * (ny * q - n * 2.0 * y) * iq * iq;
+  }
+}
+
+// Per-cell slopes and face states (cell_states): Sm = P0 - sigma hdl, Sp = P0 + sigma hdl,
+// sigma = phi((Pp - P0) rlp, (P0 - Pm) rlm).  Adjoints Smb / Spb -> Pmb, P0b, Ppb.
+template <int LIM>
+__device__ __forceinline__ void cell_states_adj(const double* Pm, const double* P0, const double* Pp, double rlm,
+                                                double rlp, double hdl, const double* Smb, const double* Spb,
+                                                double* Pmb, double* P0b, double* Ppb) {
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double sm = (P0[v] - Pm[v]) * rlm;
+    const double sp = (Pp[v] - P0[v]) * rlp;
+    const double gb = (Spb[v] - Smb[v]) * hdl;
+    double spb = 0.0, smb = 0.0;
+    limiter_adj<LIM>(sp, sm, gb, spb, smb);
+    P0b[v] += Smb[v] + Spb[v] - spb * rlp + smb * rlm;
+    Ppb[v] += spb * rlp;
+    Pmb[v] -= smb * rlm;
+  }
+}
+
+// ------------------------------------------------------------------------------ HLLC
+__device__ void hllc_adj(const double* L, const double* R, double nx, double ny, double nz, const GasT<double>& g,
+                         const double* Fb, double* Lb, double* Rb) {
+  const double rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
+  const double rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
+  // ---- forward
+  const double irl = 1.0 / rl, irr = 1.0 / rr;
+  const double al = sqrt(g.gam * pl * irl), ar = sqrt(g.gam * pr * irr);
+  const double unl = ul * nx + vl * ny + wl * nz, unr = ur * nx + vr * ny + wr * nz;
+  const double kg = g.gam * g.rgm1;
+  const double h0l = kg * pl * irl + 0.5 * (ul * ul + vl * vl + wl * wl);
+  const double h0r = kg * pr * irr + 0.5 * (ur * ur + vr * vr + wr * wr);
+  const double srl = sqrt(rl), srr = sqrt(rr), isr = 1.0 / (srl + srr);
+  const double u_roe = (srr * ur + srl * ul) * isr, v_roe = (srr * vr + srl * vl) * isr, w_roe = (srr * wr + srl * wl) * isr;
+  const double h0_roe = (srr * h0r + srl * h0l) * isr;
+  const double a_roe = sqrt(g.gm1 * (h0_roe - 0.5 * (u_roe * u_roe + v_roe * v_roe + w_roe * w_roe)));
+  const double un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+  const double e0l = h0l - pl * irl, e0r = h0r - pr * irr;
+  const double x1 = unl - al, y1 = un_roe - a_roe;
+  const double m1 = min_(x1, y1);
+  const double S1 = min_(0.0, m1);
+  const double x4 = unr + ar, y4 = un_roe + a_roe;
+  const double m4 = max_(x4, y4);
+  const double S4 = max_(0.0, m4);
+  const double dl = rl * (S1 - unl), dr = rr * (S4 - unr);
+  const double Nn = pr - pl + dl * unl - dr * unr, Dd = dl - dr;
+  const double Sstar = Nn / Dd;
+  const double pstar = rl * (unl - S1) * (unl - Sstar) + pl;
+  const bool left = (S1 >= 0.0) || (Sstar >= 0.0);
+  const double r_ = left ? rl : rr, u_ = left ? ul : ur, v_ = left ? vl : vr, w_ = left ? wl : wr;
+  const double p_ = left ? pl : pr, un_ = left ? unl : unr, h0_ = left ? h0l : h0r, e0_ = left ? e0l : e0r;
+  const double S = left ? S1 : S4;
+  const bool star = left ? (S1 < 0.0) : (S4 > 0.0);
+  const double Fr = r_ * un_;
+  // ---- reverse: side quantities
+  double r_b = 0, u_b = 0, v_b = 0, w_b = 0, p_b = 0, un_b = 0, h0_b = 0, e0_b = 0, S_b = 0;
+  double pstar_b = 0, Sstar_b = 0;
+  if (star) {
+    const double q = S - Sstar, inv = 1.0 / q;
+    const double sd = (S - un_) * r_;
+    const double dp = pstar - p_;
+    const double n2 = sd * u_ + dp * nx, n3 = sd * v_ + dp * ny, n4 = sd * w_ + dp * nz;
+    const double n5 = sd * e0_ - p_ * un_ + pstar * Sstar;
+    const double U1S = sd * inv, U2S = n2 * inv, U3S = n3 * inv, U4S = n4 * inv, U5S = n5 * inv;
+    // f_k += S (UkS - Uk), U = (r, r u, r v, r w, r e0)
+    S_b += Fb[0] * (U1S - r_) + Fb[1] * (U2S - r_ * u_) + Fb[2] * (U3S - r_ * v_) + Fb[3] * (U4S - r_ * w_) +
+           Fb[4] * (U5S - r_ * e0_);
+    const double U1b = Fb[0] * S, U2b = Fb[1] * S, U3b = Fb[2] * S, U4b = Fb[3] * S, U5b = Fb[4] * S;
+    r_b -= S * (Fb[0] + Fb[1] * u_ + Fb[2] * v_ + Fb[3] * w_ + Fb[4] * e0_);
+    u_b -= S * Fb[1] * r_;
+    v_b -= S * Fb[2] * r_;
+    w_b -= S * Fb[3] * r_;
+    e0_b -= S * Fb[4] * r_;
+    const double sd_b = (U1b + U2b * u_ + U3b * v_ + U4b * w_ + U5b * e0_) * inv;
+    const double inv_b = U1b * sd + U2b * n2 + U3b * n3 + U4b * n4 + U5b * n5;
+    u_b += U2b * inv * sd;
+    v_b += U3b * inv * sd;
+    w_b += U4b * inv * sd;
+    e0_b += U5b * inv * sd;
+    const double dp_b = (U2b * nx + U3b * ny + U4b * nz) * inv;
+    p_b -= U5b * inv * un_;
+    un_b -= U5b * inv * p_;
+    pstar_b += U5b * inv * Sstar;
+    Sstar_b += U5b * inv * pstar;
+    pstar_b += dp_b;
+    p_b -= dp_b;
+    S_b += sd_b * r_;
+    un_b -= sd_b * r_;
+    r_b += sd_b * (S - un_);
+    const double q_b = -inv * inv * inv_b;
+    S_b += q_b;
+    Sstar_b -= q_b;
+  }
+  {
+    const double Fr_b = Fb[0] + Fb[1] * u_ + Fb[2] * v_ + Fb[3] * w_ + Fb[4] * h0_;
+    u_b += Fb[1] * Fr;
+    v_b += Fb[2] * Fr;
+    w_b += Fb[3] * Fr;
+    h0_b += Fb[4] * Fr;
+    p_b += Fb[1] * nx + Fb[2] * ny + Fb[3] * nz;
+    r_b += Fr_b * un_;
+    un_b += Fr_b * r_;
+  }
+  // scatter the side adjoints
+  double rlb = 0, ulb = 0, vlb = 0, wlb = 0, plb = 0, unlb = 0, h0lb = 0, e0lb = 0, S1b = 0;
+  double rrb = 0, urb = 0, vrb = 0, wrb = 0, prb = 0, unrb = 0, h0rb = 0, e0rb = 0, S4b = 0;
+  if (left) { rlb = r_b; ulb = u_b; vlb = v_b; wlb = w_b; plb = p_b; unlb = un_b; h0lb = h0_b; e0lb = e0_b; S1b = S_b; }
+  else { rrb = r_b; urb = u_b; vrb = v_b; wrb = w_b; prb = p_b; unrb = un_b; h0rb = h0_b; e0rb = e0_b; S4b = S_b; }
+  // pstar = rl (unl - S1)(unl - Sstar) + pl
+  {
+    const double a1 = unl - S1, a2 = unl - Sstar;
+    rlb += pstar_b * a1 * a2;
+    unlb += pstar_b * rl * (a2 + a1);
+    S1b -= pstar_b * rl * a2;
+    Sstar_b -= pstar_b * rl * a1;
+    plb += pstar_b;
+  }
+  // Sstar = Nn / Dd
+  double dlb = 0, drb = 0;
+  {
+    const double Nb = Sstar_b / Dd, Db = -Sstar_b * Sstar / Dd;
+    prb += Nb;
+    plb -= Nb;
+    dlb += Nb * unl;
+    unlb += Nb * dl;
+    drb -= Nb * unr;
+    unrb -= Nb * dr;
+    dlb += Db;
+    drb -= Db;
+  }
+  rlb += dlb * (S1 - unl);
+  S1b += dlb * rl;
+  unlb -= dlb * rl;
+  rrb += drb * (S4 - unr);
+  S4b += drb * rr;
+  unrb -= drb * rr;
+  // S1 = min_(0, min_(x1, y1)), S4 = max_(0, max_(x4, y4)): the selected argument gets the adjoint
+  double alb = 0, arb = 0, un_roe_b = 0, a_roe_b = 0;
+  if (!(0.0 < m1)) {
+    if (x1 < y1) { unlb += S1b; alb -= S1b; }
+    else { un_roe_b += S1b; a_roe_b -= S1b; }
+  }
+  if (!(0.0 > m4)) {
+    if (x4 > y4) { unrb += S4b; arb += S4b; }
+    else { un_roe_b += S4b; a_roe_b += S4b; }
+  }
+  // e0 = h0 - p / rho
+  double irlb = 0, irrb = 0;
+  h0lb += e0lb; plb -= e0lb * irl; irlb -= e0lb * pl;
+  h0rb += e0rb; prb -= e0rb * irr; irrb -= e0rb * pr;
+  // Roe averages
+  double u_roe_b = un_roe_b * nx, v_roe_b = un_roe_b * ny, w_roe_b = un_roe_b * nz;
+  if (a_roe > 0.0) {
+    const double tb = a_roe_b * 0.5 / a_roe * g.gm1;
+    const double h0_roe_b0 = tb;
+    u_roe_b -= tb * u_roe;
+    v_roe_b -= tb * v_roe;
+    w_roe_b -= tb * w_roe;
+    // h0_roe
+    const double hb = h0_roe_b0;
+    h0rb += hb * srr * isr;
+    h0lb += hb * srl * isr;
+    double srrb = hb * h0r * isr, srlb = hb * h0l * isr;
+    double isrb = hb * (srr * h0r + srl * h0l);
+    srrb += (u_roe_b * ur + v_roe_b * vr + w_roe_b * wr) * isr;
+    srlb += (u_roe_b * ul + v_roe_b * vl + w_roe_b * wl) * isr;
+    urb += u_roe_b * srr * isr; vrb += v_roe_b * srr * isr; wrb += w_roe_b * srr * isr;
+    ulb += u_roe_b * srl * isr; vlb += v_roe_b * srl * isr; wlb += w_roe_b * srl * isr;
+    isrb += u_roe_b * (srr * ur + srl * ul) + v_roe_b * (srr * vr + srl * vl) + w_roe_b * (srr * wr + srl * wl);
+    const double sb = -isr * isr * isrb;
+    srlb += sb;
+    srrb += sb;
+    rlb += srlb * 0.5 / srl;
+    rrb += srrb * 0.5 / srr;
+  } else {
+    // a_roe = 0: only the un_roe path (through u_roe) carries sensitivity
+    urb += u_roe_b * srr * isr; vrb += v_roe_b * srr * isr; wrb += w_roe_b * srr * isr;
+    ulb += u_roe_b * srl * isr; vlb += v_roe_b * srl * isr; wlb += w_roe_b * srl * isr;
+    double srrb = (u_roe_b * ur + v_roe_b * vr + w_roe_b * wr) * isr;
+    double srlb = (u_roe_b * ul + v_roe_b * vl + w_roe_b * wl) * isr;
+    const double isrb = u_roe_b * (srr * ur + srl * ul) + v_roe_b * (srr * vr + srl * vl) + w_roe_b * (srr * wr + srl * wl);
+    const double sb = -isr * isr * isrb;
+    srlb += sb;
+    srrb += sb;
+    rlb += srlb * 0.5 / srl;
+    rrb += srrb * 0.5 / srr;
+  }
+  // h0 = kg p / rho + |u|^2 / 2
+  plb += h0lb * kg * irl; irlb += h0lb * kg * pl;
+  ulb += h0lb * ul; vlb += h0lb * vl; wlb += h0lb * wl;
+  prb += h0rb * kg * irr; irrb += h0rb * kg * pr;
+  urb += h0rb * ur; vrb += h0rb * vr; wrb += h0rb * wr;
+  // un = u . n
+  ulb += unlb * nx; vlb += unlb * ny; wlb += unlb * nz;
+  urb += unrb * nx; vrb += unrb * ny; wrb += unrb * nz;
+  // a = sqrt(gam p / rho)
+  {
+    const double tl = alb * 0.5 / al * g.gam, tr = arb * 0.5 / ar * g.gam;
+    plb += tl * irl; irlb += tl * pl;
+    prb += tr * irr; irrb += tr * pr;
+  }
+  rlb -= irlb * irl * irl;
+  rrb -= irrb * irr * irr;
+  Lb[0] += rlb; Lb[1] += ulb; Lb[2] += vlb; Lb[3] += wlb; Lb[4] += plb;
+  Rb[0] += rrb; Rb[1] += urb; Rb[2] += vrb; Rb[3] += wrb; Rb[4] += prb;
+}
+
+// ------------------------------------------------------------------------------ AUSM+
+__device__ void ausmp_adj(const double* L, const double* R, double nx, double ny, double nz, const GasT<double>& g,
+                          const double* Fb, double* Lb, double* Rb) {
+  const double beta = 0.125, alpha = 0.1875;
+  const double rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
+  const double rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
+  // ---- forward
+  const double irl = 1.0 / rl, irr = 1.0 / rr;
+  const double al2 = g.gam * pl * irl, ar2 = g.gam * pr * irr;
+  const double unl = ul * nx + vl * ny + wl * nz, unr = ur * nx + vr * ny + wr * nz;
+  const double kg = g.gam * g.rgm1;
+  const double H_L = kg * pl * irl + 0.5 * (ul * ul + vl * vl + wl * wl);
+  const double H_R = kg * pr * irr + 0.5 * (ur * ur + vr * vr + wr * wr);
+  const double prod = al2 * ar2, sp = sqrt(prod), a_half = sqrt(sp);
+  const double ia = 1.0 / a_half;
+  const double ML = unl * ia, MR = unr * ia;
+  double MplusL, pplusL, dMplusL, dpplusL;   // values and d/dML
+  if (fabs(ML) < 1.0) {
+    const double a = (ML + 1.0) * (ML + 1.0), b = (ML * ML - 1.0) * (ML * ML - 1.0);
+    const double da = 2.0 * (ML + 1.0), db = 4.0 * ML * (ML * ML - 1.0);
+    MplusL = 0.25 * a + beta * b;
+    pplusL = 0.25 * a * (2.0 - ML) + alpha * ML * b;
+    dMplusL = 0.25 * da + beta * db;
+    dpplusL = 0.25 * da * (2.0 - ML) - 0.25 * a + alpha * b + alpha * ML * db;
+  } else {
+    MplusL = 0.5 * (ML + fabs(ML));
+    pplusL = ML >= 0.0 ? 1.0 : 0.0;
+    dMplusL = ML >= 0.0 ? 1.0 : 0.0;
+    dpplusL = 0.0;
+  }
+  double MminusR, pminusR, dMminusR, dpminusR;
+  if (fabs(MR) < 1.0) {
+    const double a = (MR - 1.0) * (MR - 1.0), b = (MR * MR - 1.0) * (MR * MR - 1.0);
+    const double da = 2.0 * (MR - 1.0), db = 4.0 * MR * (MR * MR - 1.0);
+    MminusR = -0.25 * a - beta * b;
+    pminusR = 0.25 * a * (2.0 + MR) - alpha * MR * b;
+    dMminusR = -0.25 * da - beta * db;
+    dpminusR = 0.25 * da * (2.0 + MR) + 0.25 * a - alpha * b - alpha * MR * db;
+  } else {
+    MminusR = 0.5 * (MR - fabs(MR));
+    pminusR = MR >= 0.0 ? 0.0 : 1.0;
+    dMminusR = MR >= 0.0 ? 0.0 : 1.0;
+    dpminusR = 0.0;
+  }
+  const double Mhalf = MplusL + MminusR;
+  const double phalf = pplusL * pl + pminusR * pr;
+  const double mp = max_(Mhalf, 0.0), mm = min_(Mhalf, 0.0);
+  const double mdotp = rl * a_half * mp, mdotm = rr * a_half * mm;
+  // ---- reverse
+  const double mdotp_b = Fb[0] + Fb[1] * ul + Fb[2] * vl + Fb[3] * wl + Fb[4] * H_L;
+  const double mdotm_b = Fb[0] + Fb[1] * ur + Fb[2] * vr + Fb[3] * wr + Fb[4] * H_R;
+  const double phalf_b = Fb[1] * nx + Fb[2] * ny + Fb[3] * nz;
+  double ulb = Fb[1] * mdotp, vlb = Fb[2] * mdotp, wlb = Fb[3] * mdotp, HLb = Fb[4] * mdotp;
+  double urb = Fb[1] * mdotm, vrb = Fb[2] * mdotm, wrb = Fb[3] * mdotm, HRb = Fb[4] * mdotm;
+  double rlb = mdotp_b * a_half * mp, rrb = mdotm_b * a_half * mm;
+  double a_half_b = mdotp_b * rl * mp + mdotm_b * rr * mm;
+  const double Mhalf_b = (Mhalf > 0.0 ? mdotp_b * rl * a_half : 0.0) + (Mhalf < 0.0 ? mdotm_b * rr * a_half : 0.0);
+  double plb = phalf_b * pplusL, prb = phalf_b * pminusR;
+  const double MLb = Mhalf_b * dMplusL + phalf_b * pl * dpplusL;
+  const double MRb = Mhalf_b * dMminusR + phalf_b * pr * dpminusR;
+  // ML = unl / a_half, MR = unr / a_half
+  const double unlb = MLb * ia, unrb = MRb * ia;
+  a_half_b -= (MLb * unl + MRb * unr) * ia * ia;
+  // a_half = (al2 ar2)^(1/4)
+  const double prod_b = a_half_b * 0.5 / a_half * 0.5 / sp;
+  const double al2b = prod_b * ar2, ar2b = prod_b * al2;
+  double irlb = al2b * g.gam * pl, irrb = ar2b * g.gam * pr;
+  plb += al2b * g.gam * irl;
+  prb += ar2b * g.gam * irr;
+  // H = kg p / rho + |u|^2/2
+  plb += HLb * kg * irl; irlb += HLb * kg * pl;
+  ulb += HLb * ul; vlb += HLb * vl; wlb += HLb * wl;
+  prb += HRb * kg * irr; irrb += HRb * kg * pr;
+  urb += HRb * ur; vrb += HRb * vr; wrb += HRb * wr;
+  ulb += unlb * nx; vlb += unlb * ny; wlb += unlb * nz;
+  urb += unrb * nx; vrb += unrb * ny; wrb += unrb * nz;
+  rlb -= irlb * irl * irl;
+  rrb -= irrb * irr * irr;
+  Lb[0] += rlb; Lb[1] += ulb; Lb[2] += vlb; Lb[3] += wlb; Lb[4] += plb;
+  Rb[0] += rrb; Rb[1] += urb; Rb[2] += vrb; Rb[3] += wrb; Rb[4] += prb;
+}
+
+template <int RS>
+__device__ __forceinline__ void riemann_adj(const double* L, const double* R, double nx, double ny, double nz,
+                                            const GasT<double>& g, const double* Fb, double* Lb, double* Rb) {
+  if constexpr (RS == RS_AUSMP) ausmp_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb);
+  else hllc_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb);
+}
+
+// Physical flux of one state (BC_Extrapolation).
+__device__ __forceinline__ void physical_flux_adj(const double* P, double nx, double ny, double nz,
+                                                  const GasT<double>& g, const double* Fb, double* Pb) {
+  const double un = P[1] * nx + P[2] * ny + P[3] * nz;
+  const double m = P[0] * un;
+  const double ir = 1.0 / P[0];
+  const double h0 = g.gam * g.rgm1 * P[4] * ir + 0.5 * (P[1] * P[1] + P[2] * P[2] + P[3] * P[3]);
+  const double mb = Fb[0] + Fb[1] * P[1] + Fb[2] * P[2] + Fb[3] * P[3] + Fb[4] * h0;
+  const double h0b = Fb[4] * m;
+  const double kg = g.gam * g.rgm1;
+  const double unb = mb * P[0];
+  Pb[0] += mb * un - h0b * kg * P[4] * ir * ir;
+  Pb[1] += Fb[1] * m + unb * nx + h0b * P[1];
+  Pb[2] += Fb[2] * m + unb * ny + h0b * P[2];
+  Pb[3] += Fb[3] * m + unb * nz + h0b * P[3];
+  Pb[4] += Fb[1] * nx + Fb[2] * ny + Fb[3] * nz + h0b * kg * ir;
+}
+
+// Symmetry face (symmetry_flux): momentum part of the mirrored Riemann problem on the
+// limited face state of the boundary cell.
+template <int LIM, int RS>
+__device__ void symmetry_flux_adj(const double* Pb, const double* Pi, double bdl, double idl, bool hi, double nx,
+                                  double ny, double nz, const GasT<double>& g, const double* Fb_in, double* Pbb,
+                                  double* Pib) {
+  const double un = Pb[1] * nx + Pb[2] * ny + Pb[3] * nz;
+  const double Gp[5] = {Pb[0], Pb[1] - 2.0 * un * nx, Pb[2] - 2.0 * un * ny, Pb[3] - 2.0 * un * nz, Pb[4]};
+  const double d12 = hi ? (bdl + idl) : 2.0 * bdl;
+  const double d23 = hi ? 2.0 * bdl : (bdl + idl);
+  const double rd12 = 1.0 / d12, rd23 = 1.0 / d23;
+  const double dist = hi ? bdl : -bdl;
+  double Fc[5], s32[5], s21[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double p1 = hi ? Pi[v] : Gp[v];
+    const double p3 = hi ? Gp[v] : Pi[v];
+    s32[v] = (p3 - Pb[v]) * rd23;
+    s21[v] = (Pb[v] - p1) * rd12;
+    Fc[v] = Pb[v] + limiter<LIM>(s32[v], s21[v]) * dist;
+  }
+  const double unf = Fc[1] * nx + Fc[2] * ny + Fc[3] * nz;
+  const double un1 = hi ? unf : -unf;
+  const double L[5] = {Fc[0], Fc[1] - unf * nx + un1 * nx, Fc[2] - unf * ny + un1 * ny, Fc[3] - unf * nz + un1 * nz, Fc[4]};
+  const double R[5] = {Fc[0], Fc[1] - unf * nx - un1 * nx, Fc[2] - unf * ny - un1 * ny, Fc[3] - unf * nz - un1 * nz, Fc[4]};
+  // reverse
+  const double Fb[5] = {0.0, Fb_in[1], Fb_in[2], Fb_in[3], 0.0};
+  double Lb[5] = {0, 0, 0, 0, 0}, Rb[5] = {0, 0, 0, 0, 0};
+  riemann_adj<RS>(L, R, nx, ny, nz, g, Fb, Lb, Rb);
+  double Fcb[5];
+  Fcb[0] = Lb[0] + Rb[0];
+  Fcb[4] = Lb[4] + Rb[4];
+  Fcb[1] = Lb[1] + Rb[1];
+  Fcb[2] = Lb[2] + Rb[2];
+  Fcb[3] = Lb[3] + Rb[3];
+  // L/R velocity = Fc - unf n +/- un1 n
+  const double nvec[3] = {nx, ny, nz};
+  double unfb = 0.0, un1b = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    unfb -= (Lb[1 + c] + Rb[1 + c]) * nvec[c];
+    un1b += (Lb[1 + c] - Rb[1 + c]) * nvec[c];
+  }
+  unfb += hi ? un1b : -un1b;
+  Fcb[1] += unfb * nx;
+  Fcb[2] += unfb * ny;
+  Fcb[3] += unfb * nz;
+  double Gpb[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    double xb = 0.0, yb = 0.0;
+    limiter_adj<LIM>(s32[v], s21[v], Fcb[v] * dist, xb, yb);
+    Pbb[v] += Fcb[v] - xb * rd23 + yb * rd12;
+    const double p3b = xb * rd23, p1b = -yb * rd12;
+    if (hi) { Pib[v] += p1b; Gpb[v] += p3b; }
+    else { Gpb[v] += p1b; Pib[v] += p3b; }
+  }
+  // Gp = mirror(Pb)
+  Pbb[0] += Gpb[0];
+  Pbb[4] += Gpb[4];
+  const double gdn = Gpb[1] * nx + Gpb[2] * ny + Gpb[3] * nz;
+  Pbb[1] += Gpb[1] - 2.0 * gdn * nx;
+  Pbb[2] += Gpb[2] - 2.0 * gdn * ny;
+  Pbb[3] += Gpb[3] - 2.0 * gdn * nz;
+}
+
+// Supersonic inflow face (inflow405_flux); returns false when the symmetry flux applies.
+template <int LIM, int RS>
+__device__ bool inflow405_adj(const double* Pb, const double* Pi, bool hi, double nx, double ny, double nz,
+                              const double* bc, const GasT<double>& g, const double* Fb, double* Pbb, double* Pib) {
+  const double un = Pb[1] * nx + Pb[2] * ny + Pb[3] * nz;
+  const double sgn = hi ? 1.0 : -1.0;
+  if (sgn * un >= 0.0) return false;
+  const double snd2 = g.gam * Pb[4] / Pb[0];
+  const double flow_a = sqrt(snd2);
+  const double vmag = sqrt(Pb[1] * Pb[1] + Pb[2] * Pb[2] + Pb[3] * Pb[3]);
+  const double flow_M = vmag / flow_a;
+  const double rel = bc[6];
+  const double M = rel * bc[0] + (1.0 - rel) * flow_M;
+  const double p = rel * bc[2] + (1.0 - rel) * Pb[4];
+  const double RT = g.R * bc[1];
+  const double rho = p / RT;
+  const double a = sqrt(g.gam * p / rho);
+  const double Sup[5] = {rho, M * a * bc[3], M * a * bc[4], M * a * bc[5], p};
+  double Fc[5], L[5], R[5], d32[5], d21[5];
+  const double half = hi ? 0.5 : -0.5;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double p1 = hi ? Pi[v] : Sup[v];
+    const double p3 = hi ? Sup[v] : Pi[v];
+    d32[v] = p3 - Pb[v];
+    d21[v] = Pb[v] - p1;
+    Fc[v] = Pb[v] + half * limiter<LIM>(d32[v], d21[v]);
+    L[v] = hi ? Fc[v] : Sup[v];
+    R[v] = hi ? Sup[v] : Fc[v];
+  }
+  double Lb[5] = {0, 0, 0, 0, 0}, Rb[5] = {0, 0, 0, 0, 0};
+  riemann_adj<RS>(L, R, nx, ny, nz, g, Fb, Lb, Rb);
+  double Supb[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const double Fcb = hi ? Lb[v] : Rb[v];
+    Supb[v] = hi ? Rb[v] : Lb[v];
+    double xb = 0.0, yb = 0.0;
+    limiter_adj<LIM>(d32[v], d21[v], Fcb * half, xb, yb);
+    Pbb[v] += Fcb - xb + yb;
+    if (hi) { Pib[v] -= yb; Supb[v] += xb; }
+    else { Supb[v] -= yb; Pib[v] += xb; }
+  }
+  if (rel != 1.0) {
+    // Sup depends on the boundary cell through M (flow Mach) and p; a = sqrt(gam R T) is fixed
+    const double Mb = a * (Supb[1] * bc[3] + Supb[2] * bc[4] + Supb[3] * bc[5]);
+    const double pb = Supb[4] + Supb[0] / RT;
+    const double fMb = (1.0 - rel) * Mb;
+    Pbb[4] += (1.0 - rel) * pb;
+    if (vmag > 0.0) {
+      const double vb = fMb / flow_a;
+      Pbb[1] += vb * Pb[1] / vmag;
+      Pbb[2] += vb * Pb[2] / vmag;
+      Pbb[3] += vb * Pb[3] / vmag;
+    }
+    // flow_a = sqrt(gam p / rho)
+    const double ab = -fMb * vmag / (flow_a * flow_a);
+    const double s2b = ab * 0.5 / flow_a;
+    Pbb[4] += s2b * g.gam / Pb[0];
+    Pbb[0] -= s2b * g.gam * Pb[4] / (Pb[0] * Pb[0]);
+  }
+  return true;
+}
+
+}  // namespace mose
+
+// =============================================================================== kernels
+// Discrete adjoint of the residual R(P) = Fill_Ghost_Cell + Fluxes + BC_Fluxes (the value
+// mose_residual returns) with respect to the primitive state P of every interior cell:
+// Pbar = (dR/dP)^T Rbar.  Four passes on the current state (ghost layers filled first):
+//   k_adj_face    one thread per interior or connection face: recompute the two face states
+//                 and the Riemann flux, reverse them, store the adjoints of the face states
+//                 (the "+" state of the left cell, the "-" state of the right cell);
+//   k_adj_bcface  one thread per non-connection boundary face: reverse of the boundary flux
+//                 into adjoints of the boundary cell and its interior neighbour;
+//   k_adj_cell    one thread per cell (ghost layers included): gathers the reverse of the
+//                 limited slopes of itself and its two neighbours along each direction plus
+//                 the boundary-face adjoints that name it (deterministic, no atomics);
+//   k_adj_ghostT  one thread per cell that feeds ghost cells: adds the transpose of the
+//                 ghost-fill map (copy, extrapolation, mirror) applied to the ghost adjoints.
+
+struct AdjSeg {       // the low faces of one block in one direction
+  int64_t start;      // first work item
+  int b, d;
+  int n0, n1, n2;     // extents: along d (n_d + 1 faces), first and second tangential
+  int pad;
+};
+
+struct AdjGhostRow {  // one source cell of the ghost fill and its entry range
+  int64_t src;
+  int begin, end;
+};
+struct AdjGhostEnt {
+  int64_t ghost;      // pool index of the ghost cell
+  double w;           // weight (kind 0) ...
+  double n[3];        // ... or mirror normal (kind 1)
+  int kind, pad;
+};
+
+struct AdjArgs {
+  const double* __restrict__ U;         // state with filled ghosts
+  const double* __restrict__ Rbar;      // 5 x npool
+  double* __restrict__ W;               // 30 x npool: face-state adjoints (3 dirs x {Sp, Sm} x 5)
+  double* __restrict__ BB;              // 10 x nslots: boundary-face adjoints (cell, neighbour)
+  double* __restrict__ Pbar;            // 5 x npool
+  const BlockDev* __restrict__ blocks;
+  const LineEnt* __restrict__ lines;
+  const double* __restrict__ geo;
+  const double* __restrict__ facebc;
+  const AdjSeg* __restrict__ segs;
+  const BcFace* __restrict__ bcfaces;
+  const AdjGhostRow* __restrict__ grows;
+  const AdjGhostEnt* __restrict__ gents;
+  int64_t npool, nwork, nslots;
+  int nseg, nbcf, ngrows, nblocks;
+  double gam, R;
+};
+
+// Geometry of the path in both layouts: reciprocal low-neighbour distance, half length and
+// low-face normal/area of cell c (padded index q along d, tangential padded indices).
+template <int GEO>
+struct AdjGeo {
+  const AdjArgs& a;
+  const BlockDev& B;
+  __device__ double rl(int d, int64_t c, int q) const {
+    if constexpr (GEO == 0) return a.lines[B.line[d] + q + 1].rl1;
+    else return a.geo[(size_t)(GRLO + d) * a.npool + c];
+  }
+  __device__ double hdl(int d, int64_t c, int q) const {
+    if constexpr (GEO == 0) return a.lines[B.line[d] + q + 1].lp;
+    else return 0.5 * a.geo[(size_t)(GDL + d) * a.npool + c];
+  }
+  __device__ void face(int d, int64_t c, int i, int j, int k, double n[3], double& area) const {
+    if constexpr (GEO == 0) {
+      n[0] = d == 0 ? 1.0 : 0.0; n[1] = d == 1 ? 1.0 : 0.0; n[2] = d == 2 ? 1.0 : 0.0;
+      const double dx = a.lines[B.line[0] + i + 1].delta, dy = a.lines[B.line[1] + j + 1].delta,
+                   dz = a.lines[B.line[2] + k + 1].delta;
+      area = d == 0 ? dy * dz : (d == 1 ? dx * dz : dx * dy);
+    } else {
+      const size_t np = (size_t)a.npool;
+      n[0] = a.geo[(GN + 4 * d) * np + c]; n[1] = a.geo[(GN + 4 * d + 1) * np + c];
+      n[2] = a.geo[(GN + 4 * d + 2) * np + c]; area = a.geo[(GN + 4 * d + 3) * np + c];
+    }
+  }
+};
+
+__device__ __forceinline__ void prim_load(const double* __restrict__ U, int64_t N, int64_t c, double gm1, double* P) {
+  double u[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) u[v] = U[v * N + c];
+  cons2prim(u, gm1, P);
+}
+
+template <int LIM, int RS, int GEO>
+__global__ void __launch_bounds__(256) k_adj_face(AdjArgs a) {
+  const double gm1 = a.gam - 1.0;
+  const GasT<double> g{a.gam, gm1, 1.0 / gm1, a.R, a.gam * a.R / gm1};
+  const int64_t N = a.npool;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.nwork; q += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = a.nseg - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.segs[mid].start <= q) lo = mid; else hi = mid - 1;
+    }
+    const AdjSeg S = a.segs[lo];
+    const BlockDev& B = a.blocks[S.b];
+    const int d = S.d;
+    int64_t r = q - S.start;
+    const int qd = (int)(r % S.n0) + 1;          // cell index along d whose low face this is
+    r /= S.n0;
+    const int t1 = (int)(r % S.n1) + 1, t2 = (int)(r / S.n1) + 1;
+    int ijk[3];
+    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
+    ijk[d] = qd; ijk[e1] = t1; ijk[e2] = t2;
+    const int nd = d == 0 ? B.ni : (d == 1 ? B.nj : B.nk);
+    const bool blo = qd == 1, bhi = qd == nd + 1;
+    if ((blo && B.bct[2 * d] != F_CONN) || (bhi && B.bct[2 * d + 1] != F_CONN)) continue;
+    const int64_t s = d == 0 ? 1 : (d == 1 ? (int64_t)B.sy : B.sz);
+    const int64_t c = B.off + (ijk[0] + 1) + (int64_t)B.sy * (ijk[1] + 1) + B.sz * (int64_t)(ijk[2] + 1);
+    double Pm2[5], Pm1[5], P0[5], Pp1[5];
+    prim_load(a.U, N, c - 2 * s, gm1, Pm2);
+    prim_load(a.U, N, c - s, gm1, Pm1);
+    prim_load(a.U, N, c, gm1, P0);
+    prim_load(a.U, N, c + s, gm1, Pp1);
+    const AdjGeo<GEO> G{a, B};
+    const double rlA = G.rl(d, c - s, qd - 1), rlB = G.rl(d, c, qd), rlC = G.rl(d, c + s, qd + 1);
+    const double hA = G.hdl(d, c - s, qd - 1), hB = G.hdl(d, c, qd);
+    double PL[5], PR[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double sgA = limiter<LIM>((P0[v] - Pm1[v]) * rlB, (Pm1[v] - Pm2[v]) * rlA) * hA;
+      const double sgB = limiter<LIM>((Pp1[v] - P0[v]) * rlC, (P0[v] - Pm1[v]) * rlB) * hB;
+      PL[v] = Pm1[v] + sgA;
+      PR[v] = P0[v] - sgB;
+    }
+    double n[3], area;
+    G.face(d, c, ijk[0], ijk[1], ijk[2], n, area);
+    double Fb[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const double rl_ = blo ? 0.0 : a.Rbar[v * N + c - s];
+      const double rr_ = bhi ? 0.0 : a.Rbar[v * N + c];
+      Fb[v] = area * (rl_ - rr_);
+    }
+    double Lb[5] = {0, 0, 0, 0, 0}, Rb[5] = {0, 0, 0, 0, 0};
+    riemann_adj<RS>(PL, PR, n[0], n[1], n[2], g, Fb, Lb, Rb);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      a.W[(size_t)(10 * d + v) * N + c - s] = Lb[v];       // "+" state of the left cell
+      a.W[(size_t)(10 * d + 5 + v) * N + c] = Rb[v];       // "-" state of the right cell
+    }
+  }
+}
+
+template <int LIM, int RS>
+__global__ void __launch_bounds__(256) k_adj_bcface(AdjArgs a) {
+  const double gm1 = a.gam - 1.0;
+  const GasT<double> g{a.gam, gm1, 1.0 / gm1, a.R, a.gam * a.R / gm1};
+  const int64_t N = a.npool;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < a.nbcf; q += gridDim.x * blockDim.x) {
+    const BcFace e = a.bcfaces[q];
+    const BlockDev& B = a.blocks[e.b];
+    const bool hi = e.f & 1;
+    const int bct = B.bct[e.f];
+    double Pbb[5] = {0, 0, 0, 0, 0}, Pib[5] = {0, 0, 0, 0, 0};
+    if (bct != F_NONE) {
+      double Pb[5], Pi[5], Fb[5];
+      prim_load(a.U, N, e.cb, gm1, Pb);
+      prim_load(a.U, N, e.cn, gm1, Pi);
+      // R(cb) += (hi ? +1 : -1) area F
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Fb[v] = (hi ? e.area : -e.area) * a.Rbar[v * N + e.cb];
+      if (bct == F_EXTRAP) {
+        physical_flux_adj(Pb, e.n[0], e.n[1], e.n[2], g, Fb, Pbb);
+      } else {
+        bool done = false;
+        if (bct == F_INFLOW)
+          done = inflow405_adj<LIM, RS>(Pb, Pi, hi, e.n[0], e.n[1], e.n[2], a.facebc + 8 * B.bcd[e.f], g, Fb, Pbb, Pib);
+        if (!done) symmetry_flux_adj<LIM, RS>(Pb, Pi, e.bdl, e.idl, hi, e.n[0], e.n[1], e.n[2], g, Fb, Pbb, Pib);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      a.BB[(size_t)v * a.nslots + e.slot] = Pbb[v];
+      a.BB[(size_t)(5 + v) * a.nslots + e.slot] = Pib[v];
+    }
+  }
+}
+
+template <int LIM, int GEO>
+__global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a) {
+  const double gm1 = a.gam - 1.0;
+  const int64_t N = a.npool;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = a.nblocks - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.blocks[mid].off <= c) lo = mid; else hi = mid - 1;
+    }
+    const BlockDev& B = a.blocks[lo];
+    const int64_t r = c - B.off;
+    if (r >= B.sz * (int64_t)(B.nk + 4)) continue;     // alignment padding between blocks
+    const int pi = (int)(r % B.sy) - 1, pj = (int)((r / B.sy) % (B.nj + 4)) - 1, pk = (int)(r / B.sz) - 1;
+    const int n3[3] = {B.ni, B.nj, B.nk};
+    const int p3[3] = {pi, pj, pk};
+    // cells outside the face-ghost cross (edges and corners) are never read
+    int outside = 0;
+    for (int d = 0; d < 3; ++d) outside += (p3[d] < 1 || p3[d] > n3[d]);
+    double acc[5] = {0, 0, 0, 0, 0};
+    if (outside <= 1) {
+      const AdjGeo<GEO> G{a, B};
+      for (int d = 0; d < 3; ++d) {
+        const int64_t s = d == 0 ? 1 : (d == 1 ? (int64_t)B.sy : B.sz);
+        const int qd = p3[d];
+        for (int m = -1; m <= 1; ++m) {
+          const int qm = qd + m;
+          if (qm < 0 || qm > n3[d] + 1) continue;         // cells whose slope is never used
+          if (outside == 1 && (p3[d] >= 1 && p3[d] <= n3[d])) {
+            // a ghost cell of another direction: only its own slope along d could matter,
+            // and ghost cells never need a slope across the ghost layer
+            continue;
+          }
+          const int64_t cm = c + m * s;
+          double Spb[5], Smb[5];
+          bool any = false;
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            Spb[v] = a.W[(size_t)(10 * d + v) * N + cm];
+            Smb[v] = a.W[(size_t)(10 * d + 5 + v) * N + cm];
+            any = any || Spb[v] != 0.0 || Smb[v] != 0.0;
+          }
+          if (!any) continue;
+          double Pm[5], P0[5], Pp[5];
+          prim_load(a.U, N, cm - s, gm1, Pm);
+          prim_load(a.U, N, cm, gm1, P0);
+          prim_load(a.U, N, cm + s, gm1, Pp);
+          double bm[5] = {0, 0, 0, 0, 0}, b0[5] = {0, 0, 0, 0, 0}, bp[5] = {0, 0, 0, 0, 0};
+          cell_states_adj<LIM>(Pm, P0, Pp, G.rl(d, cm, qm), G.rl(d, cm + s, qm + 1), G.hdl(d, cm, qm), Smb, Spb,
+                               bm, b0, bp);
+          const double* src = m == -1 ? bp : (m == 0 ? b0 : bm);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) acc[v] += src[v];
+        }
+      }
+      // non-connection boundary faces naming this cell as boundary cell or neighbour
+      if (outside == 0) {
+        for (int f = 0; f < 6; ++f) {
+          if (B.bct[f] == F_CONN) continue;
+          const int d = f >> 1, d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+          const bool fh = f & 1;
+          const int nd = n3[d];
+          const int qb = fh ? nd : 1, qn = fh ? nd - 1 : 2;
+          const int64_t slot = (int64_t)B.bcs[f] + (p3[d1] - 1) + (int64_t)n3[d1] * (p3[d2] - 1);
+          if (p3[d] == qb) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) acc[v] += a.BB[(size_t)v * a.nslots + slot];
+          }
+          if (p3[d] == qn) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) acc[v] += a.BB[(size_t)(5 + v) * a.nslots + slot];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) a.Pbar[v * N + c] = acc[v];
+  }
+}
+
+// Transpose of the ghost fill: each source cell adds M^T of the ghost adjoints it feeds.
+__global__ void __launch_bounds__(256) k_adj_ghostT(AdjArgs a) {
+  const int64_t N = a.npool;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < a.ngrows; q += gridDim.x * blockDim.x) {
+    const AdjGhostRow row = a.grows[q];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int e = row.begin; e < row.end; ++e) {
+      const AdjGhostEnt E = a.gents[e];
+      double gb[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) gb[v] = a.Pbar[v * N + E.ghost];
+      if (E.kind == 0) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) acc[v] += E.w * gb[v];
+      } else {   // velocity mirror (I - 2 n n^T), symmetric
+        const double dn = gb[1] * E.n[0] + gb[2] * E.n[1] + gb[3] * E.n[2];
+        acc[0] += gb[0];
+        acc[1] += gb[1] - 2.0 * dn * E.n[0];
+        acc[2] += gb[2] - 2.0 * dn * E.n[1];
+        acc[3] += gb[3] - 2.0 * dn * E.n[2];
+        acc[4] += gb[4];
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) a.Pbar[v * N + row.src] += acc[v];
+  }
+}

@@ -11,8 +11,10 @@
 // BC_Extrapolate_Metrics :535-797).  For a rectilinear block these reduce exactly to one
 // line of values per direction: dl(i) from the reference's own summation order on the two
 // node planes, the face area of an i-face = (y_j - y_{j-1}) (z_k - z_{k-1}) and the
-// normal = the axis unit vector.  This setup verifies rectilinearity bit-for-bit and
-// builds those lines; curvilinear blocks are rejected with MOSE_E_ARG.
+// normal = the axis unit vector.  This setup verifies rectilinearity bit-for-bit and builds
+// those lines.  If any block is not rectilinear the whole context uses the general metrics
+// instead (mose_metrics.hpp): per-cell arrays of face normals and areas, cell lengths,
+// volumes and the inverse-Jacobian rows.
 #pragma once
 #pragma clang fp contract(off)
 
@@ -22,6 +24,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+
+#include "mose_metrics.hpp"
 
 struct HostBlock {
   int ni = 0, nj = 0, nk = 0;
@@ -41,6 +45,8 @@ struct Layout {
   int64_t nbcslots = 0;
   int64_t npool = 0;
   int ndir = 3;
+  int curvi = 0;                 // 1: general metrics in `geo` (NGEO x npool, SoA)
+  std::vector<double> geo;
 };
 
 struct GeoError {
@@ -111,6 +117,145 @@ static void build_tiles(Layout& L, int TXc, int TYc, int kc, int xcd) {
     const int64_t start = x * q + std::min<int64_t>(x, r);
     L.tiles[w] = S[start + slot];
   }
+}
+
+// General metrics of every block into L.geo (see mose_metrics.hpp).  off: pool offset of
+// each block, ftype: BC type per block face.
+static bool build_curvilinear(const std::vector<HostBlock>& hb, const std::vector<mose_bc_entry>& bc,
+                              const std::vector<int64_t>& off, Layout& L, GeoError& err) {
+  using namespace mose;
+  using metrics::V3;
+  const int nb = (int)hb.size();
+  const int64_t np = L.npool;
+  L.geo.assign((size_t)NGEO * np, 0.0);
+  double* G = L.geo.data();
+  auto at = [&](int g, int64_t c) -> double& { return G[(size_t)g * np + c]; };
+  auto pidx = [&](int b, int i, int j, int k) -> int64_t {
+    const HostBlock& B = hb[b];
+    return off[b] + (int64_t)(i + 1) + (int64_t)(B.ni + 4) * (j + 1) + (int64_t)(B.ni + 4) * (B.nj + 4) * (k + 1);
+  };
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock& B = hb[b];
+    const metrics::NodeView N{B.nodes.data(), B.ni, B.nj, B.nk};
+    // interior cells: dl, volume, unit rows of M (Compute_Dt versors)
+    for (int k = 1; k <= B.nk; ++k)
+      for (int j = 1; j <= B.nj; ++j)
+        for (int i = 1; i <= B.ni; ++i) {
+          const V3 nd[8] = {N(i - 1, j - 1, k - 1), N(i - 1, j - 1, k), N(i - 1, j, k - 1), N(i - 1, j, k),
+                            N(i, j - 1, k - 1),     N(i, j - 1, k),     N(i, j, k - 1),     N(i, j, k)};
+          double M[9], dl[3], vol;
+          if (!metrics::cell_metric(nd, M, dl, &vol)) return geo::set_err(err, MOSE_E_METRIC, b + 1, i, j, k);
+          const int64_t c = pidx(b, i, j, k);
+          for (int d = 0; d < 3; ++d) {
+            at(GDL + d, c) = dl[d];
+            const double* m = M + 3 * d;
+            const double nm = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+            for (int q = 0; q < 3; ++q) at(GVS + 3 * d + q, c) = m[q] / nm;
+          }
+          at(GVOL, c) = vol;
+        }
+    // faces: the low face of cell (i,j,k) in direction d is the reference's face d at
+    // node plane (index along d) - 1
+    const double sg[3] = {metrics::face_sign(N, 0), metrics::face_sign(N, 1), metrics::face_sign(N, 2)};
+    for (int d = 0; d < 3; ++d) {
+      const int i0 = d == 0 ? 0 : 1, j0 = d == 1 ? 0 : 1, k0 = d == 2 ? 0 : 1;
+      for (int k = k0; k <= B.nk; ++k)
+        for (int j = j0; j <= B.nj; ++j)
+          for (int i = i0; i <= B.ni; ++i) {
+            double n[3], A;
+            metrics::face_normal(N, d, i, j, k, sg[d], n, &A);
+            const int64_t c = pidx(b, i + (d == 0), j + (d == 1), k + (d == 2));
+            for (int q = 0; q < 3; ++q) at(GN + 4 * d + q, c) = n[q];
+            at(GN + 4 * d + 3, c) = A;
+          }
+    }
+  }
+  // ghost-cell lengths, one bc record at a time (Mod_Metrics.f90:74-105)
+  for (const mose_bc_entry& e : bc) {
+    const int b = e.b - 1, d = (e.f - 1) / 2;
+    const int* gm = geo::guide[e.f - 1];
+    const HostBlock& B = hb[b];
+    if (e.type == 101) {
+      const int* gs = geo::guide[e.fs - 1];
+      const int ds = (e.fs - 1) / 2;
+      for (int g = 1; g <= 2; ++g) {
+        const int64_t dst = pidx(b, e.i - gm[0] * g, e.j - gm[1] * g, e.k - gm[2] * g);
+        const int64_t src = pidx(e.bs - 1, e.is + gs[0] * (g - 1), e.js + gs[1] * (g - 1), e.ks + gs[2] * (g - 1));
+        double dl[3] = {at(GDL, src), at(GDL + 1, src), at(GDL + 2, src)};
+        if (d != ds) std::swap(dl[d], dl[ds]);
+        for (int q = 0; q < 3; ++q) at(GDL + q, dst) = dl[q];
+      }
+    } else if (e.type == 300) {
+      for (int g = 1; g <= 2; ++g) {
+        const int64_t dst = pidx(b, e.i - gm[0] * g, e.j - gm[1] * g, e.k - gm[2] * g);
+        const int64_t src = pidx(b, e.i + gm[0] * (g - 1), e.j + gm[1] * (g - 1), e.k + gm[2] * (g - 1));
+        for (int q = 0; q < 3; ++q) at(GDL + q, dst) = at(GDL + q, src);
+      }
+    } else {
+      const int n[3] = {B.ni, B.nj, B.nk};
+      const bool hi = (e.f % 2) == 0;
+      bool cubic = d == 0 ? true : (d == 1 ? B.nj > 2 : L.ndir == 3);
+      if (n[d] < 2) cubic = false;
+      const metrics::NodeView N{B.nodes.data(), B.ni, B.nj, B.nk};
+      const int cell[3] = {e.i, e.j, e.k};
+      const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+      // node planes along d, slot 2 = the boundary face, slots 3, 4 outward
+      V3 pl[2][2][5];
+      for (int a1 = 0; a1 < 2; ++a1)
+        for (int a2 = 0; a2 < 2; ++a2) {
+          int q[3];
+          q[t1] = cell[t1] - 1 + a1;
+          q[t2] = cell[t2] - 1 + a2;
+          const int face = hi ? cell[d] : cell[d] - 1;
+          const int s = hi ? -1 : 1;      // inward step along d
+          const int known = cubic ? 3 : 2;
+          for (int g = 0; g < known; ++g) {
+            q[d] = face + s * g;
+            pl[a1][a2][2 - g] = N(q[0], q[1], q[2]);
+          }
+          for (int g = 3; g <= 4; ++g) {   // outward planes
+            const V3 a = pl[a1][a2][g - 1], bb = pl[a1][a2][g - 2];
+            if (cubic) {
+              const V3 c = pl[a1][a2][g - 3];
+              pl[a1][a2][g] = V3{3.0 * a.x - 3.0 * bb.x + c.x, 3.0 * a.y - 3.0 * bb.y + c.y, 3.0 * a.z - 3.0 * bb.z + c.z};
+            } else {
+              pl[a1][a2][g] = V3{2.0 * a.x - bb.x, 2.0 * a.y - bb.y, 2.0 * a.z - bb.z};
+            }
+          }
+        }
+      for (int g = 1; g <= 2; ++g) {
+        V3 nd[8];
+        for (int n8 = 0; n8 < 8; ++n8) {
+          const int ab[3] = {(n8 >> 2) & 1, (n8 >> 1) & 1, n8 & 1};
+          // along d: the ghost cell spans slots 1+g (inner) and 2+g (outer); its low-index
+          // node plane is the outer one on a low face
+          const int slot = hi ? (1 + g + ab[d]) : (2 + g - ab[d]);
+          nd[n8] = pl[ab[t1]][ab[t2]][slot];
+        }
+        double M[9], dl[3], vol;
+        int gc[3] = {e.i, e.j, e.k};
+        gc[d] += hi ? g : -g;
+        if (!metrics::cell_metric(nd, M, dl, &vol)) return geo::set_err(err, MOSE_E_METRIC, 0, gc[0], gc[1], gc[2]);
+        const int64_t c = pidx(b, gc[0], gc[1], gc[2]);
+        for (int q = 0; q < 3; ++q) at(GDL + q, c) = dl[q];
+      }
+    }
+  }
+  // reciprocal centre distances to the low neighbour: 1 / ((dl(c - e_d) + dl(c)) / 2)
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock& B = hb[b];
+    const int64_t st[3] = {1, (int64_t)(B.ni + 4), (int64_t)(B.ni + 4) * (B.nj + 4)};
+    for (int k = 0; k <= B.nk + 2; ++k)
+      for (int j = 0; j <= B.nj + 2; ++j)
+        for (int i = 0; i <= B.ni + 2; ++i) {
+          const int64_t c = pidx(b, i, j, k);
+          for (int d = 0; d < 3; ++d) {
+            const double s = at(GDL + d, c - st[d]) + at(GDL + d, c);
+            at(GRLO + d, c) = s > 0.0 ? 1.0 / (0.5 * s) : 0.0;
+          }
+        }
+  }
+  return true;
 }
 
 static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mose_bc_entry>& bc, Layout& L,
@@ -186,14 +331,16 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
         coord[3 * b + d][q] = B.node(ii, jj, kk)[d];
       }
       for (int q = 1; q <= n[d]; ++q)
-        if (!(coord[3 * b + d][q] > coord[3 * b + d][q - 1])) return geo::set_err(err, MOSE_E_ARG, b + 1);
+        if (!(coord[3 * b + d][q] > coord[3 * b + d][q - 1])) L.curvi = 1;
     }
-    for (int k = 0; k <= B.nk; ++k)
-      for (int j = 0; j <= B.nj; ++j)
+    for (int k = 0; k <= B.nk && !L.curvi; ++k)
+      for (int j = 0; j <= B.nj && !L.curvi; ++j)
         for (int i = 0; i <= B.ni; ++i) {
           const double* p = B.node(i, j, k);
-          if (p[0] != coord[3 * b][i] || p[1] != coord[3 * b + 1][j] || p[2] != coord[3 * b + 2][k])
-            return geo::set_err(err, MOSE_E_ARG, b + 1, i, j, k);   // curvilinear: not supported
+          if (p[0] != coord[3 * b][i] || p[1] != coord[3 * b + 1][j] || p[2] != coord[3 * b + 2][k]) {
+            L.curvi = 1;
+            break;
+          }
         }
     for (int d = 0; d < 3; ++d) {
       lineoff[3 * b + d] = (int)L.lines.size();
@@ -269,6 +416,10 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
     }
   }
 
+  // ---- general metrics when any block is not rectilinear (the lines above are then unused)
+  if (L.curvi && !build_curvilinear(hb, bc, off, L, err)) return false;
+  auto G = [&](int g, int64_t c) -> double { return L.geo[(size_t)g * L.npool + c]; };
+
   // ---- block descriptors
   L.blocks.resize(nb);
   L.facebc.clear();
@@ -320,7 +471,12 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
       case 300: {
         op.type = G_SYM;
         const int d = (e.f - 1) / 2;
-        op.n[d] = 1.0;
+        if (L.curvi) {   // normal of the boundary face (Ghost_Symmetry, Lib_Ghost.f90:193-225)
+          const int64_t fc = (e.f % 2) ? op.b : op.b - op.s;
+          for (int q = 0; q < 3; ++q) op.n[q] = G(GN + 4 * d + q, fc);
+        } else {
+          op.n[d] = 1.0;
+        }
         break;
       }
       case 400: op.type = G_EXTRAP; break;
@@ -353,8 +509,18 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
           e.cb = pidx(b, c3[0], c3[1], c3[2]);
           e.cn = pidx(b, m3[0], m3[1], m3[2]);
           e.b = b; e.f = f; e.slot = (int)(slot + (t1 - 1) + (int64_t)n[d1] * (t2 - 1));
-          e.bdl = 0.5 * LE(b, d, c3[d]).dl;
-          e.idl = 0.5 * LE(b, d, m3[d]).dl;
+          if (L.curvi) {
+            e.bdl = 0.5 * G(GDL + d, e.cb);
+            e.idl = 0.5 * G(GDL + d, e.cn);
+            const int64_t fc = hi ? e.cb + (e.cb - e.cn) : e.cb;    // cell holding the face as its low face
+            for (int q = 0; q < 3; ++q) e.n[q] = G(GN + 4 * d + q, fc);
+            e.area = G(GN + 4 * d + 3, fc);
+          } else {
+            e.bdl = 0.5 * LE(b, d, c3[d]).dl;
+            e.idl = 0.5 * LE(b, d, m3[d]).dl;
+            e.n[d] = 1.0;
+            e.area = LE(b, d1, c3[d1]).delta * LE(b, d2, c3[d2]).delta;
+          }
           L.bcfaces.push_back(e);
         }
       slot += (int64_t)n[d1] * n[d2];

@@ -36,12 +36,14 @@ __device__ __forceinline__ void set_error(int* err, int code, int b, int i, int 
 }
 
 #include "mose_stage.hpp"
+#include "mose_adjoint.hpp"
 
 struct DtArgs {
   const double* U;
   const Tile* tiles;
   const BlockDev* blocks;
   const LineEnt* lines;
+  const double* geo;   // general metrics (GEO 1)
   unsigned long long* dtmin;
   double* dtloc;
   const int* iter;
@@ -52,8 +54,10 @@ struct DtArgs {
   double cfl, gam;
 };
 
-// CFL time step of every interior cell (Compute_Dt, Mod_dt.f90:61-85 with rectilinear
-// metrics: versor = axis, dl = cell length) and its global minimum.
+// CFL time step of every interior cell (Compute_Dt, Mod_dt.f90:61-85) and its global
+// minimum.  GEO 0: rectilinear metrics (versor = axis, dl = cell length); GEO 1: the unit
+// rows of M and dl from the geometry arrays.
+template <int GEO>
 __global__ void __launch_bounds__(256) k_dt(DtArgs a) {
   const int tid = threadIdx.x, tx = tid % a.tx, ty = tid / a.tx;
   const double gm1 = a.gam - 1.0;
@@ -76,9 +80,19 @@ __global__ void __launch_bounds__(256) k_dt(DtArgs a) {
       cons2prim(u, gm1, P);
       const double snd = sqrt(a.gam * P[4] / P[0]);
       double dtc = 1e8;
-      dtc = fmin(Lx[i + 1].dl / (fabs(P[1]) + snd) * a.cfl, dtc);
-      if (a.ndir > 1) dtc = fmin(Ly[j + 1].dl / (fabs(P[2]) + snd) * a.cfl, dtc);
-      if (a.ndir > 2) dtc = fmin(Lz[k + 1].dl / (fabs(P[3]) + snd) * a.cfl, dtc);
+      if constexpr (GEO == 0) {
+        dtc = fmin(Lx[i + 1].dl / (fabs(P[1]) + snd) * a.cfl, dtc);
+        if (a.ndir > 1) dtc = fmin(Ly[j + 1].dl / (fabs(P[2]) + snd) * a.cfl, dtc);
+        if (a.ndir > 2) dtc = fmin(Lz[k + 1].dl / (fabs(P[3]) + snd) * a.cfl, dtc);
+      } else {
+        const double* G = a.geo;
+        const size_t np = (size_t)a.npool;
+        for (int dd = 0; dd < a.ndir; ++dd) {
+          const double vel = fabs(P[1] * G[(GVS + 3 * dd) * np + c] + P[2] * G[(GVS + 3 * dd + 1) * np + c] +
+                                  P[3] * G[(GVS + 3 * dd + 2) * np + c]);
+          dtc = fmin(G[(GDL + dd) * np + c] / (vel + snd) * a.cfl, dtc);
+        }
+      }
       if (it < a.rampa) dtc = dtc * it / a.rampa;
       if (a.local_dt) a.dtloc[c] = dtc;
       mymin = fmin(mymin, dtc);
@@ -216,6 +230,23 @@ __global__ void k_get_raw(const double* __restrict__ R, int64_t N, int64_t off, 
 #pragma unroll
     for (int v = 0; v < 5; ++v) Raos[5 * c + v] = R[v * N + off + c];
 }
+// SoA pool -> blk%R-layout AoS of one block with the ghost layers set to zero.
+__global__ void k_get_interior(const double* __restrict__ R, int64_t N, int64_t off, int ni, int nj, int nk,
+                               double* __restrict__ Raos) {
+  const int64_t sy = ni + 4, sz = sy * (nj + 4), ncell = sz * (nk + 4);
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(c % sy) - 1, j = (int)((c / sy) % (nj + 4)) - 1, k = (int)(c / sz) - 1;
+    const bool in = i >= 1 && i <= ni && j >= 1 && j <= nj && k >= 1 && k <= nk;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Raos[5 * c + v] = in ? R[v * N + off + c] : 0.0;
+  }
+}
+// blk%R-layout AoS of one block -> SoA pool (plain copy).
+__global__ void k_put_raw(const double* __restrict__ Raos, double* R, int64_t N, int64_t off, int64_t ncell) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) R[v * N + off + c] = Raos[5 * c + v];
+}
 __global__ void k_fill(double* p, int64_t n, double x) {
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) p[c] = x;
 }
@@ -236,11 +267,11 @@ typedef void (*BcFn)(StageArgs, const BcFace*, int);
 constexpr int NSHAPE = 3;
 constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 8}, {8, 16}}, {{32, 8}, {16, 16}, {16, 8}}};
 
-template <int LIM, int RS, typename FT, int TXc, int TYc>
+template <int LIM, int RS, typename FT, int TXc, int TYc, int GEO = 0>
 void stage_modes(StageFn* t) {
-  t[0] = k_stage<LIM, RS, FT, 0, TXc, TYc>;
-  t[1] = k_stage<LIM, RS, FT, 1, TXc, TYc>;
-  t[2] = k_stage<LIM, RS, FT, 2, TXc, TYc>;
+  t[0] = k_stage<LIM, RS, FT, 0, TXc, TYc, GEO>;
+  t[1] = k_stage<LIM, RS, FT, 1, TXc, TYc, GEO>;
+  t[2] = k_stage<LIM, RS, FT, 2, TXc, TYc, GEO>;
 }
 
 template <int LIM, typename FT>
@@ -249,14 +280,18 @@ void bc_table(BcFn* t) {
   t[1] = k_bcface<LIM, RS_AUSMP, FT>;
 }
 
-// kernel table [prec][lim][rs][shape][mode]; null where a shape is not compiled
+// kernel table [prec][lim][rs][shape][mode]; null where a shape is not compiled.
+// g: the general-metrics kernels (shape 0 only).
 struct KTable {
   StageFn f[2][4][2][NSHAPE][3] = {};
+  StageFn g[2][4][2][3] = {};
   BcFn bc[2][4][2] = {};
   template <int LIM, int RS>
   void all_shapes() {
     stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1]>(f[0][LIM][RS][0]);
     stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1]>(f[1][LIM][RS][0]);
+    stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 1>(g[0][LIM][RS]);
+    stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1], 1>(g[1][LIM][RS]);
   }
   KTable() {
     bc_table<LIM_MINMOD, double>(bc[0][0]);
@@ -290,6 +325,33 @@ const KTable& ktable() {
   return t;
 }
 
+typedef void (*AdjFn)(AdjArgs);
+// adjoint kernels [lim][rs][geo] (FP64)
+struct ATable {
+  AdjFn face[4][2][2] = {}, bc[4][2] = {}, cell[4][2] = {};
+  template <int LIM>
+  void lim() {
+    face[LIM][0][0] = k_adj_face<LIM, RS_HLLC, 0>;
+    face[LIM][0][1] = k_adj_face<LIM, RS_HLLC, 1>;
+    face[LIM][1][0] = k_adj_face<LIM, RS_AUSMP, 0>;
+    face[LIM][1][1] = k_adj_face<LIM, RS_AUSMP, 1>;
+    bc[LIM][0] = k_adj_bcface<LIM, RS_HLLC>;
+    bc[LIM][1] = k_adj_bcface<LIM, RS_AUSMP>;
+    cell[LIM][0] = k_adj_cell<LIM, 0>;
+    cell[LIM][1] = k_adj_cell<LIM, 1>;
+  }
+  ATable() {
+    lim<LIM_MINMOD>();
+    lim<LIM_VANLEER>();
+    lim<LIM_VANALBADA>();
+    lim<LIM_FIRST>();
+  }
+};
+const ATable& atable() {
+  static ATable t;
+  return t;
+}
+
 const double kRK[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};   // Lib_RK.f90:8-11
 
 }  // namespace
@@ -313,6 +375,7 @@ struct mose_ctx {
   int64_t stage_n = 0;
   BlockDev* d_blocks = nullptr;
   LineEnt* d_lines = nullptr;
+  double* d_geo = nullptr;
   Tile* d_tiles = nullptr;
   GhostOp* d_ghost = nullptr;
   BcFace* d_bcfaces = nullptr;
@@ -326,6 +389,14 @@ struct mose_ctx {
   int* d_iter = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // discrete adjoint (allocated on first use)
+  bool adj_ready = false;
+  double *d_Rbar = nullptr, *d_W = nullptr, *d_BB = nullptr, *d_Pbar = nullptr;
+  AdjSeg* d_segs = nullptr;
+  AdjGhostRow* d_grows = nullptr;
+  AdjGhostEnt* d_gents = nullptr;
+  int nseg = 0, ngrows = 0;
+  int64_t nwork = 0;
   bool dt_primed = false;   // d_dtmin holds the CFL step of the current state
   int grid_stage = 0;
   int shape = 0;       // index into kShape of the stage kernel in use
@@ -361,6 +432,7 @@ static StageArgs stage_args(mose_ctx* c, const double* Uin, const double* Un, do
   StageArgs a{};
   a.Uin = Uin; a.Un = Un; a.Uout = Uout; a.Rout = c->Rbuf;
   a.tiles = c->d_tiles; a.blocks = c->d_blocks; a.lines = c->d_lines; a.facebc = c->d_facebc;
+  a.geo = c->d_geo;
   a.scal = c->d_scal; a.dtloc = c->d_dtloc; a.partials = c->d_partials; a.err = c->d_err;
   a.bcbuf = c->d_bcbuf;
   a.dtmin = c->d_dtmin; a.dtnext = c->d_dtloc; a.iter = c->d_iter;
@@ -389,12 +461,20 @@ static void enqueue_bcfaces(mose_ctx* c, const StageArgs& a) {
 
 static StageFn stage_fn(mose_ctx* c, int mode) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
+  if (c->lay.curvi) return ktable().g[p][c->cfg.limiter][c->cfg.riemann][mode];
   return ktable().f[p][c->cfg.limiter][c->cfg.riemann][c->shape][mode];
+}
+
+static void launch_dt(mose_ctx* c, const DtArgs& da) {
+  if (c->lay.curvi)
+    hipLaunchKernelGGL(k_dt<1>, dim3(da.ntiles), dim3(c->tx * c->ty), 0, c->stream, da);
+  else
+    hipLaunchKernelGGL(k_dt<0>, dim3(da.ntiles), dim3(c->tx * c->ty), 0, c->stream, da);
 }
 
 static DtArgs dt_args(mose_ctx* c, int iter_add) {
   DtArgs da{};
-  da.U = c->U[0]; da.tiles = c->d_tiles; da.blocks = c->d_blocks; da.lines = c->d_lines;
+  da.U = c->U[0]; da.tiles = c->d_tiles; da.blocks = c->d_blocks; da.lines = c->d_lines; da.geo = c->d_geo;
   da.dtmin = c->d_dtmin; da.dtloc = c->d_dtloc; da.iter = c->d_iter; da.npool = c->npool;
   da.ntiles = (int)c->lay.tiles.size(); da.ndir = c->ndir; da.rampa = c->cfg.rampa_cfl_iter;
   da.local_dt = c->cfg.time_accurate ? 0 : 1; da.cfl = c->cfg.cfl; da.gam = c->cfg.gamma;
@@ -407,7 +487,7 @@ static DtArgs dt_args(mose_ctx* c, int iter_add) {
 static void enqueue_prime_dt(mose_ctx* c) {
   hipLaunchKernelGGL(k_dtreset, dim3(1), dim3(1), 0, c->stream, c->d_dtmin);
   DtArgs da = dt_args(c, 1);
-  hipLaunchKernelGGL(k_dt, dim3(da.ntiles), dim3(c->tx * c->ty), 0, c->stream, da);
+  launch_dt(c, da);
 }
 
 // One explicit step enqueued on the context stream (no host synchronisation).  The CFL
@@ -471,6 +551,7 @@ static int ensure_stage(mose_ctx* c, int64_t n) {
 
 static bool shape_compiled(const mose_ctx* c, int shape) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
+  if (c->lay.curvi) return shape == 0;
   return shape >= 0 && shape < NSHAPE && ktable().f[p][c->cfg.limiter][c->cfg.riemann][shape][0] != nullptr;
 }
 
@@ -501,6 +582,111 @@ static int retile(mose_ctx* c, int shape, int kchunk, int xcd) {
   c->graph = nullptr;
   c->dt_primed = false;
   return 0;
+}
+
+// Discrete adjoint set-up: face work segments, the transposed ghost-fill map, buffers.
+static int ensure_adjoint(mose_ctx* c) {
+  if (c->adj_ready) return 0;
+  std::vector<AdjSeg> segs;
+  int64_t start = 0;
+  for (int b = 0; b < c->nb; ++b) {
+    const BlockDev& B = c->lay.blocks[b];
+    const int n[3] = {B.ni, B.nj, B.nk};
+    for (int d = 0; d < 3; ++d) {
+      AdjSeg sg{};
+      sg.start = start; sg.b = b; sg.d = d;
+      sg.n0 = n[d] + 1;
+      sg.n1 = d == 0 ? n[1] : n[0];
+      sg.n2 = d == 2 ? n[1] : n[2];
+      segs.push_back(sg);
+      start += (int64_t)sg.n0 * sg.n1 * sg.n2;
+    }
+  }
+  c->nwork = start;
+  c->nseg = (int)segs.size();
+  // ghost-fill map per source cell: ghost = sum_w w P(src) (copy, zero gradient,
+  // extrapolation 3a-3b+c and its second layer 6a-8b+3c) or the velocity mirror
+  std::map<int64_t, std::vector<AdjGhostEnt>> rows;
+  auto add = [&](int64_t src, int64_t ghost, double w, int kind, const double* n) {
+    AdjGhostEnt e{};
+    e.ghost = ghost; e.w = w; e.kind = kind;
+    if (n) { e.n[0] = n[0]; e.n[1] = n[1]; e.n[2] = n[2]; }
+    rows[src].push_back(e);
+  };
+  for (const GhostOp& op : c->lay.ghost) {
+    const int64_t g1 = op.b - op.s, g2 = op.b - 2 * op.s;
+    switch (op.type) {
+      case G_COPY: add(op.src1, g1, 1.0, 0, nullptr); add(op.src2, g2, 1.0, 0, nullptr); break;
+      case G_ZG: add(op.b, g1, 1.0, 0, nullptr); add(op.b, g2, 1.0, 0, nullptr); break;
+      case G_EXTRAP:
+        add(op.b, g1, 3.0, 0, nullptr); add(op.b + op.s, g1, -3.0, 0, nullptr); add(op.b + 2 * op.s, g1, 1.0, 0, nullptr);
+        add(op.b, g2, 6.0, 0, nullptr); add(op.b + op.s, g2, -8.0, 0, nullptr); add(op.b + 2 * op.s, g2, 3.0, 0, nullptr);
+        break;
+      default: add(op.b, g1, 0.0, 1, op.n); break;   // symmetry: one layer
+    }
+  }
+  std::vector<AdjGhostRow> grows;
+  std::vector<AdjGhostEnt> gents;
+  for (auto& kv : rows) {
+    AdjGhostRow r{};
+    r.src = kv.first;
+    r.begin = (int)gents.size();
+    gents.insert(gents.end(), kv.second.begin(), kv.second.end());
+    r.end = (int)gents.size();
+    grows.push_back(r);
+  }
+  c->ngrows = (int)grows.size();
+  if (upload(c, &c->d_segs, segs)) return c->err.code;
+  if (upload(c, &c->d_grows, grows)) return c->err.code;
+  if (upload(c, &c->d_gents, gents)) return c->err.code;
+  const int64_t ns = std::max<int64_t>(c->lay.nbcslots, 1);
+  HIPCHK(c, hipMalloc((void**)&c->d_Rbar, 5 * c->npool * sizeof(double)));
+  HIPCHK(c, hipMemset(c->d_Rbar, 0, 5 * c->npool * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_W, (size_t)30 * c->npool * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_BB, (size_t)10 * ns * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_Pbar, 5 * c->npool * sizeof(double)));
+  c->adj_ready = true;
+  return 0;
+}
+
+static AdjArgs adj_args(mose_ctx* c) {
+  AdjArgs a{};
+  a.U = c->U[0]; a.Rbar = c->d_Rbar; a.W = c->d_W; a.BB = c->d_BB; a.Pbar = c->d_Pbar;
+  a.blocks = c->d_blocks; a.lines = c->d_lines; a.geo = c->d_geo; a.facebc = c->d_facebc;
+  a.segs = c->d_segs; a.bcfaces = c->d_bcfaces; a.grows = c->d_grows; a.gents = c->d_gents;
+  a.npool = c->npool; a.nwork = c->nwork; a.nslots = std::max<int64_t>(c->lay.nbcslots, 1);
+  a.nseg = c->nseg; a.nbcf = (int)c->lay.bcfaces.size(); a.ngrows = c->ngrows; a.nblocks = c->nb;
+  a.gam = c->cfg.gamma; a.R = c->cfg.Rgas;
+  return a;
+}
+
+// Pbar = (dR/dP)^T Rbar on the current state (the ghost layers are filled first).
+static void enqueue_vjp(mose_ctx* c) {
+  enqueue_ghost(c, c->U[0]);
+  const int64_t nW = (int64_t)30 * c->npool, nB = 10 * std::max<int64_t>(c->lay.nbcslots, 1);
+  hipLaunchKernelGGL(k_fill, dim3(launch_blocks(nW)), dim3(256), 0, c->stream, c->d_W, nW, 0.0);
+  hipLaunchKernelGGL(k_fill, dim3(launch_blocks(nB)), dim3(256), 0, c->stream, c->d_BB, nB, 0.0);
+  AdjArgs a = adj_args(c);
+  const int lim = c->cfg.limiter, rs = c->cfg.riemann, geo = c->lay.curvi ? 1 : 0;
+  void* args[] = {&a};
+  (void)hipLaunchKernel((const void*)atable().face[lim][rs][geo], dim3(launch_blocks(c->nwork)), dim3(256), args, 0,
+                        c->stream);
+  if (a.nbcf > 0)
+    (void)hipLaunchKernel((const void*)atable().bc[lim][rs], dim3(launch_blocks(a.nbcf)), dim3(256), args, 0, c->stream);
+  (void)hipLaunchKernel((const void*)atable().cell[lim][geo], dim3(launch_blocks(c->npool)), dim3(256), args, 0,
+                        c->stream);
+  if (c->ngrows > 0)
+    hipLaunchKernelGGL(k_adj_ghostT, dim3(launch_blocks(c->ngrows)), dim3(256), 0, c->stream, a);
+}
+
+// One residual evaluation (what mose_residual computes) without the copy-out.
+static void enqueue_residual(mose_ctx* c) {
+  const int64_t n5 = 5 * c->npool;
+  hipLaunchKernelGGL(k_fill, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->Rbuf, n5, 0.0);
+  enqueue_ghost(c, c->U[0]);
+  StageArgs a = stage_args(c, c->U[0], c->U[0], c->U[1], 0.0);
+  enqueue_bcfaces(c, a);
+  launch_stage(c, stage_fn(c, 2), a);
 }
 
 // ============================================================================= C ABI
@@ -567,6 +753,8 @@ int mose_finalize(mose_ctx* c) {
   HIPCHK(c, hipMalloc((void**)&c->Rbuf, 5 * c->npool * sizeof(double)));
   if (upload(c, &c->d_blocks, c->lay.blocks)) return c->err.code;
   if (upload(c, &c->d_lines, c->lay.lines)) return c->err.code;
+  if (c->lay.curvi && upload(c, &c->d_geo, c->lay.geo)) return c->err.code;
+  std::vector<double>().swap(c->lay.geo);   // host copy no longer needed
   if (upload(c, &c->d_ghost, c->lay.ghost)) return c->err.code;
   if (upload(c, &c->d_facebc, c->lay.facebc)) return c->err.code;
   if (upload(c, &c->d_bcfaces, c->lay.bcfaces)) return c->err.code;
@@ -625,7 +813,7 @@ int mose_compute_dt(mose_ctx* c, int32_t iter, double* dtglobal) {
   HIPCHK(c, hipMemcpy(&cur, c->d_iter, sizeof cur, hipMemcpyDeviceToHost));
   hipLaunchKernelGGL(k_dtreset, dim3(1), dim3(1), 0, c->stream, c->d_dtmin);
   DtArgs da = dt_args(c, iter - cur);
-  hipLaunchKernelGGL(k_dt, dim3(da.ntiles), dim3(c->tx * c->ty), 0, c->stream, da);
+  launch_dt(c, da);
   unsigned long long bits = 0;
   HIPCHK(c, hipMemcpyAsync(&bits, c->d_dtmin, sizeof bits, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -665,12 +853,7 @@ int mose_step(mose_ctx* c, double* dt_out, double* time_out, double* norms) {
 
 int mose_residual(mose_ctx* c, int32_t b, double* R) {
   if (!c || !c->finalized || b < 1 || b > c->nb || !R) return fail(c, MOSE_E_ARG, b);
-  const int64_t n5 = 5 * c->npool;
-  hipLaunchKernelGGL(k_fill, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->Rbuf, n5, 0.0);
-  enqueue_ghost(c, c->U[0]);
-  StageArgs a = stage_args(c, c->U[0], c->U[0], c->U[1], 0.0);
-  enqueue_bcfaces(c, a);
-  launch_stage(c, stage_fn(c, 2), a);
+  enqueue_residual(c);
   const BlockDev& B = c->lay.blocks[b - 1];
   const int64_t ncell = B.sz * (B.nk + 4);
   if (ensure_stage(c, 5 * ncell)) return c->err.code;
@@ -711,7 +894,7 @@ int mose_time_kernels(mose_ctx* c, int32_t nrep, double ms[4]) {
       const int reps = pass == 0 ? 1 : nrep;
       HIPCHK(c, hipEventRecord(e0, c->stream));
       for (int r = 0; r < reps; ++r) {
-        if (which == 0) hipLaunchKernelGGL(k_dt, dim3(nt), dim3(c->tx * c->ty), 0, c->stream, da);
+        if (which == 0) launch_dt(c, da);
         else if (which == 1) enqueue_ghost(c, c->U[0]);
         else {
           StageArgs sa = stage_args(c, c->U[0], c->U[0], c->U[1], 0.25);
@@ -779,6 +962,65 @@ int mose_autotune(mose_ctx* c, int32_t nrep, int32_t choice[3], double* best_ms)
   return sync_and_check(c);
 }
 
+int mose_put_residual_bar(mose_ctx* c, int32_t b, const double* Rbar) {
+  if (!c || !c->finalized || b < 1 || b > c->nb || !Rbar) return fail(c, MOSE_E_ARG, b);
+  if (ensure_adjoint(c)) return c->err.code;
+  const BlockDev& B = c->lay.blocks[b - 1];
+  const int64_t ncell = B.sz * (B.nk + 4);
+  if (ensure_stage(c, 5 * ncell)) return c->err.code;
+  HIPCHK(c, hipMemcpyAsync(c->stage, Rbar, 5 * ncell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_put_raw, dim3(launch_blocks(ncell)), dim3(256), 0, c->stream, c->stage, c->d_Rbar, c->npool,
+                     B.off, ncell);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mose_residual_vjp(mose_ctx* c) {
+  if (!c || !c->finalized) return fail(c, MOSE_E_ARG);
+  if (c->err.code) return c->err.code;
+  if (ensure_adjoint(c)) return c->err.code;
+  enqueue_vjp(c);
+  return sync_and_check(c);
+}
+
+int mose_get_state_bar(mose_ctx* c, int32_t b, double* Pbar) {
+  if (!c || !c->finalized || b < 1 || b > c->nb || !Pbar) return fail(c, MOSE_E_ARG, b);
+  if (ensure_adjoint(c)) return c->err.code;
+  const BlockDev& B = c->lay.blocks[b - 1];
+  const int64_t ncell = B.sz * (B.nk + 4);
+  if (ensure_stage(c, 5 * ncell)) return c->err.code;
+  hipLaunchKernelGGL(k_get_interior, dim3(launch_blocks(ncell)), dim3(256), 0, c->stream, c->d_Pbar, c->npool,
+                     B.off, B.ni, B.nj, B.nk, c->stage);
+  HIPCHK(c, hipMemcpyAsync(Pbar, c->stage, 5 * ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mose_time_residual_vjp(mose_ctx* c, int32_t nrep, double ms[2]) {
+  if (!c || !c->finalized || nrep < 1 || !ms) return fail(c, MOSE_E_ARG);
+  if (ensure_adjoint(c)) return c->err.code;
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  for (int which = 0; which < 2; ++which)
+    for (int pass = 0; pass < 2; ++pass) {
+      const int reps = pass == 0 ? 1 : nrep;
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+      for (int r = 0; r < reps; ++r) {
+        if (which == 0) enqueue_residual(c);
+        else enqueue_vjp(c);
+      }
+      HIPCHK(c, hipEventRecord(e1, c->stream));
+      HIPCHK(c, hipEventSynchronize(e1));
+      float t = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&t, e0, e1));
+      if (pass == 1) ms[which] = (double)t / nrep;
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return sync_and_check(c);
+}
+
 int64_t mose_num_cells(const mose_ctx* c) {
   if (!c) return 0;
   int64_t n = 0;
@@ -811,9 +1053,9 @@ void mose_destroy(mose_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
-  void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_tiles,
+  void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_geo, c->d_tiles,
                   c->d_ghost, c->d_bcfaces, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
-                  c->d_iter};
+                  c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_segs, c->d_grows, c->d_gents};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -7,6 +7,11 @@
 //   * Geometry of rectilinear blocks: per block and direction a line of `LineEnt`
 //     (cell length dl as the reference computes it, node spacing, reconstruction spacings
 //     of the face on the cell's low side).  Normals are the axis unit vectors.
+//   * Geometry of curvilinear blocks (any block not rectilinear switches the context to it):
+//     per pool cell, SoA over the pool, NGEO FP64 arrays: the unit normal and area of the
+//     cell's low face in each direction, the cell lengths dl, the volume, the unit rows of
+//     the inverse Jacobian (CFL time step) and the reciprocal distance to the low neighbour's
+//     centre (reconstruction), all from the reference's metric definitions.
 //   * Work: a list of tiles (block, i0, j0, k-range).  Each workgroup of TX x TY threads
 //     owns a TX x TY column patch and marches along k; the current k-plane's primitive
 //     state with a 2-cell i/j halo is staged in LDS, the k-direction stencil of each column
@@ -23,6 +28,10 @@ enum { G_COPY = 0, G_ZG = 1, G_EXTRAP = 2, G_SYM = 3 };
 enum { F_CONN = 0, F_NONE = 1, F_EXTRAP = 2, F_SYM = 3, F_INFLOW = 4 };
 // Scalars kept on the device.
 enum { S_DT = 0, S_TIME = 1, S_NORM = 2, S_COUNT = 8 };
+// Curvilinear geometry arrays (index g: array g * npool + cell).
+//   GN + 4d + {0,1,2}: unit normal of the low face in direction d, GN + 4d + 3: its area;
+//   GDL + d: dl; GVOL: volume; GVS + 3d + {0,1,2}: M(d,:)/|M(d,:)|; GRLO + d: 2/(dl(c-e_d)+dl(c)).
+enum { GN = 0, GDL = 12, GVOL = 15, GVS = 16, GRLO = 25, NGEO = 28 };
 
 struct LineEnt {
   double dl;      // cell length along the line (Compute_Metric_Tensor dl)
@@ -47,6 +56,8 @@ struct BcFace {
   int b, f;          // block (0-based), face (0..5)
   int slot, pad;     // output slot in the boundary-flux buffer
   double bdl, idl;   // half lengths of the two cells along the face normal
+  double n[3];       // unit normal of the boundary face (increasing-index orientation)
+  double area;       // face area
 };
 
 struct Tile {
@@ -70,6 +81,7 @@ struct StageArgs {
   const BlockDev* __restrict__ blocks;
   const LineEnt* __restrict__ lines;
   const double* __restrict__ facebc;
+  const double* __restrict__ geo;      // curvilinear geometry arrays (NGEO x npool) or null
   const double* __restrict__ scal;
   const double* dtloc;          // local time stepping: dt of each cell (same array as dtnext)
   double* __restrict__ partials;

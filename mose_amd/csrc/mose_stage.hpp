@@ -53,7 +53,6 @@ __global__ void __launch_bounds__(256) k_bcface(StageArgs a, const BcFace* __res
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nfaces; q += gridDim.x * blockDim.x) {
     const BcFace e = faces[q];
     const BlockDev* Bp = a.blocks + e.b;
-    const int d = e.f >> 1;
     const bool hi = e.f & 1;
     const int bct = Bp->bct[e.f];
     FT F[5] = {FT(0), FT(0), FT(0), FT(0), FT(0)};
@@ -61,7 +60,7 @@ __global__ void __launch_bounds__(256) k_bcface(StageArgs a, const BcFace* __res
       FT Pb[5], Pi[5];
       prim_at(a.Uin, (unsigned)a.npool, (unsigned)e.cb, gm1, Pb);
       prim_at(a.Uin, (unsigned)a.npool, (unsigned)e.cn, gm1, Pi);
-      const FT nx = d == 0 ? FT(1) : FT(0), ny = d == 1 ? FT(1) : FT(0), nz = d == 2 ? FT(1) : FT(0);
+      const FT nx = FT(e.n[0]), ny = FT(e.n[1]), nz = FT(e.n[2]);
       if (bct == F_EXTRAP) {
         physical_flux(Pb, nx, ny, nz, g, F);
       } else {
@@ -76,7 +75,9 @@ __global__ void __launch_bounds__(256) k_bcface(StageArgs a, const BcFace* __res
 }
 
 // MODE 0: stage update; 1: last stage (update + residual-norm partials); 2: residual only.
-template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc>
+// GEO 0: rectilinear blocks (geometry lines); 1: general metrics from the geometry arrays,
+// with face fluxes scaled by their areas in the face loop.
+template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc, int GEO>
 __global__ void __launch_bounds__(TXc* TYc)
     __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 && TXc * TYc == 256 ? 3 : 1)))
     k_stage(StageArgs a) {
@@ -102,6 +103,8 @@ __global__ void __launch_bounds__(TXc* TYc)
   const GasT<FT> g{FT(a.gam), FT(gm1), FT(1.0 / gm1), FT(a.R), FT(a.gam * a.R / gm1)};
   const unsigned N = (unsigned)a.npool;
   const double* __restrict__ Uin = a.Uin;
+  // geometry array g at pool cell c (general metrics)
+  auto GA = [&](int g, unsigned c) -> double { return a.geo[(size_t)g * N + c]; };
 
   // the halo cell this thread stages (LDS coordinates, -2..TX+1 / -2..TY+1)
   int hx = 0, hy = 0;
@@ -153,7 +156,12 @@ __global__ void __launch_bounds__(TXc* TYc)
       prim_at(Uin, N, cbase + sz * (unsigned)(k0), gm1, Q0);         // P(k0-1)
       prim_at(Uin, N, cbase + sz * (unsigned)(k0 + 1), gm1, Q1);     // P(k0)
       prim_at(Uin, N, cbase + sz * (unsigned)(k0 + 2), gm1, Q2);     // P(k0+1)
-      cell_states<LIM>(Qm, Q0, Q1, FT(Lz[k0].rl1), FT(Lz[k0 + 1].rl1), FT(Lz[k0].lp), Sm_, Skp);   // P+(k0-1)
+      if constexpr (GEO == 0) {
+        cell_states<LIM>(Qm, Q0, Q1, FT(Lz[k0].rl1), FT(Lz[k0 + 1].rl1), FT(Lz[k0].lp), Sm_, Skp);   // P+(k0-1)
+      } else {
+        const unsigned cm = cbase + sz * (unsigned)(k0);
+        cell_states<LIM>(Qm, Q0, Q1, FT(GA(GRLO + 2, cm)), FT(GA(GRLO + 2, cm + sz)), FT(0.5 * GA(GDL + 2, cm)), Sm_, Skp);
+      }
     }
     if (hval) {
 #pragma unroll
@@ -189,7 +197,12 @@ __global__ void __launch_bounds__(TXc* TYc)
       // ---- face states: k face k+1/2 of the own column (registers), then i and j from LDS
       if (active) {
         FT Sm_[5], Sp_[5];
-        cell_states<LIM>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), Sm_, Sp_);
+        if constexpr (GEO == 0) {
+          cell_states<LIM>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), Sm_, Sp_);
+        } else {
+          const unsigned c1 = cbase + sz * (unsigned)(k + 2);
+          cell_states<LIM>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)), Sm_, Sp_);
+        }
 #pragma unroll
         for (int v = 0; v < 5; ++v) { sL[v][KOFF + tid] = Skp[v]; sR[v][KOFF + tid] = Sm_[v]; Skp[v] = Sp_[v]; }
       }
@@ -214,9 +227,16 @@ __global__ void __launch_bounds__(TXc* TYc)
               c[v] = sP[v][ly + 2][lx + 2];
               p[v] = sP[v][ly + 2 + sy][lx + 2 + sx];
             }
-            const LineEnt* L = d == 0 ? Lx : Ly;
-            const int cc = d == 0 ? T.i0 + lx : T.j0 + ly;
-            cell_states<LIM>(m, c, p, FT(L[cc + 1].rl1), FT(L[cc + 2].rl1), FT(L[cc + 1].lp), Sm_, Sp_);
+            if constexpr (GEO == 0) {
+              const LineEnt* L = d == 0 ? Lx : Ly;
+              const int cc = d == 0 ? T.i0 + lx : T.j0 + ly;
+              cell_states<LIM>(m, c, p, FT(L[cc + 1].rl1), FT(L[cc + 2].rl1), FT(L[cc + 1].lp), Sm_, Sp_);
+            } else {
+              const unsigned pc = off + (unsigned)(T.i0 + lx + 1) + (unsigned)sy_ * (unsigned)(T.j0 + ly + 1) +
+                                  sz * (unsigned)(k + 1);
+              const unsigned st = d == 0 ? 1u : (unsigned)sy_;
+              cell_states<LIM>(m, c, p, FT(GA(GRLO + d, pc)), FT(GA(GRLO + d, pc + st)), FT(0.5 * GA(GDL + d, pc)), Sm_, Sp_);
+            }
             // slots: i face (s, r) -> r (TX+1) + s ; j face (c, s) -> NI + s TX + c
             const int flo = d == 0 ? ly * (TXc + 1) + lx : NI + ly * TXc + lx;
             const int fhi = d == 0 ? flo + 1 : flo + TXc;
@@ -246,6 +266,13 @@ __global__ void __launch_bounds__(TXc* TYc)
         if (!valid) continue;
         const int bf = idx == 1 ? 2 * d : (idx == nd + 1 ? 2 * d + 1 : -1);
         FT F[5];
+        FT nx = d == 0 ? FT(1) : FT(0), ny = d == 1 ? FT(1) : FT(0), nz = d == 2 ? FT(1) : FT(0), area = FT(1);
+        if constexpr (GEO == 1) {
+          const unsigned pf = off + (unsigned)(gi + 1) + (unsigned)sy_ * (unsigned)(gj + 1) +
+                              sz * (unsigned)(d == 2 ? k + 2 : k + 1);
+          nx = FT(GA(GN + 4 * d, pf)); ny = FT(GA(GN + 4 * d + 1, pf)); nz = FT(GA(GN + 4 * d + 2, pf));
+          area = FT(GA(GN + 4 * d + 3, pf));
+        }
         if (bf >= 0 && Bp->bct[bf] != F_CONN) {
           // tangential position on the face, cyclic order (d+1, d+2)
           const int kk = d == 2 ? k + 1 : k;
@@ -261,7 +288,11 @@ __global__ void __launch_bounds__(TXc* TYc)
           for (int v = 0; v < 5; ++v) { Lq[v] = sL[v][f]; Rq[v] = sR[v][f]; }
           if (!(gas_ok(Lq) && gas_ok(Rq)))
             set_error(a.err, MOSE_E_INTERFACE, T.b + 1, d == 0 ? gi - 1 : gi, d == 1 ? gj - 1 : gj, k);
-          riemann<RS>(Lq, Rq, d == 0 ? FT(1) : FT(0), d == 1 ? FT(1) : FT(0), d == 2 ? FT(1) : FT(0), g, F);
+          riemann<RS>(Lq, Rq, nx, ny, nz, g, F);
+        }
+        if constexpr (GEO == 1) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) F[v] = F[v] * area;
         }
 #pragma unroll
         for (int v = 0; v < 5; ++v) sL[v][f] = F[v];
@@ -270,14 +301,14 @@ __global__ void __launch_bounds__(TXc* TYc)
 
       // ---- residual of the own cell and the RK update of plane k
       if (active) {
-        const double Az = Lx[i + 1].delta * Ly[j + 1].delta;
+        const double Az = GEO == 0 ? Lx[i + 1].delta * Ly[j + 1].delta : 1.0;
         FT Fk[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) { Fk[v] = sL[v][KOFF + tid]; Racc[v] = Az * ((double)Fk[v] - (double)Fkm[v]); Fkm[v] = Fk[v]; }
         if (plane) {
           const int fi = ty * (TXc + 1) + tx, fj = NI + ty * TXc + tx;
-          const double Ax = Ly[j + 1].delta * Lz[k + 1].delta;
-          const double Ay = Lx[i + 1].delta * Lz[k + 1].delta;
+          const double Ax = GEO == 0 ? Ly[j + 1].delta * Lz[k + 1].delta : 1.0;
+          const double Ay = GEO == 0 ? Lx[i + 1].delta * Lz[k + 1].delta : 1.0;
 #pragma unroll
           for (int v = 0; v < 5; ++v)
             Racc[v] += Ax * ((double)sL[v][fi + 1] - (double)sL[v][fi]) + Ay * ((double)sL[v][fj + TXc] - (double)sL[v][fj]);
@@ -290,7 +321,7 @@ __global__ void __launch_bounds__(TXc* TYc)
 #pragma unroll
             for (int v = 0; v < 5; ++v) { un[v] = a.Un[v * N + c]; ui[v] = Uin[v * N + c]; }
             const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
-            const double vol = Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta;
+            const double vol = GEO == 0 ? Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta : GA(GVOL, c);
             const double rv = dt / vol;
 #pragma unroll
             for (int v = 0; v < 5; ++v) unew[v] = un[v] + a.rkc * (ui[v] - un[v] - Racc[v] * rv);
@@ -304,9 +335,17 @@ __global__ void __launch_bounds__(TXc* TYc)
               // CFL time step of the new state for the next step (Compute_dt, Mod_dt.f90:61-85)
               const double snd = sqrt(a.gam * P[4] / P[0]);
               double dtc = 1e8;
-              dtc = fmin(Lx[i + 1].dl / (fabs(P[1]) + snd) * a.cfl, dtc);
-              if (a.ndir > 1) dtc = fmin(Ly[j + 1].dl / (fabs(P[2]) + snd) * a.cfl, dtc);
-              if (a.ndir > 2) dtc = fmin(Lz[k + 1].dl / (fabs(P[3]) + snd) * a.cfl, dtc);
+              if constexpr (GEO == 0) {
+                dtc = fmin(Lx[i + 1].dl / (fabs(P[1]) + snd) * a.cfl, dtc);
+                if (a.ndir > 1) dtc = fmin(Ly[j + 1].dl / (fabs(P[2]) + snd) * a.cfl, dtc);
+                if (a.ndir > 2) dtc = fmin(Lz[k + 1].dl / (fabs(P[3]) + snd) * a.cfl, dtc);
+              } else {
+                for (int dd = 0; dd < a.ndir; ++dd) {
+                  const double vel = fabs(P[1] * GA(GVS + 3 * dd, c) + P[2] * GA(GVS + 3 * dd + 1, c) +
+                                          P[3] * GA(GVS + 3 * dd + 2, c));
+                  dtc = fmin(GA(GDL + dd, c) / (vel + snd) * a.cfl, dtc);
+                }
+              }
               const int itn = *a.iter + 1;
               if (itn < a.rampa) dtc = dtc * itn / a.rampa;
               if (a.local_dt) a.dtnext[c] = dtc;

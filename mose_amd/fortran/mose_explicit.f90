@@ -116,7 +116,8 @@ contains
     domain%ctx = c_null_ptr
   end subroutine Domain_Destroy
 
-  !> Uniform Cartesian nodes of a block spanning [lo, hi].
+  !> Uniform Cartesian nodes of a block spanning [lo, hi]: node q = q * step + lo along each
+  !> axis, step = (hi - lo) / n, the last node exactly hi.
   subroutine Cartesian_Nodes(blk, lo, hi)
     type(mose_block), intent(inout) :: blk
     real(c_double), intent(in)      :: lo(3), hi(3)
@@ -124,12 +125,22 @@ contains
     do k = 0, blk%nk
       do j = 0, blk%nj
         do i = 0, blk%ni
-          blk%node(1, i, j, k) = lo(1) + (hi(1) - lo(1)) * real(i, c_double) / real(blk%ni, c_double)
-          blk%node(2, i, j, k) = lo(2) + (hi(2) - lo(2)) * real(j, c_double) / real(blk%nj, c_double)
-          blk%node(3, i, j, k) = lo(3) + (hi(3) - lo(3)) * real(k, c_double) / real(blk%nk, c_double)
+          blk%node(1, i, j, k) = axis_node(i, blk%ni, lo(1), hi(1))
+          blk%node(2, i, j, k) = axis_node(j, blk%nj, lo(2), hi(2))
+          blk%node(3, i, j, k) = axis_node(k, blk%nk, lo(3), hi(3))
         enddo
       enddo
     enddo
+  contains
+    pure real(c_double) function axis_node(q, n, a, b)
+      integer, intent(in)        :: q, n
+      real(c_double), intent(in) :: a, b
+      if (q == n) then
+        axis_node = b
+      else
+        axis_node = real(q, c_double) * ((b - a) / real(n, c_double)) + a
+      endif
+    end function axis_node
   end subroutine Cartesian_Nodes
 
   !> Boundary records of one block whose faces carry non-connection types ftype(1:6)

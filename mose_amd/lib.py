@@ -18,7 +18,8 @@ _LIB = None
 
 EXPORTS = ["mose_create", "mose_add_block", "mose_set_bc", "mose_finalize", "mose_put_state",
            "mose_get_state", "mose_compute_dt", "mose_step", "mose_advance", "mose_residual",
-           "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_num_cells", "mose_last_error", "mose_strerror",
+           "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_put_residual_bar",
+           "mose_residual_vjp", "mose_get_state_bar", "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
            "mose_destroy"]
 
 
@@ -46,6 +47,10 @@ def load():
     L.mose_set_time.argtypes = [v, C.c_double, C.c_int32]
     L.mose_time_kernels.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_autotune.argtypes = [v, C.c_int32, P(C.c_int32), P(C.c_double)]
+    L.mose_put_residual_bar.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_residual_vjp.argtypes = [v]
+    L.mose_get_state_bar.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_time_residual_vjp.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_num_cells.argtypes = [v]
     L.mose_num_cells.restype = C.c_int64
     L.mose_last_error.argtypes = [v, P(abi.MoseErr)]
@@ -125,6 +130,28 @@ class GpuSolver:
         R = np.zeros(abi.padded_shape(self.dims[b - 1]))
         self._check(load().mose_residual(self.h, b, _dp(R)))
         return R
+
+    def residual_vjp(self, Rbars):
+        """Discrete adjoint of the residual: Rbars, one blk%R-layout array per block ->
+        Pbar = (dR/dP)^T Rbar, one blk%P-layout array per block (ghost entries 0)."""
+        L = load()
+        for b, Rb in enumerate(Rbars, start=1):
+            Rb = np.ascontiguousarray(Rb, dtype=np.float64)
+            assert Rb.shape == abi.padded_shape(self.dims[b - 1])
+            self._check(L.mose_put_residual_bar(self.h, b, _dp(Rb)))
+        self._check(L.mose_residual_vjp(self.h))
+        out = []
+        for b in range(1, len(self.dims) + 1):
+            Pb = np.zeros(abi.padded_shape(self.dims[b - 1]))
+            self._check(L.mose_get_state_bar(self.h, b, _dp(Pb)))
+            out.append(Pb)
+        return out
+
+    def time_residual_vjp(self, nrep=10):
+        """Average device ms of one residual and one adjoint evaluation."""
+        ms = np.zeros(2)
+        self._check(load().mose_time_residual_vjp(self.h, nrep, _dp(ms)))
+        return float(ms[0]), float(ms[1])
 
     def residuals(self):
         return [self.residual(b) for b in range(1, len(self.dims) + 1)]

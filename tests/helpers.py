@@ -39,3 +39,15 @@ def perturb(case, amp=0.02, seed=0):
         Pi[..., 1:4] += amp * noise[..., 1:4]
         Pi[..., 4] *= 1.0 + amp * noise[..., 4]
     return case
+
+
+def skewed_nodes(dims, amp=0.08, hi=(1.0, 0.8, 0.6)):
+    """Smoothly distorted (curvilinear) grid of a box: node (3, 0:ni, 0:nj, 0:nk) array in
+    numpy (k, j, i, 3) order."""
+    from mose_amd import cases
+    nodes = cases.cartesian_nodes(dims, (0, 0, 0), hi)
+    X, Y, Z = nodes[..., 0].copy(), nodes[..., 1].copy(), nodes[..., 2].copy()
+    nodes[..., 0] = X + amp * np.sin(2 * np.pi * Y) * np.sin(np.pi * Z)
+    nodes[..., 1] = Y + amp * np.sin(2 * np.pi * X) * np.cos(np.pi * Z)
+    nodes[..., 2] = Z + 0.5 * amp * np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y)
+    return np.ascontiguousarray(nodes)

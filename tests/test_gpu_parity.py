@@ -183,15 +183,16 @@ def test_bad_bc_list_rejected():
 
 @pytest.mark.gpu
 def test_autotune_keeps_results():
-    """The autotuner only changes the work decomposition: states after a few steps are
-    bit-identical, the norms (summed per tile) agree to round-off."""
-    case = cases.multiblock_chain(nblocks=4, seed=3, lo=12, hi=40)
+    """The autotuner only changes the work decomposition (tile shape, k chunk, order): FP64
+    states after a few steps agree to round-off (another tile shape is another compiled
+    kernel, whose floating-point contractions may differ), the norms likewise."""
+    case = cases.multiblock_chain(nblocks=4, seed=3, lo=12, hi=40, precision=abi.PREC_FP64)
     g1, g2 = GpuSolver(case), GpuSolver(case)
     choice, ms = g2.autotune(2)
     assert ms > 0 and choice["kchunk"] in (8, 16, 32, 64)
     t1, n1 = g1.advance(3)
     t2, n2 = g2.advance(3)
-    assert t1 == t2
+    assert abs(t1 - t2) <= 1e-13 * t1
     for b in range(1, len(case.blocks) + 1):
-        np.testing.assert_array_equal(interior(g1.get_state(b)), interior(g2.get_state(b)))
-    np.testing.assert_allclose(n1, n2, rtol=1e-12, atol=1e-14 * max(np.max(np.abs(n1)), 1e-300))
+        assert rel_l2(g1.get_state(b), g2.get_state(b)) < 1e-13
+    np.testing.assert_allclose(n1, n2, rtol=1e-10, atol=1e-13 * max(np.max(np.abs(n1)), 1e-300))
