@@ -13,7 +13,8 @@ import numpy as np
 from . import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmose_hip.so")
+# MOSE_HIP_LIB may name another build of the same library (profiling of kernel variants)
+LIB_PATH = os.environ.get("MOSE_HIP_LIB") or os.path.join(_HERE, "libmose_hip.so")
 _LIB = None
 
 EXPORTS = ["mose_create", "mose_add_block", "mose_set_bc", "mose_finalize", "mose_put_state",
