@@ -511,6 +511,7 @@ struct AdjGhostEnt {
 
 struct AdjArgs {
   const double* __restrict__ U;         // state with filled ghosts
+  const double* __restrict__ Pp;        // 5 x npool: its primitive variables
   const double* __restrict__ Rbar;      // 5 x npool
   double* __restrict__ W;               // 30 x npool: face-state adjoints (3 dirs x {Sp, Sm} x 5)
   double* __restrict__ BB;              // 10 x nslots: boundary-face adjoints (cell, neighbour)
@@ -556,11 +557,23 @@ struct AdjGeo {
   }
 };
 
-__device__ __forceinline__ void prim_load(const double* __restrict__ U, int64_t N, int64_t c, double gm1, double* P) {
-  double u[5];
+__device__ __forceinline__ void prim_load(const double* __restrict__ Pp, int64_t N, int64_t c, double* P) {
 #pragma unroll
-  for (int v = 0; v < 5; ++v) u[v] = U[v * N + c];
-  cons2prim(u, gm1, P);
+  for (int v = 0; v < 5; ++v) P[v] = Pp[v * N + c];
+}
+
+// Primitive variables of every pool cell (ghost layers filled), read by the adjoint passes.
+__global__ void __launch_bounds__(256) k_adj_prim(const double* __restrict__ U, double* __restrict__ Pp, int64_t N,
+                                                  double gam) {
+  const double gm1 = gam - 1.0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
+    double u[5], P[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = U[v * N + c];
+    if (u[0] != 0.0) cons2prim(u, gm1, P);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Pp[v * N + c] = P[v];
+  }
 }
 
 template <int LIM, int RS, int GEO>
@@ -586,40 +599,45 @@ __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a) {
     ijk[d] = qd; ijk[e1] = t1; ijk[e2] = t2;
     const int nd = d == 0 ? B.ni : (d == 1 ? B.nj : B.nk);
     const bool blo = qd == 1, bhi = qd == nd + 1;
-    if ((blo && B.bct[2 * d] != F_CONN) || (bhi && B.bct[2 * d + 1] != F_CONN)) continue;
+    const bool skip = (blo && B.bct[2 * d] != F_CONN) || (bhi && B.bct[2 * d + 1] != F_CONN);
     const int64_t s = d == 0 ? 1 : (d == 1 ? (int64_t)B.sy : B.sz);
     const int64_t c = B.off + (ijk[0] + 1) + (int64_t)B.sy * (ijk[1] + 1) + B.sz * (int64_t)(ijk[2] + 1);
-    double Pm2[5], Pm1[5], P0[5], Pp1[5];
-    prim_load(a.U, N, c - 2 * s, gm1, Pm2);
-    prim_load(a.U, N, c - s, gm1, Pm1);
-    prim_load(a.U, N, c, gm1, P0);
-    prim_load(a.U, N, c + s, gm1, Pp1);
-    const AdjGeo<GEO> G{a, B};
-    const double rlA = G.rl(d, c - s, qd - 1), rlB = G.rl(d, c, qd), rlC = G.rl(d, c + s, qd + 1);
-    const double hA = G.hdl(d, c - s, qd - 1), hB = G.hdl(d, c, qd);
-    double PL[5], PR[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      const double sgA = limiter<LIM>((P0[v] - Pm1[v]) * rlB, (Pm1[v] - Pm2[v]) * rlA) * hA;
-      const double sgB = limiter<LIM>((Pp1[v] - P0[v]) * rlC, (P0[v] - Pm1[v]) * rlB) * hB;
-      PL[v] = Pm1[v] + sgA;
-      PR[v] = P0[v] - sgB;
-    }
-    double n[3], area;
-    G.face(d, c, ijk[0], ijk[1], ijk[2], n, area);
-    double Fb[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      const double rl_ = blo ? 0.0 : a.Rbar[v * N + c - s];
-      const double rr_ = bhi ? 0.0 : a.Rbar[v * N + c];
-      Fb[v] = area * (rl_ - rr_);
-    }
     double Lb[5] = {0, 0, 0, 0, 0}, Rb[5] = {0, 0, 0, 0, 0};
-    riemann_adj<RS>(PL, PR, n[0], n[1], n[2], g, Fb, Lb, Rb);
+    if (!skip) {
+      double Pm2[5], Pm1[5], P0[5], Pp1[5];
+      prim_load(a.Pp, N, c - 2 * s, Pm2);
+      prim_load(a.Pp, N, c - s, Pm1);
+      prim_load(a.Pp, N, c, P0);
+      prim_load(a.Pp, N, c + s, Pp1);
+      const AdjGeo<GEO> G{a, B};
+      const double rlA = G.rl(d, c - s, qd - 1), rlB = G.rl(d, c, qd), rlC = G.rl(d, c + s, qd + 1);
+      const double hA = G.hdl(d, c - s, qd - 1), hB = G.hdl(d, c, qd);
+      double PL[5], PR[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double sgA = limiter<LIM>((P0[v] - Pm1[v]) * rlB, (Pm1[v] - Pm2[v]) * rlA) * hA;
+        const double sgB = limiter<LIM>((Pp1[v] - P0[v]) * rlC, (P0[v] - Pm1[v]) * rlB) * hB;
+        PL[v] = Pm1[v] + sgA;
+        PR[v] = P0[v] - sgB;
+      }
+      double n[3], area;
+      G.face(d, c, ijk[0], ijk[1], ijk[2], n, area);
+      double Fb[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double rl_ = blo ? 0.0 : a.Rbar[v * N + c - s];
+        const double rr_ = bhi ? 0.0 : a.Rbar[v * N + c];
+        Fb[v] = area * (rl_ - rr_);
+      }
+      riemann_adj<RS>(PL, PR, n[0], n[1], n[2], g, Fb, Lb, Rb);
+    }
+    // every "+" / "-" slot that the cell pass reads is written here (zero when no face uses it)
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       a.W[(size_t)(10 * d + v) * N + c - s] = Lb[v];       // "+" state of the left cell
       a.W[(size_t)(10 * d + 5 + v) * N + c] = Rb[v];       // "-" state of the right cell
+      if (blo) a.W[(size_t)(10 * d + 5 + v) * N + c - s] = 0.0;   // "-" slot of cell 0
+      if (bhi) a.W[(size_t)(10 * d + v) * N + c] = 0.0;           // "+" slot of cell n+1
     }
   }
 }
@@ -637,8 +655,8 @@ __global__ void __launch_bounds__(256) k_adj_bcface(AdjArgs a) {
     double Pbb[5] = {0, 0, 0, 0, 0}, Pib[5] = {0, 0, 0, 0, 0};
     if (bct != F_NONE) {
       double Pb[5], Pi[5], Fb[5];
-      prim_load(a.U, N, e.cb, gm1, Pb);
-      prim_load(a.U, N, e.cn, gm1, Pi);
+      prim_load(a.Pp, N, e.cb, Pb);
+      prim_load(a.Pp, N, e.cn, Pi);
       // R(cb) += (hi ? +1 : -1) area F
 #pragma unroll
       for (int v = 0; v < 5; ++v) Fb[v] = (hi ? e.area : -e.area) * a.Rbar[v * N + e.cb];
@@ -703,9 +721,9 @@ __global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a) {
           }
           if (!any) continue;
           double Pm[5], P0[5], Pp[5];
-          prim_load(a.U, N, cm - s, gm1, Pm);
-          prim_load(a.U, N, cm, gm1, P0);
-          prim_load(a.U, N, cm + s, gm1, Pp);
+          prim_load(a.Pp, N, cm - s, Pm);
+          prim_load(a.Pp, N, cm, P0);
+          prim_load(a.Pp, N, cm + s, Pp);
           double bm[5] = {0, 0, 0, 0, 0}, b0[5] = {0, 0, 0, 0, 0}, bp[5] = {0, 0, 0, 0, 0};
           cell_states_adj<LIM>(Pm, P0, Pp, G.rl(d, cm, qm), G.rl(d, cm + s, qm + 1), G.hdl(d, cm, qm), Smb, Spb,
                                bm, b0, bp);

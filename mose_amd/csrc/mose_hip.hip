@@ -391,7 +391,7 @@ struct mose_ctx {
   hipGraphExec_t gexec = nullptr;
   // discrete adjoint (allocated on first use)
   bool adj_ready = false;
-  double *d_Rbar = nullptr, *d_W = nullptr, *d_BB = nullptr, *d_Pbar = nullptr;
+  double *d_Rbar = nullptr, *d_W = nullptr, *d_BB = nullptr, *d_Pbar = nullptr, *d_Pp = nullptr;
   AdjSeg* d_segs = nullptr;
   AdjGhostRow* d_grows = nullptr;
   AdjGhostEnt* d_gents = nullptr;
@@ -645,13 +645,14 @@ static int ensure_adjoint(mose_ctx* c) {
   HIPCHK(c, hipMalloc((void**)&c->d_W, (size_t)30 * c->npool * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_BB, (size_t)10 * ns * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_Pbar, 5 * c->npool * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_Pp, 5 * c->npool * sizeof(double)));
   c->adj_ready = true;
   return 0;
 }
 
 static AdjArgs adj_args(mose_ctx* c) {
   AdjArgs a{};
-  a.U = c->U[0]; a.Rbar = c->d_Rbar; a.W = c->d_W; a.BB = c->d_BB; a.Pbar = c->d_Pbar;
+  a.U = c->U[0]; a.Pp = c->d_Pp; a.Rbar = c->d_Rbar; a.W = c->d_W; a.BB = c->d_BB; a.Pbar = c->d_Pbar;
   a.blocks = c->d_blocks; a.lines = c->d_lines; a.geo = c->d_geo; a.facebc = c->d_facebc;
   a.segs = c->d_segs; a.bcfaces = c->d_bcfaces; a.grows = c->d_grows; a.gents = c->d_gents;
   a.npool = c->npool; a.nwork = c->nwork; a.nslots = std::max<int64_t>(c->lay.nbcslots, 1);
@@ -663,9 +664,8 @@ static AdjArgs adj_args(mose_ctx* c) {
 // Pbar = (dR/dP)^T Rbar on the current state (the ghost layers are filled first).
 static void enqueue_vjp(mose_ctx* c) {
   enqueue_ghost(c, c->U[0]);
-  const int64_t nW = (int64_t)30 * c->npool, nB = 10 * std::max<int64_t>(c->lay.nbcslots, 1);
-  hipLaunchKernelGGL(k_fill, dim3(launch_blocks(nW)), dim3(256), 0, c->stream, c->d_W, nW, 0.0);
-  hipLaunchKernelGGL(k_fill, dim3(launch_blocks(nB)), dim3(256), 0, c->stream, c->d_BB, nB, 0.0);
+  hipLaunchKernelGGL(k_adj_prim, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, c->U[0], c->d_Pp, c->npool,
+                     c->cfg.gamma);
   AdjArgs a = adj_args(c);
   const int lim = c->cfg.limiter, rs = c->cfg.riemann, geo = c->lay.curvi ? 1 : 0;
   void* args[] = {&a};
@@ -1055,7 +1055,7 @@ void mose_destroy(mose_ctx* c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_geo, c->d_tiles,
                   c->d_ghost, c->d_bcfaces, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
-                  c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_segs, c->d_grows, c->d_gents};
+                  c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

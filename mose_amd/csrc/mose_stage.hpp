@@ -5,16 +5,17 @@
 // of one block and marches along k over a chunk [k0, k1].  Per k-plane:
 //   * the plane's primitive state with a 2-cell i/j halo is staged in LDS; the k-direction
 //     neighbours of each column live in registers, prefetched one plane ahead;
-//   * for each direction in turn (i, j, k): every cell computes its limited slope once,
-//     sigma = phi(s+, s-), and its two face states P -/+ sigma dl/2 (the reference's per-face
-//     reconstruction, Lib_Reconstruction.f90:24-28, evaluated once per cell: the right state
-//     of face i-1/2 and the left state of face i+1/2 come from the same limiter call); the
-//     face states go to LDS, one Riemann solve per face follows, and each cell adds its two
-//     face fluxes (FP64);
+//   * every cell computes its limited slope once per direction, sigma = phi(s+, s-), and its
+//     two face states P -/+ sigma dl/2 (the reference's per-face reconstruction,
+//     Lib_Reconstruction.f90:24-28, evaluated once per cell: the right state of face i-1/2 and
+//     the left state of face i+1/2 come from the same limiter call);
+//   * each thread solves the Riemann problems of its own cell's low i and j faces (the "+"
+//     state of the neighbour comes through LDS) and of the k face above it (registers only);
+//     the faces on the patch's high edges are spread over the workgroup;
 //   * faces on a block boundary of non-connection type take the flux computed by
 //     k_bcface (the BC_Fluxes pass);
-//   * the k faces lag by one plane so every face is solved exactly once; the RK stage
-//     update (Lib_Newstate.f90:134-160, Lib_RK.f90:15-27) of plane k-1 runs at plane k.
+//   * each cell adds its six face fluxes in FP64 and takes the RK stage update
+//     (Lib_Newstate.f90:134-160, Lib_RK.f90:15-27).
 #pragma once
 
 namespace mose {
@@ -74,25 +75,70 @@ __global__ void __launch_bounds__(256) k_bcface(StageArgs a, const BcFace* __res
   }
 }
 
+// Flux of the face on the low side (direction d) of cell (fi, fj, fk) of block B (pool
+// offset off, strides sy / sz): the boundary-face buffer on non-connection block faces,
+// else the Riemann problem of (L, R) — a non-physical state raises MOSE_E_INTERFACE at the
+// cell on the low side.  GEO 1: normal from the geometry arrays, flux scaled by the area.
+template <int RS, typename FT, int GEO>
+__device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const FT* L, const FT* R, FT* F,
+                                           const BlockDev* __restrict__ Bp, int bnum, int ni, int nj, int nk,
+                                           unsigned off, unsigned sy, unsigned sz, const double* __restrict__ bcbuf,
+                                           const double* __restrict__ geo, unsigned N, int* err,
+                                           const GasT<FT>& g) {
+  int idx, nd, t1, t2, n1;
+  if (d == 0) { idx = fi; nd = ni; t1 = fj; t2 = fk; n1 = nj; }
+  else if (d == 1) { idx = fj; nd = nj; t1 = fk; t2 = fi; n1 = nk; }
+  else { idx = fk; nd = nk; t1 = fi; t2 = fj; n1 = ni; }
+  const int bf = idx == 1 ? 2 * d : (idx == nd + 1 ? 2 * d + 1 : -1);
+  const unsigned pf = off + (unsigned)(fi + 1) + sy * (unsigned)(fj + 1) + sz * (unsigned)(fk + 1);
+  if (bf >= 0 && Bp->bct[bf] != F_CONN) {
+    // tangential position on the face, cyclic order (d+1, d+2)
+    const unsigned s = (unsigned)Bp->bcs[bf] + (unsigned)(t1 - 1) + (unsigned)n1 * (unsigned)(t2 - 1);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F[v] = (FT)bcbuf[5 * s + v];
+  } else {
+    if (!(gas_ok(L) && gas_ok(R)))
+      set_error(err, MOSE_E_INTERFACE, bnum, d == 0 ? fi - 1 : fi, d == 1 ? fj - 1 : fj, d == 2 ? fk - 1 : fk);
+    FT nx = d == 0 ? FT(1) : FT(0), ny = d == 1 ? FT(1) : FT(0), nz = d == 2 ? FT(1) : FT(0);
+    if constexpr (GEO == 1) {
+      nx = FT(geo[(size_t)(GN + 4 * d) * N + pf]);
+      ny = FT(geo[(size_t)(GN + 4 * d + 1) * N + pf]);
+      nz = FT(geo[(size_t)(GN + 4 * d + 2) * N + pf]);
+    }
+    riemann<RS>(L, R, nx, ny, nz, g, F);
+  }
+  if constexpr (GEO == 1) {
+    const FT area = FT(geo[(size_t)(GN + 4 * d + 3) * N + pf]);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F[v] = F[v] * area;
+  }
+}
+
 // MODE 0: stage update; 1: last stage (update + residual-norm partials); 2: residual only.
 // GEO 0: rectilinear blocks (geometry lines); 1: general metrics from the geometry arrays,
-// with face fluxes scaled by their areas in the face loop.
+// with face fluxes scaled by their areas where they are computed.
+//
+// Face ownership: every thread solves the low i face and the low j face of its own cell
+// and the k face above it; the TX + TY faces on the high edges of the patch are solved by
+// TX + TY threads whose choice rotates with the plane so no wave always carries them.  The
+// k faces need only the thread's own registers.
 template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc, int GEO>
 __global__ void __launch_bounds__(TXc* TYc)
     __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 && TXc * TYc == 256 ? 3 : 1)))
     k_stage(StageArgs a) {
   constexpr int NTc = TXc * TYc;
-  constexpr int NI = (TXc + 1) * TYc;            // i faces of a plane patch
-  constexpr int NJ = TXc * (TYc + 1);            // j faces
-  constexpr int NK = TXc * TYc;                  // k faces (one per column)
-  constexpr int KOFF = NI + NJ;
-  constexpr int NF = NI + NJ + NK;
+  constexpr int NW = NTc / 64;                   // waves per workgroup
   constexpr int NH = 4 * (TXc + TYc);            // halo cells of a plane
   constexpr int NX = 2 * (TXc + TYc);            // cells just outside the patch needing a slope
-  static_assert(NH <= NTc && NX <= NTc, "one halo / edge cell per thread at most");
+  constexpr int NE = TXc + TYc;                  // faces on the high edges of the patch
+  static_assert(NH <= NTc && NX <= NTc && NTc % 64 == 0, "one halo / edge cell per thread at most");
   __shared__ FT sP[5][TYc + 4][TXc + 4];
-  __shared__ FT sL[5][NF];
-  __shared__ FT sR[5][NF];
+  __shared__ FT sSx[5][TYc][TXc + 1];    // [y][x]: "+" i-state of cell x-1 (x = 0: the halo cell)
+  __shared__ FT sSy[5][TYc + 1][TXc];    // [y][x]: "+" j-state of cell y-1
+  __shared__ FT sMx[5][TYc];             // "-" i-state of the halo cell x = TX
+  __shared__ FT sMy[5][TXc];             // "-" j-state of the halo cell y = TY
+  __shared__ FT sFi[5][TYc][TXc + 1];    // [y][x]: flux of the low i face of cell x
+  __shared__ FT sFj[5][TYc + 1][TXc];    // [y][x]: flux of the low j face of cell y
   // residual-norm sums: LDS for the FP32-flux kernels (register-bound), registers for FP64
   // (LDS-bound); reduced in a fixed order either way
   constexpr bool NLDS = MODE == 1 && sizeof(FT) == 4;
@@ -103,8 +149,13 @@ __global__ void __launch_bounds__(TXc* TYc)
   const GasT<FT> g{FT(a.gam), FT(gm1), FT(1.0 / gm1), FT(a.R), FT(a.gam * a.R / gm1)};
   const unsigned N = (unsigned)a.npool;
   const double* __restrict__ Uin = a.Uin;
+  // kernel-argument fields used inside the lambdas, as values (a reference to the argument
+  // struct would force a private copy of it)
+  const double* __restrict__ geo = a.geo;
+  const double* __restrict__ bcbuf = a.bcbuf;
+  int* const err = a.err;
   // geometry array g at pool cell c (general metrics)
-  auto GA = [&](int g, unsigned c) -> double { return a.geo[(size_t)g * N + c]; };
+  auto GA = [geo, N](int g, unsigned c) -> double { return geo[(size_t)g * N + c]; };
 
   // the halo cell this thread stages (LDS coordinates, -2..TX+1 / -2..TY+1)
   int hx = 0, hy = 0;
@@ -128,6 +179,8 @@ __global__ void __launch_bounds__(TXc* TYc)
     const int i = T.i0 + tx, j = T.j0 + ty;
     const bool colmem = (i <= ni + 2) && (j <= nj + 2);
     const bool active = (i <= ni) && (j <= nj);
+    const bool iface = (i <= ni + 1) && (j <= nj);     // own low i face exists
+    const bool jface = (i <= ni) && (j <= nj + 1);     // own low j face exists
     const unsigned cbase = off + (unsigned)(i + 1) + (unsigned)sy_ * (unsigned)(j + 1);
     const LineEnt* Lx = a.lines + Bp->line[0];
     const LineEnt* Ly = a.lines + Bp->line[1];
@@ -139,10 +192,11 @@ __global__ void __launch_bounds__(TXc* TYc)
     const int xci = T.i0 + xlx, xcj = T.j0 + xly;
     const bool xval = tid < NX && (xd == 0 ? (xcj <= nj && xci <= ni + 1) : (xci <= ni && xcj <= nj + 1));
 
-    // column queue P(k), P(k+1), P(k+2); face state P+(k); flux of face k-1/2;
+
+    // column queue P(k), P(k+1), P(k+2); "+" k-state of cell k; flux of the k face below;
     // prefetched U(k+3) of the column and U of the next plane's halo cell
     FT Q0[5], Q1[5], Q2[5], Skp[5], Fkm[5];
-    double Un3[5], Uh[5], Racc[5], nsum[5];
+    double Un3[5], Uh[5], nsum[5];
     double dtmin_t = 1e300;     // last stage: min CFL time step of the new state
 #pragma unroll
     for (int v = 0; v < 5; ++v) { Fkm[v] = FT(0); Skp[v] = FT(0); Un3[v] = 0.0; Uh[v] = 0.0; nsum[v] = 0.0; }
@@ -192,31 +246,21 @@ __global__ void __launch_bounds__(TXc* TYc)
 #pragma unroll
         for (int v = 0; v < 5; ++v) Uh[v] = Uin[v * N + hbase + sz * (unsigned)(k + 2)];
       }
-      __syncthreads();
 
-      // ---- face states: k face k+1/2 of the own column (registers), then i and j from LDS
-      if (active) {
-        FT Sm_[5], Sp_[5];
-        if constexpr (GEO == 0) {
-          cell_states<LIM>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), Sm_, Sp_);
-        } else {
-          const unsigned c1 = cbase + sz * (unsigned)(k + 2);
-          cell_states<LIM>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)), Sm_, Sp_);
-        }
-#pragma unroll
-        for (int v = 0; v < 5; ++v) { sL[v][KOFF + tid] = Skp[v]; sR[v][KOFF + tid] = Sm_[v]; Skp[v] = Sp_[v]; }
-      }
       if (plane) {
+        __syncthreads();       // plane k staged
+
+        // ---- i and j face states: own cell, then the cells just outside the patch
+        FT Smx[5], Smy[5];
         const int npass = tid < NX ? 3 : 2;
 #pragma unroll 1
         for (int pass = 0; pass < npass; ++pass) {
           int d, lx, ly;
-          bool ok, wl = true, wr = true;
+          bool ok;
           if (pass < 2) {
-            d = pass; lx = tx; ly = ty;
-            ok = d == 0 ? (j <= nj && i <= ni + 1) : (i <= ni && j <= nj + 1);
+            d = pass; lx = tx; ly = ty; ok = d == 0 ? iface : jface;
           } else {
-            d = xd; lx = xlx; ly = xly; ok = xval; wl = xplus; wr = !xplus;
+            d = xd; lx = xlx; ly = xly; ok = xval;
           }
           if (ok) {
             const int sx = d == 0 ? 1 : 0, sy = d == 0 ? 0 : 1;
@@ -237,81 +281,103 @@ __global__ void __launch_bounds__(TXc* TYc)
               const unsigned st = d == 0 ? 1u : (unsigned)sy_;
               cell_states<LIM>(m, c, p, FT(GA(GRLO + d, pc)), FT(GA(GRLO + d, pc + st)), FT(0.5 * GA(GDL + d, pc)), Sm_, Sp_);
             }
-            // slots: i face (s, r) -> r (TX+1) + s ; j face (c, s) -> NI + s TX + c
-            const int flo = d == 0 ? ly * (TXc + 1) + lx : NI + ly * TXc + lx;
-            const int fhi = d == 0 ? flo + 1 : flo + TXc;
-            if (wl) {
+            if (pass == 0) {
 #pragma unroll
-              for (int v = 0; v < 5; ++v) sL[v][fhi] = Sp_[v];
-            }
-            if (wr) {
+              for (int v = 0; v < 5; ++v) { Smx[v] = Sm_[v]; sSx[v][ly][lx + 1] = Sp_[v]; }
+            } else if (pass == 1) {
 #pragma unroll
-              for (int v = 0; v < 5; ++v) sR[v][flo] = Sm_[v];
+              for (int v = 0; v < 5; ++v) { Smy[v] = Sm_[v]; sSy[v][ly + 1][lx] = Sp_[v]; }
+            } else if (d == 0) {
+              if (xplus) {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sSx[v][ly][0] = Sp_[v];
+              } else {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sMx[v][ly] = Sm_[v];
+              }
+            } else {
+              if (xplus) {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sSy[v][0][lx] = Sp_[v];
+              } else {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sMy[v][lx] = Sm_[v];
+              }
             }
           }
         }
-      }
-      __syncthreads();
+        __syncthreads();
 
-      // ---- one Riemann problem per face of the plane
-      for (int f = (plane ? 0 : KOFF) + tid; f < NF; f += NTc) {
-        int d, gi, gj, idx, nd;
-        bool valid;
-        if (f < NI) { d = 0; gi = T.i0 + f % (TXc + 1); gj = T.j0 + f / (TXc + 1); idx = gi; nd = ni;
-                      valid = gj <= nj && gi <= ni + 1; }
-        else if (f < KOFF) { const int h = f - NI; d = 1; gi = T.i0 + h % TXc; gj = T.j0 + h / TXc; idx = gj; nd = nj;
-                             valid = gi <= ni && gj <= nj + 1; }
-        else { const int h = f - KOFF; d = 2; gi = T.i0 + h % TXc; gj = T.j0 + h / TXc; idx = k + 1; nd = nk;
-               valid = gi <= ni && gj <= nj; }
-        if (!valid) continue;
-        const int bf = idx == 1 ? 2 * d : (idx == nd + 1 ? 2 * d + 1 : -1);
-        FT F[5];
-        FT nx = d == 0 ? FT(1) : FT(0), ny = d == 1 ? FT(1) : FT(0), nz = d == 2 ? FT(1) : FT(0), area = FT(1);
-        if constexpr (GEO == 1) {
-          const unsigned pf = off + (unsigned)(gi + 1) + (unsigned)sy_ * (unsigned)(gj + 1) +
-                              sz * (unsigned)(d == 2 ? k + 2 : k + 1);
-          nx = FT(GA(GN + 4 * d, pf)); ny = FT(GA(GN + 4 * d + 1, pf)); nz = FT(GA(GN + 4 * d + 2, pf));
-          area = FT(GA(GN + 4 * d + 3, pf));
+        // ---- one Riemann problem per i / j face: the own low faces, then the patch edges
+        const int e = (tid + 64 * (k & (NW - 1))) % NTc;     // rotating patch-edge assignment
+        const int nr = e < NE ? 3 : 2;
+#pragma unroll 1
+        for (int r = 0; r < nr; ++r) {
+          FT Lq[5], Rq[5], F[5];
+          int d, fi, fj;
+          bool ok;
+          if (r == 0) {
+            d = 0; fi = i; fj = j; ok = iface;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) { Lq[v] = sSx[v][ty][tx]; Rq[v] = Smx[v]; }
+          } else if (r == 1) {
+            d = 1; fi = i; fj = j; ok = jface;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) { Lq[v] = sSy[v][ty][tx]; Rq[v] = Smy[v]; }
+          } else if (e < TYc) {
+            d = 0; fi = T.i0 + TXc; fj = T.j0 + e; ok = fi <= ni + 1 && fj <= nj;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) { Lq[v] = sSx[v][e][TXc]; Rq[v] = sMx[v][e]; }
+          } else {
+            const int x = e - TYc;
+            d = 1; fi = T.i0 + x; fj = T.j0 + TYc; ok = fi <= ni && fj <= nj + 1;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) { Lq[v] = sSy[v][TYc][x]; Rq[v] = sMy[v][x]; }
+          }
+          if (!ok) continue;
+          stage_face<RS, FT, GEO>(d, fi, fj, k, Lq, Rq, F, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g);
+          if (r == 0) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sFi[v][ty][tx] = F[v];
+          } else if (r == 1) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sFj[v][ty][tx] = F[v];
+          } else if (d == 0) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sFi[v][e][TXc] = F[v];
+          } else {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sFj[v][TYc][e - TYc] = F[v];
+          }
         }
-        if (bf >= 0 && Bp->bct[bf] != F_CONN) {
-          // tangential position on the face, cyclic order (d+1, d+2)
-          const int kk = d == 2 ? k + 1 : k;
-          const int t1 = d == 0 ? gj : (d == 1 ? kk : gi);
-          const int t2 = d == 0 ? kk : (d == 1 ? gi : gj);
-          const int n1 = d == 0 ? nj : (d == 1 ? nk : ni);
-          const unsigned s = (unsigned)Bp->bcs[bf] + (unsigned)(t1 - 1) + (unsigned)n1 * (unsigned)(t2 - 1);
-#pragma unroll
-          for (int v = 0; v < 5; ++v) F[v] = (FT)a.bcbuf[5 * s + v];
-        } else {
-          FT Lq[5], Rq[5];
-#pragma unroll
-          for (int v = 0; v < 5; ++v) { Lq[v] = sL[v][f]; Rq[v] = sR[v][f]; }
-          if (!(gas_ok(Lq) && gas_ok(Rq)))
-            set_error(a.err, MOSE_E_INTERFACE, T.b + 1, d == 0 ? gi - 1 : gi, d == 1 ? gj - 1 : gj, k);
-          riemann<RS>(Lq, Rq, nx, ny, nz, g, F);
-        }
-        if constexpr (GEO == 1) {
-#pragma unroll
-          for (int v = 0; v < 5; ++v) F[v] = F[v] * area;
-        }
-#pragma unroll
-        for (int v = 0; v < 5; ++v) sL[v][f] = F[v];
-      }
-      __syncthreads();
+        __syncthreads();
 
-      // ---- residual of the own cell and the RK update of plane k
-      if (active) {
-        const double Az = GEO == 0 ? Lx[i + 1].delta * Ly[j + 1].delta : 1.0;
+        // ---- k face k+1/2 of the own column, entirely in registers
         FT Fk[5];
+        if (active) {
+          FT Sm_[5], Sp_[5];
+          if constexpr (GEO == 0) {
+            cell_states<LIM>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), Sm_, Sp_);
+          } else {
+            const unsigned c1 = cbase + sz * (unsigned)(k + 2);
+            cell_states<LIM>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)), Sm_, Sp_);
+          }
+          stage_face<RS, FT, GEO>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g);
 #pragma unroll
-        for (int v = 0; v < 5; ++v) { Fk[v] = sL[v][KOFF + tid]; Racc[v] = Az * ((double)Fk[v] - (double)Fkm[v]); Fkm[v] = Fk[v]; }
-        if (plane) {
-          const int fi = ty * (TXc + 1) + tx, fj = NI + ty * TXc + tx;
+          for (int v = 0; v < 5; ++v) Skp[v] = Sp_[v];
+        }
+
+        // ---- residual of the own cell and the RK update of plane k
+        if (active) {
+          const double Az = GEO == 0 ? Lx[i + 1].delta * Ly[j + 1].delta : 1.0;
           const double Ax = GEO == 0 ? Ly[j + 1].delta * Lz[k + 1].delta : 1.0;
           const double Ay = GEO == 0 ? Lx[i + 1].delta * Lz[k + 1].delta : 1.0;
+          double Racc[5];
 #pragma unroll
           for (int v = 0; v < 5; ++v)
-            Racc[v] += Ax * ((double)sL[v][fi + 1] - (double)sL[v][fi]) + Ay * ((double)sL[v][fj + TXc] - (double)sL[v][fj]);
+            Racc[v] = Az * ((double)Fk[v] - (double)Fkm[v]) +
+                      Ax * ((double)sFi[v][ty][tx + 1] - (double)sFi[v][ty][tx]) +
+                      Ay * ((double)sFj[v][ty + 1][tx] - (double)sFj[v][ty][tx]);
           const unsigned c = cbase + sz * (unsigned)(k + 1);
           if constexpr (MODE == 2) {
 #pragma unroll
@@ -362,6 +428,28 @@ __global__ void __launch_bounds__(TXc* TYc)
 #pragma unroll
             for (int v = 0; v < 5; ++v) a.Uout[v * N + c] = unew[v];
           }
+#pragma unroll
+          for (int v = 0; v < 5; ++v) Fkm[v] = Fk[v];
+        }
+      }
+      if (!plane) {      // k = k0-1: only the face k0-1/2
+        // ---- k face k+1/2 of the own column, entirely in registers
+        FT Fk[5];
+        if (active) {
+          FT Sm_[5], Sp_[5];
+          if constexpr (GEO == 0) {
+            cell_states<LIM>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), Sm_, Sp_);
+          } else {
+            const unsigned c1 = cbase + sz * (unsigned)(k + 2);
+            cell_states<LIM>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)), Sm_, Sp_);
+          }
+          stage_face<RS, FT, GEO>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) Skp[v] = Sp_[v];
+        }
+        if (active) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) Fkm[v] = Fk[v];
         }
       }
       // advance the column queue
@@ -387,7 +475,7 @@ __global__ void __launch_bounds__(TXc* TYc)
         if (tid < 5) tot = sN[tid][0];
       } else {
         const int lane = tid & 63, wave = tid >> 6;
-        double* sW = reinterpret_cast<double*>(&sL[0][0]);     // face states are dead here
+        double* sW = reinterpret_cast<double*>(&sSx[0][0][0]);     // face states are dead here
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
           double x = nsum[v];
@@ -396,7 +484,7 @@ __global__ void __launch_bounds__(TXc* TYc)
         }
         __syncthreads();
         if (tid < 5)
-          for (int w = 0; w < (NTc + 63) / 64; ++w) tot += sW[5 * w + tid];
+          for (int w = 0; w < NW; ++w) tot += sW[5 * w + tid];
       }
       if (tid < 5) a.partials[5 * (int64_t)t + tid] = tot;
       for (int o = 32; o > 0; o >>= 1) dtmin_t = fmin(dtmin_t, __shfl_xor(dtmin_t, o, 64));

@@ -43,6 +43,8 @@ typedef struct {
   double *fn[3], *fa[3];        /* dir(d)%f(i,j,k)%n / %a, face between cell i and i+1 ... */
   double *dl, *M, *vol;         /* (3)/(9)/(1) per padded cell                              */
   double* dt;                   /* dtlocal per padded cell                                  */
+  int visc;                     /* laminar viscous terms on                                 */
+  double mu_ref, T_ref, S_suth, Pr, vnn;
 } OBlk;
 
 struct ob_domain {
@@ -302,6 +304,69 @@ static int convective_flux(const ob_domain* d, const double dl[4], const double*
   return 0;
 }
 
+/* --------------------------------- viscous terms -------------------------------------
+ * Laminar viscosity: Sutherland's law (the configuration's transport model; the reference
+ * takes mu and k from FLINT transport tables, absent here) and k = mu cp / Pr. */
+static double sutherland(const OBlk* B, double T) {
+  return B->mu_ref * pow(T / B->T_ref, 1.5) * (B->T_ref + B->S_suth) / (T + B->S_suth);
+}
+/* viscous variables of a primitive state: u, v, w, T (the single species' mass fraction is
+ * 1 everywhere, so its gradient vanishes) */
+static void visc_vars(const ob_domain* d, const double* P, double* V) {
+  V[0] = P[1]; V[1] = P[2]; V[2] = P[3]; V[3] = P[4] / (P[0] * d->Rg);
+}
+/* Compute_Diffusive_Flux, src/lib/numerics/fluxes/Lib_Diffusive.f90:90-174 (laminar, one
+ * species), with Stress_Vector_Std (src/lib/physics/turbulence/Lib_Fluid.f90:23-46).
+ * Gp[var][x]: physical gradient of (u, v, w, T). */
+static void diffusive_flux_face(const ob_domain* d, const OBlk* B, const double* Prim, double Gp[4][3], double area,
+                                const double* n, double* Flux) {
+  double rho = Prim[0];
+  double T = Prim[4] / (rho * d->Rg);
+  double mil = sutherland(B, T);
+  double kappa = mil * d->cp / B->Pr;
+  double lam = -2.0 / 3.0 * mil;
+  double div = Gp[0][0] + Gp[1][1] + Gp[2][2];
+  double ld = lam * div;
+  double txx = ld + 2.0 * mil * Gp[0][0], tyy = ld + 2.0 * mil * Gp[1][1], tzz = ld + 2.0 * mil * Gp[2][2];
+  double txy = mil * (Gp[0][1] + Gp[1][0]), txz = mil * (Gp[0][2] + Gp[2][0]), tyz = mil * (Gp[1][2] + Gp[2][1]);
+  double st[3];
+  st[0] = txx * n[0] + txy * n[1] + txz * n[2];
+  st[1] = txy * n[0] + tyy * n[1] + tyz * n[2];
+  st[2] = txz * n[0] + tyz * n[1] + tzz * n[2];
+  Flux[0] = 0.0;
+  Flux[1] = area * st[0];
+  Flux[2] = area * st[1];
+  Flux[3] = area * st[2];
+  Flux[4] = area * ((st[0] * Prim[1] + st[1] * Prim[2] + st[2] * Prim[3]) +
+                    kappa * (Gp[3][0] * n[0] + Gp[3][1] * n[1] + Gp[3][2] * n[2]) + 0.0);
+}
+/* Gradient (index space, G[var][a]) times the averaged metric tensor: Gp = G M */
+static void grad_to_phys(double G[4][3], const double* M1, const double* M2, double Gp[4][3]) {
+  double M[9];
+  for (int q = 0; q < 9; ++q) M[q] = 0.5 * (M1[q] + M2[q]);
+  for (int v = 0; v < 4; ++v)
+    for (int x = 0; x < 3; ++x) Gp[v][x] = G[v][0] * M[0 + x] + G[v][1] * M[3 + x] + G[v][2] * M[6 + x];
+}
+/* Diffusive_Flux, Lib_Diffusive.f90:10-54 with Tangential_Gradient :57-87: cells 1, 2 across
+ * the face (normal direction a), tangential pairs 3-6 (direction b) and 7-10 (direction c). */
+static void diffusive_flux(const ob_domain* d, const OBlk* B, const double* normal, double area, const double* P[10],
+                           const double* M1, const double* M2, double* Res1, double* Res2, int a, int b, int c) {
+  double V[10][4];
+  for (int q = 0; q < 10; ++q) visc_vars(d, P[q], V[q]);
+  double G[4][3];
+  for (int v = 0; v < 4; ++v) {
+    G[v][a] = V[1][v] - V[0][v];
+    G[v][b] = (V[3][v] - V[2][v] + V[5][v] - V[4][v]) * 0.25;
+    G[v][c] = (V[7][v] - V[6][v] + V[9][v] - V[8][v]) * 0.25;
+  }
+  double Gp[4][3];
+  grad_to_phys(G, M1, M2, Gp);
+  double Prim[NV], Flux[NV];
+  for (int v = 0; v < NV; ++v) Prim[v] = 0.5 * (P[0][v] + P[1][v]);
+  diffusive_flux_face(d, B, Prim, Gp, area, normal, Flux);
+  for (int v = 0; v < NV; ++v) { Res1[v] -= Flux[v]; Res2[v] += Flux[v]; }
+}
+
 /* ---------------------------------- Fluxes_blk -------------------------------------
  * src/lib/numerics/fluxes/Mod_Fluxes.f90:46-209 (inviscid, no shock detector) */
 static int fluxes_blk(ob_domain* d, int b) {
@@ -321,6 +386,20 @@ static int fluxes_blk(ob_domain* d, int b) {
                                 &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i + 1, j, k)]);
         if (e && !err) { err = e; set_err(d, e, b + 1, i, j, k); }
       }
+  if (B->visc)   /* :101-127 */
+    for (int k = 1; k <= nk; ++k)
+      for (int j = 1; j <= nj; ++j)
+        for (int i = 1; i <= ni - 1; ++i) {
+          int64_t f = cidx(B, i, j, k);
+          const double* P[10] = {&B->P[NV * cidx(B, i, j, k)],     &B->P[NV * cidx(B, i + 1, j, k)],
+                                 &B->P[NV * cidx(B, i, j - 1, k)], &B->P[NV * cidx(B, i, j + 1, k)],
+                                 &B->P[NV * cidx(B, i + 1, j - 1, k)], &B->P[NV * cidx(B, i + 1, j + 1, k)],
+                                 &B->P[NV * cidx(B, i, j, k - 1)], &B->P[NV * cidx(B, i, j, k + 1)],
+                                 &B->P[NV * cidx(B, i + 1, j, k - 1)], &B->P[NV * cidx(B, i + 1, j, k + 1)]};
+          diffusive_flux(d, B, &B->fn[0][3 * f], B->fa[0][f], P, &B->M[9 * cidx(B, i, j, k)],
+                         &B->M[9 * cidx(B, i + 1, j, k)], &B->R[NV * cidx(B, i, j, k)],
+                         &B->R[NV * cidx(B, i + 1, j, k)], 0, 1, 2);
+        }
   /* j sweep, :129-139 */
   for (int k = 1; k <= nk; ++k)
     for (int i = 1; i <= ni; ++i)
@@ -334,6 +413,20 @@ static int fluxes_blk(ob_domain* d, int b) {
                                 &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i, j + 1, k)]);
         if (e && !err) { err = e; set_err(d, e, b + 1, i, j, k); }
       }
+  if (B->visc)   /* :141-167 */
+    for (int k = 1; k <= nk; ++k)
+      for (int i = 1; i <= ni; ++i)
+        for (int j = 1; j <= nj - 1; ++j) {
+          int64_t f = cidx(B, i, j, k);
+          const double* P[10] = {&B->P[NV * cidx(B, i, j, k)],     &B->P[NV * cidx(B, i, j + 1, k)],
+                                 &B->P[NV * cidx(B, i - 1, j, k)], &B->P[NV * cidx(B, i + 1, j, k)],
+                                 &B->P[NV * cidx(B, i - 1, j + 1, k)], &B->P[NV * cidx(B, i + 1, j + 1, k)],
+                                 &B->P[NV * cidx(B, i, j, k - 1)], &B->P[NV * cidx(B, i, j, k + 1)],
+                                 &B->P[NV * cidx(B, i, j + 1, k - 1)], &B->P[NV * cidx(B, i, j + 1, k + 1)]};
+          diffusive_flux(d, B, &B->fn[1][3 * f], B->fa[1][f], P, &B->M[9 * cidx(B, i, j, k)],
+                         &B->M[9 * cidx(B, i, j + 1, k)], &B->R[NV * cidx(B, i, j, k)],
+                         &B->R[NV * cidx(B, i, j + 1, k)], 1, 0, 2);
+        }
   /* k sweep, :169-179 */
   for (int j = 1; j <= nj; ++j)
     for (int i = 1; i <= ni; ++i)
@@ -347,6 +440,20 @@ static int fluxes_blk(ob_domain* d, int b) {
                                 &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i, j, k + 1)]);
         if (e && !err) { err = e; set_err(d, e, b + 1, i, j, k); }
       }
+  if (B->visc)   /* :181-207 */
+    for (int j = 1; j <= nj; ++j)
+      for (int i = 1; i <= ni; ++i)
+        for (int k = 1; k <= nk - 1; ++k) {
+          int64_t f = cidx(B, i, j, k);
+          const double* P[10] = {&B->P[NV * cidx(B, i, j, k)],     &B->P[NV * cidx(B, i, j, k + 1)],
+                                 &B->P[NV * cidx(B, i - 1, j, k)], &B->P[NV * cidx(B, i + 1, j, k)],
+                                 &B->P[NV * cidx(B, i - 1, j, k + 1)], &B->P[NV * cidx(B, i + 1, j, k + 1)],
+                                 &B->P[NV * cidx(B, i, j - 1, k)], &B->P[NV * cidx(B, i, j + 1, k)],
+                                 &B->P[NV * cidx(B, i, j - 1, k + 1)], &B->P[NV * cidx(B, i, j + 1, k + 1)]};
+          diffusive_flux(d, B, &B->fn[2][3 * f], B->fa[2][f], P, &B->M[9 * cidx(B, i, j, k)],
+                         &B->M[9 * cidx(B, i, j, k + 1)], &B->R[NV * cidx(B, i, j, k)],
+                         &B->R[NV * cidx(B, i, j, k + 1)], 2, 0, 1);
+        }
   return err;
 }
 
@@ -403,6 +510,60 @@ static int bc_connection(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm) 
   double* R = &B->R[NV * cidx(B, Im, Jm, Km)];
   for (int v = 0; v < NV; ++v) R[v] += Flux[v] * modfm2;
   return 0;
+}
+
+/* BC_Connection_Visc, Lib_BC_Fluxes_Connection.f90:115-228 with Pg(:,3:6) from
+ * Fill_BC_Ghost_Connection (Lib_Ghost.f90:405-460): gradients at the boundary cell and at the
+ * ghost (donor) cell, each from central tangential differences, fluxes averaged. */
+static void bc_connection_visc(ob_domain* d, OBlk* B, const mose_bc_entry* e) {
+  int Im = e->i, Jm = e->j, Km = e->k, Fm = e->f;
+  int modfm2 = 1 - 2 * (Fm % 2);
+  int dir, fi, fj, fk;
+  face_index(Fm, Im, Jm, Km, &dir, &fi, &fj, &fk);
+  const int* g = guide[Fm - 1];
+  int Ig = Im - g[0], Jg = Jm - g[1], Kg = Km - g[2];
+  int64_t f = cidx(B, fi, fj, fk);
+  const double* normal = &B->fn[dir][3 * f];
+  double area = B->fa[dir][f];
+  const double* Ploc = &B->P[NV * cidx(B, Im, Jm, Km)];
+  const double* Pgh = &B->P[NV * cidx(B, Ig, Jg, Kg)];
+  /* donor tangential neighbours */
+  const OBlk* S = &d->blk[e->bs - 1];
+  int Is = e->is, Js = e->js, Ks = e->ks, q[4][3];
+  if (e->fs <= 2) {
+    int a[4][3] = {{Is, Js - e->d11, Ks - e->d12}, {Is, Js + e->d11, Ks + e->d12}, {Is, Js - e->d21, Ks - e->d22}, {Is, Js + e->d21, Ks + e->d22}};
+    memcpy(q, a, sizeof q);
+  } else if (e->fs <= 4) {
+    int a[4][3] = {{Is - e->d11, Js, Ks - e->d12}, {Is + e->d11, Js, Ks + e->d12}, {Is - e->d21, Js, Ks - e->d22}, {Is + e->d21, Js, Ks + e->d22}};
+    memcpy(q, a, sizeof q);
+  } else {
+    int a[4][3] = {{Is - e->d11, Js - e->d12, Ks}, {Is + e->d11, Js + e->d12, Ks}, {Is - e->d21, Js - e->d22, Ks}, {Is + e->d21, Js + e->d22, Ks}};
+    memcpy(q, a, sizeof q);
+  }
+  double Vloc[4], Vgh[4], Vn[6][4], Vg[4][4];
+  visc_vars(d, Ploc, Vloc);
+  visc_vars(d, Pgh, Vgh);
+  const int nb6[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  for (int m = 0; m < 6; ++m) visc_vars(d, &B->P[NV * cidx(B, Im + nb6[m][0], Jm + nb6[m][1], Km + nb6[m][2])], Vn[m]);
+  for (int m = 0; m < 4; ++m) visc_vars(d, &S->P[NV * cidx(S, q[m][0], q[m][1], q[m][2])], Vg[m]);
+  double Gl[4][3], Gg[4][3];
+  for (int v = 0; v < 4; ++v) {
+    double gn = (Vgh[v] - Vloc[v]) * modfm2, gn2 = (Vloc[v] - Vgh[v]) * (-modfm2);
+    double ti = (Vn[0][v] - Vn[1][v]) * 0.5, tj = (Vn[2][v] - Vn[3][v]) * 0.5, tk = (Vn[4][v] - Vn[5][v]) * 0.5;
+    double g1 = (Vg[1][v] - Vg[0][v]) * 0.5, g2 = (Vg[3][v] - Vg[2][v]) * 0.5;
+    if (dir == 0) { Gl[v][0] = gn; Gl[v][1] = tj; Gl[v][2] = tk; Gg[v][0] = gn2; Gg[v][1] = g1; Gg[v][2] = g2; }
+    else if (dir == 1) { Gl[v][0] = ti; Gl[v][1] = gn; Gl[v][2] = tk; Gg[v][0] = g1; Gg[v][1] = gn2; Gg[v][2] = g2; }
+    else { Gl[v][0] = ti; Gl[v][1] = tj; Gl[v][2] = gn; Gg[v][0] = g1; Gg[v][1] = g2; Gg[v][2] = gn2; }
+  }
+  double Gpl[4][3], Gpg[4][3];
+  grad_to_phys(Gl, &B->M[9 * cidx(B, Im, Jm, Km)], &B->M[9 * cidx(B, Ig, Jg, Kg)], Gpl);
+  grad_to_phys(Gg, &B->M[9 * cidx(B, Im, Jm, Km)], &B->M[9 * cidx(B, Ig, Jg, Kg)], Gpg);
+  double Prim[NV], Fl[NV], Fg[NV];
+  for (int v = 0; v < NV; ++v) Prim[v] = (Ploc[v] + Pgh[v]) * 0.5;
+  diffusive_flux_face(d, B, Prim, Gpl, area, normal, Fl);
+  diffusive_flux_face(d, B, Prim, Gpg, area, normal, Fg);
+  double* R = &B->R[NV * cidx(B, Im, Jm, Km)];
+  for (int v = 0; v < NV; ++v) R[v] = R[v] - 0.5 * modfm2 * (Fl[v] + Fg[v]);
 }
 
 /* BC_Extrapolation, Lib_BC_Fluxes_Extrapolation.f90:13-47 */
@@ -537,10 +698,20 @@ static int bc_fluxes(ob_domain* d) {
     const mose_bc_entry* e = &d->bc[q];
     OBlk* B = &d->blk[e->b - 1];
     switch (e->type) {
-      case 101: case 201: bc_connection(d, B, e->i, e->j, e->k, e->f); break;
-      case 300: bc_symmetry(d, B, e->i, e->j, e->k, e->f); break;
-      case 400: bc_extrapolation(d, B, e->i, e->j, e->k, e->f); break;
+      case 101: case 201:
+        bc_connection(d, B, e->i, e->j, e->k, e->f);
+        if (B->visc) bc_connection_visc(d, B, e);
+        break;
+      case 300:
+        if (B->visc) return set_err(d, MOSE_E_BC, e->b, e->i, e->j, e->k);   /* viscous walls: not restated */
+        bc_symmetry(d, B, e->i, e->j, e->k, e->f);
+        break;
+      case 400:
+        if (B->visc) return set_err(d, MOSE_E_BC, e->b, e->i, e->j, e->k);
+        bc_extrapolation(d, B, e->i, e->j, e->k, e->f);
+        break;
       case 405:
+        if (B->visc) return set_err(d, MOSE_E_BC, e->b, e->i, e->j, e->k);
         if (bc_inflow405(d, B, e->i, e->j, e->k, e->f, e)) bc_symmetry(d, B, e->i, e->j, e->k, e->f);
         break;
       case 0: break;
@@ -841,6 +1012,13 @@ int ob_compute_dt(ob_domain* d, int32_t iter, double* dtglobal_out) {
             double dt = B->dl[3 * c + dd] / lambda * d->cfg.cfl;
             dtmin = fmin(dt, dtmin);
           }
+          if (B->visc) {   /* VNN condition, Mod_dt.f90:88-103 */
+            double mi = sutherland(B, P[4] / (P[0] * d->Rg));
+            for (int dd = 0; dd < d->ndir; ++dd) {
+              double dt = (P[0] * B->dl[3 * c + dd] * B->dl[3 * c + dd] * B->vnn) / mi;
+              dtmin = fmin(dt, dtmin);
+            }
+          }
           if (iter < d->cfg.rampa_cfl_iter) dtmin = dtmin * iter / d->cfg.rampa_cfl_iter;
           B->dt[c] = dtmin;
           dtglobal = fmin(dtmin, dtglobal);
@@ -981,6 +1159,16 @@ int ob_add_block(ob_domain* d, int32_t b, const int32_t dims[3], const double* n
   B->M = (double*)calloc((size_t)(9 * B->npad), sizeof(double));
   B->vol = (double*)calloc((size_t)B->npad, sizeof(double));
   B->dt = (double*)calloc((size_t)B->npad, sizeof(double));
+  return 0;
+}
+int ob_set_viscous(ob_domain* d, int32_t b, const double params[5]) {
+  if (b < 0 || b > d->nb) return set_err(d, MOSE_E_ARG, b, 0, 0, 0);
+  for (int q = 0; q < d->nb; ++q) {
+    if (b != 0 && q != b - 1) continue;
+    OBlk* B = &d->blk[q];
+    B->visc = 1;
+    B->mu_ref = params[0]; B->T_ref = params[1]; B->S_suth = params[2]; B->Pr = params[3]; B->vnn = params[4];
+  }
   return 0;
 }
 int ob_set_bc(ob_domain* d, const mose_bc_entry* entries, int64_t n) {

@@ -22,6 +22,10 @@ ob_domain* ob_create(const mose_cfg* cfg, int32_t nblocks);
 int  ob_add_block(ob_domain* d, int32_t b, const int32_t dims[3], const double* nodes);
 int  ob_set_bc(ob_domain* d, const mose_bc_entry* entries, int64_t n);
 int  ob_setup(ob_domain* d);
+/* Laminar viscous terms (model > 0) for block b (0: every block), before ob_setup:
+ * params = {mu_ref, T_ref, S, Pr, vnn}: Sutherland mu(T) = mu_ref (T/T_ref)^1.5 (T_ref+S)/(T+S),
+ * conductivity mu cp / Pr, VNN number of the time step. */
+int  ob_set_viscous(ob_domain* d, int32_t b, const double params[5]);
 int  ob_put_state(ob_domain* d, int32_t b, const double* P);
 int  ob_get_state(ob_domain* d, int32_t b, double* P);
 int  ob_fill_ghost(ob_domain* d);

@@ -28,6 +28,7 @@ def lib():
         L.ob_add_block.argtypes = [d, C.c_int32, P(C.c_int32), P(C.c_double)]
         L.ob_set_bc.argtypes = [d, C.c_void_p, C.c_int64]
         L.ob_setup.argtypes = [d]
+        L.ob_set_viscous.argtypes = [d, C.c_int32, P(C.c_double)]
         L.ob_put_state.argtypes = [d, C.c_int32, P(C.c_double)]
         L.ob_get_state.argtypes = [d, C.c_int32, P(C.c_double)]
         L.ob_fill_ghost.argtypes = [d]
@@ -76,6 +77,9 @@ class OracleSolver:
             self._check(L.ob_add_block(self.h, q, dims, _dp(nodes)))
         bc = np.ascontiguousarray(case.bc)
         self._check(L.ob_set_bc(self.h, bc.ctypes.data, len(bc)))
+        for b, prm in case.meta.get("viscous", []):
+            prm = np.ascontiguousarray(prm, dtype=np.float64)
+            self._check(L.ob_set_viscous(self.h, b, _dp(prm)))
         self._check(L.ob_setup(self.h))
         for q, blk in enumerate(case.blocks, start=1):
             self.put_state(q, blk.P)

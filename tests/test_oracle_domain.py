@@ -204,3 +204,59 @@ def test_nonphysical_state_stops():
     with pytest.raises(OracleError) as ei:
         OracleSolver(case).step()
     assert ei.value.err.code in (abi.E_INTERFACE, abi.E_STATE)
+
+
+# ------------------------------------------------------------------------ viscous terms
+def _shear_case(n=32, mu=0.01, amp=0.01, split=False):
+    dims = (n, 4, 4)
+    nodes = cases.cartesian_nodes(dims, (0, 0, 0), (1.0, 0.25, 0.25))
+    xc = cases.cell_centres(nodes)
+    prim = np.zeros(dims[::-1] + (5,))
+    prim[..., 0] = 1.0
+    prim[..., 2] = amp * np.sin(2 * np.pi * xc[..., 0])
+    prim[..., 4] = 1.0
+    blk = cases.Block(dims, nodes, cases.padded_state(dims, prim))
+    bc = cases.bc_records([{"dims": dims, "faces": cases.periodic_faces(1)}])
+    visc = [(0, [mu, 1.4, 0.0, 0.71, 0.3])]    # T = p / (rho R) = 1.4 = T_ref: mu = mu_ref
+    inv = cases.Case("shear", abi.make_cfg(limiter="vanleer"), [blk], bc)
+    vis = cases.Case("shear", abi.make_cfg(limiter="vanleer"), [blk], bc, meta={"viscous": visc})
+    return inv, vis, prim
+
+
+def test_viscous_shear_wave_matches_second_difference():
+    """v = a sin(2 pi x), uniform rho, p, mu: the viscous y-momentum residual of every cell
+    (interior faces and the periodic connection faces alike) is -A mu (v[i+1] - 2 v[i] + v[i-1]) / dx,
+    and the continuity residual vanishes."""
+    inv, vis, prim = _shear_case()
+    Rv = interior(OracleSolver(vis).residual(1)) - interior(OracleSolver(inv).residual(1))
+    n, dx, A, mu = 32, 1.0 / 32, 0.25 * 0.25 / 16, 0.01
+    v = prim[..., 2]
+    expect = -A * mu * (np.roll(v, -1, axis=2) - 2 * v + np.roll(v, 1, axis=2)) / dx
+    assert np.max(np.abs(Rv[..., 2] - expect)) < 1e-12 * np.max(np.abs(expect))
+    assert np.max(np.abs(Rv[..., 0])) == 0.0
+    assert np.max(np.abs(Rv[..., 1])) < 1e-12 * np.max(np.abs(expect))
+
+
+def test_viscous_conservation_periodic_multiblock():
+    """Connection faces compute one flux from both sides (gradients at the boundary and ghost
+    cells averaged): the sum of the viscous residual over a periodic split domain is zero."""
+    case = perturb(cases.vortex(dims=(12, 10, 8), riemann="AUSM+"), seed=11)
+    split = cases.split_block_case(case, axis=0, parts=2)
+    split.meta["viscous"] = [(0, [0.02, 1.0, 0.4, 0.71, 0.3])]
+    o = OracleSolver(split)
+    tot = sum(interior(o.residual(b)).sum(axis=(0, 1, 2)) for b in (1, 2))
+    scale = max(np.abs(interior(o.residual(b))).max() for b in (1, 2))
+    assert np.all(np.abs(tot) < 1e-12 * scale * 12 * 10 * 8)
+
+
+def test_viscous_time_step_closed_form():
+    """Compute_dt with the VNN term: dt = min(CFL dl / (|u_d| + a), rho dl^2 vnn / mu) over d."""
+    inv, vis, prim = _shear_case(mu=0.5)
+    dt = OracleSolver(vis).compute_dt(1)
+    dx, dy = 1.0 / 32, 0.25 / 4
+    a = np.sqrt(1.4 * 1.0 / 1.0)
+    cfl = vis.cfg.cfl
+    v = prim[..., 2]
+    dt_conv = np.min(np.minimum(dx / a * cfl, dy / (np.abs(v) + a) * cfl))
+    dt_vnn = 1.0 * min(dx, dy) ** 2 * 0.3 / 0.5
+    assert abs(dt - min(dt_conv, dt_vnn)) <= 1e-14 * dt
