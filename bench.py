@@ -12,8 +12,9 @@ Tile shapes are picked by the library's autotuner before the warm-up steps.
 Also reported on the same line: the FP32-flux variant of the same workload, the L2 error
 against the advected exact vortex, the roofline of the dominant kernel (the fused
 flux + RK-stage kernel) measured with HIP events on the library's stream, the 64-block
-batched case (seeded block sizes in [16,128], FP32 flux vs FP64), and a CPU baseline
-(the C oracle, single thread, on a bounded sample of the same workload).
+batched case (seeded block sizes in [16,128], FP32 flux vs FP64), the discrete adjoint's
+cost relative to one residual, the batched viscous Taylor-Green case (8 x 192^3), and a CPU
+baseline (the C oracle, single thread, on a bounded sample of the same workload).
 
 roofline.traffic is the HBM bytes per launch of the stage kernel read from
 profiles/k_stage_vortex256_traffic.txt (two rocprofv3 --pmc runs, FETCH_SIZE and
@@ -132,6 +133,38 @@ def batched(steps, warmup):
     return out
 
 
+def taylor_green_batch(steps, warmup, n=192, nbox=8):
+    """8 periodic 192^3 Taylor-Green boxes (Mach 0.1, Re per box from {400, 800, 1600, 3200},
+    Sutherland viscosity, Pr 0.71), van Leer + AUSM+ with the laminar viscous terms, all boxes
+    in one launch per stage: FP32 flux against FP64, kinetic energy and its decay compared."""
+    from mose_amd import abi, cases
+    from mose_amd.lib import GpuSolver
+    t0 = time.perf_counter()
+    case = cases.taylor_green(n=n, nbox=nbox, precision=abi.PREC_FP32_FLUX)
+    log("[bench] tgv: case built in %.1f s, %d cells" % (time.perf_counter() - t0, case.ncells))
+    ke0 = np.array(cases.kinetic_energy([b.P for b in case.blocks]))
+    out = {"cells": case.ncells, "boxes": nbox, "steps": steps, "Re": [int(r) for r in case.meta["Re"]]}
+    ke = {}
+    for prec, name in ((abi.PREC_FP32_FLUX, "fp32_flux"), (abi.PREC_FP64, "fp64")):
+        cfg = abi.make_cfg(cases.GAMMA, cases.RGAS_NONDIM, case.cfg.cfl, "vanleer", "AUSM+", 3, True, precision=prec)
+        g = GpuSolver(case, cfg=cfg)
+        g.advance(warmup)
+        t0 = time.perf_counter()
+        t_sim, _ = g.advance(steps)
+        secs = time.perf_counter() - t0
+        ms = g.time_kernels(nrep=5)
+        ke[name] = np.array(cases.kinetic_energy([g.get_state(b + 1) for b in range(nbox)]))
+        out[name] = {"value": case.ncells * steps / secs, "ms_per_step": 1e3 * secs / steps,
+                     "stage_kernel_ms": ms[2], "sim_time": t_sim}
+        log("[bench] tgv %s: %.3f ms/step" % (name, 1e3 * secs / steps))
+        g.close()
+    out["speedup_fp32_vs_fp64"] = out["fp32_flux"]["value"] / out["fp64"]["value"]
+    out["ke_rel_diff_max"] = float(np.max(np.abs(ke["fp32_flux"] - ke["fp64"]) / ke["fp64"]))
+    d64, d32 = ke0 - ke["fp64"], ke0 - ke["fp32_flux"]
+    out["ke_decay_rel_diff_max"] = float(np.max(np.abs(d32 - d64) / np.abs(d64)))
+    return out
+
+
 def cpu_baseline(n=64, steps=20):
     """The C oracle (single thread, no fast-math) on the same workload at n^3."""
     from mose_amd import cases
@@ -160,6 +193,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-tgv", action="store_true")
+    ap.add_argument("--tgv-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; sharding blocks over several GPUs is not "
@@ -229,6 +264,8 @@ def main():
                              "note": "device ms of one mose_residual_vjp / one mose_residual, FP64"}
     if not args.no_batched:
         result["batched_64_blocks"] = batched(args.batched_steps, min(args.warmup, 2))
+    if not args.no_tgv:
+        result["tgv_8x192"] = taylor_green_batch(args.tgv_steps, 2)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)

@@ -87,6 +87,14 @@ int  mose_create(const mose_cfg* cfg, mose_ctx** out);
 int  mose_add_block(mose_ctx* ctx, int32_t b, const int32_t dims[3], const double* nodes);
 /* Read_BCfile (IO_BC.f90:136-414) — the entries in file order. */
 int  mose_set_bc(mose_ctx* ctx, const mose_bc_entry* entries, int64_t n);
+/* Laminar viscous terms (the reference's model > 0 without turbulence: Diffusive_Flux,
+ * Lib_Diffusive.f90:10-174; BC_Connection_Visc, Lib_BC_Fluxes_Connection.f90:115-228; the VNN
+ * time-step limit, Mod_dt.f90:88-103) for block b (0: every block added so far), after
+ * mose_add_block and before mose_finalize.  params = {mu_ref, T_ref, S, Pr, vnn}: Sutherland
+ * viscosity mu(T) = mu_ref (T/T_ref)^1.5 (T_ref + S)/(T + S), conductivity mu cp / Pr.  The
+ * reference takes mu from FLINT transport tables; Sutherland's law stands in for them.
+ * Viscous blocks must be rectilinear with face types 0 / 101 / 201 (aligned connections). */
+int  mose_set_viscous(mose_ctx* ctx, int32_t b, const double params[5]);
 /* Setup_Metrics (Mod_Metrics.f90:8-107) + device allocation; call once after blocks and BCs. */
 int  mose_finalize(mose_ctx* ctx);
 

@@ -373,14 +373,17 @@ def bump_channel(ni=512, nj=256, height=0.04, mach=2.0, seed=7, ncontrol=64, lim
 
 
 def taylor_green(n=192, nbox=8, mach=0.1, seed=11, limiter="vanleer", riemann="AUSM+", cfl=0.5,
-                 precision=abi.PREC_FP32_FLUX):
-    """Batch of `nbox` periodic n^3 Taylor-Green boxes over [0, 2 pi]^3 (inviscid part of
-    the configuration; each box is its own self-periodic block)."""
+                 precision=abi.PREC_FP32_FLUX, viscous=True, pr=0.71, vnn=0.3):
+    """Batch of `nbox` periodic n^3 Taylor-Green boxes over [0, 2 pi]^3 at Mach `mach`
+    (rho0 = 1, U0 = 1, L = 1, p0 = 1/(gamma M^2), T0 = p0/(rho0 R) = 1/M^2 with R = 1/gamma),
+    each box its own self-periodic block with Re drawn from {400, 800, 1600, 3200} by `seed`:
+    Sutherland viscosity mu(T0) = 1/Re with S = 110.4/273.15 T0 (air), Pr = `pr`."""
     rng = np.random.default_rng(seed)
     res = rng.choice([400, 800, 1600, 3200], size=nbox)
-    blocks, specs = [], []
+    blocks, specs, visc = [], [], []
     L = 2.0 * np.pi
     p0 = 1.0 / (GAMMA * mach * mach)
+    T0 = p0 / RGAS_NONDIM
     for q in range(nbox):
         dims = (n, n, n)
         nodes = cartesian_nodes(dims, (0.0, 0.0, 0.0), (L, L, L))
@@ -393,6 +396,20 @@ def taylor_green(n=192, nbox=8, mach=0.1, seed=11, limiter="vanleer", riemann="A
         prim[..., 4] = p0 + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
         blocks.append(Block(dims, nodes, padded_state(dims, prim)))
         specs.append({"dims": dims, "faces": periodic_faces(q + 1)})
+        visc.append((q + 1, [1.0 / float(res[q]), T0, 110.4 / 273.15 * T0, pr, vnn]))
     bc = bc_records(specs)
     cfg = abi.make_cfg(GAMMA, RGAS_NONDIM, cfl, limiter, riemann, 3, True, precision=precision)
-    return Case("tgv", cfg, blocks, bc, meta={"Re": res})
+    meta = {"Re": res, "T0": T0}
+    if viscous:
+        meta["viscous"] = visc
+    return Case("tgv", cfg, blocks, bc, meta=meta)
+
+
+def kinetic_energy(states):
+    """Mean kinetic energy rho |u|^2 / 2 over the cells of each block (uniform cells;
+    primitive blk%P-layout states)."""
+    out = []
+    for P in states:
+        Pi = P[2:-2, 2:-2, 2:-2, :]
+        out.append(float(np.mean(0.5 * Pi[..., 0] * (Pi[..., 1] ** 2 + Pi[..., 2] ** 2 + Pi[..., 3] ** 2))))
+    return out

@@ -290,6 +290,50 @@ __device__ __forceinline__ bool inflow405_flux(const T* Pb, const T* Pi, bool hi
   return true;
 }
 
+// ---------------------------------------------------------------------------- viscous
+// Laminar viscosity by Sutherland's law, mu = mu_ref (T/T_ref)^1.5 (T_ref + S)/(T + S).
+template <typename T>
+__device__ __forceinline__ T sutherland(T temp, T mu_ref, T T_ref, T S) {
+  const T t = temp * rcp_(T_ref);
+  return mu_ref * t * sqrt_(t) * (T_ref + S) * rcp_(temp + S);
+}
+
+// Compute_Diffusive_Flux (Lib_Diffusive.f90:90-174, laminar, one species) with
+// Stress_Vector_Std (Lib_Fluid.f90:23-46) on an axis-aligned face: G[v][a] are the
+// index-space gradients of (u, v, w, T) along axis a, M[a] the averaged diagonal metric
+// (1/dl), Pm = (rho, u, v, w, p) the face-averaged primitive state, vis = {mu_ref, T_ref, S,
+// cp/Pr}.  Returns the flux through the face (area-scaled); the face adds -Flux to the cell
+// on its low side and +Flux to the cell on its high side.
+template <typename T>
+__device__ __forceinline__ void visc_face_flux(int d, const T G[4][3], const T* M, const T* Pm, T area, T Rg,
+                                               const T* vis, T* F) {
+  T Gp[4][3];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) Gp[v][a] = G[v][a] * M[a];
+  const T temp = Pm[4] * rcp_(Pm[0] * Rg);
+  const T mu = sutherland(temp, vis[0], vis[1], vis[2]);
+  const T kap = mu * vis[3];
+  const T lam = T(-2.0 / 3.0) * mu;
+  const T ld = lam * (Gp[0][0] + Gp[1][1] + Gp[2][2]);
+  // stress vector tau . n with n the axis d: column d of the (symmetric) stress tensor
+  T st[3];
+  if (d == 0) {
+    st[0] = ld + T(2) * mu * Gp[0][0]; st[1] = mu * (Gp[0][1] + Gp[1][0]); st[2] = mu * (Gp[0][2] + Gp[2][0]);
+  } else if (d == 1) {
+    st[0] = mu * (Gp[0][1] + Gp[1][0]); st[1] = ld + T(2) * mu * Gp[1][1]; st[2] = mu * (Gp[1][2] + Gp[2][1]);
+  } else {
+    st[0] = mu * (Gp[0][2] + Gp[2][0]); st[1] = mu * (Gp[1][2] + Gp[2][1]); st[2] = ld + T(2) * mu * Gp[2][2];
+  }
+  const T gTn = d == 0 ? Gp[3][0] : (d == 1 ? Gp[3][1] : Gp[3][2]);
+  F[0] = T(0);
+  F[1] = area * st[0];
+  F[2] = area * st[1];
+  F[3] = area * st[2];
+  F[4] = area * ((st[0] * Pm[1] + st[1] * Pm[2] + st[2] * Pm[3]) + kap * gTn);
+}
+
 // --------------------------------------------------------------------------- thermo (FP64)
 // Conservative <-> primitive for the calorically perfect gas; always FP64 (the pressure
 // recovery p = (gamma-1)(E - |m|^2/(2 rho)) stays FP64 near vacuum).

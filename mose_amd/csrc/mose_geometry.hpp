@@ -30,6 +30,8 @@
 struct HostBlock {
   int ni = 0, nj = 0, nk = 0;
   std::vector<double> nodes;   // (3, 0:ni, 0:nj, 0:nk)
+  int visc = 0;                // laminar viscous terms: mu_ref, T_ref, S, Pr, vnn
+  double vis[5] = {0, 0, 0, 0, 0};
   const double* node(int i, int j, int k) const {
     return &nodes[3 * ((size_t)i + (size_t)(ni + 1) * ((size_t)j + (size_t)(nj + 1) * k))];
   }
@@ -47,6 +49,9 @@ struct Layout {
   int ndir = 3;
   int curvi = 0;                 // 1: general metrics in `geo` (NGEO x npool, SoA)
   std::vector<double> geo;
+  int visc = 0;                  // 1: some block has viscous terms
+  std::vector<mose::ViscFace> vfaces;
+  int64_t nvslots = 0;
 };
 
 struct GeoError {
@@ -258,8 +263,9 @@ static bool build_curvilinear(const std::vector<HostBlock>& hb, const std::vecto
   return true;
 }
 
-static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mose_bc_entry>& bc, Layout& L,
+static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mose_bc_entry>& bc, double cp, Layout& L,
                          GeoError& err) {
+  auto cp_of = [cp](const HostBlock&) { return cp; };
   using namespace mose;
   const int nb = (int)hb.size();
   if (nb < 1) return geo::set_err(err, MOSE_E_ARG, 0);
@@ -431,10 +437,25 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
     D.sy = B.ni + 4;
     D.sz = (int64_t)(B.ni + 4) * (B.nj + 4);
     for (int d = 0; d < 3; ++d) D.line[d] = lineoff[3 * b + d];
+    D.visc = B.visc;
+    for (int q = 0; q < 5; ++q) D.vis[q] = B.vis[q];
+    if (B.visc) {
+      // the viscous path: rectilinear blocks, faces 0 / 101 / 201 (the wall and inflow/outflow
+      // viscous conditions of the reference are not part of it)
+      L.visc = 1;
+      if (L.curvi) return geo::set_err(err, MOSE_E_ARG, b + 1);
+      for (int f = 0; f < 6; ++f) {
+        const int t = ftype[6 * b + f];
+        if (t != 0 && t != 101 && t != 201) return geo::set_err(err, MOSE_E_BC, b + 1);
+      }
+      // cp / Pr in place of Pr
+      D.vis[3] = cp_of(B) / B.vis[3];
+    }
     for (int f = 0; f < 6; ++f) {
       D.bct[f] = geo::face_kind(ftype[6 * b + f]);
       D.bcd[f] = -1;
       D.bcs[f] = -1;
+      D.bvs[f] = -1;
       if (ftype[6 * b + f] == 405) {
         const mose_bc_entry& e = bc[fentry[6 * b + f]];
         D.bcd[f] = (int)(L.facebc.size() / 8);
@@ -527,5 +548,53 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
     }
   }
   L.nbcslots = slot;
+
+  // ---- viscous flux of connection faces (BC_Connection_Visc), one slot per face cell
+  L.vfaces.clear();
+  int64_t vslot = 0;
+  if (L.visc) {
+    std::vector<int> seen(6 * nb, 0);
+    for (const mose_bc_entry& e : bc) {
+      if (e.type != 101 && e.type != 201) continue;
+      const int b = e.b - 1, f = e.f - 1, d = f / 2;
+      if (!hb[b].visc) continue;
+      if ((e.fs - 1) / 2 != d || e.d11 != 1 || e.d12 != 0 || e.d21 != 0 || e.d22 != 1)
+        return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);   // aligned, same-direction only
+      BlockDev& D = L.blocks[b];
+      const int n[3] = {D.ni, D.nj, D.nk};
+      const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+      if (!seen[6 * b + f]) {
+        seen[6 * b + f] = 1;
+        D.bvs[f] = (int)vslot;
+        vslot += (int64_t)n[d1] * n[d2];
+      }
+      const int c3[3] = {e.i, e.j, e.k};
+      const int* g = geo::guide[f];
+      mose::ViscFace v{};
+      v.b = b; v.d = d;
+      v.modfm2 = 1 - 2 * (e.f % 2);
+      v.slot = D.bvs[f] + (c3[d1] - 1) + n[d1] * (c3[d2] - 1);
+      v.loc = pidx(b, e.i, e.j, e.k);
+      v.gh = pidx(b, e.i - g[0], e.j - g[1], e.k - g[2]);
+      const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;    // tangential axes in increasing order
+      int q[3];
+      for (int s2 = 0; s2 < 4; ++s2) {
+        q[0] = e.i; q[1] = e.j; q[2] = e.k;
+        q[s2 < 2 ? ta : tb] += (s2 % 2) ? 1 : -1;
+        v.tn[s2] = pidx(b, q[0], q[1], q[2]);
+        q[0] = e.is; q[1] = e.js; q[2] = e.ks;                 // aligned: same axes in the donor
+        q[s2 < 2 ? ta : tb] += (s2 % 2) ? 1 : -1;
+        v.dn[s2] = pidx(e.bs - 1, q[0], q[1], q[2]);
+      }
+      const int gc[3] = {e.i - g[0], e.j - g[1], e.k - g[2]};
+      for (int a = 0; a < 3; ++a) {
+        v.ml[a] = 1.0 / LE(b, a, c3[a]).dl;
+        v.mg[a] = 1.0 / LE(b, a, gc[a]).dl;
+      }
+      v.area = LE(b, d1, c3[d1]).delta * LE(b, d2, c3[d2]).delta;
+      L.vfaces.push_back(v);
+    }
+  }
+  L.nvslots = vslot;
   return true;
 }

@@ -51,7 +51,7 @@ struct DtArgs {
   int ntiles, ndir, rampa, local_dt;
   int tx, ty;        // patch shape of the tiles
   int iter_add;      // the ramp uses *iter + iter_add
-  double cfl, gam;
+  double cfl, gam, R;
 };
 
 // CFL time step of every interior cell (Compute_Dt, Mod_dt.f90:61-85) and its global
@@ -84,6 +84,11 @@ __global__ void __launch_bounds__(256) k_dt(DtArgs a) {
         dtc = fmin(Lx[i + 1].dl / (fabs(P[1]) + snd) * a.cfl, dtc);
         if (a.ndir > 1) dtc = fmin(Ly[j + 1].dl / (fabs(P[2]) + snd) * a.cfl, dtc);
         if (a.ndir > 2) dtc = fmin(Lz[k + 1].dl / (fabs(P[3]) + snd) * a.cfl, dtc);
+        if (B.visc) {   // VNN condition (Mod_dt.f90:88-103)
+          const double mi = sutherland(P[4] / (P[0] * a.R), B.vis[0], B.vis[1], B.vis[2]);
+          const double dls[3] = {Lx[i + 1].dl, Ly[j + 1].dl, Lz[k + 1].dl};
+          for (int dd = 0; dd < a.ndir; ++dd) dtc = fmin(P[0] * dls[dd] * dls[dd] * B.vis[4] / mi, dtc);
+        }
       } else {
         const double* G = a.geo;
         const size_t np = (size_t)a.npool;
@@ -284,14 +289,16 @@ void bc_table(BcFn* t) {
 // g: the general-metrics kernels (shape 0 only).
 struct KTable {
   StageFn f[2][4][2][NSHAPE][3] = {};
-  StageFn g[2][4][2][3] = {};
+  StageFn g[2][2][4][2][3] = {};     // [geo - 1][prec][lim][rs][mode]
   BcFn bc[2][4][2] = {};
   template <int LIM, int RS>
   void all_shapes() {
     stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1]>(f[0][LIM][RS][0]);
     stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1]>(f[1][LIM][RS][0]);
-    stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 1>(g[0][LIM][RS]);
-    stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1], 1>(g[1][LIM][RS]);
+    stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 1>(g[0][0][LIM][RS]);
+    stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1], 1>(g[0][1][LIM][RS]);
+    stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 2>(g[1][0][LIM][RS]);
+    stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1], 2>(g[1][1][LIM][RS]);
   }
   KTable() {
     bc_table<LIM_MINMOD, double>(bc[0][0]);
@@ -380,6 +387,8 @@ struct mose_ctx {
   GhostOp* d_ghost = nullptr;
   BcFace* d_bcfaces = nullptr;
   double* d_bcbuf = nullptr;
+  ViscFace* d_vfaces = nullptr;
+  double* d_vbuf = nullptr;
   double* d_facebc = nullptr;
   double* d_scal = nullptr;
   double* d_partials = nullptr;
@@ -435,6 +444,7 @@ static StageArgs stage_args(mose_ctx* c, const double* Uin, const double* Un, do
   a.geo = c->d_geo;
   a.scal = c->d_scal; a.dtloc = c->d_dtloc; a.partials = c->d_partials; a.err = c->d_err;
   a.bcbuf = c->d_bcbuf;
+  a.vbuf = c->d_vbuf;
   a.dtmin = c->d_dtmin; a.dtnext = c->d_dtloc; a.iter = c->d_iter;
   a.ndir = c->ndir; a.rampa = c->cfg.rampa_cfl_iter; a.cfl = c->cfg.cfl;
   a.npool = c->npool; a.ntiles = (int)c->lay.tiles.size(); a.local_dt = c->cfg.time_accurate ? 0 : 1;
@@ -449,6 +459,14 @@ static void launch_stage(mose_ctx* c, StageFn fn, StageArgs a) {
 
 // Boundary-face fluxes of the stage input (the BC_Fluxes pass), before the stage kernel.
 static void enqueue_bcfaces(mose_ctx* c, const StageArgs& a) {
+  const int nv = (int)c->lay.vfaces.size();
+  if (nv > 0) {   // viscous flux of the connection faces
+    StageArgs av = a;
+    const ViscFace* vf = c->d_vfaces;
+    void* vargs[] = {&av, &vf, (void*)&nv};
+    const void* fn = c->cfg.precision == MOSE_PREC_FP32_FLUX ? (const void*)k_viscbc<float> : (const void*)k_viscbc<double>;
+    (void)hipLaunchKernel(fn, dim3(launch_blocks(nv)), dim3(256), vargs, 0, c->stream);
+  }
   const int n = (int)c->lay.bcfaces.size();
   if (n == 0) return;
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
@@ -461,7 +479,8 @@ static void enqueue_bcfaces(mose_ctx* c, const StageArgs& a) {
 
 static StageFn stage_fn(mose_ctx* c, int mode) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
-  if (c->lay.curvi) return ktable().g[p][c->cfg.limiter][c->cfg.riemann][mode];
+  if (c->lay.curvi) return ktable().g[0][p][c->cfg.limiter][c->cfg.riemann][mode];
+  if (c->lay.visc) return ktable().g[1][p][c->cfg.limiter][c->cfg.riemann][mode];
   return ktable().f[p][c->cfg.limiter][c->cfg.riemann][c->shape][mode];
 }
 
@@ -478,7 +497,7 @@ static DtArgs dt_args(mose_ctx* c, int iter_add) {
   da.dtmin = c->d_dtmin; da.dtloc = c->d_dtloc; da.iter = c->d_iter; da.npool = c->npool;
   da.ntiles = (int)c->lay.tiles.size(); da.ndir = c->ndir; da.rampa = c->cfg.rampa_cfl_iter;
   da.local_dt = c->cfg.time_accurate ? 0 : 1; da.cfl = c->cfg.cfl; da.gam = c->cfg.gamma;
-  da.tx = c->tx; da.ty = c->ty; da.iter_add = iter_add;
+  da.tx = c->tx; da.ty = c->ty; da.iter_add = iter_add; da.R = c->cfg.Rgas;
   return da;
 }
 
@@ -551,7 +570,7 @@ static int ensure_stage(mose_ctx* c, int64_t n) {
 
 static bool shape_compiled(const mose_ctx* c, int shape) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
-  if (c->lay.curvi) return shape == 0;
+  if (c->lay.curvi || c->lay.visc) return shape == 0;
   return shape >= 0 && shape < NSHAPE && ktable().f[p][c->cfg.limiter][c->cfg.riemann][shape][0] != nullptr;
 }
 
@@ -721,6 +740,20 @@ int mose_add_block(mose_ctx* c, int32_t b, const int32_t dims[3], const double* 
   return 0;
 }
 
+int mose_set_viscous(mose_ctx* c, int32_t b, const double params[5]) {
+  if (!c || c->finalized || !params || b < 0) return fail(c, MOSE_E_ARG, b);
+  if (!(params[0] >= 0.0) || !(params[1] > 0.0) || !(params[2] >= 0.0) || !(params[3] > 0.0) || !(params[4] > 0.0))
+    return fail(c, MOSE_E_ARG, b);
+  bool any = false;
+  for (auto& kv : c->hb) {
+    if (b != 0 && kv.first != b) continue;
+    kv.second.visc = 1;
+    for (int q = 0; q < 5; ++q) kv.second.vis[q] = params[q];
+    any = true;
+  }
+  return any ? 0 : fail(c, MOSE_E_ARG, b);
+}
+
 int mose_set_bc(mose_ctx* c, const mose_bc_entry* e, int64_t n) {
   if (!c || c->finalized || (n > 0 && !e) || n < 0) return fail(c, MOSE_E_ARG);
   c->bc.assign(e, e + n);
@@ -741,7 +774,8 @@ int mose_finalize(mose_ctx* c) {
   c->tx = kShape[prec][0][0];
   c->ty = kShape[prec][0][1];
   GeoError ge;
-  if (!build_layout(blocks, c->bc, c->lay, ge)) return fail(c, ge.code, ge.b, ge.i, ge.j, ge.k);
+  const double cp = c->cfg.gamma * c->cfg.Rgas / (c->cfg.gamma - 1.0);
+  if (!build_layout(blocks, c->bc, cp, c->lay, ge)) return fail(c, ge.code, ge.b, ge.i, ge.j, ge.k);
   if (5 * c->lay.npool >= (int64_t)1 << 31) return fail(c, MOSE_E_ARG);   // 32-bit element offsets
   c->npool = c->lay.npool;
   c->ndir = c->lay.ndir;
@@ -758,6 +792,9 @@ int mose_finalize(mose_ctx* c) {
   if (upload(c, &c->d_ghost, c->lay.ghost)) return c->err.code;
   if (upload(c, &c->d_facebc, c->lay.facebc)) return c->err.code;
   if (upload(c, &c->d_bcfaces, c->lay.bcfaces)) return c->err.code;
+  if (upload(c, &c->d_vfaces, c->lay.vfaces)) return c->err.code;
+  HIPCHK(c, hipMalloc((void**)&c->d_vbuf, 5 * std::max<int64_t>(c->lay.nvslots, 1) * sizeof(double)));
+  HIPCHK(c, hipMemset(c->d_vbuf, 0, 5 * std::max<int64_t>(c->lay.nvslots, 1) * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_bcbuf, 5 * std::max<int64_t>(c->lay.nbcslots, 1) * sizeof(double)));
   HIPCHK(c, hipMemset(c->d_bcbuf, 0, 5 * std::max<int64_t>(c->lay.nbcslots, 1) * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_scal, S_COUNT * sizeof(double)));
@@ -1054,7 +1091,7 @@ void mose_destroy(mose_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_geo, c->d_tiles,
-                  c->d_ghost, c->d_bcfaces, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
+                  c->d_ghost, c->d_bcfaces, c->d_vfaces, c->d_vbuf, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
                   c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents};
   for (void* p : bufs)
     if (p) (void)hipFree(p);

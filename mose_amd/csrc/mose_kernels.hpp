@@ -48,6 +48,20 @@ struct BlockDev {
   int bct[6];        // F_* per face
   int bcd[6];        // index of the 8-double data record (inflow), or -1
   int bcs[6];        // first slot of the face in the boundary-flux buffer, or -1
+  int bvs[6];        // first slot of a connection face in the viscous-flux buffer, or -1
+  int visc, pad_;    // laminar viscous terms on
+  double vis[5];     // mu_ref, T_ref, S (Sutherland), cp / Pr, vnn
+};
+
+// One boundary face cell of a block connection when viscous terms are on: the stencil of
+// BC_Connection_Visc (boundary cell, its ghost, the tangential neighbours of both sides).
+struct ViscFace {
+  int64_t loc, gh;     // pool index of the boundary cell and of its first ghost cell
+  int64_t tn[4];       // boundary cell's neighbours: 1st tangential -, +, 2nd tangential -, +
+  int64_t dn[4];       // the donor cell's neighbours in the same order (Pg(:,3:6))
+  int b, d, slot, modfm2;
+  double area;
+  double ml[3], mg[3]; // diagonal metric (1/dl) of the boundary and ghost cells
 };
 
 // One boundary face cell of a non-connection face (flux computed by k_bcface).
@@ -86,6 +100,7 @@ struct StageArgs {
   const double* dtloc;          // local time stepping: dt of each cell (same array as dtnext)
   double* __restrict__ partials;
   double* __restrict__ bcbuf;
+  double* __restrict__ vbuf;    // viscous flux of connection faces (per unit area), viscous path
   int* err;
   unsigned long long* dtmin;   // last stage: CFL time step of the next step (min as bits)
   double* dtnext;              // last stage, local time stepping: dtlocal of the next step

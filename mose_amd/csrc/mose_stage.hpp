@@ -75,6 +75,141 @@ __global__ void __launch_bounds__(256) k_bcface(StageArgs a, const BcFace* __res
   }
 }
 
+// Viscous flux of the connection faces (BC_Connection_Visc, Lib_BC_Fluxes_Connection.f90:
+// 115-228): gradients at the boundary cell (normal difference to the ghost, central
+// tangential differences of its neighbours) and at the ghost cell (central tangential
+// differences of the donor cell's neighbours), one face flux from each, averaged.  One thread
+// per boundary face cell; vbuf receives the face flux per unit area in the low-to-high
+// convention of the stage kernel (-(F_loc + F_ghost)/2).
+template <typename FT>
+__global__ void __launch_bounds__(256) k_viscbc(StageArgs a, const ViscFace* __restrict__ vf, int nvf) {
+  const double gm1 = a.gam - 1.0;
+  const unsigned N = (unsigned)a.npool;
+  const FT Rg = FT(a.R);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nvf; q += gridDim.x * blockDim.x) {
+    const ViscFace e = vf[q];
+    const BlockDev* Bp = a.blocks + e.b;
+    FT Pl[5], Pg[5], Vl[4], Vg[4], Vt[4][4], Vd[4][4];
+    prim_at(a.Uin, N, (unsigned)e.loc, gm1, Pl);
+    prim_at(a.Uin, N, (unsigned)e.gh, gm1, Pg);
+    auto vv = [Rg](const FT* P, FT* V) { V[0] = P[1]; V[1] = P[2]; V[2] = P[3]; V[3] = P[4] * rcp_(P[0] * Rg); };
+    vv(Pl, Vl);
+    vv(Pg, Vg);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      FT P[5];
+      prim_at(a.Uin, N, (unsigned)e.tn[m], gm1, P);
+      vv(P, Vt[m]);
+      prim_at(a.Uin, N, (unsigned)e.dn[m], gm1, P);
+      vv(P, Vd[m]);
+    }
+    const int d = e.d, ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    const FT m2 = FT(e.modfm2);
+    FT Gl[4][3], Gg[4][3];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const FT gn = (Vg[v] - Vl[v]) * m2, gn2 = (Vl[v] - Vg[v]) * (-m2);
+      const FT la = (Vt[1][v] - Vt[0][v]) * FT(0.5), lb = (Vt[3][v] - Vt[2][v]) * FT(0.5);
+      const FT ga = (Vd[1][v] - Vd[0][v]) * FT(0.5), gb = (Vd[3][v] - Vd[2][v]) * FT(0.5);
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        Gl[v][x] = x == d ? gn : (x == ta ? la : lb);
+        Gg[v][x] = x == d ? gn2 : (x == ta ? ga : gb);
+      }
+    }
+    FT M[3], Pm[5], vis[4], Fl[5], Fg[5];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) M[x] = FT(0.5 * (e.ml[x] + e.mg[x]));
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Pm[v] = (Pl[v] + Pg[v]) * FT(0.5);
+#pragma unroll
+    for (int x = 0; x < 4; ++x) vis[x] = FT(Bp->vis[x]);
+    visc_face_flux<FT>(d, Gl, M, Pm, FT(1), Rg, vis, Fl);
+    visc_face_flux<FT>(d, Gg, M, Pm, FT(1), Rg, vis, Fg);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) a.vbuf[5 * (int64_t)e.slot + v] = -0.5 * ((double)Fl[v] + (double)Fg[v]);
+  }
+}
+
+// Viscous flux of an interior face (Diffusive_Flux, Lib_Diffusive.f90:10-54, with
+// Tangential_Gradient :57-87) from the LDS ring of viscous variables (u, v, w, T) of planes
+// k-1, k, k+1 (slots s0, s1, s2; LDS coordinates shifted by one).  The face joins the cell at
+// patch position (lx, ly) of plane k and its neighbour in direction d (for d = 2 the same
+// column in plane k+1).  rho / p of the two cells in rl, pl, rr, pr; M the averaged
+// diagonal metric.  Returns -Flux per unit area (the low-to-high face flux).
+template <typename FT, int TXc, int TYc>
+__device__ __forceinline__ void visc_interior(int d, int lx, int ly, const FT (*sV)[4][TYc + 2][TXc + 2], int s0,
+                                              int s1, int s2, FT rl, FT pl, FT rr, FT pr, const FT* M, FT Rg,
+                                              const FT* vis, FT* Fv) {
+  FT G[4][3];
+  const int x = lx + 1, y = ly + 1;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    FT VL, VR, p3, p4, p5, p6, p7, p8, p9, p10;
+    if (d == 0) {
+      VL = sV[s1][v][y][x]; VR = sV[s1][v][y][x + 1];
+      p3 = sV[s1][v][y - 1][x]; p4 = sV[s1][v][y + 1][x]; p5 = sV[s1][v][y - 1][x + 1]; p6 = sV[s1][v][y + 1][x + 1];
+      p7 = sV[s0][v][y][x]; p8 = sV[s2][v][y][x]; p9 = sV[s0][v][y][x + 1]; p10 = sV[s2][v][y][x + 1];
+    } else if (d == 1) {
+      VL = sV[s1][v][y][x]; VR = sV[s1][v][y + 1][x];
+      p3 = sV[s1][v][y][x - 1]; p4 = sV[s1][v][y][x + 1]; p5 = sV[s1][v][y + 1][x - 1]; p6 = sV[s1][v][y + 1][x + 1];
+      p7 = sV[s0][v][y][x]; p8 = sV[s2][v][y][x]; p9 = sV[s0][v][y + 1][x]; p10 = sV[s2][v][y + 1][x];
+    } else {
+      VL = sV[s1][v][y][x]; VR = sV[s2][v][y][x];
+      p3 = sV[s1][v][y][x - 1]; p4 = sV[s1][v][y][x + 1]; p5 = sV[s2][v][y][x - 1]; p6 = sV[s2][v][y][x + 1];
+      p7 = sV[s1][v][y - 1][x]; p8 = sV[s1][v][y + 1][x]; p9 = sV[s2][v][y - 1][x]; p10 = sV[s2][v][y + 1][x];
+    }
+    const FT gn = VR - VL;
+    const FT gb = (p4 - p3 + p6 - p5) * FT(0.25);
+    const FT gc = (p8 - p7 + p10 - p9) * FT(0.25);
+    // (a, b, c) = (0, 1, 2), (1, 0, 2), (2, 0, 1)
+    G[v][0] = d == 0 ? gn : gb;
+    G[v][1] = d == 1 ? gn : (d == 0 ? gb : gc);
+    G[v][2] = d == 2 ? gn : gc;
+  }
+  FT Pm[5];
+  Pm[0] = (rl + rr) * FT(0.5);
+  Pm[4] = (pl + pr) * FT(0.5);
+  {
+    const int xr = d == 0 ? x + 1 : x, yr = d == 1 ? y + 1 : y, sr = d == 2 ? s2 : s1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Pm[1 + c] = (sV[s1][c][y][x] + sV[sr][c][yr][xr]) * FT(0.5);
+  }
+  FT F[5];
+  visc_face_flux<FT>(d, G, M, Pm, FT(1), Rg, vis, F);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) Fv[v] = -F[v];
+}
+
+// Viscous part of the k face k+1/2 of column (i, j): the connection-face buffer on a block
+// face, else the interior formula from the LDS ring (rho / p of the two cells from the
+// column queue Q0 = P(k), Q1 = P(k+1)).  Added to the face flux Fk (per unit area).
+template <typename FT, int TXc, int TYc>
+__device__ __forceinline__ void visc_kface(const BlockDev* __restrict__ Bp, int i, int j, int k, int ni, int nk, int tx,
+                                           int ty, const FT (*sV)[4][TYc + 2][TXc + 2], const FT* Q0, const FT* Q1,
+                                           const LineEnt* Lx, const LineEnt* Ly, const LineEnt* Lz,
+                                           const double* __restrict__ vbuf, FT Rg, const FT* vis, FT* Fk) {
+  if (k + 1 == 1 || k + 1 == nk + 1) {
+    const int bf = k + 1 == 1 ? 4 : 5;
+    if (Bp->bct[bf] == F_CONN) {
+      const unsigned sl = (unsigned)Bp->bvs[bf] + (unsigned)(i - 1) + (unsigned)ni * (unsigned)(j - 1);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) Fk[v] += (FT)vbuf[5 * sl + v];
+    }
+  } else {
+    FT M[3];
+    const double rzL = 1.0 / Lz[k + 1].dl, rzR = 1.0 / Lz[k + 2].dl;
+    M[0] = FT(1.0 / Lx[i + 1].dl);
+    M[1] = FT(1.0 / Ly[j + 1].dl);
+    M[2] = FT(0.5 * (rzL + rzR));
+    FT Fv[5];
+    visc_interior<FT, TXc, TYc>(2, tx, ty, sV, (k + 2) % 3, (k + 3) % 3, (k + 4) % 3, Q0[0], Q0[4], Q1[0], Q1[4], M, Rg,
+                                vis, Fv);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Fk[v] += Fv[v];
+  }
+}
+
 // Flux of the face on the low side (direction d) of cell (fi, fj, fk) of block B (pool
 // offset off, strides sy / sz): the boundary-face buffer on non-connection block faces,
 // else the Riemann problem of (L, R) — a non-physical state raises MOSE_E_INTERFACE at the
@@ -116,7 +251,11 @@ __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const 
 
 // MODE 0: stage update; 1: last stage (update + residual-norm partials); 2: residual only.
 // GEO 0: rectilinear blocks (geometry lines); 1: general metrics from the geometry arrays,
-// with face fluxes scaled by their areas where they are computed.
+// with face fluxes scaled by their areas where they are computed; 2: rectilinear blocks
+// with the laminar viscous terms (Fluxes_blk viscous sweeps, Mod_Fluxes.f90:101-207): the
+// viscous variables (u, v, w, T) of planes k-1, k, k+1 with a one-cell halo (corners
+// included) are kept in an LDS ring, and the connection faces take the viscous flux
+// k_viscbc computed.
 //
 // Face ownership: every thread solves the low i face and the low j face of its own cell
 // and the k face above it; the TX + TY faces on the high edges of the patch are solved by
@@ -124,7 +263,7 @@ __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const 
 // k faces need only the thread's own registers.
 template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc, int GEO>
 __global__ void __launch_bounds__(TXc* TYc)
-    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 && TXc * TYc == 256 ? 3 : 1)))
+    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 && TXc * TYc == 256 && GEO != 2 ? 3 : 1)))
     k_stage(StageArgs a) {
   constexpr int NTc = TXc * TYc;
   constexpr int NW = NTc / 64;                   // waves per workgroup
@@ -139,6 +278,9 @@ __global__ void __launch_bounds__(TXc* TYc)
   __shared__ FT sMy[5][TXc];             // "-" j-state of the halo cell y = TY
   __shared__ FT sFi[5][TYc][TXc + 1];    // [y][x]: flux of the low i face of cell x
   __shared__ FT sFj[5][TYc + 1][TXc];    // [y][x]: flux of the low j face of cell y
+  constexpr bool VIS = GEO == 2;
+  constexpr int VR = VIS ? 3 : 1, VY = VIS ? TYc + 2 : 1, VX = VIS ? TXc + 2 : 1;
+  __shared__ FT sV[VR][4][VY][VX];       // viscous variables, ring over planes k-1, k, k+1
   // residual-norm sums: LDS for the FP32-flux kernels (register-bound), registers for FP64
   // (LDS-bound); reduced in a fixed order either way
   constexpr bool NLDS = MODE == 1 && sizeof(FT) == 4;
@@ -210,7 +352,7 @@ __global__ void __launch_bounds__(TXc* TYc)
       prim_at(Uin, N, cbase + sz * (unsigned)(k0), gm1, Q0);         // P(k0-1)
       prim_at(Uin, N, cbase + sz * (unsigned)(k0 + 1), gm1, Q1);     // P(k0)
       prim_at(Uin, N, cbase + sz * (unsigned)(k0 + 2), gm1, Q2);     // P(k0+1)
-      if constexpr (GEO == 0) {
+      if constexpr (GEO != 1) {
         cell_states<LIM>(Qm, Q0, Q1, FT(Lz[k0].rl1), FT(Lz[k0 + 1].rl1), FT(Lz[k0].lp), Sm_, Skp);   // P+(k0-1)
       } else {
         const unsigned cm = cbase + sz * (unsigned)(k0);
@@ -220,6 +362,39 @@ __global__ void __launch_bounds__(TXc* TYc)
     if (hval) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) Uh[v] = Uin[v * N + hbase + sz * (unsigned)(k0 + 1)];
+    }
+    // viscous path: one-cell-halo plane kk of the viscous variables into ring slot (kk+3)%3
+    const double Rgas = a.R;
+    const FT Rg = FT(Rgas);
+    auto stage_visc = [=](int kk) {
+      if constexpr (VIS) {
+        const int slot = (kk + 3) % 3;
+        for (int q = tid; q < VX * VY; q += NTc) {
+          const int x = q % VX - 1, y = q / VX - 1;
+          const int gi = T.i0 + x, gj = T.j0 + y;
+          FT V[4] = {FT(0), FT(0), FT(0), FT(0)};
+          if (gi <= ni + 2 && gj <= nj + 2) {
+            double u[5], p[5];
+            const unsigned c = off + (unsigned)(gi + 1) + (unsigned)sy_ * (unsigned)(gj + 1) + sz * (unsigned)(kk + 1);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) u[v] = Uin[v * N + c];
+            if (u[0] != 0.0) {
+              cons2prim(u, gm1, p);
+              V[0] = (FT)p[1]; V[1] = (FT)p[2]; V[2] = (FT)p[3]; V[3] = (FT)(p[4] / (p[0] * Rgas));
+            }
+          }
+#pragma unroll
+          for (int v = 0; v < 4; ++v) sV[slot][v][y + 1][x + 1] = V[v];
+        }
+      }
+    };
+    FT vis[4] = {FT(0), FT(0), FT(0), FT(0)};
+    if constexpr (VIS) {
+      stage_visc(k0 - 1);
+      stage_visc(k0);
+#pragma unroll
+      for (int x = 0; x < 4; ++x) vis[x] = FT(Bp->vis[x]);
+      __syncthreads();
     }
 
     // k = k0-1 only produces the face k0-1/2; planes k0..k1-1 are updated
@@ -248,6 +423,7 @@ __global__ void __launch_bounds__(TXc* TYc)
       }
 
       if (plane) {
+        if constexpr (VIS) stage_visc(k + 1);
         __syncthreads();       // plane k staged
 
         // ---- i and j face states: own cell, then the cells just outside the patch
@@ -271,7 +447,7 @@ __global__ void __launch_bounds__(TXc* TYc)
               c[v] = sP[v][ly + 2][lx + 2];
               p[v] = sP[v][ly + 2 + sy][lx + 2 + sx];
             }
-            if constexpr (GEO == 0) {
+            if constexpr (GEO != 1) {
               const LineEnt* L = d == 0 ? Lx : Ly;
               const int cc = d == 0 ? T.i0 + lx : T.j0 + ly;
               cell_states<LIM>(m, c, p, FT(L[cc + 1].rl1), FT(L[cc + 2].rl1), FT(L[cc + 1].lp), Sm_, Sp_);
@@ -336,6 +512,36 @@ __global__ void __launch_bounds__(TXc* TYc)
           }
           if (!ok) continue;
           stage_face<RS, FT, GEO>(d, fi, fj, k, Lq, Rq, F, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g);
+          if constexpr (VIS) {
+            if (Bp->visc) {
+              // cell on the low side of the face, patch coordinates
+              const int lxL = fi - T.i0 - (d == 0 ? 1 : 0), lyL = fj - T.j0 - (d == 1 ? 1 : 0);
+              const int idx = d == 0 ? fi : fj, nd = d == 0 ? ni : nj;
+              if (idx == 1 || idx == nd + 1) {
+                const int bf = idx == 1 ? 2 * d : 2 * d + 1;
+                if (Bp->bct[bf] == F_CONN) {
+                  const int t1 = d == 0 ? fj : k, t2 = d == 0 ? k : fi, n1 = d == 0 ? nj : nk;
+                  const unsigned sl = (unsigned)Bp->bvs[bf] + (unsigned)(t1 - 1) + (unsigned)n1 * (unsigned)(t2 - 1);
+#pragma unroll
+                  for (int v = 0; v < 5; ++v) F[v] += (FT)a.vbuf[5 * sl + v];
+                }
+              } else {
+                const int s1 = (k + 3) % 3, s0 = (k + 2) % 3, s2 = (k + 4) % 3;
+                const FT rl = sP[0][lyL + 2][lxL + 2], pl = sP[4][lyL + 2][lxL + 2];
+                const FT rr = sP[0][lyL + 2 + (d == 1)][lxL + 2 + (d == 0)], pr = sP[4][lyL + 2 + (d == 1)][lxL + 2 + (d == 0)];
+                // averaged diagonal metric 1 / dl of the two cells
+                FT M[3];
+                const double rxL = 1.0 / Lx[fi - (d == 0) + 1].dl, rxR = 1.0 / Lx[fi + 1].dl;
+                const double ryL = 1.0 / Ly[fj - (d == 1) + 1].dl, ryR = 1.0 / Ly[fj + 1].dl;
+                const double rz = 1.0 / Lz[k + 1].dl;
+                M[0] = FT(0.5 * (rxL + rxR)); M[1] = FT(0.5 * (ryL + ryR)); M[2] = FT(rz);
+                FT Fv[5];
+                visc_interior<FT, TXc, TYc>(d, lxL, lyL, sV, s0, s1, s2, rl, pl, rr, pr, M, Rg, vis, Fv);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) F[v] += Fv[v];
+              }
+            }
+          }
           if (r == 0) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) sFi[v][ty][tx] = F[v];
@@ -356,22 +562,26 @@ __global__ void __launch_bounds__(TXc* TYc)
         FT Fk[5];
         if (active) {
           FT Sm_[5], Sp_[5];
-          if constexpr (GEO == 0) {
+          if constexpr (GEO != 1) {
             cell_states<LIM>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), Sm_, Sp_);
           } else {
             const unsigned c1 = cbase + sz * (unsigned)(k + 2);
             cell_states<LIM>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)), Sm_, Sp_);
           }
           stage_face<RS, FT, GEO>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g);
+          if constexpr (VIS) {
+            if (Bp->visc)
+              visc_kface<FT, TXc, TYc>(Bp, i, j, k, ni, nk, tx, ty, sV, Q0, Q1, Lx, Ly, Lz, a.vbuf, Rg, vis, Fk);
+          }
 #pragma unroll
           for (int v = 0; v < 5; ++v) Skp[v] = Sp_[v];
         }
 
         // ---- residual of the own cell and the RK update of plane k
         if (active) {
-          const double Az = GEO == 0 ? Lx[i + 1].delta * Ly[j + 1].delta : 1.0;
-          const double Ax = GEO == 0 ? Ly[j + 1].delta * Lz[k + 1].delta : 1.0;
-          const double Ay = GEO == 0 ? Lx[i + 1].delta * Lz[k + 1].delta : 1.0;
+          const double Az = GEO != 1 ? Lx[i + 1].delta * Ly[j + 1].delta : 1.0;
+          const double Ax = GEO != 1 ? Ly[j + 1].delta * Lz[k + 1].delta : 1.0;
+          const double Ay = GEO != 1 ? Lx[i + 1].delta * Lz[k + 1].delta : 1.0;
           double Racc[5];
 #pragma unroll
           for (int v = 0; v < 5; ++v)
@@ -387,7 +597,7 @@ __global__ void __launch_bounds__(TXc* TYc)
 #pragma unroll
             for (int v = 0; v < 5; ++v) { un[v] = a.Un[v * N + c]; ui[v] = Uin[v * N + c]; }
             const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
-            const double vol = GEO == 0 ? Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta : GA(GVOL, c);
+            const double vol = GEO != 1 ? Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta : GA(GVOL, c);
             const double rv = dt / vol;
 #pragma unroll
             for (int v = 0; v < 5; ++v) unew[v] = un[v] + a.rkc * (ui[v] - un[v] - Racc[v] * rv);
@@ -401,10 +611,15 @@ __global__ void __launch_bounds__(TXc* TYc)
               // CFL time step of the new state for the next step (Compute_dt, Mod_dt.f90:61-85)
               const double snd = sqrt(a.gam * P[4] / P[0]);
               double dtc = 1e8;
-              if constexpr (GEO == 0) {
+              if constexpr (GEO != 1) {
                 dtc = fmin(Lx[i + 1].dl / (fabs(P[1]) + snd) * a.cfl, dtc);
                 if (a.ndir > 1) dtc = fmin(Ly[j + 1].dl / (fabs(P[2]) + snd) * a.cfl, dtc);
                 if (a.ndir > 2) dtc = fmin(Lz[k + 1].dl / (fabs(P[3]) + snd) * a.cfl, dtc);
+                if (GEO == 2 && Bp->visc) {   // VNN condition (Mod_dt.f90:88-103)
+                  const double mi = sutherland(P[4] / (P[0] * a.R), Bp->vis[0], Bp->vis[1], Bp->vis[2]);
+                  const double dls[3] = {Lx[i + 1].dl, Ly[j + 1].dl, Lz[k + 1].dl};
+                  for (int dd = 0; dd < a.ndir; ++dd) dtc = fmin(P[0] * dls[dd] * dls[dd] * Bp->vis[4] / mi, dtc);
+                }
               } else {
                 for (int dd = 0; dd < a.ndir; ++dd) {
                   const double vel = fabs(P[1] * GA(GVS + 3 * dd, c) + P[2] * GA(GVS + 3 * dd + 1, c) +
@@ -437,13 +652,17 @@ __global__ void __launch_bounds__(TXc* TYc)
         FT Fk[5];
         if (active) {
           FT Sm_[5], Sp_[5];
-          if constexpr (GEO == 0) {
+          if constexpr (GEO != 1) {
             cell_states<LIM>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), Sm_, Sp_);
           } else {
             const unsigned c1 = cbase + sz * (unsigned)(k + 2);
             cell_states<LIM>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)), Sm_, Sp_);
           }
           stage_face<RS, FT, GEO>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g);
+          if constexpr (VIS) {
+            if (Bp->visc)
+              visc_kface<FT, TXc, TYc>(Bp, i, j, k, ni, nk, tx, ty, sV, Q0, Q1, Lx, Ly, Lz, a.vbuf, Rg, vis, Fk);
+          }
 #pragma unroll
           for (int v = 0; v < 5; ++v) Skp[v] = Sp_[v];
         }

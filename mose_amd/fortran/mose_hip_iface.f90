@@ -43,7 +43,7 @@ module mose_hip
     integer(c_int32_t) :: code = 0, block = 0, i = 0, j = 0, k = 0
   end type mose_err
 
-  public :: mose_create, mose_add_block, mose_set_bc, mose_finalize, mose_put_state, &
+  public :: mose_create, mose_add_block, mose_set_viscous, mose_set_bc, mose_finalize, mose_put_state, &
             mose_get_state, mose_compute_dt, mose_step, mose_advance, mose_residual, &
             mose_get_time, mose_set_time, mose_num_cells, mose_last_error, mose_destroy
 
@@ -59,6 +59,12 @@ module mose_hip
       integer(c_int32_t), value       :: b
       integer(c_int32_t), intent(in)  :: dims(3)
       real(c_double), intent(in)      :: nodes(*)
+    end function
+    integer(c_int) function mose_set_viscous(ctx, b, params) bind(C, name="mose_set_viscous")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value         :: ctx
+      integer(c_int32_t), value  :: b
+      real(c_double), intent(in) :: params(5)
     end function
     integer(c_int) function mose_set_bc(ctx, entries, n) bind(C, name="mose_set_bc")
       import :: c_int, c_ptr, c_int64_t, mose_bc_entry

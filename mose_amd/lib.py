@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("MOSE_HIP_LIB") or os.path.join(_HERE, "libmose_hip.so")
 _LIB = None
 
-EXPORTS = ["mose_create", "mose_add_block", "mose_set_bc", "mose_finalize", "mose_put_state",
+EXPORTS = ["mose_create", "mose_add_block", "mose_set_viscous", "mose_set_bc", "mose_finalize", "mose_put_state",
            "mose_get_state", "mose_compute_dt", "mose_step", "mose_advance", "mose_residual",
            "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_put_residual_bar",
            "mose_residual_vjp", "mose_get_state_bar", "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
@@ -37,6 +37,7 @@ def load():
     L.mose_create.argtypes = [P(abi.MoseCfg), P(v)]
     L.mose_add_block.argtypes = [v, C.c_int32, P(C.c_int32), P(C.c_double)]
     L.mose_set_bc.argtypes = [v, C.c_void_p, C.c_int64]
+    L.mose_set_viscous.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_finalize.argtypes = [v]
     L.mose_put_state.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_get_state.argtypes = [v, C.c_int32, P(C.c_double)]
@@ -96,6 +97,9 @@ class GpuSolver:
             self._check(L.mose_add_block(self.h, q, dims, _dp(nodes)))
         bc = np.ascontiguousarray(case.bc)
         self._check(L.mose_set_bc(self.h, bc.ctypes.data, len(bc)))
+        for b, prm in case.meta.get("viscous", []):
+            prm = np.ascontiguousarray(prm, dtype=np.float64)
+            self._check(L.mose_set_viscous(self.h, b, _dp(prm)))
         self._check(L.mose_finalize(self.h))
         for q, blk in enumerate(case.blocks, start=1):
             self.put_state(q, blk.P)
