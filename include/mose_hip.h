@@ -131,6 +131,19 @@ int  mose_residual(mose_ctx* ctx, int32_t b, double* R);
 int  mose_put_residual_bar(mose_ctx* ctx, int32_t b, const double* Rbar);
 int  mose_residual_vjp(mose_ctx* ctx);
 int  mose_get_state_bar(mose_ctx* ctx, int32_t b, double* Pbar);
+/* Geometry adjoint: mose_residual_vjp plus the adjoints of the residual with respect to the
+ * block nodes and the imposed values of supersonic inflow faces (shape and boundary-value
+ * sensitivities, the Xbar of the residual's backward).  Needs general metrics (some block
+ * not rectilinear), else MOSE_E_ARG.  After it, mose_get_state_bar as above and
+ *   mose_get_node_bar(b, Xbar): Xbar = (dR/dX)^T Rbar in the node layout (3,0:ni,0:nj,0:nk),
+ *     through every metric the residual reads (face normals and areas, cell lengths incl.
+ *     the ghost-cell lengths of each bc type, symmetry mirror normals);
+ *   mose_get_inflow_bar(b, f, out): adjoints with respect to the 405 record of face f (1..6)
+ *     of block b: out = (mach, T, p, alpha, beta).
+ * No counterpart in the reference (no adjoint). */
+int  mose_residual_vjp_geometry(mose_ctx* ctx);
+int  mose_get_node_bar(mose_ctx* ctx, int32_t b, double* Xbar);
+int  mose_get_inflow_bar(mose_ctx* ctx, int32_t b, int32_t f, double out[5]);
 /* Average device ms of one residual evaluation (ms[0]) and one adjoint evaluation (ms[1])
  * over nrep back-to-back runs on the current state (profiling). */
 int  mose_time_residual_vjp(mose_ctx* ctx, int32_t nrep, double ms[2]);

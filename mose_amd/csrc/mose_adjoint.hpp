@@ -33,11 +33,12 @@ __device__ __forceinline__ void limiter_adj(double x, double y, double gb, doubl
 }
 
 // Per-cell slopes and face states (cell_states): Sm = P0 - sigma hdl, Sp = P0 + sigma hdl,
-// sigma = phi((Pp - P0) rlp, (P0 - Pm) rlm).  Adjoints Smb / Spb -> Pmb, P0b, Ppb.
+// sigma = phi((Pp - P0) rlp, (P0 - Pm) rlm).  Adjoints Smb / Spb -> Pmb, P0b, Ppb and, when
+// gb3 is given, the spacing adjoints gb3 += (rlm_bar, rlp_bar, hdl_bar).
 template <int LIM>
 __device__ __forceinline__ void cell_states_adj(const double* Pm, const double* P0, const double* Pp, double rlm,
                                                 double rlp, double hdl, const double* Smb, const double* Spb,
-                                                double* Pmb, double* P0b, double* Ppb) {
+                                                double* Pmb, double* P0b, double* Ppb, double* gb3 = nullptr) {
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const double sm = (P0[v] - Pm[v]) * rlm;
@@ -48,12 +49,17 @@ __device__ __forceinline__ void cell_states_adj(const double* Pm, const double* 
     P0b[v] += Smb[v] + Spb[v] - spb * rlp + smb * rlm;
     Ppb[v] += spb * rlp;
     Pmb[v] -= smb * rlm;
+    if (gb3) {
+      gb3[0] += smb * (P0[v] - Pm[v]);
+      gb3[1] += spb * (Pp[v] - P0[v]);
+      gb3[2] += (Spb[v] - Smb[v]) * limiter<LIM>(sp, sm);
+    }
   }
 }
 
 // ------------------------------------------------------------------------------ HLLC
 __device__ void hllc_adj(const double* L, const double* R, double nx, double ny, double nz, const GasT<double>& g,
-                         const double* Fb, double* Lb, double* Rb) {
+                         const double* Fb, double* Lb, double* Rb, double* nb) {
   const double rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   const double rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
   // ---- forward
@@ -111,6 +117,9 @@ __device__ void hllc_adj(const double* L, const double* R, double nx, double ny,
     w_b += U4b * inv * sd;
     e0_b += U5b * inv * sd;
     const double dp_b = (U2b * nx + U3b * ny + U4b * nz) * inv;
+    nb[0] += U2b * inv * dp;
+    nb[1] += U3b * inv * dp;
+    nb[2] += U4b * inv * dp;
     p_b -= U5b * inv * un_;
     un_b -= U5b * inv * p_;
     pstar_b += U5b * inv * Sstar;
@@ -131,6 +140,9 @@ __device__ void hllc_adj(const double* L, const double* R, double nx, double ny,
     w_b += Fb[3] * Fr;
     h0_b += Fb[4] * Fr;
     p_b += Fb[1] * nx + Fb[2] * ny + Fb[3] * nz;
+    nb[0] += Fb[1] * p_;
+    nb[1] += Fb[2] * p_;
+    nb[2] += Fb[3] * p_;
     r_b += Fr_b * un_;
     un_b += Fr_b * r_;
   }
@@ -183,6 +195,9 @@ __device__ void hllc_adj(const double* L, const double* R, double nx, double ny,
   h0rb += e0rb; prb -= e0rb * irr; irrb -= e0rb * pr;
   // Roe averages
   double u_roe_b = un_roe_b * nx, v_roe_b = un_roe_b * ny, w_roe_b = un_roe_b * nz;
+  nb[0] += un_roe_b * u_roe;
+  nb[1] += un_roe_b * v_roe;
+  nb[2] += un_roe_b * w_roe;
   if (a_roe > 0.0) {
     const double tb = a_roe_b * 0.5 / a_roe * g.gm1;
     const double h0_roe_b0 = tb;
@@ -226,6 +241,9 @@ __device__ void hllc_adj(const double* L, const double* R, double nx, double ny,
   // un = u . n
   ulb += unlb * nx; vlb += unlb * ny; wlb += unlb * nz;
   urb += unrb * nx; vrb += unrb * ny; wrb += unrb * nz;
+  nb[0] += unlb * ul + unrb * ur;
+  nb[1] += unlb * vl + unrb * vr;
+  nb[2] += unlb * wl + unrb * wr;
   // a = sqrt(gam p / rho)
   {
     const double tl = alb * 0.5 / al * g.gam, tr = arb * 0.5 / ar * g.gam;
@@ -240,7 +258,7 @@ __device__ void hllc_adj(const double* L, const double* R, double nx, double ny,
 
 // ------------------------------------------------------------------------------ AUSM+
 __device__ void ausmp_adj(const double* L, const double* R, double nx, double ny, double nz, const GasT<double>& g,
-                          const double* Fb, double* Lb, double* Rb) {
+                          const double* Fb, double* Lb, double* Rb, double* nb) {
   const double beta = 0.125, alpha = 0.1875;
   const double rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   const double rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
@@ -290,6 +308,9 @@ __device__ void ausmp_adj(const double* L, const double* R, double nx, double ny
   const double mdotp_b = Fb[0] + Fb[1] * ul + Fb[2] * vl + Fb[3] * wl + Fb[4] * H_L;
   const double mdotm_b = Fb[0] + Fb[1] * ur + Fb[2] * vr + Fb[3] * wr + Fb[4] * H_R;
   const double phalf_b = Fb[1] * nx + Fb[2] * ny + Fb[3] * nz;
+  nb[0] += Fb[1] * phalf;
+  nb[1] += Fb[2] * phalf;
+  nb[2] += Fb[3] * phalf;
   double ulb = Fb[1] * mdotp, vlb = Fb[2] * mdotp, wlb = Fb[3] * mdotp, HLb = Fb[4] * mdotp;
   double urb = Fb[1] * mdotm, vrb = Fb[2] * mdotm, wrb = Fb[3] * mdotm, HRb = Fb[4] * mdotm;
   double rlb = mdotp_b * a_half * mp, rrb = mdotm_b * a_half * mm;
@@ -314,22 +335,29 @@ __device__ void ausmp_adj(const double* L, const double* R, double nx, double ny
   urb += HRb * ur; vrb += HRb * vr; wrb += HRb * wr;
   ulb += unlb * nx; vlb += unlb * ny; wlb += unlb * nz;
   urb += unrb * nx; vrb += unrb * ny; wrb += unrb * nz;
+  nb[0] += unlb * ul + unrb * ur;
+  nb[1] += unlb * vl + unrb * vr;
+  nb[2] += unlb * wl + unrb * wr;
   rlb -= irlb * irl * irl;
   rrb -= irrb * irr * irr;
   Lb[0] += rlb; Lb[1] += ulb; Lb[2] += vlb; Lb[3] += wlb; Lb[4] += plb;
   Rb[0] += rrb; Rb[1] += urb; Rb[2] += vrb; Rb[3] += wrb; Rb[4] += prb;
 }
 
+// nb (may be null) receives the adjoint of the face normal.
 template <int RS>
 __device__ __forceinline__ void riemann_adj(const double* L, const double* R, double nx, double ny, double nz,
-                                            const GasT<double>& g, const double* Fb, double* Lb, double* Rb) {
-  if constexpr (RS == RS_AUSMP) ausmp_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb);
-  else hllc_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb);
+                                            const GasT<double>& g, const double* Fb, double* Lb, double* Rb,
+                                            double* nb = nullptr) {
+  double nd[3] = {0.0, 0.0, 0.0};
+  if constexpr (RS == RS_AUSMP) ausmp_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb, nd);
+  else hllc_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb, nd);
+  if (nb) { nb[0] += nd[0]; nb[1] += nd[1]; nb[2] += nd[2]; }
 }
 
 // Physical flux of one state (BC_Extrapolation).
 __device__ __forceinline__ void physical_flux_adj(const double* P, double nx, double ny, double nz,
-                                                  const GasT<double>& g, const double* Fb, double* Pb) {
+                                                  const GasT<double>& g, const double* Fb, double* Pb, double* nb) {
   const double un = P[1] * nx + P[2] * ny + P[3] * nz;
   const double m = P[0] * un;
   const double ir = 1.0 / P[0];
@@ -343,6 +371,9 @@ __device__ __forceinline__ void physical_flux_adj(const double* P, double nx, do
   Pb[2] += Fb[2] * m + unb * ny + h0b * P[2];
   Pb[3] += Fb[3] * m + unb * nz + h0b * P[3];
   Pb[4] += Fb[1] * nx + Fb[2] * ny + Fb[3] * nz + h0b * kg * ir;
+  nb[0] += unb * P[1] + Fb[1] * P[4];
+  nb[1] += unb * P[2] + Fb[2] * P[4];
+  nb[2] += unb * P[3] + Fb[3] * P[4];
 }
 
 // Symmetry face (symmetry_flux): momentum part of the mirrored Riemann problem on the
@@ -350,7 +381,7 @@ __device__ __forceinline__ void physical_flux_adj(const double* P, double nx, do
 template <int LIM, int RS>
 __device__ void symmetry_flux_adj(const double* Pb, const double* Pi, double bdl, double idl, bool hi, double nx,
                                   double ny, double nz, const GasT<double>& g, const double* Fb_in, double* Pbb,
-                                  double* Pib) {
+                                  double* Pib, double* nb, double* bdlb, double* idlb) {
   const double un = Pb[1] * nx + Pb[2] * ny + Pb[3] * nz;
   const double Gp[5] = {Pb[0], Pb[1] - 2.0 * un * nx, Pb[2] - 2.0 * un * ny, Pb[3] - 2.0 * un * nz, Pb[4]};
   const double d12 = hi ? (bdl + idl) : 2.0 * bdl;
@@ -373,7 +404,7 @@ __device__ void symmetry_flux_adj(const double* Pb, const double* Pi, double bdl
   // reverse
   const double Fb[5] = {0.0, Fb_in[1], Fb_in[2], Fb_in[3], 0.0};
   double Lb[5] = {0, 0, 0, 0, 0}, Rb[5] = {0, 0, 0, 0, 0};
-  riemann_adj<RS>(L, R, nx, ny, nz, g, Fb, Lb, Rb);
+  riemann_adj<RS>(L, R, nx, ny, nz, g, Fb, Lb, Rb, nb);
   double Fcb[5];
   Fcb[0] = Lb[0] + Rb[0];
   Fcb[4] = Lb[4] + Rb[4];
@@ -391,15 +422,30 @@ __device__ void symmetry_flux_adj(const double* Pb, const double* Pi, double bdl
   Fcb[1] += unfb * nx;
   Fcb[2] += unfb * ny;
   Fcb[3] += unfb * nz;
+  // normal: L/R velocity terms (un1 - unf) n, -(unf + un1) n, and unf = Fc . n
+  for (int c = 0; c < 3; ++c)
+    nb[c] += (un1 - unf) * Lb[1 + c] - (unf + un1) * Rb[1 + c] + unfb * Fc[1 + c];
   double Gpb[5] = {0, 0, 0, 0, 0};
+  double distb = 0.0, rd12b = 0.0, rd23b = 0.0;
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     double xb = 0.0, yb = 0.0;
+    const double phi = limiter<LIM>(s32[v], s21[v]);
+    distb += Fcb[v] * phi;
     limiter_adj<LIM>(s32[v], s21[v], Fcb[v] * dist, xb, yb);
     Pbb[v] += Fcb[v] - xb * rd23 + yb * rd12;
     const double p3b = xb * rd23, p1b = -yb * rd12;
+    rd23b += xb * s32[v] / rd23;
+    rd12b += yb * s21[v] / rd12;
     if (hi) { Pib[v] += p1b; Gpb[v] += p3b; }
     else { Gpb[v] += p1b; Pib[v] += p3b; }
+  }
+  // spacings: dist = +/-bdl, d12 / d23 = 2 bdl or bdl + idl, rd = 1/d
+  {
+    const double d12b = -rd12b * rd12 * rd12, d23b = -rd23b * rd23 * rd23;
+    *bdlb += (hi ? 1.0 : -1.0) * distb;
+    if (hi) { *bdlb += d12b + 2.0 * d23b; *idlb += d12b; }
+    else { *bdlb += 2.0 * d12b + d23b; *idlb += d23b; }
   }
   // Gp = mirror(Pb)
   Pbb[0] += Gpb[0];
@@ -408,12 +454,16 @@ __device__ void symmetry_flux_adj(const double* Pb, const double* Pi, double bdl
   Pbb[1] += Gpb[1] - 2.0 * gdn * nx;
   Pbb[2] += Gpb[2] - 2.0 * gdn * ny;
   Pbb[3] += Gpb[3] - 2.0 * gdn * nz;
+  for (int c = 0; c < 3; ++c) nb[c] += -2.0 * (Pb[1 + c] * gdn + un * Gpb[1 + c]);
 }
 
 // Supersonic inflow face (inflow405_flux); returns false when the symmetry flux applies.
+// bcb (6) receives the adjoints of the imposed Mach number, temperature, pressure and the
+// three components of the velocity direction.
 template <int LIM, int RS>
 __device__ bool inflow405_adj(const double* Pb, const double* Pi, bool hi, double nx, double ny, double nz,
-                              const double* bc, const GasT<double>& g, const double* Fb, double* Pbb, double* Pib) {
+                              const double* bc, const GasT<double>& g, const double* Fb, double* Pbb, double* Pib,
+                              double* nb, double* bcb) {
   const double un = Pb[1] * nx + Pb[2] * ny + Pb[3] * nz;
   const double sgn = hi ? 1.0 : -1.0;
   if (sgn * un >= 0.0) return false;
@@ -441,7 +491,7 @@ __device__ bool inflow405_adj(const double* Pb, const double* Pi, bool hi, doubl
     R[v] = hi ? Sup[v] : Fc[v];
   }
   double Lb[5] = {0, 0, 0, 0, 0}, Rb[5] = {0, 0, 0, 0, 0};
-  riemann_adj<RS>(L, R, nx, ny, nz, g, Fb, Lb, Rb);
+  riemann_adj<RS>(L, R, nx, ny, nz, g, Fb, Lb, Rb, nb);
   double Supb[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
@@ -453,10 +503,20 @@ __device__ bool inflow405_adj(const double* Pb, const double* Pi, bool hi, doubl
     if (hi) { Pib[v] -= yb; Supb[v] += xb; }
     else { Supb[v] -= yb; Pib[v] += xb; }
   }
+  // Sup = (p / RT, M a dir, p) with a = sqrt(gam p / rho) = sqrt(gam R T)
+  const double Mb = a * (Supb[1] * bc[3] + Supb[2] * bc[4] + Supb[3] * bc[5]);
+  const double pb = Supb[4] + Supb[0] / RT;
+  {
+    const double ab = M * (Supb[1] * bc[3] + Supb[2] * bc[4] + Supb[3] * bc[5]);
+    bcb[0] += rel * Mb;
+    bcb[1] += -Supb[0] * rho / bc[1] + ab * 0.5 * a / bc[1];
+    bcb[2] += rel * pb;
+    bcb[3] += M * a * Supb[1];
+    bcb[4] += M * a * Supb[2];
+    bcb[5] += M * a * Supb[3];
+  }
   if (rel != 1.0) {
-    // Sup depends on the boundary cell through M (flow Mach) and p; a = sqrt(gam R T) is fixed
-    const double Mb = a * (Supb[1] * bc[3] + Supb[2] * bc[4] + Supb[3] * bc[5]);
-    const double pb = Supb[4] + Supb[0] / RT;
+    // Sup depends on the boundary cell through M (flow Mach) and p
     const double fMb = (1.0 - rel) * Mb;
     Pbb[4] += (1.0 - rel) * pb;
     if (vmag > 0.0) {
@@ -490,6 +550,12 @@ __device__ bool inflow405_adj(const double* Pb, const double* Pi, bool hi, doubl
 //                 the boundary-face adjoints that name it (deterministic, no atomics);
 //   k_adj_ghostT  one thread per cell that feeds ghost cells: adds the transpose of the
 //                 ghost-fill map (copy, extrapolation, mirror) applied to the ghost adjoints.
+// With the geometry outputs set (general metrics only), the same passes also write the
+// adjoints of every geometric quantity the residual reads: face normals and areas (GF, by
+// the face and boundary-face passes), reciprocal centre distances and cell lengths (GL, by
+// the cell pass, which gathers the boundary-face half lengths from BG), the mirror normals
+// of the symmetry ghost fill (GE) and the imposed values of inflow faces (IB).  The host
+// turns them into node adjoints by the reverse of the metric computation.
 
 struct AdjSeg {       // the low faces of one block in one direction
   int64_t start;      // first work item
@@ -524,8 +590,14 @@ struct AdjArgs {
   const BcFace* __restrict__ bcfaces;
   const AdjGhostRow* __restrict__ grows;
   const AdjGhostEnt* __restrict__ gents;
+  // geometry adjoints (null: not requested)
+  double* __restrict__ GF;              // 12 x npool: low-face normal (3) and area, per direction
+  double* __restrict__ GL;              // 6 x npool: rl(d) and dl(d) per direction
+  double* __restrict__ BG;              // 2 x nslots: boundary-face bdl, idl
+  double* __restrict__ GE;              // 3 x nents: mirror normal of the ghost-fill entries
+  double* __restrict__ IB;              // 6 x nslots: inflow M, T, p, direction (3)
   int64_t npool, nwork, nslots;
-  int nseg, nbcf, ngrows, nblocks;
+  int nseg, nbcf, ngrows, nblocks, nents, pad_;
   double gam, R;
 };
 
@@ -622,14 +694,37 @@ __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a) {
       }
       double n[3], area;
       G.face(d, c, ijk[0], ijk[1], ijk[2], n, area);
-      double Fb[5];
+      double Fb[5], dR[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         const double rl_ = blo ? 0.0 : a.Rbar[v * N + c - s];
         const double rr_ = bhi ? 0.0 : a.Rbar[v * N + c];
-        Fb[v] = area * (rl_ - rr_);
+        dR[v] = rl_ - rr_;
+        Fb[v] = area * dR[v];
       }
-      riemann_adj<RS>(PL, PR, n[0], n[1], n[2], g, Fb, Lb, Rb);
+      if constexpr (GEO == 1) {
+        if (a.GF) {
+          double nb[3] = {0.0, 0.0, 0.0}, F[5];
+          riemann_adj<RS>(PL, PR, n[0], n[1], n[2], g, Fb, Lb, Rb, nb);
+          riemann<RS>(PL, PR, n[0], n[1], n[2], g, F);
+          double Ab = 0.0;
+#pragma unroll
+          for (int v = 0; v < 5; ++v) Ab += F[v] * dR[v];
+          const size_t np = (size_t)N;
+          a.GF[(4 * d) * np + c] = nb[0];
+          a.GF[(4 * d + 1) * np + c] = nb[1];
+          a.GF[(4 * d + 2) * np + c] = nb[2];
+          a.GF[(4 * d + 3) * np + c] = Ab;
+        } else {
+          riemann_adj<RS>(PL, PR, n[0], n[1], n[2], g, Fb, Lb, Rb);
+        }
+      } else {
+        riemann_adj<RS>(PL, PR, n[0], n[1], n[2], g, Fb, Lb, Rb);
+      }
+    } else if (GEO == 1 && a.GF) {
+      const size_t np = (size_t)N;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a.GF[(4 * d + q) * np + c] = 0.0;   // a non-connection face: k_adj_bcface
     }
     // every "+" / "-" slot that the cell pass reads is written here (zero when no face uses it)
 #pragma unroll
@@ -647,32 +742,61 @@ __global__ void __launch_bounds__(256) k_adj_bcface(AdjArgs a) {
   const double gm1 = a.gam - 1.0;
   const GasT<double> g{a.gam, gm1, 1.0 / gm1, a.R, a.gam * a.R / gm1};
   const int64_t N = a.npool;
+  const bool want_geo = a.GF != nullptr;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < a.nbcf; q += gridDim.x * blockDim.x) {
     const BcFace e = a.bcfaces[q];
     const BlockDev& B = a.blocks[e.b];
     const bool hi = e.f & 1;
     const int bct = B.bct[e.f];
     double Pbb[5] = {0, 0, 0, 0, 0}, Pib[5] = {0, 0, 0, 0, 0};
+    double nb[3] = {0, 0, 0}, bcb[6] = {0, 0, 0, 0, 0, 0}, Ab = 0.0, bdlb = 0.0, idlb = 0.0;
     if (bct != F_NONE) {
-      double Pb[5], Pi[5], Fb[5];
+      double Pb[5], Pi[5], Fb[5], F[5];
       prim_load(a.Pp, N, e.cb, Pb);
       prim_load(a.Pp, N, e.cn, Pi);
+      const double sg = hi ? 1.0 : -1.0;
       // R(cb) += (hi ? +1 : -1) area F
 #pragma unroll
-      for (int v = 0; v < 5; ++v) Fb[v] = (hi ? e.area : -e.area) * a.Rbar[v * N + e.cb];
+      for (int v = 0; v < 5; ++v) Fb[v] = sg * e.area * a.Rbar[v * N + e.cb];
       if (bct == F_EXTRAP) {
-        physical_flux_adj(Pb, e.n[0], e.n[1], e.n[2], g, Fb, Pbb);
+        physical_flux_adj(Pb, e.n[0], e.n[1], e.n[2], g, Fb, Pbb, nb);
+        if (want_geo) physical_flux(Pb, e.n[0], e.n[1], e.n[2], g, F);
       } else {
+        const double* bc = a.facebc + 8 * B.bcd[e.f];
         bool done = false;
         if (bct == F_INFLOW)
-          done = inflow405_adj<LIM, RS>(Pb, Pi, hi, e.n[0], e.n[1], e.n[2], a.facebc + 8 * B.bcd[e.f], g, Fb, Pbb, Pib);
-        if (!done) symmetry_flux_adj<LIM, RS>(Pb, Pi, e.bdl, e.idl, hi, e.n[0], e.n[1], e.n[2], g, Fb, Pbb, Pib);
+          done = inflow405_adj<LIM, RS>(Pb, Pi, hi, e.n[0], e.n[1], e.n[2], bc, g, Fb, Pbb, Pib, nb, bcb);
+        if (!done)
+          symmetry_flux_adj<LIM, RS>(Pb, Pi, e.bdl, e.idl, hi, e.n[0], e.n[1], e.n[2], g, Fb, Pbb, Pib, nb, &bdlb,
+                                     &idlb);
+        if (want_geo) {
+          bool fdone = false;
+          if (bct == F_INFLOW) fdone = inflow405_flux<LIM, RS>(Pb, Pi, hi, e.n[0], e.n[1], e.n[2], bc, g, F);
+          if (!fdone) symmetry_flux<LIM, RS>(Pb, Pi, e.bdl, e.idl, hi, e.n[0], e.n[1], e.n[2], g, F);
+        }
+      }
+      if (want_geo) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Ab += sg * F[v] * a.Rbar[v * N + e.cb];
       }
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       a.BB[(size_t)v * a.nslots + e.slot] = Pbb[v];
       a.BB[(size_t)(5 + v) * a.nslots + e.slot] = Pib[v];
+    }
+    if (want_geo) {
+      const size_t np = (size_t)N;
+      const int d = e.f >> 1;
+      const int64_t fc = hi ? e.cb + (e.cb - e.cn) : e.cb;   // the cell holding the face as its low face
+      a.GF[(4 * d) * np + fc] = nb[0];
+      a.GF[(4 * d + 1) * np + fc] = nb[1];
+      a.GF[(4 * d + 2) * np + fc] = nb[2];
+      a.GF[(4 * d + 3) * np + fc] = Ab;
+      a.BG[e.slot] = bdlb;
+      a.BG[(size_t)a.nslots + e.slot] = idlb;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) a.IB[(size_t)r * a.nslots + e.slot] = bcb[r];
     }
   }
 }
@@ -697,6 +821,7 @@ __global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a) {
     int outside = 0;
     for (int d = 0; d < 3; ++d) outside += (p3[d] < 1 || p3[d] > n3[d]);
     double acc[5] = {0, 0, 0, 0, 0};
+    double rlb[3] = {0, 0, 0}, dlb[3] = {0, 0, 0};
     if (outside <= 1) {
       const AdjGeo<GEO> G{a, B};
       for (int d = 0; d < 3; ++d) {
@@ -725,8 +850,13 @@ __global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a) {
           prim_load(a.Pp, N, cm, P0);
           prim_load(a.Pp, N, cm + s, Pp);
           double bm[5] = {0, 0, 0, 0, 0}, b0[5] = {0, 0, 0, 0, 0}, bp[5] = {0, 0, 0, 0, 0};
+          double gb3[3] = {0.0, 0.0, 0.0};
+          const bool sp_geo = GEO == 1 && a.GL && m <= 0;
           cell_states_adj<LIM>(Pm, P0, Pp, G.rl(d, cm, qm), G.rl(d, cm + s, qm + 1), G.hdl(d, cm, qm), Smb, Spb,
-                               bm, b0, bp);
+                               bm, b0, bp, sp_geo ? gb3 : nullptr);
+          // rl(c) is rlm of c's own slope and rlp of the slope of c - e_d; hdl(c) = dl(c) / 2
+          if (m == 0) { rlb[d] += gb3[0]; dlb[d] += 0.5 * gb3[2]; }
+          else if (m == -1) rlb[d] += gb3[1];
           const double* src = m == -1 ? bp : (m == 0 ? b0 : bm);
 #pragma unroll
           for (int v = 0; v < 5; ++v) acc[v] += src[v];
@@ -744,16 +874,25 @@ __global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a) {
           if (p3[d] == qb) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) acc[v] += a.BB[(size_t)v * a.nslots + slot];
+            if (GEO == 1 && a.GL) dlb[d] += 0.5 * a.BG[slot];                    // bdl = dl / 2
           }
           if (p3[d] == qn) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) acc[v] += a.BB[(size_t)(5 + v) * a.nslots + slot];
+            if (GEO == 1 && a.GL) dlb[d] += 0.5 * a.BG[(size_t)a.nslots + slot];  // idl = dl / 2
           }
         }
       }
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) a.Pbar[v * N + c] = acc[v];
+    if (GEO == 1 && a.GL) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        a.GL[(size_t)d * N + c] = rlb[d];
+        a.GL[(size_t)(3 + d) * N + c] = dlb[d];
+      }
+    }
   }
 }
 
@@ -768,6 +907,7 @@ __global__ void __launch_bounds__(256) k_adj_ghostT(AdjArgs a) {
       double gb[5];
 #pragma unroll
       for (int v = 0; v < 5; ++v) gb[v] = a.Pbar[v * N + E.ghost];
+      double enb[3] = {0.0, 0.0, 0.0};
       if (E.kind == 0) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) acc[v] += E.w * gb[v];
@@ -778,6 +918,17 @@ __global__ void __launch_bounds__(256) k_adj_ghostT(AdjArgs a) {
         acc[2] += gb[2] - 2.0 * dn * E.n[1];
         acc[3] += gb[3] - 2.0 * dn * E.n[2];
         acc[4] += gb[4];
+        if (a.GE) {   // d/dn of gb . (u - 2 (u.n) n)
+          const double u1 = a.Pp[N + row.src], u2 = a.Pp[2 * N + row.src], u3 = a.Pp[3 * N + row.src];
+          const double un = u1 * E.n[0] + u2 * E.n[1] + u3 * E.n[2];
+          enb[0] = -2.0 * (un * gb[1] + dn * u1);
+          enb[1] = -2.0 * (un * gb[2] + dn * u2);
+          enb[2] = -2.0 * (un * gb[3] + dn * u3);
+        }
+      }
+      if (a.GE) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) a.GE[(size_t)r * a.nents + e] = enb[r];
       }
     }
 #pragma unroll

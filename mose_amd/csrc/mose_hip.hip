@@ -261,6 +261,7 @@ __global__ void k_copy(const double* __restrict__ s, double* __restrict__ d, int
 
 // ============================================================================== host
 #include "mose_geometry.hpp"
+#include "mose_metrics_adj.hpp"
 
 namespace {
 
@@ -404,8 +405,14 @@ struct mose_ctx {
   AdjSeg* d_segs = nullptr;
   AdjGhostRow* d_grows = nullptr;
   AdjGhostEnt* d_gents = nullptr;
-  int nseg = 0, ngrows = 0;
+  int nseg = 0, ngrows = 0, nents = 0;
   int64_t nwork = 0;
+  // geometry adjoints (general metrics only; allocated on first use)
+  std::vector<int64_t> gent_face;   // per ghost-fill entry: 3 * (cell holding the face) + d, or -1
+  double *d_GF = nullptr, *d_GL = nullptr, *d_BG = nullptr, *d_GE = nullptr, *d_IB = nullptr;
+  bool geo_valid = false;           // the last vjp wrote the geometry adjoints
+  bool xbar_valid = false;
+  std::vector<std::vector<double>> xbar;
   bool dt_primed = false;   // d_dtmin holds the CFL step of the current state
   int grid_stage = 0;
   int shape = 0;       // index into kShape of the stage kernel in use
@@ -625,36 +632,51 @@ static int ensure_adjoint(mose_ctx* c) {
   c->nseg = (int)segs.size();
   // ghost-fill map per source cell: ghost = sum_w w P(src) (copy, zero gradient,
   // extrapolation 3a-3b+c and its second layer 6a-8b+3c) or the velocity mirror
-  std::map<int64_t, std::vector<AdjGhostEnt>> rows;
-  auto add = [&](int64_t src, int64_t ghost, double w, int kind, const double* n) {
+  // (each entry is tagged with the face whose normal a mirror entry uses, for the geometry
+  // adjoint: 3 * (cell holding the face as its low face) + direction)
+  std::map<int64_t, std::vector<std::pair<AdjGhostEnt, int64_t>>> rows;
+  auto add = [&](int64_t src, int64_t ghost, double w, int kind, const double* n, int64_t tag) {
     AdjGhostEnt e{};
     e.ghost = ghost; e.w = w; e.kind = kind;
     if (n) { e.n[0] = n[0]; e.n[1] = n[1]; e.n[2] = n[2]; }
-    rows[src].push_back(e);
+    rows[src].push_back({e, tag});
   };
-  for (const GhostOp& op : c->lay.ghost) {
+  for (size_t q = 0; q < c->lay.ghost.size(); ++q) {
+    const GhostOp& op = c->lay.ghost[q];
     const int64_t g1 = op.b - op.s, g2 = op.b - 2 * op.s;
     switch (op.type) {
-      case G_COPY: add(op.src1, g1, 1.0, 0, nullptr); add(op.src2, g2, 1.0, 0, nullptr); break;
-      case G_ZG: add(op.b, g1, 1.0, 0, nullptr); add(op.b, g2, 1.0, 0, nullptr); break;
+      case G_COPY: add(op.src1, g1, 1.0, 0, nullptr, -1); add(op.src2, g2, 1.0, 0, nullptr, -1); break;
+      case G_ZG: add(op.b, g1, 1.0, 0, nullptr, -1); add(op.b, g2, 1.0, 0, nullptr, -1); break;
       case G_EXTRAP:
-        add(op.b, g1, 3.0, 0, nullptr); add(op.b + op.s, g1, -3.0, 0, nullptr); add(op.b + 2 * op.s, g1, 1.0, 0, nullptr);
-        add(op.b, g2, 6.0, 0, nullptr); add(op.b + op.s, g2, -8.0, 0, nullptr); add(op.b + 2 * op.s, g2, 3.0, 0, nullptr);
+        add(op.b, g1, 3.0, 0, nullptr, -1); add(op.b + op.s, g1, -3.0, 0, nullptr, -1);
+        add(op.b + 2 * op.s, g1, 1.0, 0, nullptr, -1);
+        add(op.b, g2, 6.0, 0, nullptr, -1); add(op.b + op.s, g2, -8.0, 0, nullptr, -1);
+        add(op.b + 2 * op.s, g2, 3.0, 0, nullptr, -1);
         break;
-      default: add(op.b, g1, 0.0, 1, op.n); break;   // symmetry: one layer
+      default: {   // symmetry: one layer, mirrored with the normal of the boundary face
+        const mose_bc_entry& e = c->bc[q];
+        const int64_t fc = (e.f % 2) ? op.b : op.b - op.s;
+        add(op.b, g1, 0.0, 1, op.n, c->lay.curvi ? 3 * fc + (e.f - 1) / 2 : -1);
+        break;
+      }
     }
   }
   std::vector<AdjGhostRow> grows;
   std::vector<AdjGhostEnt> gents;
+  c->gent_face.clear();
   for (auto& kv : rows) {
     AdjGhostRow r{};
     r.src = kv.first;
     r.begin = (int)gents.size();
-    gents.insert(gents.end(), kv.second.begin(), kv.second.end());
+    for (auto& et : kv.second) {
+      gents.push_back(et.first);
+      c->gent_face.push_back(et.second);
+    }
     r.end = (int)gents.size();
     grows.push_back(r);
   }
   c->ngrows = (int)grows.size();
+  c->nents = (int)gents.size();
   if (upload(c, &c->d_segs, segs)) return c->err.code;
   if (upload(c, &c->d_grows, grows)) return c->err.code;
   if (upload(c, &c->d_gents, gents)) return c->err.code;
@@ -676,16 +698,38 @@ static AdjArgs adj_args(mose_ctx* c) {
   a.segs = c->d_segs; a.bcfaces = c->d_bcfaces; a.grows = c->d_grows; a.gents = c->d_gents;
   a.npool = c->npool; a.nwork = c->nwork; a.nslots = std::max<int64_t>(c->lay.nbcslots, 1);
   a.nseg = c->nseg; a.nbcf = (int)c->lay.bcfaces.size(); a.ngrows = c->ngrows; a.nblocks = c->nb;
+  a.nents = std::max(c->nents, 1);
   a.gam = c->cfg.gamma; a.R = c->cfg.Rgas;
   return a;
 }
 
-// Pbar = (dR/dP)^T Rbar on the current state (the ghost layers are filled first).
-static void enqueue_vjp(mose_ctx* c) {
+// Buffers of the geometry adjoints (general metrics only).
+static int ensure_adjoint_geometry(mose_ctx* c) {
+  if (c->d_GF) return 0;
+  const int64_t ns = std::max<int64_t>(c->lay.nbcslots, 1);
+  HIPCHK(c, hipMalloc((void**)&c->d_GF, (size_t)12 * c->npool * sizeof(double)));
+  HIPCHK(c, hipMemset(c->d_GF, 0, (size_t)12 * c->npool * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_GL, (size_t)6 * c->npool * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_BG, (size_t)2 * ns * sizeof(double)));
+  HIPCHK(c, hipMemset(c->d_BG, 0, (size_t)2 * ns * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_GE, (size_t)3 * std::max(c->nents, 1) * sizeof(double)));
+  HIPCHK(c, hipMemset(c->d_GE, 0, (size_t)3 * std::max(c->nents, 1) * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_IB, (size_t)6 * ns * sizeof(double)));
+  HIPCHK(c, hipMemset(c->d_IB, 0, (size_t)6 * ns * sizeof(double)));
+  return 0;
+}
+
+// Pbar = (dR/dP)^T Rbar on the current state (the ghost layers are filled first); with
+// geometry, also the adjoints of the geometric quantities (face normals and areas, lengths,
+// mirror normals, inflow values).
+static void enqueue_vjp(mose_ctx* c, bool geometry = false) {
   enqueue_ghost(c, c->U[0]);
   hipLaunchKernelGGL(k_adj_prim, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, c->U[0], c->d_Pp, c->npool,
                      c->cfg.gamma);
   AdjArgs a = adj_args(c);
+  if (geometry) {
+    a.GF = c->d_GF; a.GL = c->d_GL; a.BG = c->d_BG; a.GE = c->d_GE; a.IB = c->d_IB;
+  }
   const int lim = c->cfg.limiter, rs = c->cfg.riemann, geo = c->lay.curvi ? 1 : 0;
   void* args[] = {&a};
   (void)hipLaunchKernel((const void*)atable().face[lim][rs][geo], dim3(launch_blocks(c->nwork)), dim3(256), args, 0,
@@ -1016,8 +1060,87 @@ int mose_residual_vjp(mose_ctx* c) {
   if (!c || !c->finalized) return fail(c, MOSE_E_ARG);
   if (c->err.code) return c->err.code;
   if (ensure_adjoint(c)) return c->err.code;
+  c->geo_valid = false;
+  c->xbar_valid = false;
   enqueue_vjp(c);
   return sync_and_check(c);
+}
+
+int mose_residual_vjp_geometry(mose_ctx* c) {
+  if (!c || !c->finalized) return fail(c, MOSE_E_ARG);
+  if (c->err.code) return c->err.code;
+  if (!c->lay.curvi) return fail(c, MOSE_E_ARG);
+  if (ensure_adjoint(c)) return c->err.code;
+  if (ensure_adjoint_geometry(c)) return c->err.code;
+  enqueue_vjp(c, true);
+  c->xbar_valid = false;
+  const int rc = sync_and_check(c);
+  c->geo_valid = rc == 0;
+  return rc;
+}
+
+// Node adjoints of every block from the geometry adjoints of the last
+// mose_residual_vjp_geometry: device arrays to the host, then the reverse of the metrics.
+static int node_adjoints(mose_ctx* c) {
+  if (c->xbar_valid) return 0;
+  const int64_t np = c->npool;
+  std::vector<double> GF((size_t)12 * np), GL((size_t)6 * np), RL((size_t)3 * np);
+  std::vector<double> GE((size_t)3 * std::max(c->nents, 1));
+  HIPCHK(c, hipMemcpy(GF.data(), c->d_GF, GF.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(GL.data(), c->d_GL, GL.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(RL.data(), c->d_geo + (size_t)GRLO * np, RL.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(GE.data(), c->d_GE, GE.size() * sizeof(double), hipMemcpyDeviceToHost));
+  // mirror normals of the symmetry ghost fill are copies of boundary-face normals
+  for (int e = 0; e < c->nents; ++e) {
+    const int64_t t = c->gent_face[e];
+    if (t < 0) continue;
+    const int64_t fc = t / 3;
+    const int d = (int)(t % 3);
+    for (int q = 0; q < 3; ++q) GF[(size_t)(GN + 4 * d + q) * np + fc] += GE[(size_t)q * c->nents + e];
+  }
+  std::vector<double> RLb(GL.begin(), GL.begin() + 3 * np), DLb(GL.begin() + 3 * np, GL.end());
+  std::vector<HostBlock> blocks;
+  for (int b = 1; b <= c->nb; ++b) blocks.push_back(c->hb.at(b));
+  geo::curvilinear_vjp(blocks, c->bc, c->lay, RL, GF, RLb, std::move(DLb), c->xbar);
+  c->xbar_valid = true;
+  return 0;
+}
+
+int mose_get_node_bar(mose_ctx* c, int32_t b, double* Xbar) {
+  if (!c || !c->finalized || b < 1 || b > c->nb || !Xbar || !c->geo_valid) return fail(c, MOSE_E_ARG, b);
+  if (node_adjoints(c)) return c->err.code;
+  const std::vector<double>& x = c->xbar[b - 1];
+  std::memcpy(Xbar, x.data(), x.size() * sizeof(double));
+  return 0;
+}
+
+int mose_get_inflow_bar(mose_ctx* c, int32_t b, int32_t f, double out[5]) {
+  if (!c || !c->finalized || b < 1 || b > c->nb || f < 1 || f > 6 || !out || !c->geo_valid)
+    return fail(c, MOSE_E_ARG, b);
+  const BlockDev& B = c->lay.blocks[b - 1];
+  if (B.bct[f - 1] != F_INFLOW) return fail(c, MOSE_E_ARG, b);
+  const int64_t ns = std::max<int64_t>(c->lay.nbcslots, 1);
+  const int d = (f - 1) / 2;
+  const int n[3] = {B.ni, B.nj, B.nk};
+  const int64_t cnt = (int64_t)n[(d + 1) % 3] * n[(d + 2) % 3];
+  std::vector<double> ib((size_t)6 * cnt);
+  for (int r = 0; r < 6; ++r)
+    HIPCHK(c, hipMemcpy(ib.data() + (size_t)r * cnt, c->d_IB + (size_t)r * ns + B.bcs[f - 1], cnt * sizeof(double),
+                        hipMemcpyDeviceToHost));
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < 6; ++r)
+    for (int64_t q = 0; q < cnt; ++q) s[r] += ib[(size_t)r * cnt + q];
+  // direction (cos a cos b, sin a cos b, sin b) of the record's angles
+  const double* v = c->lay.facebc.data() + 8 * B.bcd[f - 1];
+  const double ca = v[3], sa = v[4];   // cos a cos b, sin a cos b
+  const double cb = std::sqrt(ca * ca + sa * sa), sb = v[5];
+  const double cosa = cb > 0.0 ? ca / cb : 1.0, sina = cb > 0.0 ? sa / cb : 0.0;
+  out[0] = s[0];
+  out[1] = s[1];
+  out[2] = s[2];
+  out[3] = s[3] * (-sina * cb) + s[4] * (cosa * cb);                        // d/d alpha
+  out[4] = s[3] * (-cosa * sb) + s[4] * (-sina * sb) + s[5] * cb;            // d/d beta
+  return 0;
 }
 
 int mose_get_state_bar(mose_ctx* c, int32_t b, double* Pbar) {
@@ -1092,7 +1215,8 @@ void mose_destroy(mose_ctx* c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_geo, c->d_tiles,
                   c->d_ghost, c->d_bcfaces, c->d_vfaces, c->d_vbuf, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
-                  c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents};
+                  c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
+                  c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -20,7 +20,8 @@ _LIB = None
 EXPORTS = ["mose_create", "mose_add_block", "mose_set_viscous", "mose_set_bc", "mose_finalize", "mose_put_state",
            "mose_get_state", "mose_compute_dt", "mose_step", "mose_advance", "mose_residual",
            "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_put_residual_bar",
-           "mose_residual_vjp", "mose_get_state_bar", "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
+           "mose_residual_vjp", "mose_get_state_bar", "mose_residual_vjp_geometry", "mose_get_node_bar",
+           "mose_get_inflow_bar", "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
            "mose_destroy"]
 
 
@@ -53,6 +54,9 @@ def load():
     L.mose_residual_vjp.argtypes = [v]
     L.mose_get_state_bar.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_time_residual_vjp.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_residual_vjp_geometry.argtypes = [v]
+    L.mose_get_node_bar.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_get_inflow_bar.argtypes = [v, C.c_int32, C.c_int32, P(C.c_double)]
     L.mose_num_cells.argtypes = [v]
     L.mose_num_cells.restype = C.c_int64
     L.mose_last_error.argtypes = [v, P(abi.MoseErr)]
@@ -151,6 +155,35 @@ class GpuSolver:
             self._check(L.mose_get_state_bar(self.h, b, _dp(Pb)))
             out.append(Pb)
         return out
+
+    def residual_vjp_geometry(self, Rbars):
+        """Adjoint of the residual with respect to the state, the block nodes and the inflow
+        values: returns (Pbar per block, Xbar per block in the node layout (nk+1, nj+1, ni+1, 3)
+        of Case blocks, {(b, f): d/d(mach, T, p, alpha, beta)} for every 405 face)."""
+        L = load()
+        for b, Rb in enumerate(Rbars, start=1):
+            Rb = np.ascontiguousarray(Rb, dtype=np.float64)
+            assert Rb.shape == abi.padded_shape(self.dims[b - 1])
+            self._check(L.mose_put_residual_bar(self.h, b, _dp(Rb)))
+        self._check(L.mose_residual_vjp_geometry(self.h))
+        Pbars, Xbars = [], []
+        for b in range(1, len(self.dims) + 1):
+            Pb = np.zeros(abi.padded_shape(self.dims[b - 1]))
+            self._check(L.mose_get_state_bar(self.h, b, _dp(Pb)))
+            Pbars.append(Pb)
+            ni, nj, nk = self.dims[b - 1]
+            Xb = np.zeros((nk + 1, nj + 1, ni + 1, 3))
+            self._check(L.mose_get_node_bar(self.h, b, _dp(Xb)))
+            Xbars.append(Xb)
+        inflow = {}
+        for row in np.asarray(self.case.bc):
+            if int(row["type"]) == 405:
+                key = (int(row["b"]), int(row["f"]))
+                if key not in inflow:
+                    out = np.zeros(5)
+                    self._check(L.mose_get_inflow_bar(self.h, key[0], key[1], _dp(out)))
+                    inflow[key] = out
+        return Pbars, Xbars, inflow
 
     def time_residual_vjp(self, nrep=10):
         """Average device ms of one residual and one adjoint evaluation."""
