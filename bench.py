@@ -88,8 +88,12 @@ def adjoint_ratio(g, case, nrep=5):
     evaluation (mose_residual) on the current state, seeded with Rbar = 1 on every cell."""
     Rbars = [np.pad(np.ones(tuple(b.dims[::-1]) + (5,)), ((2, 2), (2, 2), (2, 2), (0, 0))) for b in case.blocks]
     g.residual_vjp(Rbars)
-    res_ms, vjp_ms = g.time_residual_vjp(nrep)
-    return {"residual_ms": res_ms, "vjp_ms": vjp_ms, "ratio": vjp_ms / res_ms}
+    res_ms, vjp_ms, geo_ms = g.time_residual_vjp(nrep, geometry=True)
+    out = {"residual_ms": res_ms, "vjp_ms": vjp_ms, "ratio": vjp_ms / res_ms}
+    if geo_ms > 0.0:   # general metrics: state + node + inflow adjoints in one evaluation
+        out["vjp_geometry_ms"] = geo_ms
+        out["ratio_geometry"] = geo_ms / res_ms
+    return out
 
 
 def bump_adjoint():
@@ -261,7 +265,8 @@ def main():
         }
     if adj_vortex is not None:
         result["adjoint"] = {"vortex": adj_vortex, "bump_512x256": bump_adjoint(),
-                             "note": "device ms of one mose_residual_vjp / one mose_residual, FP64"}
+                             "note": "device ms of one mose_residual_vjp / one mose_residual, FP64; "
+                                     "*_geometry: mose_residual_vjp_geometry (state + node + inflow adjoints)"}
     if not args.no_batched:
         result["batched_64_blocks"] = batched(args.batched_steps, min(args.warmup, 2))
     if not args.no_tgv:

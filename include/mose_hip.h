@@ -144,9 +144,11 @@ int  mose_get_state_bar(mose_ctx* ctx, int32_t b, double* Pbar);
 int  mose_residual_vjp_geometry(mose_ctx* ctx);
 int  mose_get_node_bar(mose_ctx* ctx, int32_t b, double* Xbar);
 int  mose_get_inflow_bar(mose_ctx* ctx, int32_t b, int32_t f, double out[5]);
-/* Average device ms of one residual evaluation (ms[0]) and one adjoint evaluation (ms[1])
- * over nrep back-to-back runs on the current state (profiling). */
-int  mose_time_residual_vjp(mose_ctx* ctx, int32_t nrep, double ms[2]);
+/* Average device ms of one residual evaluation (ms[0]), one adjoint evaluation (ms[1]) and,
+ * with general metrics, one geometry adjoint evaluation (ms[2], device passes only; 0 on
+ * rectilinear layouts) over nrep back-to-back runs on the current state and residual
+ * adjoint (profiling); afterwards the adjoint results are those of the last run. */
+int  mose_time_residual_vjp(mose_ctx* ctx, int32_t nrep, double ms[3]);
 
 /* Simulation clock and iteration counter (domain%time, domain%iter). */
 int  mose_get_time(mose_ctx* ctx, double* time, int32_t* iter);

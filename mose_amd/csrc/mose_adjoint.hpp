@@ -560,7 +560,7 @@ __device__ bool inflow405_adj(const double* Pb, const double* Pi, bool hi, doubl
 struct AdjSeg {       // the low faces of one block in one direction
   int64_t start;      // first work item
   int b, d;
-  int n0, n1, n2;     // extents: along d (n_d + 1 faces), first and second tangential
+  int n0, n1, n2;     // extents along i, j, k (n + 1 along d: the faces), i fastest
   int pad;
 };
 
@@ -662,13 +662,14 @@ __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a) {
     const AdjSeg S = a.segs[lo];
     const BlockDev& B = a.blocks[S.b];
     const int d = S.d;
+    // i fastest for every direction, so that a wavefront reads consecutive cells
     int64_t r = q - S.start;
-    const int qd = (int)(r % S.n0) + 1;          // cell index along d whose low face this is
-    r /= S.n0;
-    const int t1 = (int)(r % S.n1) + 1, t2 = (int)(r / S.n1) + 1;
     int ijk[3];
-    const int e1 = d == 0 ? 1 : 0, e2 = d == 2 ? 1 : 2;
-    ijk[d] = qd; ijk[e1] = t1; ijk[e2] = t2;
+    ijk[0] = (int)(r % S.n0) + 1;
+    r /= S.n0;
+    ijk[1] = (int)(r % S.n1) + 1;
+    ijk[2] = (int)(r / S.n1) + 1;
+    const int qd = ijk[d];                       // cell index along d whose low face this is
     const int nd = d == 0 ? B.ni : (d == 1 ? B.nj : B.nk);
     const bool blo = qd == 1, bhi = qd == nd + 1;
     const bool skip = (blo && B.bct[2 * d] != F_CONN) || (bhi && B.bct[2 * d + 1] != F_CONN);

@@ -621,9 +621,9 @@ static int ensure_adjoint(mose_ctx* c) {
     for (int d = 0; d < 3; ++d) {
       AdjSeg sg{};
       sg.start = start; sg.b = b; sg.d = d;
-      sg.n0 = n[d] + 1;
-      sg.n1 = d == 0 ? n[1] : n[0];
-      sg.n2 = d == 2 ? n[1] : n[2];
+      sg.n0 = n[0] + (d == 0);
+      sg.n1 = n[1] + (d == 1);
+      sg.n2 = n[2] + (d == 2);
       segs.push_back(sg);
       start += (int64_t)sg.n0 * sg.n1 * sg.n2;
     }
@@ -1156,19 +1156,22 @@ int mose_get_state_bar(mose_ctx* c, int32_t b, double* Pbar) {
   return 0;
 }
 
-int mose_time_residual_vjp(mose_ctx* c, int32_t nrep, double ms[2]) {
+int mose_time_residual_vjp(mose_ctx* c, int32_t nrep, double ms[3]) {
   if (!c || !c->finalized || nrep < 1 || !ms) return fail(c, MOSE_E_ARG);
   if (ensure_adjoint(c)) return c->err.code;
+  const bool geo = c->lay.curvi;
+  if (geo && ensure_adjoint_geometry(c)) return c->err.code;
+  ms[2] = 0.0;
   hipEvent_t e0, e1;
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
-  for (int which = 0; which < 2; ++which)
+  for (int which = 0; which < (geo ? 3 : 2); ++which)
     for (int pass = 0; pass < 2; ++pass) {
       const int reps = pass == 0 ? 1 : nrep;
       HIPCHK(c, hipEventRecord(e0, c->stream));
       for (int r = 0; r < reps; ++r) {
         if (which == 0) enqueue_residual(c);
-        else enqueue_vjp(c);
+        else enqueue_vjp(c, which == 2);
       }
       HIPCHK(c, hipEventRecord(e1, c->stream));
       HIPCHK(c, hipEventSynchronize(e1));
@@ -1178,6 +1181,8 @@ int mose_time_residual_vjp(mose_ctx* c, int32_t nrep, double ms[2]) {
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  c->geo_valid = geo;
+  c->xbar_valid = false;
   return sync_and_check(c);
 }
 

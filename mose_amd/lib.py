@@ -185,10 +185,13 @@ class GpuSolver:
                     inflow[key] = out
         return Pbars, Xbars, inflow
 
-    def time_residual_vjp(self, nrep=10):
-        """Average device ms of one residual and one adjoint evaluation."""
-        ms = np.zeros(2)
+    def time_residual_vjp(self, nrep=10, geometry=False):
+        """Average device ms of one residual and one adjoint evaluation (and, with
+        geometry=True, of one geometry adjoint evaluation: 0 on rectilinear layouts)."""
+        ms = np.zeros(3)
         self._check(load().mose_time_residual_vjp(self.h, nrep, _dp(ms)))
+        if geometry:
+            return float(ms[0]), float(ms[1]), float(ms[2])
         return float(ms[0]), float(ms[1])
 
     def residuals(self):
