@@ -111,6 +111,59 @@ module mose_hip
       integer(c_int32_t), value     :: b
       real(c_double), intent(inout) :: R(*)
     end function
+    ! discrete adjoint of mose_residual (state; with geometry also nodes and 405 records)
+    integer(c_int) function mose_put_residual_bar(ctx, b, Rbar) bind(C, name="mose_put_residual_bar")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value         :: ctx
+      integer(c_int32_t), value  :: b
+      real(c_double), intent(in) :: Rbar(*)
+    end function
+    integer(c_int) function mose_residual_vjp(ctx) bind(C, name="mose_residual_vjp")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function mose_residual_vjp_geometry(ctx) bind(C, name="mose_residual_vjp_geometry")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function mose_get_state_bar(ctx, b, Pbar) bind(C, name="mose_get_state_bar")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value          :: ctx
+      integer(c_int32_t), value   :: b
+      real(c_double), intent(out) :: Pbar(*)
+    end function
+    integer(c_int) function mose_get_node_bar(ctx, b, Xbar) bind(C, name="mose_get_node_bar")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value          :: ctx
+      integer(c_int32_t), value   :: b
+      real(c_double), intent(out) :: Xbar(*)
+    end function
+    integer(c_int) function mose_get_inflow_bar(ctx, b, f, dinflow) bind(C, name="mose_get_inflow_bar")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value          :: ctx
+      integer(c_int32_t), value   :: b, f
+      real(c_double), intent(out) :: dinflow(5)
+    end function
+    integer(c_int) function mose_time_residual_vjp(ctx, nrep, ms) bind(C, name="mose_time_residual_vjp")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value          :: ctx
+      integer(c_int32_t), value   :: nrep
+      real(c_double), intent(out) :: ms(3)
+    end function
+    ! profiling and tile autotuning
+    integer(c_int) function mose_time_kernels(ctx, nrep, ms) bind(C, name="mose_time_kernels")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value          :: ctx
+      integer(c_int32_t), value   :: nrep
+      real(c_double), intent(out) :: ms(4)
+    end function
+    integer(c_int) function mose_autotune(ctx, nrep, choice, best_ms) bind(C, name="mose_autotune")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value              :: ctx
+      integer(c_int32_t), value       :: nrep
+      integer(c_int32_t), intent(out) :: choice(3)
+      real(c_double), intent(out)     :: best_ms
+    end function
     integer(c_int) function mose_get_time(ctx, time, iter) bind(C, name="mose_get_time")
       import :: c_int, c_ptr, c_int32_t, c_double
       type(c_ptr), value              :: ctx

@@ -29,6 +29,14 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(lib.EXPORTS)
 
 
+def test_fortran_binding_covers_the_abi():
+    """The ISO_C_BINDING module binds every entry point (mose_strerror returns a C string
+    and is not needed from Fortran, where mose_explicit prints the reference's messages)."""
+    text = open(os.path.join(ROOT, "mose_amd", "fortran", "mose_hip_iface.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(mose_[a-z_]+)"\)', text))
+    assert set(declared_functions()) - {"mose_strerror"} == bound
+
+
 def test_struct_layouts_match_header():
     # mose_bc_entry: 16 int32 + 6 double = 112 bytes; mose_cfg: 3 double + 6 int32 + 4 int32 = 64
     assert C.sizeof(abi.MoseBcEntry) == 112
