@@ -539,7 +539,19 @@ static int enqueue_step(mose_ctx* c) {
   return 0;
 }
 
+// A kernel launch that the runtime refused (launch configuration, resources) leaves its
+// error for hipGetLastError; every enqueue sequence is followed by this check.
+static int launch_check(mose_ctx* c, const char* where) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    fprintf(stderr, "[mose_hip] kernel launch failed in %s: %s\n", where, hipGetErrorString(e));
+    return fail(c, MOSE_E_HIP);
+  }
+  return 0;
+}
+
 static int sync_and_check(mose_ctx* c) {
+  if (launch_check(c, "enqueue")) return c->err.code;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int e[5];
   HIPCHK(c, hipMemcpy(e, c->d_err, sizeof e, hipMemcpyDeviceToHost));
@@ -554,7 +566,14 @@ static int ensure_graph(mose_ctx* c) {
   if (c->gexec) return 0;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   enqueue_step(c);
+  const hipError_t le = hipGetLastError();
   HIPCHK(c, hipStreamEndCapture(c->stream, &c->graph));
+  if (le != hipSuccess) {
+    fprintf(stderr, "[mose_hip] kernel launch failed during step capture: %s\n", hipGetErrorString(le));
+    (void)hipGraphDestroy(c->graph);
+    c->graph = nullptr;
+    return fail(c, MOSE_E_HIP);
+  }
   HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
   return 0;
 }
@@ -1024,6 +1043,10 @@ int mose_autotune(mose_ctx* c, int32_t nrep, int32_t choice[3], double* best_ms)
           float ms = 0.f;
           HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
           t = (double)ms / reps;
+          if (hipGetLastError() != hipSuccess) {   // a configuration the runtime refused
+            t = 1e300;
+            break;
+          }
         }
         if (t < best) {
           best = t;
