@@ -271,7 +271,7 @@ typedef void (*BcFn)(StageArgs, const BcFace*, int);
 // Patch shapes (TX x TY threads) of the stage kernel per precision; shape 0 is the default,
 // the others are compiled for the van Leer limiter and selected by the tile tuner.
 constexpr int NSHAPE = 3;
-constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 8}, {8, 16}}, {{32, 8}, {16, 16}, {16, 8}}};
+constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 16}, {8, 16}}, {{32, 8}, {16, 16}, {16, 8}}};
 
 template <int LIM, int RS, typename FT, int TXc, int TYc, int GEO = 0>
 void stage_modes(StageFn* t) {

@@ -285,6 +285,9 @@ __global__ void __launch_bounds__(TXc* TYc)
   // (LDS-bound); reduced in a fixed order either way
   constexpr bool NLDS = MODE == 1 && sizeof(FT) == 4;
   __shared__ double sN[5][NLDS ? NTc : 1];
+  // 512-thread tiles (32 x 16) failed at launch ("unspecified launch failure") with the cause
+  // not identified; only tiles of at most 256 threads are compiled
+  static_assert(NTc <= 256, "tile shapes above 256 threads are not supported");
 
   const int tid = threadIdx.x, tx = tid % TXc, ty = tid / TXc;
   const double gm1 = a.gam - 1.0;
