@@ -142,6 +142,12 @@ int  mose_get_state_bar(mose_ctx* ctx, int32_t b, double* Pbar);
  *     of block b: out = (mach, T, p, alpha, beta).
  * No counterpart in the reference (no adjoint). */
 int  mose_residual_vjp_geometry(mose_ctx* ctx);
+/* After either vjp on a layout with viscous blocks: the adjoint of Rbar . R with respect to
+ * the Sutherland reference viscosity mu_ref of block b (d/dRe = -mu_ref^2 d/dmu_ref for
+ * mu_ref = 1/Re); 0 for an inviscid block.  The viscous terms of the residual (interior
+ * faces and BC_Connection_Visc faces) are part of both vjps.  No counterpart in the
+ * reference (no adjoint). */
+int  mose_get_viscous_bar(mose_ctx* ctx, int32_t b, double* mu_ref_bar);
 int  mose_get_node_bar(mose_ctx* ctx, int32_t b, double* Xbar);
 int  mose_get_inflow_bar(mose_ctx* ctx, int32_t b, int32_t f, double out[5]);
 /* Average device ms of one residual evaluation (ms[0]), one adjoint evaluation (ms[1]) and,

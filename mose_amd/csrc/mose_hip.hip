@@ -410,6 +410,8 @@ struct mose_ctx {
   // geometry adjoints (general metrics only; allocated on first use)
   std::vector<int64_t> gent_face;   // per ghost-fill entry: 3 * (cell holding the face) + d, or -1
   double *d_GF = nullptr, *d_GL = nullptr, *d_BG = nullptr, *d_GE = nullptr, *d_IB = nullptr;
+  double *d_VW = nullptr, *d_VB = nullptr;   // viscous adjoint buffers (viscous layouts only)
+  bool vjp_valid = false;                    // the last vjp left Pbar and the viscous buffers
   bool geo_valid = false;           // the last vjp wrote the geometry adjoints
   bool xbar_valid = false;
   std::vector<std::vector<double>> xbar;
@@ -706,6 +708,10 @@ static int ensure_adjoint(mose_ctx* c) {
   HIPCHK(c, hipMalloc((void**)&c->d_BB, (size_t)10 * ns * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_Pbar, 5 * c->npool * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_Pp, 5 * c->npool * sizeof(double)));
+  if (c->lay.visc) {
+    HIPCHK(c, hipMalloc((void**)&c->d_VW, (size_t)3 * 18 * c->npool * sizeof(double)));
+    HIPCHK(c, hipMalloc((void**)&c->d_VB, std::max<size_t>(c->lay.vfaces.size(), 1) * sizeof(double)));
+  }
   c->adj_ready = true;
   return 0;
 }
@@ -718,6 +724,7 @@ static AdjArgs adj_args(mose_ctx* c) {
   a.npool = c->npool; a.nwork = c->nwork; a.nslots = std::max<int64_t>(c->lay.nbcslots, 1);
   a.nseg = c->nseg; a.nbcf = (int)c->lay.bcfaces.size(); a.ngrows = c->ngrows; a.nblocks = c->nb;
   a.nents = std::max(c->nents, 1);
+  a.VW = c->d_VW; a.VB = c->d_VB; a.vfaces = c->d_vfaces; a.nvf = (int)c->lay.vfaces.size();
   a.gam = c->cfg.gamma; a.R = c->cfg.Rgas;
   return a;
 }
@@ -757,6 +764,11 @@ static void enqueue_vjp(mose_ctx* c, bool geometry = false) {
     (void)hipLaunchKernel((const void*)atable().bc[lim][rs], dim3(launch_blocks(a.nbcf)), dim3(256), args, 0, c->stream);
   (void)hipLaunchKernel((const void*)atable().cell[lim][geo], dim3(launch_blocks(c->npool)), dim3(256), args, 0,
                         c->stream);
+  if (c->lay.visc) {   // viscous terms: interior faces (gathered per cell), then connection faces
+    hipLaunchKernelGGL(k_adj_vface, dim3(launch_blocks(c->nwork)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_adj_vcell, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, a);
+    if (a.nvf > 0) hipLaunchKernelGGL(k_adj_vbc, dim3(launch_blocks(a.nvf)), dim3(256), 0, c->stream, a);
+  }
   if (c->ngrows > 0)
     hipLaunchKernelGGL(k_adj_ghostT, dim3(launch_blocks(c->ngrows)), dim3(256), 0, c->stream, a);
 }
@@ -1086,6 +1098,7 @@ int mose_residual_vjp(mose_ctx* c) {
   c->geo_valid = false;
   c->xbar_valid = false;
   enqueue_vjp(c);
+  c->vjp_valid = true;
   return sync_and_check(c);
 }
 
@@ -1096,6 +1109,7 @@ int mose_residual_vjp_geometry(mose_ctx* c) {
   if (ensure_adjoint(c)) return c->err.code;
   if (ensure_adjoint_geometry(c)) return c->err.code;
   enqueue_vjp(c, true);
+  c->vjp_valid = true;
   c->xbar_valid = false;
   const int rc = sync_and_check(c);
   c->geo_valid = rc == 0;
@@ -1166,6 +1180,38 @@ int mose_get_inflow_bar(mose_ctx* c, int32_t b, int32_t f, double out[5]) {
   return 0;
 }
 
+int mose_get_viscous_bar(mose_ctx* c, int32_t b, double* mu_ref_bar) {
+  if (!c || !c->finalized || b < 1 || b > c->nb || !mu_ref_bar || !c->vjp_valid) return fail(c, MOSE_E_ARG, b);
+  const BlockDev& B = c->lay.blocks[b - 1];
+  *mu_ref_bar = 0.0;
+  if (!B.visc) return 0;
+  const int64_t np = c->npool, ncell = B.sz * (B.nk + 4);
+  std::vector<double> row((size_t)ncell);
+  const int n3[3] = {B.ni, B.nj, B.nk};
+  double sum = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    HIPCHK(c, hipMemcpy(row.data(), c->d_VW + ((size_t)d * 18 + 17) * np + B.off, ncell * sizeof(double),
+                        hipMemcpyDeviceToHost));
+    const int tb = d == 0 ? 1 : 0, tc = d == 2 ? 1 : 2;
+    for (int k = 1; k <= B.nk; ++k)
+      for (int j = 1; j <= B.nj; ++j)
+        for (int i = 1; i <= B.ni; ++i) {
+          const int q[3] = {i, j, k};
+          if (q[d] > n3[d] - 1 || q[tb] > n3[tb] || q[tc] > n3[tc]) continue;
+          sum += row[(size_t)(i + 1) + (size_t)B.sy * (j + 1) + (size_t)B.sz * (k + 1)];
+        }
+  }
+  const size_t nvf = c->lay.vfaces.size();
+  if (nvf > 0) {
+    std::vector<double> vb(nvf);
+    HIPCHK(c, hipMemcpy(vb.data(), c->d_VB, nvf * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < nvf; ++q)
+      if (c->lay.vfaces[q].b == b - 1) sum += vb[q];
+  }
+  *mu_ref_bar = sum;
+  return 0;
+}
+
 int mose_get_state_bar(mose_ctx* c, int32_t b, double* Pbar) {
   if (!c || !c->finalized || b < 1 || b > c->nb || !Pbar) return fail(c, MOSE_E_ARG, b);
   if (ensure_adjoint(c)) return c->err.code;
@@ -1206,6 +1252,7 @@ int mose_time_residual_vjp(mose_ctx* c, int32_t nrep, double ms[3]) {
   (void)hipEventDestroy(e1);
   c->geo_valid = geo;
   c->xbar_valid = false;
+  c->vjp_valid = true;
   return sync_and_check(c);
 }
 
@@ -1244,7 +1291,7 @@ void mose_destroy(mose_ctx* c) {
   void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_geo, c->d_tiles,
                   c->d_ghost, c->d_bcfaces, c->d_vfaces, c->d_vbuf, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
                   c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
-                  c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB};
+                  c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -45,6 +45,9 @@ module mose_hip
 
   public :: mose_create, mose_add_block, mose_set_viscous, mose_set_bc, mose_finalize, mose_put_state, &
             mose_get_state, mose_compute_dt, mose_step, mose_advance, mose_residual, &
+            mose_put_residual_bar, mose_residual_vjp, mose_residual_vjp_geometry, mose_get_viscous_bar, &
+            mose_get_state_bar, mose_get_node_bar, mose_get_inflow_bar, mose_time_residual_vjp, &
+            mose_time_kernels, mose_autotune, &
             mose_get_time, mose_set_time, mose_num_cells, mose_last_error, mose_destroy
 
   interface
@@ -125,6 +128,12 @@ module mose_hip
     integer(c_int) function mose_residual_vjp_geometry(ctx) bind(C, name="mose_residual_vjp_geometry")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function mose_get_viscous_bar(ctx, b, mu_ref_bar) bind(C, name="mose_get_viscous_bar")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value          :: ctx
+      integer(c_int32_t), value   :: b
+      real(c_double), intent(out) :: mu_ref_bar
     end function
     integer(c_int) function mose_get_state_bar(ctx, b, Pbar) bind(C, name="mose_get_state_bar")
       import :: c_int, c_ptr, c_int32_t, c_double

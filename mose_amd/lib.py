@@ -20,7 +20,7 @@ _LIB = None
 EXPORTS = ["mose_create", "mose_add_block", "mose_set_viscous", "mose_set_bc", "mose_finalize", "mose_put_state",
            "mose_get_state", "mose_compute_dt", "mose_step", "mose_advance", "mose_residual",
            "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_put_residual_bar",
-           "mose_residual_vjp", "mose_get_state_bar", "mose_residual_vjp_geometry", "mose_get_node_bar",
+           "mose_residual_vjp", "mose_get_state_bar", "mose_residual_vjp_geometry", "mose_get_viscous_bar", "mose_get_node_bar",
            "mose_get_inflow_bar", "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
            "mose_destroy"]
 
@@ -55,6 +55,7 @@ def load():
     L.mose_get_state_bar.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_time_residual_vjp.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_residual_vjp_geometry.argtypes = [v]
+    L.mose_get_viscous_bar.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_get_node_bar.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_get_inflow_bar.argtypes = [v, C.c_int32, C.c_int32, P(C.c_double)]
     L.mose_num_cells.argtypes = [v]
@@ -155,6 +156,12 @@ class GpuSolver:
             self._check(L.mose_get_state_bar(self.h, b, _dp(Pb)))
             out.append(Pb)
         return out
+
+    def viscous_bar(self, b):
+        """After residual_vjp: d(Rbar . R)/d mu_ref of block b (0 for an inviscid block)."""
+        x = C.c_double()
+        self._check(load().mose_get_viscous_bar(self.h, b, C.byref(x)))
+        return x.value
 
     def residual_vjp_geometry(self, Rbars):
         """Adjoint of the residual with respect to the state, the block nodes and the inflow

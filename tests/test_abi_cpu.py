@@ -35,6 +35,8 @@ def test_fortran_binding_covers_the_abi():
     text = open(os.path.join(ROOT, "mose_amd", "fortran", "mose_hip_iface.f90")).read()
     bound = set(re.findall(r'bind\(C,\s*name="(mose_[a-z_]+)"\)', text))
     assert set(declared_functions()) - {"mose_strerror"} == bound
+    public = re.search(r"public ::\s*(mose_create.*?)\n\n", text, re.S).group(1)
+    assert bound == set(re.findall(r"mose_[a-z_]+", public))
 
 
 def test_struct_layouts_match_header():

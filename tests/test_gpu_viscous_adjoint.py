@@ -1,0 +1,133 @@
+"""Discrete adjoint of the viscous terms (interior Diffusive_Flux faces and BC_Connection_Visc
+connection faces, Sutherland viscosity) against central differences.
+
+The viscous part is isolated: the same state and Rbar on a viscous and an inviscid context,
+and the difference of the two adjoints is compared with central differences of the
+difference of the two residuals (GPU, and the oracle's residuals).  The adjoint with respect
+to the Sutherland reference viscosity mu_ref (mu_ref = 1/Re) is checked the same way.
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from mose_amd import abi, cases
+from mose_amd.lib import GpuSolver
+from oracle.oracle import OracleSolver
+
+from helpers import perturb
+
+pytestmark = pytest.mark.gpu
+
+
+def _mask(dims):
+    m = np.zeros(abi.padded_shape(dims))
+    m[2:-2, 2:-2, 2:-2, :] = 1.0
+    return m
+
+
+def _tgv(n=6, nbox=2, mu=0.05, seed=3):
+    """Small Taylor-Green boxes (self-periodic 201 blocks) at a low Reynolds number, FP64, with
+    a seeded perturbation so every direction and variable carries a gradient."""
+    case = cases.taylor_green(n=n, nbox=nbox, mach=0.3, precision=abi.PREC_FP64)
+    case.meta["viscous"] = [(b, [mu * (1 + 0.3 * b)] + list(p[1:])) for b, p in case.meta["viscous"]]
+    return perturb(case, amp=0.05, seed=seed)
+
+
+def _inviscid(case):
+    c = copy.deepcopy(case)
+    c.meta.pop("viscous", None)
+    return c
+
+
+def _J(case, Rbar, solver_cls, P=None):
+    s = solver_cls(case)
+    try:
+        if P is not None:
+            for b, Pb in enumerate(P, start=1):
+                s.put_state(b, Pb)
+        return sum(float(np.sum(rb * s.residual(b))) for b, rb in enumerate(Rbar, start=1))
+    finally:
+        if hasattr(s, "close"):
+            s.close()
+
+
+def _state_check(case, solver_cls=GpuSolver, h=1e-7, tol=2e-6, seed=0):
+    rng = np.random.default_rng(seed)
+    dims = [b.dims for b in case.blocks]
+    Rbar = [rng.standard_normal(abi.padded_shape(d)) * _mask(d) for d in dims]
+    P0 = [b.P.copy() for b in case.blocks]
+    V = [rng.standard_normal(abi.padded_shape(d)) * _mask(d) * np.abs(P) for d, P in zip(dims, P0)]
+    inv = _inviscid(case)
+    gv, gi = GpuSolver(case), GpuSolver(inv)
+    dPv, dPi = gv.residual_vjp(Rbar), gi.residual_vjp(Rbar)
+    gv.close()
+    gi.close()
+    adj = sum(float(np.sum((a - b) * v)) for a, b, v in zip(dPv, dPi, V))
+
+    def Jd(t):
+        P = [p + t * v for p, v in zip(P0, V)]
+        return _J(case, Rbar, solver_cls, P) - _J(inv, Rbar, solver_cls, P)
+
+    fd = (Jd(h) - Jd(-h)) / (2 * h)
+    scale = sum(float(np.sum(np.abs((a - b) * v))) for a, b, v in zip(dPv, dPi, V))
+    assert scale > 0.0
+    assert abs(adj - fd) <= tol * scale, (adj, fd, scale)
+
+
+def test_viscous_state_adjoint_periodic_boxes():
+    _state_check(_tgv())
+
+
+def test_viscous_state_adjoint_against_oracle_residual():
+    _state_check(_tgv(n=5, nbox=1, seed=5), solver_cls=OracleSolver, seed=1)
+
+
+def test_viscous_state_adjoint_block_connections():
+    """One box split into two blocks along x (aligned 101 connections between them, 201
+    periodic across the ends): connection faces between different blocks."""
+    base = _tgv(n=6, nbox=1, seed=7)
+    prm = base.meta["viscous"][0][1]
+    case = cases.split_block_case(base, axis=0, parts=2)
+    case.meta["viscous"] = [(0, prm)]
+    _state_check(case, seed=2)
+
+
+@pytest.mark.parametrize("lim,rs", [("vanleer", "AUSM+"), ("minmod", "HLLC")])
+def test_viscous_adjoint_other_schemes(lim, rs):
+    case = _tgv(n=5, nbox=2, seed=9)
+    case.cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.5, lim, rs, 3, True, precision=abi.PREC_FP64)
+    _state_check(case, seed=4)
+
+
+def test_mu_ref_adjoint():
+    """d(Rbar . R)/d mu_ref per block (the Reynolds-number sensitivity: d/dRe = -mu_ref^2
+    d/dmu_ref) against central differences."""
+    case = _tgv(n=6, nbox=2, seed=11)
+    rng = np.random.default_rng(6)
+    Rbar = [rng.standard_normal(abi.padded_shape(b.dims)) * _mask(b.dims) for b in case.blocks]
+    g = GpuSolver(case)
+    g.residual_vjp(Rbar)
+    grads = [g.viscous_bar(b) for b in range(1, len(case.blocks) + 1)]
+    g.close()
+    for b in range(1, len(case.blocks) + 1):
+        h = 1e-6 * case.meta["viscous"][b - 1][1][0]
+
+        def with_mu(dmu):
+            c = copy.deepcopy(case)
+            c.meta["viscous"] = [(bb, [p[0] + (dmu if bb == b else 0.0)] + list(p[1:])) for bb, p in c.meta["viscous"]]
+            return c
+
+        fd = (_J(with_mu(h), Rbar, GpuSolver) - _J(with_mu(-h), Rbar, GpuSolver)) / (2 * h)
+        assert abs(grads[b - 1] - fd) <= 1e-6 * abs(fd) + 1e-10, (b, grads[b - 1], fd)
+
+
+def test_inviscid_block_has_zero_mu_ref_adjoint():
+    case = _tgv(n=5, nbox=2, seed=13)
+    case.meta["viscous"] = case.meta["viscous"][:1]        # block 2 inviscid
+    rng = np.random.default_rng(2)
+    Rbar = [rng.standard_normal(abi.padded_shape(b.dims)) * _mask(b.dims) for b in case.blocks]
+    g = GpuSolver(case)
+    g.residual_vjp(Rbar)
+    assert g.viscous_bar(2) == 0.0 and g.viscous_bar(1) != 0.0
+    g.close()
