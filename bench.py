@@ -14,7 +14,7 @@ against the advected exact vortex, the roofline of the dominant kernel (the fuse
 flux + RK-stage kernel) measured with HIP events on the library's stream, the 64-block
 batched case (seeded block sizes in [16,128], FP32 flux vs FP64), the discrete adjoint's
 cost relative to one residual, the batched viscous Taylor-Green case (8 x 192^3), and a CPU
-baseline (the C oracle, single thread, on a bounded sample of the same workload).
+baseline (the C oracle, OpenMP over lines of cells, on a bounded sample of the same workload).
 
 roofline.traffic is the HBM bytes per launch of the stage kernel read from
 profiles/k_stage_vortex256_traffic.txt (two rocprofv3 --pmc runs, FETCH_SIZE and
@@ -169,10 +169,11 @@ def taylor_green_batch(steps, warmup, n=192, nbox=8):
     return out
 
 
-def cpu_baseline(n=64, steps=20):
-    """The C oracle (single thread, no fast-math) on the same workload at n^3."""
+def cpu_baseline(n=96, steps=20):
+    """The C oracle (no fast-math; OpenMP over lines of cells like the reference's
+    collapse(2) loops, OMP_NUM_THREADS threads) on the same workload at n^3."""
     from mose_amd import cases
-    from oracle.oracle import OracleSolver
+    from oracle.oracle import OracleSolver, num_threads
     case = cases.vortex(n=n)
     o = OracleSolver(case)
     o.step()
@@ -180,9 +181,10 @@ def cpu_baseline(n=64, steps=20):
     for _ in range(steps):
         o.step()
     dt = time.perf_counter() - t0
-    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": "vortex %d^3, %d steps after 1 warm-up step, C oracle (gcc -O2, 1 thread) on %s"
-                      % (n, steps, platform.processor() or platform.machine())}
+    nt = num_threads()
+    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": nt, "kind": "port",
+            "sample": "vortex %d^3, %d steps after 1 warm-up step, C oracle (gcc -O2 -fopenmp, %d threads) on %s"
+                      % (n, steps, nt, platform.processor() or platform.machine())}
 
 
 def main():

@@ -24,6 +24,9 @@
  * SSP-RK coefficient order test; Sod against the reference's committed exact.txt.
  */
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 #include <stdio.h>
@@ -369,11 +372,27 @@ static void diffusive_flux(const ob_domain* d, const OBlk* B, const double* norm
 
 /* ---------------------------------- Fluxes_blk -------------------------------------
  * src/lib/numerics/fluxes/Mod_Fluxes.f90:46-209 (inviscid, no shock detector) */
+/* First failing face of a sweep in the serial loop order (the sweeps run their lines in
+ * parallel; each line touches only its own cells, so every cell accumulates its face fluxes
+ * in the serial order and the result is the serial one bit for bit). */
+typedef struct { int64_t key; int code, i, j, k; } first_err;
+static void first_err_note(first_err* fe, int64_t key, int code, int i, int j, int k) {
+#pragma omp critical(ob_first_err)
+  {
+    if (key < fe->key) { fe->key = key; fe->code = code; fe->i = i; fe->j = j; fe->k = k; }
+  }
+}
+static void first_err_flush(ob_domain* d, const first_err* fe, int b, int* err) {
+  if (fe->code && !*err) { *err = fe->code; set_err(d, fe->code, b + 1, fe->i, fe->j, fe->k); }
+}
+
 static int fluxes_blk(ob_domain* d, int b) {
   OBlk* B = &d->blk[b];
   memset(B->R, 0, sizeof(double) * NV * B->npad);
   int ni = B->ni, nj = B->nj, nk = B->nk, err = 0;
+  first_err fe = {INT64_MAX, 0, 0, 0, 0};
   /* i sweep, :89-99 */
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nk; ++k)
     for (int j = 1; j <= nj; ++j)
       for (int i = 1; i <= ni - 1; ++i) {
@@ -384,9 +403,11 @@ static int fluxes_blk(ob_domain* d, int b) {
                                 &B->P[NV * cidx(B, i - 1, j, k)], &B->P[NV * cidx(B, i, j, k)],
                                 &B->P[NV * cidx(B, i + 1, j, k)], &B->P[NV * cidx(B, i + 2, j, k)],
                                 &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i + 1, j, k)]);
-        if (e && !err) { err = e; set_err(d, e, b + 1, i, j, k); }
+        if (e) first_err_note(&fe, ((int64_t)k * (nj + 2) + j) * (ni + 2) + i, e, i, j, k);
       }
+  first_err_flush(d, &fe, b, &err);
   if (B->visc)   /* :101-127 */
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= nk; ++k)
       for (int j = 1; j <= nj; ++j)
         for (int i = 1; i <= ni - 1; ++i) {
@@ -401,6 +422,8 @@ static int fluxes_blk(ob_domain* d, int b) {
                          &B->R[NV * cidx(B, i + 1, j, k)], 0, 1, 2);
         }
   /* j sweep, :129-139 */
+  fe.key = INT64_MAX; fe.code = 0;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nk; ++k)
     for (int i = 1; i <= ni; ++i)
       for (int j = 1; j <= nj - 1; ++j) {
@@ -411,9 +434,11 @@ static int fluxes_blk(ob_domain* d, int b) {
                                 &B->P[NV * cidx(B, i, j - 1, k)], &B->P[NV * cidx(B, i, j, k)],
                                 &B->P[NV * cidx(B, i, j + 1, k)], &B->P[NV * cidx(B, i, j + 2, k)],
                                 &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i, j + 1, k)]);
-        if (e && !err) { err = e; set_err(d, e, b + 1, i, j, k); }
+        if (e) first_err_note(&fe, ((int64_t)k * (ni + 2) + i) * (nj + 2) + j, e, i, j, k);
       }
+  first_err_flush(d, &fe, b, &err);
   if (B->visc)   /* :141-167 */
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= nk; ++k)
       for (int i = 1; i <= ni; ++i)
         for (int j = 1; j <= nj - 1; ++j) {
@@ -428,6 +453,8 @@ static int fluxes_blk(ob_domain* d, int b) {
                          &B->R[NV * cidx(B, i, j + 1, k)], 1, 0, 2);
         }
   /* k sweep, :169-179 */
+  fe.key = INT64_MAX; fe.code = 0;
+#pragma omp parallel for collapse(2) schedule(static)
   for (int j = 1; j <= nj; ++j)
     for (int i = 1; i <= ni; ++i)
       for (int k = 1; k <= nk - 1; ++k) {
@@ -438,9 +465,11 @@ static int fluxes_blk(ob_domain* d, int b) {
                                 &B->P[NV * cidx(B, i, j, k - 1)], &B->P[NV * cidx(B, i, j, k)],
                                 &B->P[NV * cidx(B, i, j, k + 1)], &B->P[NV * cidx(B, i, j, k + 2)],
                                 &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i, j, k + 1)]);
-        if (e && !err) { err = e; set_err(d, e, b + 1, i, j, k); }
+        if (e) first_err_note(&fe, ((int64_t)j * (ni + 2) + i) * (nk + 2) + k, e, i, j, k);
       }
+  first_err_flush(d, &fe, b, &err);
   if (B->visc)   /* :181-207 */
+#pragma omp parallel for collapse(2) schedule(static)
     for (int j = 1; j <= nj; ++j)
       for (int i = 1; i <= ni; ++i)
         for (int k = 1; k <= nk - 1; ++k) {
@@ -996,6 +1025,7 @@ int ob_compute_dt(ob_domain* d, int32_t iter, double* dtglobal_out) {
   double dtglobal = 1e5;   /* Mod_Explicit.f90:44 */
   for (int b = 0; b < d->nb; ++b) {
     OBlk* B = &d->blk[b];
+#pragma omp parallel for collapse(2) schedule(static) reduction(min : dtglobal)
     for (int k = 1; k <= B->nk; ++k)
       for (int j = 1; j <= B->nj; ++j)
         for (int i = 1; i <= B->ni; ++i) {
@@ -1042,6 +1072,8 @@ static int newstate(ob_domain* d, int irk) {
   double c = RKcoeff[d->cfg.n_rk - 1][irk - 1];
   for (int b = 0; b < d->nb; ++b) {
     OBlk* B = &d->blk[b];
+    first_err fe = {INT64_MAX, 0, 0, 0, 0};
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = 1; k <= B->nk; ++k)
       for (int j = 1; j <= B->nj; ++j)
         for (int i = 1; i <= B->ni; ++i) {
@@ -1060,26 +1092,37 @@ static int newstate(ob_domain* d, int irk) {
           cons2prim_d(d, cons, P);
           int bad = 0;
           for (int v = 0; v < NV; ++v) if (isnan(P[v])) bad = 1;
-          if (bad || P[4] < 0.0) return set_err(d, MOSE_E_STATE, b + 1, i, j, k);
+          if (bad || P[4] < 0.0) {
+            first_err_note(&fe, ((int64_t)k * (B->nj + 2) + j) * (B->ni + 2) + i, MOSE_E_STATE, i, j, k);
+            continue;
+          }
           if (P[0] < 1e-20) P[0] = 1e-20;
         }
+    if (fe.code) return set_err(d, MOSE_E_STATE, b + 1, fe.i, fe.j, fe.k);
   }
   return 0;
 }
 
 /* Compute_Residual, src/lib/diagnostic/Mod_Diagnostic.f90:13-50 (+ sqrt, Mod_Explicit.f90:112) */
 static void compute_residual(ob_domain* d, double* norms) {
+  /* sums per k-plane (one thread each), then over the planes in order: the same value on
+   * any number of threads */
   double tot[NV] = {0, 0, 0, 0, 0};
   for (int b = 0; b < d->nb; ++b) {
     OBlk* B = &d->blk[b];
+    double* part = (double*)calloc((size_t)NV * (B->nk + 1), sizeof(double));
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= B->nk; ++k)
       for (int j = 1; j <= B->nj; ++j)
         for (int i = 1; i <= B->ni; ++i) {
           int64_t q = cidx(B, i, j, k);
           const double* n = &B->P[NV * q];
           const double* o = &B->PO[NV * q];
-          for (int v = 0; v < NV; ++v) { double r = fabs(n[v] - o[v]); tot[v] += r * r; }
+          for (int v = 0; v < NV; ++v) { double r = fabs(n[v] - o[v]); part[NV * k + v] += r * r; }
         }
+    for (int k = 1; k <= B->nk; ++k)
+      for (int v = 0; v < NV; ++v) tot[v] += part[NV * k + v];
+    free(part);
   }
   for (int v = 0; v < NV; ++v) norms[v] = sqrt(tot[v]);
 }
@@ -1092,6 +1135,7 @@ int ob_step(ob_domain* d, double* dt_out, double* time_out, double* norms) {
   if (d->cfg.time_accurate) d->time += d->dtglobal;
   for (int b = 0; b < d->nb; ++b) {          /* Copy_State (interior), :149-170 */
     OBlk* B = &d->blk[b];
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= B->nk; ++k)
       for (int j = 1; j <= B->nj; ++j)
         for (int i = 1; i <= B->ni; ++i) {
@@ -1210,6 +1254,13 @@ int ob_get_dtlocal(ob_domain* d, int32_t b, double* dt) {
 }
 int ob_last_error(ob_domain* d, mose_err* e) { *e = d->err; return d->err.code; }
 int ob_ndir(ob_domain* d) { return d->ndir; }
+int ob_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 void ob_destroy(ob_domain* d) {
   if (!d) return;
   for (int b = 0; b < d->nb; ++b) {

@@ -38,6 +38,7 @@ int  ob_get_metrics(ob_domain* d, int32_t b, double* fn, double* fa, double* dl,
 int  ob_get_dtlocal(ob_domain* d, int32_t b, double* dt);
 int  ob_last_error(ob_domain* d, mose_err* e);
 int  ob_ndir(ob_domain* d);
+int  ob_num_threads(void);   /* OpenMP threads the oracle runs its sweeps on */
 void ob_destroy(ob_domain* d);
 
 /* Point kernels, exposed for unit tests. */

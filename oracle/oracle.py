@@ -41,6 +41,7 @@ def lib():
         L.ob_get_dtlocal.argtypes = [d, C.c_int32, P(C.c_double)]
         L.ob_last_error.argtypes = [d, P(abi.MoseErr)]
         L.ob_ndir.argtypes = [d]
+        L.ob_num_threads.argtypes = []
         L.ob_destroy.argtypes = [d]
         L.ob_limiter.restype = C.c_double
         L.ob_limiter.argtypes = [C.c_int32, C.c_double, C.c_double]
@@ -182,3 +183,8 @@ def cons2prim(cfg, U):
     U = np.ascontiguousarray(U, dtype=np.float64); P = np.zeros(5)
     lib().ob_cons2prim(C.byref(cfg), _dp(U), _dp(P))
     return P
+
+
+def num_threads():
+    """OpenMP threads the oracle runs its sweeps on."""
+    return int(lib().ob_num_threads())

@@ -260,3 +260,42 @@ def test_viscous_time_step_closed_form():
     dt_conv = np.min(np.minimum(dx / a * cfl, dy / (np.abs(v) + a) * cfl))
     dt_vnn = 1.0 * min(dx, dy) ** 2 * 0.3 / 0.5
     assert abs(dt - min(dt_conv, dt_vnn)) <= 1e-14 * dt
+
+
+_THREAD_PROBE = r'''
+import hashlib, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from mose_amd import cases
+from oracle.oracle import OracleSolver
+from helpers import perturb
+h = hashlib.sha256()
+for case in (perturb(cases.multiblock_chain(nblocks=3, seed=2, lo=6, hi=12), seed=1),
+             perturb(cases.taylor_green(n=6, nbox=2, precision=0), seed=2),
+             perturb(cases.bump_channel(ni=16, nj=8), seed=3)):
+    o = OracleSolver(case)
+    for b in range(1, len(case.blocks) + 1):
+        h.update(o.residual(b).tobytes())
+    for _ in range(3):
+        o.step()
+    for b in range(1, len(case.blocks) + 1):
+        h.update(o.get_state(b).tobytes())
+print(h.hexdigest())
+'''
+
+
+def test_oracle_results_do_not_depend_on_thread_count():
+    """The oracle's OpenMP sweeps keep each cell's summation order: residuals and states after
+    three steps are bit-identical on 1 and 4 threads."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = []
+    for nt in ("1", "4"):
+        env = dict(os.environ, OMP_NUM_THREADS=nt, PYTHONPATH=os.path.join(root, "tests"))
+        r = subprocess.run([sys.executable, "-c", _THREAD_PROBE, root], env=env, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        out.append(r.stdout.strip().splitlines()[-1])
+    assert out[0] == out[1]
