@@ -9,7 +9,8 @@ with the state resident in HBM.  value = cell-updates/s = cells x steps / second
 
 Tile shapes are picked by the library's autotuner before the warm-up steps.
 
-Also reported on the same line: the FP32-flux variant of the same workload, the L2 error
+Also reported on the same line: the FP32-flux variant of the same workload, the same
+workload with the other Riemann solver (AUSM+, FP64), the L2 error
 against the advected exact vortex, the roofline of the dominant kernel (the fused
 flux + RK-stage kernel) measured with HIP events on the library's stream, the 64-block
 batched case (seeded block sizes in [16,128], FP32 flux vs FP64), the discrete adjoint's
@@ -201,6 +202,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-tgv", action="store_true")
     ap.add_argument("--tgv-steps", type=int, default=20)
+    ap.add_argument("--no-ausmp", action="store_true")
+    ap.add_argument("--ausmp-steps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; sharding blocks over several GPUs is not "
@@ -265,6 +268,17 @@ def main():
             "stage_frac": STAGE_BYTES_PER_CELL * ncell / (ms32[2] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "step_roofline_frac": STEP_BYTES_PER_CELL * cups32 / (HBM_PEAK_GBS * 1e9),
         }
+    if not args.no_ausmp:
+        # the same workload with the other Riemann solver of the case (van Leer + AUSM+), FP64
+        other = "AUSM+" if args.riemann == "HLLC" else "HLLC"
+        ca = cases.vortex(n=n, riemann=other, precision=abi.PREC_FP64)
+        ga, secsa, ta, msa = run_gpu(ca, args.ausmp_steps, min(args.warmup, 2), "fp64-" + other)
+        result["other_riemann"] = {
+            "riemann": other, "value": ncell * args.ausmp_steps / secsa, "steps": args.ausmp_steps,
+            "ms_per_step": 1e3 * secsa / args.ausmp_steps, "stage_kernel_ms": msa[2], "tiling": ga.tiling,
+            "stage_frac": STAGE_BYTES_PER_CELL * ncell / (msa[2] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "l2_rho_vs_exact": l2_vs_exact(ga, ca, ta)}
+        ga.close()
     if adj_vortex is not None:
         result["adjoint"] = {"vortex": adj_vortex, "bump_512x256": bump_adjoint(),
                              "note": "device ms of one mose_residual_vjp / one mose_residual, FP64; "
