@@ -67,7 +67,11 @@ typedef struct {
 
 /* One boundary-face-cell record of bc.txt (src/lib/io/IO_BC.f90:136-414).
  * type 101/201 use (bs,is,js,ks,fs,d11,d12,d21,d22); type 405 uses (mach,T,p,alpha,beta,rel_fac)
- * with alpha/beta in radians.  Supported types: 0, 101, 201, 300, 400, 405. */
+ * with alpha/beta in radians, or an angle >= DBL_MAX/2 for bc.txt's 'normal' token (the
+ * reference's huge() sentinel, parse_dir_tok IO_BC.f90:443-456): the angle then follows the
+ * boundary face normal (Compute_Inflow_Direction, Lib_BC_Fluxes_Inflow.f90:479-496).  All
+ * records of one block face share one type, and for 405 one set of values.
+ * Supported types: 0, 101, 201, 300, 400, 405. */
 typedef struct {
   int32_t b, i, j, k, f, type;
   int32_t bs, is, js, ks, fs;

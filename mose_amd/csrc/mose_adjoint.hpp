@@ -477,7 +477,9 @@ __device__ bool inflow405_adj(const double* Pb, const double* Pi, bool hi, doubl
   const double RT = g.R * bc[1];
   const double rho = p / RT;
   const double a = sqrt(g.gam * p / rho);
-  const double Sup[5] = {rho, M * a * bc[3], M * a * bc[4], M * a * bc[5], p};
+  double dir[3];
+  inflow_dir(bc, nx, ny, nz, dir);
+  const double Sup[5] = {rho, M * a * dir[0], M * a * dir[1], M * a * dir[2], p};
   double Fc[5], L[5], R[5], d32[5], d21[5];
   const double half = hi ? 0.5 : -0.5;
 #pragma unroll
@@ -504,16 +506,43 @@ __device__ bool inflow405_adj(const double* Pb, const double* Pi, bool hi, doubl
     else { Supb[v] -= yb; Pib[v] += xb; }
   }
   // Sup = (p / RT, M a dir, p) with a = sqrt(gam p / rho) = sqrt(gam R T)
-  const double Mb = a * (Supb[1] * bc[3] + Supb[2] * bc[4] + Supb[3] * bc[5]);
+  const double Mb = a * (Supb[1] * dir[0] + Supb[2] * dir[1] + Supb[3] * dir[2]);
   const double pb = Supb[4] + Supb[0] / RT;
   {
-    const double ab = M * (Supb[1] * bc[3] + Supb[2] * bc[4] + Supb[3] * bc[5]);
+    const double ab = M * (Supb[1] * dir[0] + Supb[2] * dir[1] + Supb[3] * dir[2]);
     bcb[0] += rel * Mb;
     bcb[1] += -Supb[0] * rho / bc[1] + ab * 0.5 * a / bc[1];
     bcb[2] += rel * pb;
-    bcb[3] += M * a * Supb[1];
-    bcb[4] += M * a * Supb[2];
-    bcb[5] += M * a * Supb[3];
+    const double db[3] = {M * a * Supb[1], M * a * Supb[2], M * a * Supb[3]};
+    const int flags = (int)bc[7];
+    if (flags == 0) {   // adjoints of the three direction components
+      bcb[3] += db[0];
+      bcb[4] += db[1];
+      bcb[5] += db[2];
+    } else {
+      // angles: explicit ones get their adjoint in bcb[3] (alpha) / bcb[4] (beta); 'normal'
+      // ones pass it to the face normal, alpha = atan(ny / (nx + 1e-20)), beta = atan(nz / ...)
+      const double al = (flags & 1) ? atan(ny / (nx + 1e-20)) : bc[3];
+      const double be = (flags & 2) ? atan(nz / (nx + 1e-20)) : bc[4];
+      const double ca = cos(al), sa = sin(al), cb = cos(be), sb = sin(be);
+      const double alb = db[0] * (-sa * cb) + db[1] * (ca * cb);
+      const double beb = db[0] * (-ca * sb) + db[1] * (-sa * sb) + db[2] * cb;
+      const double nx_ = nx + 1e-20;
+      if (flags & 1) {
+        const double q = 1.0 / (nx_ * nx_ + ny * ny);
+        nb[0] += alb * (-ny * q);
+        nb[1] += alb * (nx_ * q);
+      } else {
+        bcb[3] += alb;
+      }
+      if (flags & 2) {
+        const double q = 1.0 / (nx_ * nx_ + nz * nz);
+        nb[0] += beb * (-nz * q);
+        nb[2] += beb * (nx_ * q);
+      } else {
+        bcb[4] += beb;
+      }
+    }
   }
   if (rel != 1.0) {
     // Sup depends on the boundary cell through M (flow Mach) and p

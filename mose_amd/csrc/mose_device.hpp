@@ -275,6 +275,24 @@ __device__ __forceinline__ void symmetry_flux(const T* Pb, const T* Pi, T bdl, T
   F[0] = T(0); F[1] = Fr[1]; F[2] = Fr[2]; F[3] = Fr[3]; F[4] = T(0);
 }
 
+// Velocity direction of a 405 record: precomputed on the host, or, where bc.txt gives the
+// 'normal' token for an angle (flag bits in bc[7]: 1 alpha, 2 beta; the explicit angles then
+// sit in bc[3], bc[4]), from the boundary face normal (Compute_Inflow_Direction,
+// Lib_BC_Fluxes_Inflow.f90:479-496: alpha = atan(n_y / (n_x + 1e-20)), beta = atan(n_z / ...)).
+template <typename T>
+__device__ __forceinline__ void inflow_dir(const double* bc, T nx, T ny, T nz, T* dir) {
+  const int flags = (int)bc[7];
+  if (flags == 0) {
+    dir[0] = T(bc[3]); dir[1] = T(bc[4]); dir[2] = T(bc[5]);
+    return;
+  }
+  const double a = (flags & 1) ? atan((double)ny / ((double)nx + 1e-20)) : bc[3];
+  const double b = (flags & 2) ? atan((double)nz / ((double)nx + 1e-20)) : bc[4];
+  dir[0] = T(cos(a) * cos(b));
+  dir[1] = T(sin(a) * cos(b));
+  dir[2] = T(sin(b));
+}
+
 // Supersonic inflow face (BC_Inlet_Supersonic_Static + Supersonic_Inflow,
 // Lib_BC_Fluxes_Inflow.f90:244-382).  Returns false when the boundary-cell flow leaves the
 // domain (the caller then uses the symmetry flux).  bc = {mach, T, p, cos(alpha) cos(beta),
@@ -292,7 +310,9 @@ __device__ __forceinline__ bool inflow405_flux(const T* Pb, const T* Pi, bool hi
   const T p = rel * T(bc[2]) + (T(1) - rel) * Pb[4];
   const T rho = p / (g.R * T(bc[1]));
   const T a = sqrt_(g.gam * p / rho);
-  T Sup[5] = {rho, M * a * T(bc[3]), M * a * T(bc[4]), M * a * T(bc[5]), p};
+  T dir[3];
+  inflow_dir(bc, nx, ny, nz, dir);
+  T Sup[5] = {rho, M * a * dir[0], M * a * dir[1], M * a * dir[2], p};
   T Fc[5], L[5], R[5];
   const T half = hi ? T(0.5) : T(-0.5);
 #pragma unroll

@@ -19,6 +19,7 @@
 #pragma clang fp contract(off)
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -293,6 +294,12 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
     if (geo::face_kind(e.type) < 0) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
     if (ftype[slot] == -9999) { ftype[slot] = e.type; fentry[slot] = (int)q; }
     else if (ftype[slot] != e.type) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
+    if (e.type == 405) {   // one inflow record per block face: every cell of the face carries the same values
+      const mose_bc_entry& e0 = bc[fentry[slot]];
+      if (e.mach != e0.mach || e.T != e0.T || e.p != e0.p || e.alpha != e0.alpha || e.beta != e0.beta ||
+          e.rel_fac != e0.rel_fac)
+        return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
+    }
     if (e.type == 101 || e.type == 201) {
       if (e.bs < 1 || e.bs > nb || e.fs < 1 || e.fs > 6) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
       const HostBlock& S = hb[e.bs - 1];
@@ -459,9 +466,18 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
       if (ftype[6 * b + f] == 405) {
         const mose_bc_entry& e = bc[fentry[6 * b + f]];
         D.bcd[f] = (int)(L.facebc.size() / 8);
-        // Sup velocity direction of Supersonic_Inflow (Lib_BC_Fluxes_Inflow.f90:336-338)
-        const double v[8] = {e.mach, e.T, e.p, std::cos(e.alpha) * std::cos(e.beta),
-                             std::sin(e.alpha) * std::cos(e.beta), std::sin(e.beta), e.rel_fac, 0.0};
+        // Sup velocity direction of Supersonic_Inflow (Lib_BC_Fluxes_Inflow.f90:336-338); an
+        // angle at or above half the largest double is bc.txt's 'normal' token, resolved per
+        // face cell from its normal (flag bits in v[7], explicit angles in v[3], v[4])
+        const int flags = (e.alpha >= 0.5 * DBL_MAX ? 1 : 0) | (e.beta >= 0.5 * DBL_MAX ? 2 : 0);
+        double v[8] = {e.mach, e.T, e.p, std::cos(e.alpha) * std::cos(e.beta),
+                       std::sin(e.alpha) * std::cos(e.beta), std::sin(e.beta), e.rel_fac, 0.0};
+        if (flags) {
+          v[3] = (flags & 1) ? 0.0 : e.alpha;
+          v[4] = (flags & 2) ? 0.0 : e.beta;
+          v[5] = 0.0;
+          v[7] = (double)flags;
+        }
         L.facebc.insert(L.facebc.end(), v, v + 8);
       }
     }

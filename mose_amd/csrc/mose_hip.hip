@@ -1167,14 +1167,21 @@ int mose_get_inflow_bar(mose_ctx* c, int32_t b, int32_t f, double out[5]) {
   double s[6] = {0, 0, 0, 0, 0, 0};
   for (int r = 0; r < 6; ++r)
     for (int64_t q = 0; q < cnt; ++q) s[r] += ib[(size_t)r * cnt + q];
-  // direction (cos a cos b, sin a cos b, sin b) of the record's angles
-  const double* v = c->lay.facebc.data() + 8 * B.bcd[f - 1];
-  const double ca = v[3], sa = v[4];   // cos a cos b, sin a cos b
-  const double cb = std::sqrt(ca * ca + sa * sa), sb = v[5];
-  const double cosa = cb > 0.0 ? ca / cb : 1.0, sina = cb > 0.0 ? sa / cb : 0.0;
   out[0] = s[0];
   out[1] = s[1];
   out[2] = s[2];
+  const double* v = c->lay.facebc.data() + 8 * B.bcd[f - 1];
+  if ((int)v[7] != 0) {
+    // angle adjoints accumulated by the kernel; a 'normal' angle has none (it follows the
+    // face normal, whose adjoint goes to the nodes)
+    out[3] = s[3];
+    out[4] = s[4];
+    return 0;
+  }
+  // direction (cos a cos b, sin a cos b, sin b) of the record's angles
+  const double ca = v[3], sa = v[4];   // cos a cos b, sin a cos b
+  const double cb = std::sqrt(ca * ca + sa * sa), sb = v[5];
+  const double cosa = cb > 0.0 ? ca / cb : 1.0, sina = cb > 0.0 ? sa / cb : 0.0;
   out[3] = s[3] * (-sina * cb) + s[4] * (cosa * cb);                        // d/d alpha
   out[4] = s[3] * (-cosa * sb) + s[4] * (-sina * sb) + s[5] * cb;            // d/d beta
   return 0;

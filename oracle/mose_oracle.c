@@ -23,6 +23,7 @@
  * multi-block grids; block-decomposition invariance; metric exactness on affine grids;
  * SSP-RK coefficient order test; Sod against the reference's committed exact.txt.
  */
+#include <float.h>
 #include <math.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -685,7 +686,10 @@ static int bc_inflow405(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm, c
   double flow_Mach = sqrt(P2[1] * P2[1] + P2[2] * P2[2] + P2[3] * P2[3]) / flow_Sound;
   double Mach_l = e->rel_fac * e->mach + (1.0 - e->rel_fac) * flow_Mach;
   double p_l = e->rel_fac * e->p + (1.0 - e->rel_fac) * P2[4];
-  double alpha = e->alpha, beta = e->beta;   /* explicit angles (Compute_Inflow_Direction :479-496) */
+  /* Compute_Inflow_Direction (Lib_BC_Fluxes_Inflow.f90:479-496): an angle at or above half
+   * the largest double is the 'normal' token of bc.txt and follows the face normal */
+  double alpha = e->alpha >= 0.5 * DBL_MAX ? atan(n[1] / (n[0] + 1e-20)) : e->alpha;
+  double beta = e->beta >= 0.5 * DBL_MAX ? atan(n[2] / (n[0] + 1e-20)) : e->beta;
   double Sup[NV];
   Sup[4] = p_l;
   double Sup_rho = Sup[4] / (d->Rg * e->T);
