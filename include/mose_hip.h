@@ -35,8 +35,9 @@ extern "C" {
 
 /* Limiters: rlimiter_MINMOD/VANLEER/VANALBADA/IORD, src/lib/numerics/space/Lib_Limiters.f90:24-75 */
 enum { MOSE_LIM_MINMOD = 0, MOSE_LIM_VANLEER = 1, MOSE_LIM_VANALBADA = 2, MOSE_LIM_FIRST_ORDER = 3 };
-/* Riemann solvers: riemann_HLLC (Lib_Riemann_HLL.f90:147-258), riemann_AUSMp (Lib_Riemann_AUSM.f90:150-238) */
-enum { MOSE_RS_HLLC = 0, MOSE_RS_AUSMP = 1 };
+/* Riemann solvers: riemann_HLLC (Lib_Riemann_HLL.f90:147-258), riemann_AUSMp
+ * (Lib_Riemann_AUSM.f90:150-238), riemann_HLLE (Lib_Riemann_HLL.f90:261-320) */
+enum { MOSE_RS_HLLC = 0, MOSE_RS_AUSMP = 1, MOSE_RS_HLLE = 2 };
 /* Arithmetic of reconstruction + Riemann flux; state, accumulation and reductions are always FP64. */
 enum { MOSE_PREC_FP64 = 0, MOSE_PREC_FP32_FLUX = 1 };
 

@@ -57,6 +57,94 @@ __device__ __forceinline__ void cell_states_adj(const double* Pm, const double* 
   }
 }
 
+// Adjoints of one side's quantities of the HLL-family front end.
+struct SideBar { double r, u, v, w, p, un, h0, ir; };
+
+// Reverse of the front end HLLC and HLLE share: a = sqrt(gam p/rho), un = u.n,
+// h0 = gam/(gam-1) p/rho + |u|^2/2, the Roe averages and Batten's speeds
+// S1 = min(0, unl - al, un_roe - a_roe), S4 = max(0, unr + ar, un_roe + a_roe).
+// l / r carry the adjoints already gathered on each side's (rho, u, v, w, p, un, h0, 1/rho);
+// the result is added to Lb, Rb and nb.
+__device__ __forceinline__ void roe_batten_adj(const double* L, const double* R, double nx, double ny, double nz,
+                                               const GasT<double>& g, double S1b, double S4b, SideBar l, SideBar r,
+                                               double* Lb, double* Rb, double* nb) {
+  const double rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
+  const double rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
+  const double irl = 1.0 / rl, irr = 1.0 / rr;
+  const double al = sqrt(g.gam * pl * irl), ar = sqrt(g.gam * pr * irr);
+  const double unl = ul * nx + vl * ny + wl * nz, unr = ur * nx + vr * ny + wr * nz;
+  const double kg = g.gam * g.rgm1;
+  const double h0l = kg * pl * irl + 0.5 * (ul * ul + vl * vl + wl * wl);
+  const double h0r = kg * pr * irr + 0.5 * (ur * ur + vr * vr + wr * wr);
+  const double srl = sqrt(rl), srr = sqrt(rr), isr = 1.0 / (srl + srr);
+  const double u_roe = (srr * ur + srl * ul) * isr, v_roe = (srr * vr + srl * vl) * isr, w_roe = (srr * wr + srl * wl) * isr;
+  const double h0_roe = (srr * h0r + srl * h0l) * isr;
+  const double a_roe = sqrt(g.gm1 * (h0_roe - 0.5 * (u_roe * u_roe + v_roe * v_roe + w_roe * w_roe)));
+  const double un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+  const double x1 = unl - al, y1 = un_roe - a_roe, m1 = min_(x1, y1);
+  const double x4 = unr + ar, y4 = un_roe + a_roe, m4 = max_(x4, y4);
+  double rlb = l.r, ulb = l.u, vlb = l.v, wlb = l.w, plb = l.p, unlb = l.un, h0lb = l.h0, irlb = l.ir;
+  double rrb = r.r, urb = r.u, vrb = r.v, wrb = r.w, prb = r.p, unrb = r.un, h0rb = r.h0, irrb = r.ir;
+  // S1 = min_(0, min_(x1, y1)), S4 = max_(0, max_(x4, y4)): the selected argument gets the adjoint
+  double alb = 0, arb = 0, un_roe_b = 0, a_roe_b = 0;
+  if (!(0.0 < m1)) {
+    if (x1 < y1) { unlb += S1b; alb -= S1b; }
+    else { un_roe_b += S1b; a_roe_b -= S1b; }
+  }
+  if (!(0.0 > m4)) {
+    if (x4 > y4) { unrb += S4b; arb += S4b; }
+    else { un_roe_b += S4b; a_roe_b += S4b; }
+  }
+  // Roe averages
+  double u_roe_b = un_roe_b * nx, v_roe_b = un_roe_b * ny, w_roe_b = un_roe_b * nz;
+  nb[0] += un_roe_b * u_roe;
+  nb[1] += un_roe_b * v_roe;
+  nb[2] += un_roe_b * w_roe;
+  double srrb = 0, srlb = 0, isrb = 0;
+  if (a_roe > 0.0) {      // a_roe = 0: only the un_roe path (through u_roe) carries sensitivity
+    const double hb = a_roe_b * 0.5 / a_roe * g.gm1;
+    u_roe_b -= hb * u_roe;
+    v_roe_b -= hb * v_roe;
+    w_roe_b -= hb * w_roe;
+    h0rb += hb * srr * isr;
+    h0lb += hb * srl * isr;
+    srrb += hb * h0r * isr;
+    srlb += hb * h0l * isr;
+    isrb += hb * (srr * h0r + srl * h0l);
+  }
+  srrb += (u_roe_b * ur + v_roe_b * vr + w_roe_b * wr) * isr;
+  srlb += (u_roe_b * ul + v_roe_b * vl + w_roe_b * wl) * isr;
+  urb += u_roe_b * srr * isr; vrb += v_roe_b * srr * isr; wrb += w_roe_b * srr * isr;
+  ulb += u_roe_b * srl * isr; vlb += v_roe_b * srl * isr; wlb += w_roe_b * srl * isr;
+  isrb += u_roe_b * (srr * ur + srl * ul) + v_roe_b * (srr * vr + srl * vl) + w_roe_b * (srr * wr + srl * wl);
+  const double sb = -isr * isr * isrb;
+  srlb += sb;
+  srrb += sb;
+  rlb += srlb * 0.5 / srl;
+  rrb += srrb * 0.5 / srr;
+  // h0 = kg p / rho + |u|^2 / 2
+  plb += h0lb * kg * irl; irlb += h0lb * kg * pl;
+  ulb += h0lb * ul; vlb += h0lb * vl; wlb += h0lb * wl;
+  prb += h0rb * kg * irr; irrb += h0rb * kg * pr;
+  urb += h0rb * ur; vrb += h0rb * vr; wrb += h0rb * wr;
+  // un = u . n
+  ulb += unlb * nx; vlb += unlb * ny; wlb += unlb * nz;
+  urb += unrb * nx; vrb += unrb * ny; wrb += unrb * nz;
+  nb[0] += unlb * ul + unrb * ur;
+  nb[1] += unlb * vl + unrb * vr;
+  nb[2] += unlb * wl + unrb * wr;
+  // a = sqrt(gam p / rho)
+  {
+    const double tl = alb * 0.5 / al * g.gam, tr = arb * 0.5 / ar * g.gam;
+    plb += tl * irl; irlb += tl * pl;
+    prb += tr * irr; irrb += tr * pr;
+  }
+  rlb -= irlb * irl * irl;
+  rrb -= irrb * irr * irr;
+  Lb[0] += rlb; Lb[1] += ulb; Lb[2] += vlb; Lb[3] += wlb; Lb[4] += plb;
+  Rb[0] += rrb; Rb[1] += urb; Rb[2] += vrb; Rb[3] += wrb; Rb[4] += prb;
+}
+
 // ------------------------------------------------------------------------------ HLLC
 __device__ void hllc_adj(const double* L, const double* R, double nx, double ny, double nz, const GasT<double>& g,
                          const double* Fb, double* Lb, double* Rb, double* nb) {
@@ -179,81 +267,75 @@ __device__ void hllc_adj(const double* L, const double* R, double nx, double ny,
   rrb += drb * (S4 - unr);
   S4b += drb * rr;
   unrb -= drb * rr;
-  // S1 = min_(0, min_(x1, y1)), S4 = max_(0, max_(x4, y4)): the selected argument gets the adjoint
-  double alb = 0, arb = 0, un_roe_b = 0, a_roe_b = 0;
-  if (!(0.0 < m1)) {
-    if (x1 < y1) { unlb += S1b; alb -= S1b; }
-    else { un_roe_b += S1b; a_roe_b -= S1b; }
-  }
-  if (!(0.0 > m4)) {
-    if (x4 > y4) { unrb += S4b; arb += S4b; }
-    else { un_roe_b += S4b; a_roe_b += S4b; }
-  }
   // e0 = h0 - p / rho
-  double irlb = 0, irrb = 0;
-  h0lb += e0lb; plb -= e0lb * irl; irlb -= e0lb * pl;
-  h0rb += e0rb; prb -= e0rb * irr; irrb -= e0rb * pr;
-  // Roe averages
-  double u_roe_b = un_roe_b * nx, v_roe_b = un_roe_b * ny, w_roe_b = un_roe_b * nz;
-  nb[0] += un_roe_b * u_roe;
-  nb[1] += un_roe_b * v_roe;
-  nb[2] += un_roe_b * w_roe;
-  if (a_roe > 0.0) {
-    const double tb = a_roe_b * 0.5 / a_roe * g.gm1;
-    const double h0_roe_b0 = tb;
-    u_roe_b -= tb * u_roe;
-    v_roe_b -= tb * v_roe;
-    w_roe_b -= tb * w_roe;
-    // h0_roe
-    const double hb = h0_roe_b0;
-    h0rb += hb * srr * isr;
-    h0lb += hb * srl * isr;
-    double srrb = hb * h0r * isr, srlb = hb * h0l * isr;
-    double isrb = hb * (srr * h0r + srl * h0l);
-    srrb += (u_roe_b * ur + v_roe_b * vr + w_roe_b * wr) * isr;
-    srlb += (u_roe_b * ul + v_roe_b * vl + w_roe_b * wl) * isr;
-    urb += u_roe_b * srr * isr; vrb += v_roe_b * srr * isr; wrb += w_roe_b * srr * isr;
-    ulb += u_roe_b * srl * isr; vlb += v_roe_b * srl * isr; wlb += w_roe_b * srl * isr;
-    isrb += u_roe_b * (srr * ur + srl * ul) + v_roe_b * (srr * vr + srl * vl) + w_roe_b * (srr * wr + srl * wl);
-    const double sb = -isr * isr * isrb;
-    srlb += sb;
-    srrb += sb;
-    rlb += srlb * 0.5 / srl;
-    rrb += srrb * 0.5 / srr;
-  } else {
-    // a_roe = 0: only the un_roe path (through u_roe) carries sensitivity
-    urb += u_roe_b * srr * isr; vrb += v_roe_b * srr * isr; wrb += w_roe_b * srr * isr;
-    ulb += u_roe_b * srl * isr; vlb += v_roe_b * srl * isr; wlb += w_roe_b * srl * isr;
-    double srrb = (u_roe_b * ur + v_roe_b * vr + w_roe_b * wr) * isr;
-    double srlb = (u_roe_b * ul + v_roe_b * vl + w_roe_b * wl) * isr;
-    const double isrb = u_roe_b * (srr * ur + srl * ul) + v_roe_b * (srr * vr + srl * vl) + w_roe_b * (srr * wr + srl * wl);
-    const double sb = -isr * isr * isrb;
-    srlb += sb;
-    srrb += sb;
-    rlb += srlb * 0.5 / srl;
-    rrb += srrb * 0.5 / srr;
-  }
-  // h0 = kg p / rho + |u|^2 / 2
-  plb += h0lb * kg * irl; irlb += h0lb * kg * pl;
-  ulb += h0lb * ul; vlb += h0lb * vl; wlb += h0lb * wl;
-  prb += h0rb * kg * irr; irrb += h0rb * kg * pr;
-  urb += h0rb * ur; vrb += h0rb * vr; wrb += h0rb * wr;
-  // un = u . n
-  ulb += unlb * nx; vlb += unlb * ny; wlb += unlb * nz;
-  urb += unrb * nx; vrb += unrb * ny; wrb += unrb * nz;
-  nb[0] += unlb * ul + unrb * ur;
-  nb[1] += unlb * vl + unrb * vr;
-  nb[2] += unlb * wl + unrb * wr;
-  // a = sqrt(gam p / rho)
+  h0lb += e0lb; plb -= e0lb * irl;
+  h0rb += e0rb; prb -= e0rb * irr;
+  const SideBar lb{rlb, ulb, vlb, wlb, plb, unlb, h0lb, -e0lb * pl};
+  const SideBar rb{rrb, urb, vrb, wrb, prb, unrb, h0rb, -e0rb * pr};
+  roe_batten_adj(L, R, nx, ny, nz, g, S1b, S4b, lb, rb, Lb, Rb, nb);
+}
+
+// HLLE (hlle in mose_device.hpp): F = a F_L + b F_R + c (U_R - U_L) with
+// a = S4/(S4-S1), b = -S1/(S4-S1), c = S1 S4/(S4-S1).
+__device__ void hlle_adj(const double* L, const double* R, double nx, double ny, double nz, const GasT<double>& g,
+                         const double* Fb, double* Lb, double* Rb, double* nb) {
+  const double rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
+  const double rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
+  // ---- forward
+  const double irl = 1.0 / rl, irr = 1.0 / rr;
+  const double al = sqrt(g.gam * pl * irl), ar = sqrt(g.gam * pr * irr);
+  const double unl = ul * nx + vl * ny + wl * nz, unr = ur * nx + vr * ny + wr * nz;
+  const double kg = g.gam * g.rgm1;
+  const double h0l = kg * pl * irl + 0.5 * (ul * ul + vl * vl + wl * wl);
+  const double h0r = kg * pr * irr + 0.5 * (ur * ur + vr * vr + wr * wr);
+  const double srl = sqrt(rl), srr = sqrt(rr), isr = 1.0 / (srl + srr);
+  const double u_roe = (srr * ur + srl * ul) * isr, v_roe = (srr * vr + srl * vl) * isr, w_roe = (srr * wr + srl * wl) * isr;
+  const double h0_roe = (srr * h0r + srl * h0l) * isr;
+  const double a_roe = sqrt(g.gm1 * (h0_roe - 0.5 * (u_roe * u_roe + v_roe * v_roe + w_roe * w_roe)));
+  const double un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+  const double S1 = min_(0.0, min_(unl - al, un_roe - a_roe));
+  const double S4 = max_(0.0, max_(unr + ar, un_roe + a_roe));
+  const double inv = 1.0 / (S4 - S1);
+  const double FL = rl * unl, FR = rr * unr;
+  const double fl[5] = {FL, FL * ul + pl * nx, FL * vl + pl * ny, FL * wl + pl * nz, FL * h0l};
+  const double fr[5] = {FR, FR * ur + pr * nx, FR * vr + pr * ny, FR * wr + pr * nz, FR * h0r};
+  const double du[5] = {rr - rl, rr * ur - rl * ul, rr * vr - rl * vl, rr * wr - rl * wl,
+                        rr * h0r - pr - (rl * h0l - pl)};
+  // ---- reverse: coefficients -> S1, S4
+  double a_b = 0, b_b = 0, c_b = 0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) { a_b += Fb[k] * fl[k]; b_b += Fb[k] * fr[k]; c_b += Fb[k] * du[k]; }
+  const double inv_b = a_b * S4 - b_b * S1 + c_b * S1 * S4;
+  const double q_b = -inv * inv * inv_b;
+  double S4b = a_b * inv + c_b * S1 * inv + q_b;
+  double S1b = -b_b * inv + c_b * S4 * inv - q_b;
+  // physical fluxes, weights a Fb (left) and b Fb (right); jumps, weight c Fb
+  const double A = S4 * inv, B = -S1 * inv, C = S1 * S4 * inv;
+  SideBar lb{0, 0, 0, 0, 0, 0, 0, 0}, rb{0, 0, 0, 0, 0, 0, 0, 0};
   {
-    const double tl = alb * 0.5 / al * g.gam, tr = arb * 0.5 / ar * g.gam;
-    plb += tl * irl; irlb += tl * pl;
-    prb += tr * irr; irrb += tr * pr;
+    const double w0 = A * Fb[0], w1 = A * Fb[1], w2 = A * Fb[2], w3 = A * Fb[3], w4 = A * Fb[4];
+    const double Fr_b = w0 + w1 * ul + w2 * vl + w3 * wl + w4 * h0l;
+    lb.u += w1 * FL; lb.v += w2 * FL; lb.w += w3 * FL; lb.h0 += w4 * FL;
+    lb.p += w1 * nx + w2 * ny + w3 * nz;
+    nb[0] += w1 * pl; nb[1] += w2 * pl; nb[2] += w3 * pl;
+    lb.r += Fr_b * unl; lb.un += Fr_b * rl;
   }
-  rlb -= irlb * irl * irl;
-  rrb -= irrb * irr * irr;
-  Lb[0] += rlb; Lb[1] += ulb; Lb[2] += vlb; Lb[3] += wlb; Lb[4] += plb;
-  Rb[0] += rrb; Rb[1] += urb; Rb[2] += vrb; Rb[3] += wrb; Rb[4] += prb;
+  {
+    const double w0 = B * Fb[0], w1 = B * Fb[1], w2 = B * Fb[2], w3 = B * Fb[3], w4 = B * Fb[4];
+    const double Fr_b = w0 + w1 * ur + w2 * vr + w3 * wr + w4 * h0r;
+    rb.u += w1 * FR; rb.v += w2 * FR; rb.w += w3 * FR; rb.h0 += w4 * FR;
+    rb.p += w1 * nx + w2 * ny + w3 * nz;
+    nb[0] += w1 * pr; nb[1] += w2 * pr; nb[2] += w3 * pr;
+    rb.r += Fr_b * unr; rb.un += Fr_b * rr;
+  }
+  {
+    const double w0 = C * Fb[0], w1 = C * Fb[1], w2 = C * Fb[2], w3 = C * Fb[3], w4 = C * Fb[4];
+    rb.r += w0 + w1 * ur + w2 * vr + w3 * wr + w4 * h0r;
+    rb.u += w1 * rr; rb.v += w2 * rr; rb.w += w3 * rr; rb.h0 += w4 * rr; rb.p -= w4;
+    lb.r -= w0 + w1 * ul + w2 * vl + w3 * wl + w4 * h0l;
+    lb.u -= w1 * rl; lb.v -= w2 * rl; lb.w -= w3 * rl; lb.h0 -= w4 * rl; lb.p += w4;
+  }
+  roe_batten_adj(L, R, nx, ny, nz, g, S1b, S4b, lb, rb, Lb, Rb, nb);
 }
 
 // ------------------------------------------------------------------------------ AUSM+
@@ -351,6 +433,7 @@ __device__ __forceinline__ void riemann_adj(const double* L, const double* R, do
                                             double* nb = nullptr) {
   double nd[3] = {0.0, 0.0, 0.0};
   if constexpr (RS == RS_AUSMP) ausmp_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb, nd);
+  else if constexpr (RS == RS_HLLE) hlle_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb, nd);
   else hllc_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb, nd);
   if (nb) { nb[0] += nd[0]; nb[1] += nd[1]; nb[2] += nd[2]; }
 }

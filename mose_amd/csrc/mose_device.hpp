@@ -19,7 +19,7 @@
 namespace mose {
 
 enum { LIM_MINMOD = 0, LIM_VANLEER = 1, LIM_VANALBADA = 2, LIM_FIRST = 3 };
-enum { RS_HLLC = 0, RS_AUSMP = 1 };
+enum { RS_HLLC = 0, RS_AUSMP = 1, RS_HLLE = 2 };
 
 template <typename T>
 struct GasT {
@@ -210,9 +210,43 @@ __device__ __forceinline__ void ausmp(const T* L, const T* R, T nx, T ny, T nz, 
   F[4] = mdotp * H_L + mdotm * H_R;
 }
 
+// HLLE: the two-wave HLL flux with Batten's (Einfeldt's) speed estimates, riemann_HLLE
+// (Lib_Riemann_HLL.f90:261-320):  F = (S4 F_L - S1 F_R + S1 S4 (U_R - U_L)) / (S4 - S1).
+template <typename T>
+__device__ __forceinline__ void hlle(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F) {
+  const T rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
+  const T rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
+  const T irl = rcp_(rl), irr = rcp_(rr);
+  const T al = sqrtp_(g.gam * pl * irl), ar = sqrtp_(g.gam * pr * irr);
+  const T unl = ul * nx + vl * ny + wl * nz;
+  const T unr = ur * nx + vr * ny + wr * nz;
+  const T kgl = g.gam * g.rgm1;
+  const T h0l = kgl * pl * irl + T(0.5) * (ul * ul + vl * vl + wl * wl);
+  const T h0r = kgl * pr * irr + T(0.5) * (ur * ur + vr * vr + wr * wr);
+  const T srl = sqrtp_(rl), srr = sqrtp_(rr);
+  const T isr = rcp_(srl + srr);
+  const T u_roe = (srr * ur + srl * ul) * isr;
+  const T v_roe = (srr * vr + srl * vl) * isr;
+  const T w_roe = (srr * wr + srl * wl) * isr;
+  const T h0_roe = (srr * h0r + srl * h0l) * isr;
+  const T a_roe = sqrtp_(g.gm1 * (h0_roe - T(0.5) * (u_roe * u_roe + v_roe * v_roe + w_roe * w_roe)));
+  const T un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+  const T S1 = min_(T(0), min_(unl - al, un_roe - a_roe));
+  const T S4 = max_(T(0), max_(unr + ar, un_roe + a_roe));
+  const T inv = rcp_(S4 - S1);
+  const T a = S4 * inv, b = -S1 * inv, c = S1 * S4 * inv;
+  const T FL = rl * unl, FR = rr * unr;
+  F[0] = a * FL + b * FR + c * (rr - rl);
+  F[1] = a * (FL * ul + pl * nx) + b * (FR * ur + pr * nx) + c * (rr * ur - rl * ul);
+  F[2] = a * (FL * vl + pl * ny) + b * (FR * vr + pr * ny) + c * (rr * vr - rl * vl);
+  F[3] = a * (FL * wl + pl * nz) + b * (FR * wr + pr * nz) + c * (rr * wr - rl * wl);
+  F[4] = a * (FL * h0l) + b * (FR * h0r) + c * (rr * h0r - pr - (rl * h0l - pl));
+}
+
 template <int RS, typename T>
 __device__ __forceinline__ void riemann(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F) {
   if constexpr (RS == RS_AUSMP) ausmp(L, R, nx, ny, nz, g, F);
+  else if constexpr (RS == RS_HLLE) hlle(L, R, nx, ny, nz, g, F);
   else hllc(L, R, nx, ny, nz, g, F);
 }
 

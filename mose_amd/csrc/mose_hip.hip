@@ -284,14 +284,16 @@ template <int LIM, typename FT>
 void bc_table(BcFn* t) {
   t[0] = k_bcface<LIM, RS_HLLC, FT>;
   t[1] = k_bcface<LIM, RS_AUSMP, FT>;
+  t[2] = k_bcface<LIM, RS_HLLE, FT>;
 }
 
 // kernel table [prec][lim][rs][shape][mode]; null where a shape is not compiled.
 // g: the general-metrics kernels (shape 0 only).
+constexpr int NRS = 3;   // HLLC, AUSM+, HLLE
 struct KTable {
-  StageFn f[2][4][2][NSHAPE][3] = {};
-  StageFn g[2][2][4][2][3] = {};     // [geo - 1][prec][lim][rs][mode]
-  BcFn bc[2][4][2] = {};
+  StageFn f[2][4][NRS][NSHAPE][3] = {};
+  StageFn g[2][2][4][NRS][3] = {};     // [geo - 1][prec][lim][rs][mode]
+  BcFn bc[2][4][NRS] = {};
   template <int LIM, int RS>
   void all_shapes() {
     stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1]>(f[0][LIM][RS][0]);
@@ -318,6 +320,10 @@ struct KTable {
     all_shapes<LIM_VANALBADA, RS_AUSMP>();
     all_shapes<LIM_FIRST, RS_HLLC>();
     all_shapes<LIM_FIRST, RS_AUSMP>();
+    all_shapes<LIM_MINMOD, RS_HLLE>();
+    all_shapes<LIM_VANLEER, RS_HLLE>();
+    all_shapes<LIM_VANALBADA, RS_HLLE>();
+    all_shapes<LIM_FIRST, RS_HLLE>();
     stage_modes<LIM_VANLEER, RS_HLLC, double, kShape[0][1][0], kShape[0][1][1]>(f[0][1][0][1]);
     stage_modes<LIM_VANLEER, RS_HLLC, double, kShape[0][2][0], kShape[0][2][1]>(f[0][1][0][2]);
     stage_modes<LIM_VANLEER, RS_AUSMP, double, kShape[0][1][0], kShape[0][1][1]>(f[0][1][1][1]);
@@ -336,7 +342,7 @@ const KTable& ktable() {
 typedef void (*AdjFn)(AdjArgs);
 // adjoint kernels [lim][rs][geo] (FP64)
 struct ATable {
-  AdjFn face[4][2][2] = {}, bc[4][2] = {}, cell[4][2] = {};
+  AdjFn face[4][NRS][2] = {}, bc[4][NRS] = {}, cell[4][2] = {};
   template <int LIM>
   void lim() {
     face[LIM][0][0] = k_adj_face<LIM, RS_HLLC, 0>;
@@ -345,6 +351,9 @@ struct ATable {
     face[LIM][1][1] = k_adj_face<LIM, RS_AUSMP, 1>;
     bc[LIM][0] = k_adj_bcface<LIM, RS_HLLC>;
     bc[LIM][1] = k_adj_bcface<LIM, RS_AUSMP>;
+    face[LIM][2][0] = k_adj_face<LIM, RS_HLLE, 0>;
+    face[LIM][2][1] = k_adj_face<LIM, RS_HLLE, 1>;
+    bc[LIM][2] = k_adj_bcface<LIM, RS_HLLE>;
     cell[LIM][0] = k_adj_cell<LIM, 0>;
     cell[LIM][1] = k_adj_cell<LIM, 1>;
   }
@@ -789,7 +798,7 @@ extern "C" {
 int mose_create(const mose_cfg* cfg, mose_ctx** out) {
   if (!cfg || !out) return MOSE_E_ARG;
   *out = nullptr;
-  if (cfg->limiter < 0 || cfg->limiter > 3 || cfg->riemann < 0 || cfg->riemann > 1 || cfg->n_rk < 1 ||
+  if (cfg->limiter < 0 || cfg->limiter > 3 || cfg->riemann < 0 || cfg->riemann > 2 || cfg->n_rk < 1 ||
       cfg->n_rk > 3 || !(cfg->gamma > 1.0) || !(cfg->Rgas > 0.0) || cfg->precision < 0 || cfg->precision > 1)
     return MOSE_E_ARG;
   int ndev = 0;

@@ -215,6 +215,29 @@ static void riemann_hllc(const ob_domain* d, double dl, double ul, double vl, do
   F[0] = Fr; F[1] = Fu; F[2] = Fv; F[3] = Fw; F[4] = FE;
 }
 
+/* riemann_HLLE: HLL flux with the Batten signal speeds of HLLC, Lib_Riemann_HLL.f90:261-320 */
+static void riemann_hlle(const ob_domain* d, double dl, double ul, double vl, double wl, double pl,
+                         double al, double dltot, double dr, double ur, double vr, double wr,
+                         double pr, double ar, double drtot, double nx, double ny, double nz,
+                         double* F) {
+  double rho_roe, u_roe, v_roe, w_roe, a_roe, h0_roe, h0l, h0r;
+  double unl = ul * nx + vl * ny + wl * nz;
+  double unr = ur * nx + vr * ny + wr * nz;
+  roe_averages(d, dl, dr, dltot, drtot, ul, vl, wl, ur, vr, wr, pl, pr, &h0l, &h0r,
+               &rho_roe, &u_roe, &v_roe, &w_roe, &a_roe, &h0_roe);
+  double un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+  double S1 = fmin(0.0, fmin(unl - al, un_roe - a_roe));
+  double S4 = fmax(0.0, fmax(unr + ar, un_roe + a_roe));
+  double inv_denom = 1.0 / (S4 - S1);
+  double FrL = dltot * unl, FrR = drtot * unr;
+  F[0] = (S4 * FrL - S1 * FrR + S1 * S4 * (drtot - dltot)) * inv_denom;
+  F[1] = (S4 * (FrL * ul + pl * nx) - S1 * (FrR * ur + pr * nx) + S1 * S4 * (drtot * ur - dltot * ul)) * inv_denom;
+  F[2] = (S4 * (FrL * vl + pl * ny) - S1 * (FrR * vr + pr * ny) + S1 * S4 * (drtot * vr - dltot * vl)) * inv_denom;
+  F[3] = (S4 * (FrL * wl + pl * nz) - S1 * (FrR * wr + pr * nz) + S1 * S4 * (drtot * wr - dltot * wl)) * inv_denom;
+  F[4] = (S4 * (FrL * h0l) - S1 * (FrR * h0r) +
+          S1 * S4 * (drtot * (h0r - pr / drtot) - dltot * (h0l - pl / dltot))) * inv_denom;
+}
+
 /* riemann_AUSMp, Lib_Riemann_AUSM.f90:150-238 (a_half = sqrt(aL aR), :196) */
 static void riemann_ausmp(const ob_domain* d, double dl, double ul, double vl, double wl, double pl,
                           double al, double dltot, double dr, double ur, double vr, double wr,
@@ -260,6 +283,9 @@ static void riemann(const ob_domain* d, const double* PL, double aL, double rL,
   if (d->cfg.riemann == MOSE_RS_AUSMP)
     riemann_ausmp(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
                   n[0], n[1], n[2], F);
+  else if (d->cfg.riemann == MOSE_RS_HLLE)
+    riemann_hlle(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
+                 n[0], n[1], n[2], F);
   else
     riemann_hllc(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
                  n[0], n[1], n[2], F);

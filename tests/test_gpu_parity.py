@@ -17,7 +17,7 @@ from helpers import flux_scale, interior, perturb, rel_l2
 pytestmark = pytest.mark.gpu
 
 LIMS = ["minmod", "vanleer", "vanalbada"]
-RSS = ["HLLC", "AUSM+"]
+RSS = ["HLLC", "AUSM+", "HLLE"]
 
 
 def _pair(case):

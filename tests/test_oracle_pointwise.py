@@ -102,7 +102,7 @@ def rand_normal(rng):
     return n / np.linalg.norm(n)
 
 
-@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP])
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE])
 def test_riemann_consistency(rs):
     cfg = abi.make_cfg(riemann=rs)
     rng = np.random.default_rng(1)
@@ -113,7 +113,7 @@ def test_riemann_consistency(rs):
         assert np.allclose(F, physical_flux(P, n), rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP])
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE])
 def test_riemann_supersonic_upwinding(rs):
     cfg = abi.make_cfg(riemann=rs)
     rng = np.random.default_rng(2)
@@ -143,7 +143,7 @@ def test_riemann_stationary_contact(rs):
         assert np.allclose(F, [0, p * n[0], p * n[1], p * n[2], 0], atol=1e-13)
 
 
-@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP])
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE])
 def test_riemann_mirror_symmetry(rs):
     """F(L, R, n) = -F(R, L, -n) componentwise (relabelling the two sides)."""
     cfg = abi.make_cfg(riemann=rs)
@@ -172,6 +172,49 @@ def test_riemann_moving_contact_is_upwinded(rs):
         F = O.riemann(cfg, L, R, n)
         up = L if un > 0 else R
         assert np.allclose(F, physical_flux(up, n), rtol=1e-12, atol=1e-13)
+
+
+def cons(P):
+    rho, u, v, w, p = P
+    return np.array([rho, rho * u, rho * v, rho * w, p / (G - 1) + 0.5 * rho * (u * u + v * v + w * w)])
+
+
+def rankine_hugoniot_pair(rng, mach, frame, n, t):
+    """Left (pre-shock) and right (post-shock) primitive states of a normal shock of the
+    u - a family with upstream Mach number `mach` in the shock frame, seen from a frame in
+    which the shock moves with normal speed -frame; t is a tangential velocity common to both
+    sides.  Returns (L, R, shock speed)."""
+    rho1, p1 = rng.uniform(0.3, 2.0), rng.uniform(0.3, 2.0)
+    a1 = np.sqrt(G * p1 / rho1)
+    u1 = mach * a1
+    m2 = mach * mach
+    rho2 = rho1 * (G + 1) * m2 / ((G - 1) * m2 + 2)
+    p2 = p1 * (1 + 2 * G / (G + 1) * (m2 - 1))
+    u2 = u1 * rho1 / rho2
+    L = np.array([rho1, *((u1 - frame) * n + t), p1])
+    R = np.array([rho2, *((u2 - frame) * n + t), p2])
+    return L, R, -frame
+
+
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_HLLE])
+def test_riemann_isolated_shock_is_exact(rs):
+    """A single shock satisfying the Rankine-Hugoniot relations: the Roe average has the shock
+    speed as an eigenvalue (Roe's property), so Batten's/Einfeldt's S1 is the shock speed and the
+    HLL-family fluxes return the exact Godunov flux, the physical flux of the state at x/t = 0:
+    the right state for a shock moving left (or standing), the left state for one moving right."""
+    cfg = abi.make_cfg(riemann=rs)
+    rng = np.random.default_rng(8)
+    for trial in range(100):
+        n = rand_normal(rng)
+        t = np.cross(n, rand_normal(rng)) * rng.uniform(0, 1)
+        mach = rng.uniform(1.05, 4.0)
+        frame = [0.0, rng.uniform(0.05, 0.5), rng.uniform(1.5, 3.0)][trial % 3]
+        L, R, s = rankine_hugoniot_pair(rng, mach, frame, n, t)
+        # the pair satisfies F(R) - F(L) = s (U(R) - U(L))
+        assert np.allclose(physical_flux(R, n) - physical_flux(L, n), s * (cons(R) - cons(L)),
+                           rtol=1e-12, atol=1e-12)
+        F = O.riemann(cfg, L, R, n)
+        assert np.allclose(F, physical_flux(R, n), rtol=1e-11, atol=1e-11), (trial, mach, frame)
 
 
 def test_prim_cons_round_trip():
