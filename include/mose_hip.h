@@ -33,8 +33,9 @@
 extern "C" {
 #endif
 
-/* Limiters: rlimiter_MINMOD/VANLEER/VANALBADA/IORD, src/lib/numerics/space/Lib_Limiters.f90:24-75 */
-enum { MOSE_LIM_MINMOD = 0, MOSE_LIM_VANLEER = 1, MOSE_LIM_VANALBADA = 2, MOSE_LIM_FIRST_ORDER = 3 };
+/* Limiters: rlimiter_MINMOD/VANLEER/VANALBADA/IORD/MC/SB, src/lib/numerics/space/Lib_Limiters.f90:24-105 */
+enum { MOSE_LIM_MINMOD = 0, MOSE_LIM_VANLEER = 1, MOSE_LIM_VANALBADA = 2, MOSE_LIM_FIRST_ORDER = 3,
+       MOSE_LIM_MC = 4, MOSE_LIM_SUPERBEE = 5 };
 /* Riemann solvers: riemann_HLLC (Lib_Riemann_HLL.f90:147-258), riemann_AUSMp
  * (Lib_Riemann_AUSM.f90:150-238), riemann_HLLE (Lib_Riemann_HLL.f90:261-320) */
 enum { MOSE_RS_HLLC = 0, MOSE_RS_AUSMP = 1, MOSE_RS_HLLE = 2 };

@@ -15,7 +15,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("MOSE_FC", "/opt/rocm/llvm/bin/amdflang")
 
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # FP32 division and square root of the FP32-flux kernels use the hardware reciprocal /
@@ -30,10 +30,40 @@ def run(cmd, cwd=None):
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
+NLIM = 6      # limiters; mose_lim.hip is compiled once for each (MOSE_LIM_ID)
+BUILD = os.path.join(HERE, "_build")
+
+
+def _jobs():
+    try:
+        return max(1, min(int(os.environ.get("MAX_JOBS", "8")), 16))
+    except ValueError:
+        return 8
+
+
 def build_hip(extra=()):
-    src = os.path.join(CSRC, "mose_hip.hip")
+    """Compile the host/ABI unit and one unit per limiter in parallel, then link."""
+    os.makedirs(BUILD, exist_ok=True)
+    inc = ["-I", os.path.join(ROOT, "include")]
+    units = [(os.path.join(CSRC, "mose_hip.hip"), [], os.path.join(BUILD, "mose_hip.o"))]
+    units += [(os.path.join(CSRC, "mose_lim.hip"), ["-DMOSE_LIM_ID=%d" % k], os.path.join(BUILD, "mose_lim%d.o" % k))
+              for k in range(NLIM)]
+    procs = []
+    pending = list(units)
+    failed = None
+    while pending or procs:
+        while pending and len(procs) < _jobs():
+            src, defs, obj = pending.pop(0)
+            cmd = [HIPCC] + HIP_FLAGS + list(extra) + inc + defs + ["-c", "-o", obj, src]
+            print("+", " ".join(cmd), flush=True)
+            procs.append((subprocess.Popen(cmd), obj))
+        p, obj = procs.pop(0)
+        if p.wait() != 0 and failed is None:
+            failed = obj
+    if failed:
+        raise subprocess.CalledProcessError(1, "hipcc -> " + failed)
     tmp = LIB + ".tmp"
-    run([HIPCC] + HIP_FLAGS + list(extra) + ["-I", os.path.join(ROOT, "include"), "-o", tmp, src])
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [u[2] for u in units])
     os.replace(tmp, LIB)
     return LIB
 

@@ -5,7 +5,7 @@
 // in the kernels that call these functions.
 //
 // Reference routines restated (algebraically identical, re-arranged for the GPU):
-//   limiters            src/lib/numerics/space/Lib_Limiters.f90:24-75
+//   limiters            src/lib/numerics/space/Lib_Limiters.f90:24-105
 //   reconstruction      src/lib/numerics/space/Lib_Reconstruction.f90:10-29
 //   HLLC (Batten)       src/lib/numerics/fluxes/riemann/Lib_Riemann_HLL.f90:147-258,550-647
 //   AUSM+               src/lib/numerics/fluxes/riemann/Lib_Riemann_AUSM.f90:150-238
@@ -18,7 +18,7 @@
 
 namespace mose {
 
-enum { LIM_MINMOD = 0, LIM_VANLEER = 1, LIM_VANALBADA = 2, LIM_FIRST = 3 };
+enum { LIM_MINMOD = 0, LIM_VANLEER = 1, LIM_VANALBADA = 2, LIM_FIRST = 3, LIM_MC = 4, LIM_SUPERBEE = 5 };
 enum { RS_HLLC = 0, RS_AUSMP = 1, RS_HLLE = 2 };
 
 template <typename T>
@@ -66,6 +66,8 @@ template <typename T> __device__ __forceinline__ T max_(T a, T b) { return a > b
 //  minmod   : reference form.
 //  van Leer : (r+|r|)/(1+|r|) x with r = y/x  ==  2xy/(x+y) if xy > 0 else 0.
 //  van Albada: (r^2+r)/(1+r^2) x              ==  xy(x+y)/(x^2+y^2).
+//  MC       : max(0, min(2r, (1+r)/2, 2)) x   ==  sgn(x) min(2|x|, 2|y|, (|x|+|y|)/2) if xy > 0.
+//  superbee : max(0, min(2r, 1), min(r, 2)) x ==  sgn(x) max(min(2|y|, |x|), min(|y|, 2|x|)) if xy > 0.
 // The reference regularises r with x+1e-30; the single-division forms agree with it to
 // round-off for every slope pair whose magnitude exceeds ~1e-14 of the other and differ
 // by less than |x| otherwise.
@@ -82,6 +84,14 @@ __device__ __forceinline__ T limiter(T x, T y) {
   } else if constexpr (LIM == LIM_VANALBADA) {
     T q = x * x + y * y;
     return (q > T(0)) ? (x * y * (x + y)) * rcp_(q) : T(0);
+  } else if constexpr (LIM == LIM_MC) {
+    const T ax = fabs_(x), ay = fabs_(y);
+    const T m = min_(T(2) * min_(ax, ay), T(0.5) * (ax + ay));
+    return (x * y > T(0)) ? (x > T(0) ? m : -m) : T(0);
+  } else if constexpr (LIM == LIM_SUPERBEE) {
+    const T ax = fabs_(x), ay = fabs_(y);
+    const T m = max_(min_(T(2) * ay, ax), min_(ay, T(2) * ax));
+    return (x * y > T(0)) ? (x > T(0) ? m : -m) : T(0);
   } else {
     return T(0);
   }

@@ -29,14 +29,10 @@
 using namespace mose;
 
 // ============================================================================ device
-__device__ __forceinline__ void set_error(int* err, int code, int b, int i, int j, int k) {
-  if (atomicCAS(err, 0, code) == 0) {
-    err[1] = b; err[2] = i; err[3] = j; err[4] = k;
-  }
-}
-
 #include "mose_stage.hpp"
 #include "mose_adjoint.hpp"
+#include "mose_adjoint_kernels.hpp"
+#include "mose_tables.hpp"
 
 struct DtArgs {
   const double* U;
@@ -265,109 +261,25 @@ __global__ void k_copy(const double* __restrict__ s, double* __restrict__ d, int
 
 namespace {
 
-typedef void (*StageFn)(StageArgs);
-typedef void (*BcFn)(StageArgs, const BcFace*, int);
-
-// Patch shapes (TX x TY threads) of the stage kernel per precision; shape 0 is the default,
-// the others are compiled for the van Leer limiter and selected by the tile tuner.
-constexpr int NSHAPE = 3;
-constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 16}, {8, 16}}, {{32, 8}, {16, 16}, {16, 8}}};
-
-template <int LIM, int RS, typename FT, int TXc, int TYc, int GEO = 0>
-void stage_modes(StageFn* t) {
-  t[0] = k_stage<LIM, RS, FT, 0, TXc, TYc, GEO>;
-  t[1] = k_stage<LIM, RS, FT, 1, TXc, TYc, GEO>;
-  t[2] = k_stage<LIM, RS, FT, 2, TXc, TYc, GEO>;
-}
-
-template <int LIM, typename FT>
-void bc_table(BcFn* t) {
-  t[0] = k_bcface<LIM, RS_HLLC, FT>;
-  t[1] = k_bcface<LIM, RS_AUSMP, FT>;
-  t[2] = k_bcface<LIM, RS_HLLE, FT>;
-}
-
-// kernel table [prec][lim][rs][shape][mode]; null where a shape is not compiled.
-// g: the general-metrics kernels (shape 0 only).
-constexpr int NRS = 3;   // HLLC, AUSM+, HLLE
-struct KTable {
-  StageFn f[2][4][NRS][NSHAPE][3] = {};
-  StageFn g[2][2][4][NRS][3] = {};     // [geo - 1][prec][lim][rs][mode]
-  BcFn bc[2][4][NRS] = {};
-  template <int LIM, int RS>
-  void all_shapes() {
-    stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1]>(f[0][LIM][RS][0]);
-    stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1]>(f[1][LIM][RS][0]);
-    stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 1>(g[0][0][LIM][RS]);
-    stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1], 1>(g[0][1][LIM][RS]);
-    stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 2>(g[1][0][LIM][RS]);
-    stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1], 2>(g[1][1][LIM][RS]);
-  }
-  KTable() {
-    bc_table<LIM_MINMOD, double>(bc[0][0]);
-    bc_table<LIM_VANLEER, double>(bc[0][1]);
-    bc_table<LIM_VANALBADA, double>(bc[0][2]);
-    bc_table<LIM_FIRST, double>(bc[0][3]);
-    bc_table<LIM_MINMOD, float>(bc[1][0]);
-    bc_table<LIM_VANLEER, float>(bc[1][1]);
-    bc_table<LIM_VANALBADA, float>(bc[1][2]);
-    bc_table<LIM_FIRST, float>(bc[1][3]);
-    all_shapes<LIM_MINMOD, RS_HLLC>();
-    all_shapes<LIM_MINMOD, RS_AUSMP>();
-    all_shapes<LIM_VANLEER, RS_HLLC>();
-    all_shapes<LIM_VANLEER, RS_AUSMP>();
-    all_shapes<LIM_VANALBADA, RS_HLLC>();
-    all_shapes<LIM_VANALBADA, RS_AUSMP>();
-    all_shapes<LIM_FIRST, RS_HLLC>();
-    all_shapes<LIM_FIRST, RS_AUSMP>();
-    all_shapes<LIM_MINMOD, RS_HLLE>();
-    all_shapes<LIM_VANLEER, RS_HLLE>();
-    all_shapes<LIM_VANALBADA, RS_HLLE>();
-    all_shapes<LIM_FIRST, RS_HLLE>();
-    stage_modes<LIM_VANLEER, RS_HLLC, double, kShape[0][1][0], kShape[0][1][1]>(f[0][1][0][1]);
-    stage_modes<LIM_VANLEER, RS_HLLC, double, kShape[0][2][0], kShape[0][2][1]>(f[0][1][0][2]);
-    stage_modes<LIM_VANLEER, RS_AUSMP, double, kShape[0][1][0], kShape[0][1][1]>(f[0][1][1][1]);
-    stage_modes<LIM_VANLEER, RS_AUSMP, double, kShape[0][2][0], kShape[0][2][1]>(f[0][1][1][2]);
-    stage_modes<LIM_VANLEER, RS_HLLC, float, kShape[1][1][0], kShape[1][1][1]>(f[1][1][0][1]);
-    stage_modes<LIM_VANLEER, RS_HLLC, float, kShape[1][2][0], kShape[1][2][1]>(f[1][1][0][2]);
-    stage_modes<LIM_VANLEER, RS_AUSMP, float, kShape[1][1][0], kShape[1][1][1]>(f[1][1][1][1]);
-    stage_modes<LIM_VANLEER, RS_AUSMP, float, kShape[1][2][0], kShape[1][2][1]>(f[1][1][1][2]);
+// every limiter's kernel variants (instantiated in mose_lim.hip, one unit per limiter)
+struct Tables {
+  KTable k;
+  ATable a;
+  Tables() {
+    fill_tables<LIM_MINMOD>(k, a);
+    fill_tables<LIM_VANLEER>(k, a);
+    fill_tables<LIM_VANALBADA>(k, a);
+    fill_tables<LIM_FIRST>(k, a);
+    fill_tables<LIM_MC>(k, a);
+    fill_tables<LIM_SUPERBEE>(k, a);
   }
 };
-const KTable& ktable() {
-  static KTable t;
+const Tables& tables() {
+  static Tables t;
   return t;
 }
-
-typedef void (*AdjFn)(AdjArgs);
-// adjoint kernels [lim][rs][geo] (FP64)
-struct ATable {
-  AdjFn face[4][NRS][2] = {}, bc[4][NRS] = {}, cell[4][2] = {};
-  template <int LIM>
-  void lim() {
-    face[LIM][0][0] = k_adj_face<LIM, RS_HLLC, 0>;
-    face[LIM][0][1] = k_adj_face<LIM, RS_HLLC, 1>;
-    face[LIM][1][0] = k_adj_face<LIM, RS_AUSMP, 0>;
-    face[LIM][1][1] = k_adj_face<LIM, RS_AUSMP, 1>;
-    bc[LIM][0] = k_adj_bcface<LIM, RS_HLLC>;
-    bc[LIM][1] = k_adj_bcface<LIM, RS_AUSMP>;
-    face[LIM][2][0] = k_adj_face<LIM, RS_HLLE, 0>;
-    face[LIM][2][1] = k_adj_face<LIM, RS_HLLE, 1>;
-    bc[LIM][2] = k_adj_bcface<LIM, RS_HLLE>;
-    cell[LIM][0] = k_adj_cell<LIM, 0>;
-    cell[LIM][1] = k_adj_cell<LIM, 1>;
-  }
-  ATable() {
-    lim<LIM_MINMOD>();
-    lim<LIM_VANLEER>();
-    lim<LIM_VANALBADA>();
-    lim<LIM_FIRST>();
-  }
-};
-const ATable& atable() {
-  static ATable t;
-  return t;
-}
+const KTable& ktable() { return tables().k; }
+const ATable& atable() { return tables().a; }
 
 const double kRK[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};   // Lib_RK.f90:8-11
 
@@ -798,7 +710,7 @@ extern "C" {
 int mose_create(const mose_cfg* cfg, mose_ctx** out) {
   if (!cfg || !out) return MOSE_E_ARG;
   *out = nullptr;
-  if (cfg->limiter < 0 || cfg->limiter > 3 || cfg->riemann < 0 || cfg->riemann > 2 || cfg->n_rk < 1 ||
+  if (cfg->limiter < 0 || cfg->limiter > 5 || cfg->riemann < 0 || cfg->riemann > 2 || cfg->n_rk < 1 ||
       cfg->n_rk > 3 || !(cfg->gamma > 1.0) || !(cfg->Rgas > 0.0) || cfg->precision < 0 || cfg->precision > 1)
     return MOSE_E_ARG;
   int ndev = 0;

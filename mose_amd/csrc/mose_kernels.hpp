@@ -112,4 +112,11 @@ struct StageArgs {
   double rkc, gam, R, cfl;
 };
 
+// First error of a launch wins (code, block, i, j, k); later ones are dropped.
+__device__ __forceinline__ void set_error(int* err, int code, int b, int i, int j, int k) {
+  if (atomicCAS(err, 0, code) == 0) {
+    err[1] = b; err[2] = i; err[3] = j; err[4] = k;
+  }
+}
+
 }  // namespace mose

@@ -1,0 +1,34 @@
+// mose_tables.hpp — tables of the kernel variants the host launches: stage kernels
+// [prec][limiter][riemann][tile shape][mode], general-metrics and viscous stage kernels,
+// boundary-face kernels and the solver-dependent adjoint passes.  The entries of each
+// limiter are instantiated in a translation unit of their own (mose_lim.hip, compiled once
+// per limiter with MOSE_LIM_ID set) so that the variants compile in parallel;
+// mose_hip.hip fills the tables once by calling fill_tables<LIM> for every limiter.
+#pragma once
+
+typedef void (*StageFn)(mose::StageArgs);
+typedef void (*BcFn)(mose::StageArgs, const mose::BcFace*, int);
+typedef void (*AdjFn)(AdjArgs);
+
+constexpr int NLIM = 6;     // minmod, van Leer, van Albada, first order, MC, superbee
+constexpr int NRS = 3;      // HLLC, AUSM+, HLLE
+// Patch shapes (TX x TY threads) of the stage kernel per precision; shape 0 is the default,
+// the others are compiled for the van Leer limiter and selected by the tile tuner.
+constexpr int NSHAPE = 3;
+constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 16}, {8, 16}}, {{32, 8}, {16, 16}, {16, 8}}};
+
+// null where a variant is not compiled
+struct KTable {
+  StageFn f[2][NLIM][NRS][NSHAPE][3] = {};
+  StageFn g[2][2][NLIM][NRS][3] = {};     // [geo - 1][prec][lim][rs][mode]
+  BcFn bc[2][NLIM][NRS] = {};
+};
+
+// adjoint kernels [lim][rs][geo] (FP64)
+struct ATable {
+  AdjFn face[NLIM][NRS][2] = {}, bc[NLIM][NRS] = {}, cell[NLIM][2] = {};
+};
+
+// Entries of limiter LIM; defined and instantiated in mose_lim.hip.
+template <int LIM>
+void fill_tables(KTable& k, ATable& a);

@@ -131,3 +131,33 @@ def probe_check(xc, rho, u, p, sie, exact_table, threshold=1.0):
             if err > threshold:
                 fails.append((name, i, err))
     return fails
+
+
+def probe_fails(cells, cfd, exact, thresholds=None):
+    """The reference's per-probe acceptance rule of its 1-D verify scripts
+    (test/1D/Ein91/verify.py:141-215, test/1D/Noh87/verify.py:141-221): at each probe cell,
+    the percent error |cfd - exact| / exact of rho, u, p and sie must not exceed 1, 1, 1.5
+    and 1 percent; the velocity's denominator is exact + 1e-10.  cfd and exact map those
+    names to arrays over the same cells.  Returns the failing (quantity, cell, percent)."""
+    th = {"rho": 1.0, "u": 1.0, "p": 1.5, "sie": 1.0}
+    th.update(thresholds or {})
+    fails = []
+    for name in ("rho", "u", "p", "sie"):
+        for i in cells:
+            denom = exact[name][i] + (1e-10 if name == "u" else 0.0)
+            err = abs((cfd[name][i] - exact[name][i]) / denom) * 100.0
+            if not err <= th[name]:
+                fails.append((name, i, err))
+    return fails
+
+
+EIN91_LEFT = (1.0, -2.0, 0.4)
+EIN91_RIGHT = (1.0, 2.0, 0.4)
+
+
+def ein91(x, t=0.15, gamma=1.4):
+    """Einfeldt's 1-2-3 problem of test/1D/Ein91 (two rarefactions, verify.py:46-49):
+    returns (rho, u, p, sie)."""
+    rho, u, p = sample(x, t, EIN91_LEFT, EIN91_RIGHT, 0.5, gamma)
+    return rho, u, p, p / ((gamma - 1.0) * rho)
+

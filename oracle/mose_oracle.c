@@ -96,7 +96,7 @@ static void cons2prim_d(const ob_domain* d, const double* U, double* P) {
 }
 
 /* ---------------------------------- limiters ----------------------------------------
- * src/lib/numerics/space/Lib_Limiters.f90:24-75 */
+ * src/lib/numerics/space/Lib_Limiters.f90:24-105 (selected by name in Mod_Space.f90:50-67) */
 double ob_limiter(int32_t kind, double x, double y) {
   double ru, phi, r;
   switch (kind) {
@@ -113,6 +113,14 @@ double ob_limiter(int32_t kind, double x, double y) {
     case MOSE_LIM_VANALBADA:
       ru = y / (x + 1e-30);
       phi = (ru * ru + ru) / (1.0 + ru * ru);
+      return phi * x;
+    case MOSE_LIM_MC:          /* rlimiter_MC, Lib_Limiters.f90:79-90 */
+      ru = y / (x + 1e-30);
+      phi = fmax(0.0, fmin(fmin(2.0 * ru, 0.5 * (1.0 + ru)), 2.0));
+      return phi * x;
+    case MOSE_LIM_SUPERBEE:    /* rlimiter_SB, Lib_Limiters.f90:94-105 */
+      ru = y / (x + 1e-30);
+      phi = fmax(fmax(0.0, fmin(2.0 * ru, 1.0)), fmin(ru, 2.0));
       return phi * x;
     default:
       return 0.0;   /* rlimiter_IORD */

@@ -2,6 +2,8 @@
 directory (data fixtures; gzip for the larger ones):
   test/1D/Sod79/INPUT/ic_x1.tec, bc_x1.txt           -> sod79_ic_x1.tec, sod79_bc_x1.txt
   test/2D/euler/prandtl-meyer/INPUT/ic.tec, bc.txt   -> prandtl_meyer_ic.tec.gz, prandtl_meyer_bc.txt.gz
+  test/1D/Ein91/INPUT/ic_x8.tec, bc_x8.txt           -> ein91_ic_x8.tec.gz, ein91_bc_x8.txt.gz
+  test/1D/Noh87/INPUT/ic_x8.tec, bc_x8.txt           -> noh87_ic_x8.tec.gz, noh87_bc_x8.txt.gz
 Run in a container that holds the reference at /root/reference."""
 import gzip
 import os
@@ -11,13 +13,21 @@ REF = "/root/reference/test"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+def _gzip(src, dst):
+    """Byte-reproducible gzip (no file name or time stamp in the header)."""
+    with open(src, "rb") as f, open(os.path.join(HERE, dst), "wb") as raw, \
+            gzip.GzipFile(filename="", mode="wb", compresslevel=9, fileobj=raw, mtime=0) as g:
+        g.write(f.read())
+
+
 def main():
     shutil.copyfile(os.path.join(REF, "1D/Sod79/INPUT/ic_x1.tec"), os.path.join(HERE, "sod79_ic_x1.tec"))
     shutil.copyfile(os.path.join(REF, "1D/Sod79/INPUT/bc_x1.txt"), os.path.join(HERE, "sod79_bc_x1.txt"))
     for src, dst in (("ic.tec", "prandtl_meyer_ic.tec.gz"), ("bc.txt", "prandtl_meyer_bc.txt.gz")):
-        with open(os.path.join(REF, "2D/euler/prandtl-meyer/INPUT", src), "rb") as f, \
-                gzip.open(os.path.join(HERE, dst), "wb", compresslevel=9) as g:
-            g.write(f.read())
+        _gzip(os.path.join(REF, "2D/euler/prandtl-meyer/INPUT", src), dst)
+    for case in ("Ein91", "Noh87"):
+        for src in ("ic_x8.tec", "bc_x8.txt"):
+            _gzip(os.path.join(REF, "1D", case, "INPUT", src), "%s_%s.gz" % (case.lower(), src))
 
 
 if __name__ == "__main__":

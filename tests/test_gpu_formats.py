@@ -1,5 +1,6 @@
-"""The reference's own input files (test/1D/Sod79 x1, test/2D/euler/prandtl-meyer) run on
-the GPU path against the oracle: residual, steps, and the node / inflow adjoints through the
+"""The reference's own input files (test/1D/Sod79 x1, test/1D/Ein91 and Noh87 x8,
+test/2D/euler/prandtl-meyer) run on the GPU path against the oracle and the exact solutions:
+residual, steps, the reference's probe rule, and the node / inflow adjoints through the
 'normal' inflow direction (angles from the boundary face normals)."""
 import os
 
@@ -10,7 +11,9 @@ from mose_amd import abi, formats
 from mose_amd.lib import GpuSolver
 from oracle.oracle import OracleSolver
 
-from helpers import perturb, rel_l2
+from oracle import exact_riemann as er
+
+from helpers import interior, perturb, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -27,7 +30,7 @@ def _pm_case(limiter="vanleer", riemann="HLLC", amp=0.0):
     return perturb(case, amp=amp, seed=3) if amp else case
 
 
-@pytest.mark.parametrize("rs", ["HLLC", "AUSM+"])
+@pytest.mark.parametrize("rs", ["HLLC", "AUSM+", "HLLE"])
 def test_prandtl_meyer_residual_and_steps(rs):
     case = _pm_case(riemann=rs, amp=0.02)
     g, o = GpuSolver(case), OracleSolver(case)
@@ -49,6 +52,62 @@ def test_sod79_input_steps():
     assert abs(tg - to) <= 1e-13 * to
     assert rel_l2(g.get_state(1), o.get_state(1)) < 1e-12
     g.close()
+
+
+def _input_1d(name, limiter, cfl):
+    ic, bc = os.path.join(GOLD, "%s_ic_x8.tec.gz" % name), os.path.join(GOLD, "%s_bc_x8.txt.gz" % name)
+    return formats.load_case(ic, bc, abi.make_cfg(1.4, 1.0 / 1.4, cfl, limiter, "HLLE", 3, True))
+
+
+def test_ein91_input_probe_rule_and_parity():
+    """test/1D/Ein91 (HLLE + van Leer, CFL 0.9, 800 cells): the GPU state matches the oracle
+    after 50 steps and passes the reference's probe rule at t = 0.15."""
+    case = _input_1d("ein91", "vanleer", 0.9)
+    g, o = GpuSolver(case), OracleSolver(case)
+    for _ in range(50):
+        g.step()
+        o.step()
+    assert rel_l2(g.get_state(1), o.get_state(1)) < 1e-12
+    t = 0.0
+    while t < 0.15:
+        _, t, _ = g.step()
+    assert 0.15 <= t < 0.151
+    P = interior(g.get_state(1))[0, 0]
+    g.close()
+    xc = (np.arange(800) + 0.5) / 800
+    rho, u, p, sie = er.ein91(xc)
+    cfd = {"rho": P[:, 0], "u": P[:, 1], "p": P[:, 4], "sie": P[:, 4] / (0.4 * P[:, 0])}
+    assert not er.probe_fails((80, 160, 240), cfd, {"rho": rho, "u": u, "p": p, "sie": sie})
+
+
+def test_noh87_input_first_order_parity():
+    """test/1D/Noh87 (first order + HLLE, CFL 0.5, colliding streams with p = 1e-15): the GPU
+    residual matches the oracle's on the initial state and on the oracle's state after 250
+    steps; the GPU's own 250 steps reach the oracle's time and Noh's post-shock plateau
+    (rho = (g+1)/(g-1), u = 0, p = (g+1)/2 u0^2 behind a shock moving at (g-1)/2 u0) away
+    from the wall-heating cells at the centre.  (The time step is set by |u| + a of the
+    pre-shock gas, whose a ~ 4e-8 comes from a pressure at round-off level, so the two
+    clocks agree to ~1e-9, not to 1e-13.)"""
+    case = _input_1d("noh87", "first-order", 0.5)
+    g, o = GpuSolver(case), OracleSolver(case)
+    assert rel_l2(g.residual(1), o.residual(1)) < 1e-12
+    for _ in range(250):
+        _, tg, _ = g.step()
+        _, to, _ = o.step()
+    assert abs(tg - to) <= 1e-8 * to
+    Pg = interior(g.get_state(1))[0, 0]
+    g.put_state(1, o.get_state(1))
+    assert rel_l2(g.residual(1), o.residual(1)) < 1e-12
+    g.close()
+    # shock at 0.5 +- 0.2 t: plateau cells between 8 cells off the centre and 4 cells
+    # inside the shock
+    xs = 0.2 * tg
+    xc = (np.arange(800) + 0.5) / 800
+    m = (np.abs(xc - 0.5) > 8 / 800) & (np.abs(xc - 0.5) < xs - 4 / 800)
+    assert m.sum() >= 20
+    assert np.allclose(Pg[m, 0], 6.0, rtol=2e-2)
+    assert np.allclose(Pg[m, 4], 1.2, rtol=2e-2)
+    assert np.max(np.abs(Pg[m, 1])) < 2e-2
 
 
 def test_node_adjoint_with_normal_inflow_direction():

@@ -16,7 +16,7 @@ from helpers import flux_scale, interior, perturb, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-LIMS = ["minmod", "vanleer", "vanalbada"]
+LIMS = ["minmod", "vanleer", "vanalbada", "mc", "superbee"]
 RSS = ["HLLC", "AUSM+", "HLLE"]
 
 

@@ -51,6 +51,53 @@ def test_probe_rule_detects_a_wrong_solution():
     assert er.probe_check(xc, rho * 1.03, u, p, sie, table, 1.0)
 
 
+def test_exact_two_rarefaction_closed_form():
+    """The exact solver's two-rarefaction branch (Ein91): for the symmetric problem u* = 0 and
+    (p*/p)^((g-1)/2g) = 1 - (g-1)|u|/(2a) (the Riemann invariant u + 2a/(g-1) across the left
+    fan); the solution is mirror-symmetric about x = 0.5 (rho, p even, u odd)."""
+    g = 1.4
+    ps, us = er.star_state(er.EIN91_LEFT, er.EIN91_RIGHT, g)
+    a = np.sqrt(g * 0.4 / 1.0)
+    assert abs(us) < 1e-14
+    assert ps == pytest.approx(0.4 * (1 - (g - 1) * 2.0 / (2 * a)) ** (2 * g / (g - 1)), rel=1e-12)
+    x = np.linspace(0.01, 0.99, 99)
+    rho, u, p, _ = er.ein91(x)
+    rho2, u2, p2, _ = er.ein91(1.0 - x)
+    assert np.allclose(rho, rho2, rtol=1e-12) and np.allclose(p, p2, rtol=1e-12)
+    assert np.allclose(u, -u2, atol=1e-12)
+
+
+def _reference_1d_input(name, limiter, cfl):
+    """A case from the reference's own 1-D input files (test/1D/<name>/INPUT, 800 cells),
+    with the numerics of its input.ini."""
+    from mose_amd import formats
+    ic = os.path.join(GOLDEN, "%s_ic_x8.tec.gz" % name)
+    bc = os.path.join(GOLDEN, "%s_bc_x8.txt.gz" % name)
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, cfl, limiter, "HLLE", 3, True)
+    return formats.load_case(ic, bc, cfg)
+
+
+def test_ein91_reference_input_probe_rule():
+    """test/1D/Ein91 (HLLE + van Leer, RK3, CFL 0.9, 800 cells to t = 0.15) run from the
+    reference's ic_x8.tec / bc_x8.txt passes the reference's probe rule against the exact
+    two-rarefaction solution; the same rule fails on a planted 2 % pressure error."""
+    case = _reference_1d_input("ein91", "vanleer", 0.9)
+    o = OracleSolver(case)
+    t = 0.0
+    while t < 0.15:
+        _, t, _ = o.step()
+    assert 0.15 <= t < 0.151
+    P = interior(o.get_state(1))[0, 0]
+    xc = cases.cell_centres(case.blocks[0].nodes)[0, 0, :, 0]
+    rho, u, p, sie = er.ein91(xc)
+    exact = {"rho": rho, "u": u, "p": p, "sie": sie}
+    cfd = {"rho": P[:, 0], "u": P[:, 1], "p": P[:, 4], "sie": P[:, 4] / (0.4 * P[:, 0])}
+    probes = (80, 160, 240)
+    assert not er.probe_fails(probes, cfd, exact)
+    cfd["p"] = cfd["p"] * 1.02
+    assert er.probe_fails(probes, cfd, exact)
+
+
 def _skewed_nodes(dims, seed=0, amp=0.08):
     """Smoothly distorted (curvilinear) grid of a unit box."""
     nodes = cases.cartesian_nodes(dims, (0, 0, 0), (1.0, 0.8, 0.6))

@@ -29,6 +29,32 @@ def vanalbada_def(x, y):
     return x * y * (x + y) / (x * x + y * y) if x * x + y * y > 0 else 0.0
 
 
+def minmod3(*a):
+    """minmod of several arguments: 0 unless all share a sign, else the one of least magnitude."""
+    if all(v > 0 for v in a):
+        return min(a)
+    if all(v < 0 for v in a):
+        return max(a)
+    return 0.0
+
+
+def maxmod(a, b):
+    """0 unless both share a sign, else the one of largest magnitude."""
+    if a * b <= 0:
+        return a + b if a * b == 0 else 0.0
+    return max(a, b) if a > 0 else min(a, b)
+
+
+def mc_def(x, y):
+    """Monotonized-central slope (van Leer 1977): minmod(2x, 2y, (x + y)/2)."""
+    return minmod3(2 * x, 2 * y, 0.5 * (x + y))
+
+
+def superbee_def(x, y):
+    """Roe's superbee slope: maxmod(minmod(x, 2y), minmod(2x, y))."""
+    return maxmod(minmod3(x, 2 * y), minmod3(2 * x, y))
+
+
 VALS = [-3.0, -1.0, -0.25, 0.0, 0.25, 1.0, 2.0, 7.5]
 
 
@@ -38,7 +64,35 @@ def test_limiters_against_definitions(x, y):
     # the reference regularises r = y/(x + 1e-30): agrees with the closed forms to round-off
     assert O.limiter(abi.LIM_VANLEER, x, y) == pytest.approx(vanleer_def(x, y), rel=1e-14, abs=1e-28)
     assert O.limiter(abi.LIM_VANALBADA, x, y) == pytest.approx(vanalbada_def(x, y), rel=1e-14, abs=1e-28)
+    assert O.limiter(abi.LIM_MC, x, y) == pytest.approx(mc_def(x, y), rel=1e-14, abs=1e-28)
+    assert O.limiter(abi.LIM_SUPERBEE, x, y) == pytest.approx(superbee_def(x, y), rel=1e-14, abs=1e-28)
     assert O.limiter(abi.LIM_FIRST_ORDER, x, y) == 0.0
+
+
+def test_limiters_in_sweby_region():
+    """Sweby's second-order TVD region: with r = y/x > 0 the limited slope phi(r) x lies
+    between minmod's and superbee's, van Leer's <= MC's; phi(1) = 1; phi(r)/r = phi(1/r)
+    (symmetric in x and y); degree-1 homogeneous; zero for slopes of opposite sign (all but
+    van Albada)."""
+    rng = np.random.default_rng(11)
+    lims = (abi.LIM_MINMOD, abi.LIM_VANLEER, abi.LIM_VANALBADA, abi.LIM_MC, abi.LIM_SUPERBEE)
+    for x, y in rng.normal(size=(2000, 2)):
+        v = {k: O.limiter(k, x, y) for k in lims}
+        if x * y < 0:
+            # (the reference's van Albada, phi = (r^2 + r)/(1 + r^2), is not clipped at r < 0)
+            assert all(s == 0.0 for k, s in v.items() if k != abi.LIM_VANALBADA)
+            continue
+        a = {k: abs(s) for k, s in v.items()}
+        tol = 1e-14 * (abs(x) + abs(y))
+        assert a[abi.LIM_MINMOD] <= a[abi.LIM_VANLEER] + tol
+        assert a[abi.LIM_VANLEER] <= a[abi.LIM_MC] + tol
+        assert a[abi.LIM_MC] <= a[abi.LIM_SUPERBEE] + tol
+        assert a[abi.LIM_SUPERBEE] <= 2 * min(abs(x), abs(y)) + tol
+        for k in (abi.LIM_MC, abi.LIM_SUPERBEE):
+            assert np.sign(v[k]) == np.sign(x)
+            assert O.limiter(k, y, x) == pytest.approx(v[k], rel=1e-13, abs=1e-15)
+            assert O.limiter(k, 2.5 * x, 2.5 * y) == pytest.approx(2.5 * v[k], rel=1e-13, abs=1e-15)
+            assert O.limiter(k, x, x) == pytest.approx(x, rel=1e-14)
 
 
 def test_limiter_invariants():
@@ -58,7 +112,7 @@ def test_limiter_invariants():
                 assert np.sign(s) == np.sign(x) and abs(s) <= 2 * min(abs(x), abs(y)) + 1e-15
 
 
-@pytest.mark.parametrize("lim", [abi.LIM_MINMOD, abi.LIM_VANLEER, abi.LIM_VANALBADA])
+@pytest.mark.parametrize("lim", [abi.LIM_MINMOD, abi.LIM_VANLEER, abi.LIM_VANALBADA, abi.LIM_MC, abi.LIM_SUPERBEE])
 def test_reconstruction_exact_for_linear_data(lim):
     # non-uniform cells: lengths dl; linear field q(x) = a + b x sampled at the centres
     dl = np.array([0.7, 1.3, 0.9, 1.6])
@@ -74,7 +128,7 @@ def test_reconstruction_exact_for_linear_data(lim):
 
 def test_reconstruction_extremum_is_first_order():
     q = np.array([0.0, 1.0, 0.0, 1.0])
-    for lim in (abi.LIM_MINMOD, abi.LIM_VANLEER):
+    for lim in (abi.LIM_MINMOD, abi.LIM_VANLEER, abi.LIM_MC, abi.LIM_SUPERBEE):
         ql, qr = O.reconstruct(lim, q, 1, 1, 1, 0.5, 0.5)
         assert (ql, qr) == (1.0, 0.0)
 
