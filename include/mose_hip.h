@@ -37,8 +37,9 @@ extern "C" {
 enum { MOSE_LIM_MINMOD = 0, MOSE_LIM_VANLEER = 1, MOSE_LIM_VANALBADA = 2, MOSE_LIM_FIRST_ORDER = 3,
        MOSE_LIM_MC = 4, MOSE_LIM_SUPERBEE = 5 };
 /* Riemann solvers: riemann_HLLC (Lib_Riemann_HLL.f90:147-258), riemann_AUSMp
- * (Lib_Riemann_AUSM.f90:150-238), riemann_HLLE (Lib_Riemann_HLL.f90:261-320) */
-enum { MOSE_RS_HLLC = 0, MOSE_RS_AUSMP = 1, MOSE_RS_HLLE = 2 };
+ * (Lib_Riemann_AUSM.f90:150-238), riemann_HLLE (Lib_Riemann_HLL.f90:261-320), riemann_SLAU and
+ * riemann_SLAU2 (Lib_Riemann_SLAU.f90:19-140) */
+enum { MOSE_RS_HLLC = 0, MOSE_RS_AUSMP = 1, MOSE_RS_HLLE = 2, MOSE_RS_SLAU = 3, MOSE_RS_SLAU2 = 4 };
 /* Arithmetic of reconstruction + Riemann flux; state, accumulation and reductions are always FP64. */
 enum { MOSE_PREC_FP64 = 0, MOSE_PREC_FP32_FLUX = 1 };
 

@@ -10,14 +10,14 @@ import ctypes as C
 import numpy as np
 
 LIM_MINMOD, LIM_VANLEER, LIM_VANALBADA, LIM_FIRST_ORDER, LIM_MC, LIM_SUPERBEE = 0, 1, 2, 3, 4, 5
-RS_HLLC, RS_AUSMP, RS_HLLE = 0, 1, 2
+RS_HLLC, RS_AUSMP, RS_HLLE, RS_SLAU, RS_SLAU2 = 0, 1, 2, 3, 4
 PREC_FP64, PREC_FP32_FLUX = 0, 1
 
 E_OK, E_ARG, E_HIP, E_STATE, E_INTERFACE, E_METRIC, E_BC, E_NOGPU = range(8)
 
 LIMITERS = {"minmod": LIM_MINMOD, "vanleer": LIM_VANLEER, "vanalbada": LIM_VANALBADA,
             "first-order": LIM_FIRST_ORDER, "mc": LIM_MC, "superbee": LIM_SUPERBEE}
-RIEMANN = {"HLLC": RS_HLLC, "AUSM+": RS_AUSMP, "HLLE": RS_HLLE}
+RIEMANN = {"HLLC": RS_HLLC, "AUSM+": RS_AUSMP, "HLLE": RS_HLLE, "SLAU": RS_SLAU, "SLAU2": RS_SLAU2}
 
 NPRIM = 5
 GC = 2

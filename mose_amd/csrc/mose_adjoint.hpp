@@ -360,6 +360,125 @@ __device__ void hlle_adj(const double* L, const double* R, double nx, double ny,
   roe_batten_adj(L, R, nx, ny, nz, g, S1b, S4b, lb, rb, Lb, Rb, nb);
 }
 
+// ------------------------------------------------------------------------------ SLAU / SLAU2
+template <bool V2>
+__device__ void slau_adj(const double* L, const double* R, double nx, double ny, double nz, const GasT<double>& g,
+                         const double* Fb, double* Lb, double* Rb, double* nb) {
+  const double rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
+  const double rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
+  // ---- forward (slau in mose_device.hpp)
+  const double irl = 1.0 / rl, irr = 1.0 / rr;
+  const double al = sqrt(g.gam * pl * irl), ar = sqrt(g.gam * pr * irr);
+  const double unl = ul * nx + vl * ny + wl * nz, unr = ur * nx + vr * ny + wr * nz;
+  const double sql = ul * ul + vl * vl + wl * wl, sqr = ur * ur + vr * vr + wr * wr;
+  const double aF = 0.5 * (al + ar), iaF = 1.0 / aF;
+  const double mL = unl * iaF, mR = unr * iaF;
+  const double vrms = sqrt(0.5 * (sql + sqr));
+  const double mraw = vrms * iaF;
+  const double mt = min_(1.0, mraw);
+  const double chi = (1.0 - mt) * (1.0 - mt);
+  const double aul = fabs(unl), aur = fabs(unr);
+  const double rs = rl + rr, irs = 1.0 / rs;
+  const double vn = (rl * aul + rr * aur) * irs;
+  const double A = max_(min_(mL, 0.0), -1.0), B = min_(max_(mR, 0.0), 1.0);
+  const double f = -A * B;
+  const double vnl = (1.0 - f) * vn + f * aul, vnr = (1.0 - f) * vn + f * aur;
+  const double Fr = 0.5 * (rl * (unl + vnl) + rr * (unr - vnr) - chi * iaF * (pr - pl));
+  const bool inL = fabs(mL) < 1.0, inR = fabs(mR) < 1.0;
+  const double bl = inL ? 0.25 * (2.0 - mL) * (mL + 1.0) * (mL + 1.0) : (mL >= 0.0 ? 1.0 : 0.0);
+  const double br = inR ? 0.25 * (2.0 + mR) * (mR - 1.0) * (mR - 1.0) : (mR >= 0.0 ? 0.0 : 1.0);
+  const double kg = g.gam * g.rgm1;
+  const bool left = Fr >= 0.0;
+  // ---- reverse: F = (Fr, Fr u_s + n ph, Fr h0_s)
+  double rlb = 0, ulb = 0, vlb = 0, wlb = 0, plb = 0, rrb = 0, urb = 0, vrb = 0, wrb = 0, prb = 0;
+  double unlb = 0, unrb = 0, sqlb = 0, sqrb = 0, aFb = 0, iaFb = 0, vrmsb = 0, chib = 0;
+  double Fr_b = Fb[0];
+  const double phb = Fb[1] * nx + Fb[2] * ny + Fb[3] * nz;
+  double ph = 0.5 * (pl + pr) + 0.5 * (bl - br) * (pl - pr);
+  if constexpr (V2) ph += vrms * (bl + br - 1.0) * aF * 0.5 * rs;
+  else ph += (1.0 - chi) * (bl + br - 1.0) * 0.5 * (pl + pr);
+  nb[0] += Fb[1] * ph; nb[1] += Fb[2] * ph; nb[2] += Fb[3] * ph;
+  if (left) {
+    const double h0 = kg * pl * irl + 0.5 * sql;
+    Fr_b += Fb[1] * ul + Fb[2] * vl + Fb[3] * wl + Fb[4] * h0;
+    ulb += Fb[1] * Fr; vlb += Fb[2] * Fr; wlb += Fb[3] * Fr;
+    const double hb = Fb[4] * Fr;
+    plb += hb * kg * irl; rlb -= hb * kg * pl * irl * irl; sqlb += 0.5 * hb;
+  } else {
+    const double h0 = kg * pr * irr + 0.5 * sqr;
+    Fr_b += Fb[1] * ur + Fb[2] * vr + Fb[3] * wr + Fb[4] * h0;
+    urb += Fb[1] * Fr; vrb += Fb[2] * Fr; wrb += Fb[3] * Fr;
+    const double hb = Fb[4] * Fr;
+    prb += hb * kg * irr; rrb -= hb * kg * pr * irr * irr; sqrb += 0.5 * hb;
+  }
+  // pressure flux
+  double blb = 0, brb = 0;
+  {
+    const double S = pl + pr, D = pl - pr, W = bl + br - 1.0;
+    plb += phb * (0.5 + 0.5 * (bl - br)); prb += phb * (0.5 - 0.5 * (bl - br));
+    blb += phb * 0.5 * D; brb -= phb * 0.5 * D;
+    if constexpr (V2) {
+      const double c = vrms * aF * 0.5 * rs;
+      blb += phb * c; brb += phb * c;
+      vrmsb += phb * W * aF * 0.5 * rs;
+      aFb += phb * vrms * W * 0.5 * rs;
+      rlb += phb * vrms * W * aF * 0.5; rrb += phb * vrms * W * aF * 0.5;
+    } else {
+      const double c = (1.0 - chi) * 0.5 * S;
+      blb += phb * c; brb += phb * c;
+      chib -= phb * W * 0.5 * S;
+      plb += phb * (1.0 - chi) * W * 0.5; prb += phb * (1.0 - chi) * W * 0.5;
+    }
+  }
+  double mLb = 0, mRb = 0;
+  if (inL) mLb += blb * 0.75 * (1.0 - mL * mL);
+  if (inR) mRb += brb * 0.75 * (mR * mR - 1.0);
+  // mass flux
+  double vnlb, vnrb;
+  {
+    const double h = 0.5 * Fr_b;
+    rlb += h * (unl + vnl); unlb += h * rl; vnlb = h * rl;
+    rrb += h * (unr - vnr); unrb += h * rr; vnrb = -h * rr;
+    chib -= h * iaF * (pr - pl); iaFb -= h * chi * (pr - pl);
+    prb -= h * chi * iaF; plb += h * chi * iaF;
+  }
+  const double vnb = (vnlb + vnrb) * (1.0 - f);
+  const double fb = vnlb * (aul - vn) + vnrb * (aur - vn);
+  double aulb = vnlb * f, aurb = vnrb * f;
+  // f = -A B, A = clip(mL, -1, 0), B = clip(mR, 0, 1)
+  if (mL < 0.0 && mL > -1.0) mLb -= fb * B;
+  if (mR > 0.0 && mR < 1.0) mRb -= fb * A;
+  // vn = (rl |unl| + rr |unr|) / (rl + rr)
+  rlb += vnb * (aul - vn) * irs; rrb += vnb * (aur - vn) * irs;
+  aulb += vnb * rl * irs; aurb += vnb * rr * irs;
+  unlb += unl >= 0.0 ? aulb : -aulb;
+  unrb += unr >= 0.0 ? aurb : -aurb;
+  // chi = (1 - mt)^2, mt = min_(1, vrms / aF)
+  if (!(1.0 < mraw)) {
+    const double mtb = -2.0 * (1.0 - mt) * chib;
+    vrmsb += mtb * iaF; iaFb += mtb * vrms;
+  }
+  if (vrms > 0.0) {
+    const double qb = vrmsb * 0.5 / vrms;
+    sqlb += 0.5 * qb; sqrb += 0.5 * qb;
+  }
+  // Mach numbers, aF
+  unlb += mLb * iaF; unrb += mRb * iaF; iaFb += mLb * unl + mRb * unr;
+  aFb -= iaFb * iaF * iaF;
+  {
+    const double tl = 0.5 * aFb * 0.5 / al * g.gam, tr = 0.5 * aFb * 0.5 / ar * g.gam;
+    plb += tl * irl; rlb -= tl * pl * irl * irl;
+    prb += tr * irr; rrb -= tr * pr * irr * irr;
+  }
+  ulb += unlb * nx + 2.0 * ul * sqlb; vlb += unlb * ny + 2.0 * vl * sqlb; wlb += unlb * nz + 2.0 * wl * sqlb;
+  urb += unrb * nx + 2.0 * ur * sqrb; vrb += unrb * ny + 2.0 * vr * sqrb; wrb += unrb * nz + 2.0 * wr * sqrb;
+  nb[0] += unlb * ul + unrb * ur;
+  nb[1] += unlb * vl + unrb * vr;
+  nb[2] += unlb * wl + unrb * wr;
+  Lb[0] += rlb; Lb[1] += ulb; Lb[2] += vlb; Lb[3] += wlb; Lb[4] += plb;
+  Rb[0] += rrb; Rb[1] += urb; Rb[2] += vrb; Rb[3] += wrb; Rb[4] += prb;
+}
+
 // ------------------------------------------------------------------------------ AUSM+
 __device__ void ausmp_adj(const double* L, const double* R, double nx, double ny, double nz, const GasT<double>& g,
                           const double* Fb, double* Lb, double* Rb, double* nb) {
@@ -456,6 +575,8 @@ __device__ __forceinline__ void riemann_adj(const double* L, const double* R, do
   double nd[3] = {0.0, 0.0, 0.0};
   if constexpr (RS == RS_AUSMP) ausmp_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb, nd);
   else if constexpr (RS == RS_HLLE) hlle_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb, nd);
+  else if constexpr (RS == RS_SLAU) slau_adj<false>(L, R, nx, ny, nz, g, Fb, Lb, Rb, nd);
+  else if constexpr (RS == RS_SLAU2) slau_adj<true>(L, R, nx, ny, nz, g, Fb, Lb, Rb, nd);
   else hllc_adj(L, R, nx, ny, nz, g, Fb, Lb, Rb, nd);
   if (nb) { nb[0] += nd[0]; nb[1] += nd[1]; nb[2] += nd[2]; }
 }

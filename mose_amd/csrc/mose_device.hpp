@@ -9,6 +9,8 @@
 //   reconstruction      src/lib/numerics/space/Lib_Reconstruction.f90:10-29
 //   HLLC (Batten)       src/lib/numerics/fluxes/riemann/Lib_Riemann_HLL.f90:147-258,550-647
 //   AUSM+               src/lib/numerics/fluxes/riemann/Lib_Riemann_AUSM.f90:150-238
+//   HLLE                src/lib/numerics/fluxes/riemann/Lib_Riemann_HLL.f90:261-320
+//   SLAU, SLAU2         src/lib/numerics/fluxes/riemann/Lib_Riemann_SLAU.f90:19-140
 //   interface flux      src/lib/numerics/fluxes/Lib_Convective.f90:10-85
 // Gas: single species, calorically perfect (the FLINT tables of the reference are absent;
 // with tables linear in T the reference's Roe averages reduce to the forms used here).
@@ -19,7 +21,7 @@
 namespace mose {
 
 enum { LIM_MINMOD = 0, LIM_VANLEER = 1, LIM_VANALBADA = 2, LIM_FIRST = 3, LIM_MC = 4, LIM_SUPERBEE = 5 };
-enum { RS_HLLC = 0, RS_AUSMP = 1, RS_HLLE = 2 };
+enum { RS_HLLC = 0, RS_AUSMP = 1, RS_HLLE = 2, RS_SLAU = 3, RS_SLAU2 = 4 };
 
 template <typename T>
 struct GasT {
@@ -253,10 +255,53 @@ __device__ __forceinline__ void hlle(const T* L, const T* R, T nx, T ny, T nz, c
   F[4] = a * (FL * h0l) + b * (FR * h0r) + c * (rr * h0r - pr - (rl * h0l - pl));
 }
 
+// SLAU (Shima & Kitamura) and SLAU2 (Kitamura & Shima), riemann_SLAU_base / riemann_SLAU /
+// riemann_SLAU2 (Lib_Riemann_SLAU.f90:19-140): mean sound speed aF = (aL + aR)/2, the
+// low-Mach weight chi = (1 - min(1, |u|_rms/aF))^2, a density-weighted |u_n| blended with
+// each side's by f = -clip(mL, -1, 0) clip(mR, 0, 1), cubic pressure weights BetaL/R; SLAU2
+// replaces the chi term of the pressure flux by |u|_rms (BetaL + BetaR - 1) aF (rhoL + rhoR)/2.
+template <bool V2, typename T>
+__device__ __forceinline__ void slau(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F) {
+  const T rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
+  const T rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
+  const T irl = rcp_(rl), irr = rcp_(rr);
+  const T al = sqrtp_(g.gam * pl * irl), ar = sqrtp_(g.gam * pr * irr);
+  const T unl = ul * nx + vl * ny + wl * nz;
+  const T unr = ur * nx + vr * ny + wr * nz;
+  const T sql = ul * ul + vl * vl + wl * wl, sqr = ur * ur + vr * vr + wr * wr;
+  const T aF = T(0.5) * (al + ar);
+  const T iaF = rcp_(aF);
+  const T mL = unl * iaF, mR = unr * iaF;
+  const T vrms = sqrt_(T(0.5) * (sql + sqr));          // may be 0: the general square root
+  const T mt = min_(T(1), vrms * iaF);
+  const T chi = (T(1) - mt) * (T(1) - mt);
+  const T aul = fabs_(unl), aur = fabs_(unr);
+  const T vn = (rl * aul + rr * aur) * rcp_(rl + rr);
+  const T f = -max_(min_(mL, T(0)), T(-1)) * min_(max_(mR, T(0)), T(1));
+  const T vnl = (T(1) - f) * vn + f * aul, vnr = (T(1) - f) * vn + f * aur;
+  const T Fr = T(0.5) * (rl * (unl + vnl) + rr * (unr - vnr) - chi * iaF * (pr - pl));
+  const T bl = fabs_(mL) < T(1) ? T(0.25) * (T(2) - mL) * (mL + T(1)) * (mL + T(1)) : (mL >= T(0) ? T(1) : T(0));
+  const T br = fabs_(mR) < T(1) ? T(0.25) * (T(2) + mR) * (mR - T(1)) * (mR - T(1)) : (mR >= T(0) ? T(0) : T(1));
+  T ph = T(0.5) * (pl + pr) + T(0.5) * (bl - br) * (pl - pr);
+  if constexpr (V2) ph += vrms * (bl + br - T(1)) * aF * T(0.5) * (rl + rr);
+  else ph += (T(1) - chi) * (bl + br - T(1)) * T(0.5) * (pl + pr);
+  const T kgl = g.gam * g.rgm1;
+  const bool left = Fr >= T(0);
+  const T u_ = left ? ul : ur, v_ = left ? vl : vr, w_ = left ? wl : wr;
+  const T h0 = left ? kgl * pl * irl + T(0.5) * sql : kgl * pr * irr + T(0.5) * sqr;
+  F[0] = Fr;
+  F[1] = Fr * u_ + nx * ph;
+  F[2] = Fr * v_ + ny * ph;
+  F[3] = Fr * w_ + nz * ph;
+  F[4] = Fr * h0;
+}
+
 template <int RS, typename T>
 __device__ __forceinline__ void riemann(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F) {
   if constexpr (RS == RS_AUSMP) ausmp(L, R, nx, ny, nz, g, F);
   else if constexpr (RS == RS_HLLE) hlle(L, R, nx, ny, nz, g, F);
+  else if constexpr (RS == RS_SLAU) slau<false>(L, R, nx, ny, nz, g, F);
+  else if constexpr (RS == RS_SLAU2) slau<true>(L, R, nx, ny, nz, g, F);
   else hllc(L, R, nx, ny, nz, g, F);
 }
 

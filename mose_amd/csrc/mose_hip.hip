@@ -710,7 +710,7 @@ extern "C" {
 int mose_create(const mose_cfg* cfg, mose_ctx** out) {
   if (!cfg || !out) return MOSE_E_ARG;
   *out = nullptr;
-  if (cfg->limiter < 0 || cfg->limiter > 5 || cfg->riemann < 0 || cfg->riemann > 2 || cfg->n_rk < 1 ||
+  if (cfg->limiter < 0 || cfg->limiter > 5 || cfg->riemann < 0 || cfg->riemann > 4 || cfg->n_rk < 1 ||
       cfg->n_rk > 3 || !(cfg->gamma > 1.0) || !(cfg->Rgas > 0.0) || cfg->precision < 0 || cfg->precision > 1)
     return MOSE_E_ARG;
   int ndev = 0;

@@ -53,6 +53,8 @@ void fill_tables(KTable& k, ATable& a) {
   solver_entries<LIM, RS_HLLC>(k, a);
   solver_entries<LIM, RS_AUSMP>(k, a);
   solver_entries<LIM, RS_HLLE>(k, a);
+  solver_entries<LIM, RS_SLAU>(k, a);
+  solver_entries<LIM, RS_SLAU2>(k, a);
   a.cell[LIM][0] = k_adj_cell<LIM, 0>;
   a.cell[LIM][1] = k_adj_cell<LIM, 1>;
 }

@@ -11,7 +11,7 @@ typedef void (*BcFn)(mose::StageArgs, const mose::BcFace*, int);
 typedef void (*AdjFn)(AdjArgs);
 
 constexpr int NLIM = 6;     // minmod, van Leer, van Albada, first order, MC, superbee
-constexpr int NRS = 3;      // HLLC, AUSM+, HLLE
+constexpr int NRS = 5;      // HLLC, AUSM+, HLLE, SLAU, SLAU2
 // Patch shapes (TX x TY threads) of the stage kernel per precision; shape 0 is the default,
 // the others are compiled for the van Leer limiter and selected by the tile tuner.
 constexpr int NSHAPE = 3;

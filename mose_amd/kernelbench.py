@@ -6,7 +6,7 @@ HIP events (mose_time_kernels), then runs a few full steps.  Used under rocprofv
     rocprofv3 --kernel-trace --stats -d <dir> -- python3 -m mose_amd.kernelbench --n 256
     rocprofv3 --pmc FETCH_SIZE -d <dir> -- python3 -m mose_amd.kernelbench --n 128 --reps 3
 
-Options: --prec fp64|fp32, --riemann HLLC|AUSM+|HLLE, --limiter vanleer|minmod|vanalbada,
+Options: --prec fp64|fp32, --riemann HLLC|AUSM+|HLLE|SLAU|SLAU2, --limiter vanleer|minmod|vanalbada,
 --autotune picks the tile shape / k chunk / tile order by timing first,
 --copy also runs a plain device-to-device copy of 5 n^3 doubles (hipMemcpy) so the HBM
 byte counters (FETCH_SIZE / WRITE_SIZE) can be calibrated against a kernel of known size.

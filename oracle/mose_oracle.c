@@ -285,6 +285,51 @@ static void riemann_ausmp(const ob_domain* d, double dl, double ul, double vl, d
   F[4] = mdotp * H_L + mdotm * H_R;
 }
 
+/* riemann_SLAU_base, Lib_Riemann_SLAU.f90:19-82: mass flux and the pressure weights shared by
+ * SLAU (:85-114, Shima & Kitamura) and SLAU2 (:117-140, Kitamura & Shima); v2 selects the
+ * pressure flux. */
+static void riemann_slau(const ob_domain* d, int v2, double ul, double vl, double wl, double pl,
+                         double al, double dltot, double ur, double vr, double wr, double pr,
+                         double ar, double drtot, double nx, double ny, double nz, double* F) {
+  double unl = ul * nx + vl * ny + wl * nz;
+  double unr = ur * nx + vr * ny + wr * nz;
+  double sq_veli = ul * ul + vl * vl + wl * wl;
+  double sq_velj = ur * ur + vr * vr + wr * wr;
+  double aF = 0.5 * (al + ar);
+  double mL = unl / aF, mR = unr / aF;
+  double Mach_tilde = fmin(1.0, sqrt(0.5 * (sq_veli + sq_velj)) / aF);
+  double Chi = (1.0 - Mach_tilde) * (1.0 - Mach_tilde);
+  double Vn_mag = (dltot * fabs(unl) + drtot * fabs(unr)) / (dltot + drtot);
+  double f_rho = -fmax(fmin(mL, 0.0), -1.0) * fmin(fmax(mR, 0.0), 1.0);
+  double Vn_magL = (1.0 - f_rho) * Vn_mag + f_rho * fabs(unl);
+  double Vn_magR = (1.0 - f_rho) * Vn_mag + f_rho * fabs(unr);
+  double F_r = 0.5 * (dltot * (unl + Vn_magL) + drtot * (unr - Vn_magR) - (Chi / aF) * (pr - pl));
+  double BetaL, BetaR;
+  if (fabs(mL) < 1.0) BetaL = 0.25 * (2.0 - mL) * (mL + 1.0) * (mL + 1.0);
+  else if (mL >= 0.0) BetaL = 1.0;
+  else BetaL = 0.0;
+  if (fabs(mR) < 1.0) BetaR = 0.25 * (2.0 + mR) * (mR - 1.0) * (mR - 1.0);
+  else if (mR >= 0.0) BetaR = 0.0;
+  else BetaR = 1.0;
+  double p_half;
+  if (!v2)
+    p_half = 0.5 * (pl + pr) + 0.5 * (BetaL - BetaR) * (pl - pr) +
+             (1.0 - Chi) * (BetaL + BetaR - 1.0) * 0.5 * (pl + pr);
+  else
+    p_half = 0.5 * (pl + pr) + 0.5 * (BetaL - BetaR) * (pl - pr) +
+             sqrt(0.5 * (sq_veli + sq_velj)) * (BetaL + BetaR - 1.0) * aF * 0.5 * (dltot + drtot);
+  double H_L = H0(d, pl, dltot, sqrt(sq_veli));
+  double H_R = H0(d, pr, drtot, sqrt(sq_velj));
+  F[0] = F_r;
+  if (F_r >= 0.0) {
+    F[1] = F_r * ul + nx * p_half; F[2] = F_r * vl + ny * p_half; F[3] = F_r * wl + nz * p_half;
+    F[4] = F_r * H_L;
+  } else {
+    F[1] = F_r * ur + nx * p_half; F[2] = F_r * vr + ny * p_half; F[3] = F_r * wr + nz * p_half;
+    F[4] = F_r * H_R;
+  }
+}
+
 /* The `Riemann` procedure pointer (Mod_Riemann.f90:9-28,32-114) */
 static void riemann(const ob_domain* d, const double* PL, double aL, double rL,
                     const double* PR, double aR, double rR, const double* n, double* F) {
@@ -294,6 +339,9 @@ static void riemann(const ob_domain* d, const double* PL, double aL, double rL,
   else if (d->cfg.riemann == MOSE_RS_HLLE)
     riemann_hlle(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
                  n[0], n[1], n[2], F);
+  else if (d->cfg.riemann == MOSE_RS_SLAU || d->cfg.riemann == MOSE_RS_SLAU2)
+    riemann_slau(d, d->cfg.riemann == MOSE_RS_SLAU2, PL[1], PL[2], PL[3], PL[4], aL, rL, PR[1], PR[2], PR[3], PR[4],
+                 aR, rR, n[0], n[1], n[2], F);
   else
     riemann_hllc(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
                  n[0], n[1], n[2], F);

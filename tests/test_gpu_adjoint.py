@@ -20,7 +20,7 @@ from helpers import interior, perturb, skewed_nodes
 pytestmark = pytest.mark.gpu
 
 LIMS = ["minmod", "vanleer", "vanalbada", "mc", "superbee"]
-RSS = ["HLLC", "AUSM+", "HLLE"]
+RSS = ["HLLC", "AUSM+", "HLLE", "SLAU", "SLAU2"]
 INFLOW = {"type": 405, "mach": 1.8, "T": 1.05, "p": 0.7, "alpha": 0.1, "beta": -0.05, "rel_fac": 0.8}
 
 

@@ -30,7 +30,7 @@ def _pm_case(limiter="vanleer", riemann="HLLC", amp=0.0):
     return perturb(case, amp=amp, seed=3) if amp else case
 
 
-@pytest.mark.parametrize("rs", ["HLLC", "AUSM+", "HLLE"])
+@pytest.mark.parametrize("rs", ["HLLC", "AUSM+", "HLLE", "SLAU", "SLAU2"])
 def test_prandtl_meyer_residual_and_steps(rs):
     case = _pm_case(riemann=rs, amp=0.02)
     g, o = GpuSolver(case), OracleSolver(case)
@@ -52,6 +52,26 @@ def test_sod79_input_steps():
     assert abs(tg - to) <= 1e-13 * to
     assert rel_l2(g.get_state(1), o.get_state(1)) < 1e-12
     g.close()
+
+
+@pytest.mark.parametrize("rs", ["SLAU", "SLAU2"])
+def test_sod79_shipped_configuration(rs):
+    """test/1D/Sod79 with its input.ini settings (MC + SLAU, RK3, CFL 0.9) on its 100-cell
+    input: GPU state equal to the oracle's at t = 0.2 and the reference's check.py rule."""
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.9, "mc", rs, 3, True)
+    case = formats.load_case(SOD_IC, SOD_BC, cfg)
+    g, o = GpuSolver(case), OracleSolver(case)
+    t = 0.0
+    while t < 0.2:
+        _, t, _ = g.step()
+        _, to, _ = o.step()
+    assert abs(t - to) <= 1e-13 * to
+    assert rel_l2(g.get_state(1), o.get_state(1)) < 1e-12
+    P = interior(g.get_state(1))[0, 0]
+    g.close()
+    xc = 0.5 * (case.blocks[0].nodes[0, 0, 1:, 0] + case.blocks[0].nodes[0, 0, :-1, 0])
+    table = np.loadtxt(os.path.join(GOLD, "sod79_exact.txt"))
+    assert not er.probe_check(xc, P[:, 0], P[:, 1], P[:, 4], P[:, 4] / (0.4 * P[:, 0]), table, 1.0)
 
 
 def _input_1d(name, limiter, cfl):

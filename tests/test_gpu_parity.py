@@ -17,7 +17,7 @@ from helpers import flux_scale, interior, perturb, rel_l2
 pytestmark = pytest.mark.gpu
 
 LIMS = ["minmod", "vanleer", "vanalbada", "mc", "superbee"]
-RSS = ["HLLC", "AUSM+", "HLLE"]
+RSS = ["HLLC", "AUSM+", "HLLE", "SLAU", "SLAU2"]
 
 
 def _pair(case):

@@ -83,7 +83,7 @@ def _box(dims, faces, limiter="vanalbada", riemann="HLLC", mach=1.5, seed=3):
     return perturb(cases.Case("box", cfg, [blk], bc), amp=0.03, seed=seed)
 
 
-@pytest.mark.parametrize("rs", ["HLLC", "AUSM+", "HLLE"])
+@pytest.mark.parametrize("rs", ["HLLC", "AUSM+", "HLLE", "SLAU", "SLAU2"])
 @pytest.mark.parametrize("lim", ["vanleer", "vanalbada"])
 def test_node_adjoint_boundary_types(lim, rs):
     case = _box((9, 7, 5), [INFLOW, 400, 300, 300, 400, 0], limiter=lim, riemann=rs)

@@ -33,7 +33,7 @@ def _residual_check(case, tol=1e-12):
     g.close()
 
 
-@pytest.mark.parametrize("rs", ["HLLC", "AUSM+", "HLLE"])
+@pytest.mark.parametrize("rs", ["HLLC", "AUSM+", "HLLE", "SLAU", "SLAU2"])
 def test_viscous_residual_periodic_box(rs):
     case = _viscous(perturb(cases.vortex(dims=(14, 11, 9), riemann=rs), seed=21))
     _residual_check(case)

@@ -67,6 +67,25 @@ def test_exact_two_rarefaction_closed_form():
     assert np.allclose(u, -u2, atol=1e-12)
 
 
+@pytest.mark.parametrize("riemann", ["SLAU", "SLAU2"])
+def test_sod79_shipped_configuration_passes_check_py(riemann):
+    """The reference's own Sod79 settings (test/1D/Sod79/input.ini: MC limiter, SLAU, RK3,
+    CFL 0.9) on its own 100-cell input (INPUT/ic_x1.tec, bc_x1.txt) pass its acceptance rule
+    (check.py:53-82) at t = 0.2, including the velocity probes where the exact velocity is 0:
+    SLAU's flux of two equal states is exact, so the undisturbed gas stays exactly at rest."""
+    from mose_amd import formats
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.9, "mc", riemann, 3, True)
+    case = formats.load_case(os.path.join(GOLDEN, "sod79_ic_x1.tec"), os.path.join(GOLDEN, "sod79_bc_x1.txt"), cfg)
+    o = OracleSolver(case)
+    t = 0.0
+    while t < 0.2:
+        _, t, _ = o.step()
+    P = interior(o.get_state(1))[0, 0]
+    xc = cases.cell_centres(case.blocks[0].nodes)[0, 0, :, 0]
+    table = np.loadtxt(os.path.join(GOLDEN, "sod79_exact.txt"))
+    assert not er.probe_check(xc, P[:, 0], P[:, 1], P[:, 4], P[:, 4] / (0.4 * P[:, 0]), table, 1.0)
+
+
 def _reference_1d_input(name, limiter, cfl):
     """A case from the reference's own 1-D input files (test/1D/<name>/INPUT, 800 cells),
     with the numerics of its input.ini."""

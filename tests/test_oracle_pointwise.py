@@ -156,7 +156,7 @@ def rand_normal(rng):
     return n / np.linalg.norm(n)
 
 
-@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE])
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE, abi.RS_SLAU, abi.RS_SLAU2])
 def test_riemann_consistency(rs):
     cfg = abi.make_cfg(riemann=rs)
     rng = np.random.default_rng(1)
@@ -183,7 +183,7 @@ def test_riemann_supersonic_upwinding(rs):
         assert np.allclose(F, physical_flux(R, n), rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP])
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_SLAU, abi.RS_SLAU2])
 def test_riemann_stationary_contact(rs):
     cfg = abi.make_cfg(riemann=rs)
     rng = np.random.default_rng(3)
@@ -197,7 +197,7 @@ def test_riemann_stationary_contact(rs):
         assert np.allclose(F, [0, p * n[0], p * n[1], p * n[2], 0], atol=1e-13)
 
 
-@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE])
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE, abi.RS_SLAU, abi.RS_SLAU2])
 def test_riemann_mirror_symmetry(rs):
     """F(L, R, n) = -F(R, L, -n) componentwise (relabelling the two sides)."""
     cfg = abi.make_cfg(riemann=rs)
@@ -209,7 +209,7 @@ def test_riemann_mirror_symmetry(rs):
         assert np.allclose(F1, -F2, rtol=1e-11, atol=1e-12)
 
 
-@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP])
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_SLAU, abi.RS_SLAU2])
 def test_riemann_moving_contact_is_upwinded(rs):
     """An isolated contact moving with velocity u (equal u and p, different rho) is resolved
     exactly: the flux is the physical flux of the upwind side."""
@@ -226,6 +226,26 @@ def test_riemann_moving_contact_is_upwinded(rs):
         F = O.riemann(cfg, L, R, n)
         up = L if un > 0 else R
         assert np.allclose(F, physical_flux(up, n), rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("rs", [abi.RS_SLAU, abi.RS_SLAU2])
+def test_slau_pressure_jump_at_rest(rs):
+    """Gas at rest with a pressure jump: the SLAU mass flux is its pressure-diffusion term
+    alone, -chi (pR - pL) / (2 aF) with chi = 1 at zero velocity, i.e. (pL - pR)/(aL + aR);
+    the pressure flux is the mean (pL + pR)/2 (BetaL = BetaR = 1/2 at M = 0, and SLAU2's
+    extra term carries |u| = 0); the energy flux carries the upwind side's total enthalpy."""
+    cfg = abi.make_cfg(riemann=rs)
+    rng = np.random.default_rng(12)
+    for _ in range(50):
+        n = rand_normal(rng)
+        L = np.array([rng.uniform(0.3, 2.0), 0, 0, 0, rng.uniform(0.3, 2.0)])
+        R = np.array([rng.uniform(0.3, 2.0), 0, 0, 0, rng.uniform(0.3, 2.0)])
+        aL, aR = np.sqrt(G * L[4] / L[0]), np.sqrt(G * R[4] / R[0])
+        m = (L[4] - R[4]) / (aL + aR)
+        h = (G / (G - 1)) * (L[4] / L[0] if m >= 0 else R[4] / R[0])
+        F = O.riemann(cfg, L, R, n)
+        expect = [m, *(0.5 * (L[4] + R[4]) * n), m * h]
+        assert np.allclose(F, expect, rtol=1e-13, atol=1e-14)
 
 
 def cons(P):
