@@ -39,7 +39,12 @@ enum { MOSE_LIM_MINMOD = 0, MOSE_LIM_VANLEER = 1, MOSE_LIM_VANALBADA = 2, MOSE_L
 /* Riemann solvers: riemann_HLLC (Lib_Riemann_HLL.f90:147-258), riemann_AUSMp
  * (Lib_Riemann_AUSM.f90:150-238), riemann_HLLE (Lib_Riemann_HLL.f90:261-320), riemann_SLAU and
  * riemann_SLAU2 (Lib_Riemann_SLAU.f90:19-140) */
-enum { MOSE_RS_HLLC = 0, MOSE_RS_AUSMP = 1, MOSE_RS_HLLE = 2, MOSE_RS_SLAU = 3, MOSE_RS_SLAU2 = 4 };
+enum { MOSE_RS_HLLC = 0, MOSE_RS_AUSMP = 1, MOSE_RS_HLLE = 2, MOSE_RS_SLAU = 3, MOSE_RS_SLAU2 = 4,
+       MOSE_RS_HLLEPP = 5, MOSE_RS_HLLCP = 6 };
+/* MOSE_RS_HLLEPP: riemann_HLLEpp (Tramel's HLLE++, Lib_Riemann_HLL.f90:321-440); MOSE_RS_HLLCP:
+ * riemann_HLLCSD ('HLLC+', :486-534).  Both take the shock-detector weight SD_Tramel
+ * (Lib_Shock_Detector.f90:9-38) of the face's low cell (of the block's own boundary cell on a
+ * connection face), computed from the pressure of the cell and its six face neighbours. */
 /* Arithmetic of reconstruction + Riemann flux; state, accumulation and reductions are always FP64. */
 enum { MOSE_PREC_FP64 = 0, MOSE_PREC_FP32_FLUX = 1 };
 
@@ -65,7 +70,9 @@ typedef struct {
   int32_t time_accurate;    /* [MOSE-Numerics] time-accurate: 1 global dt, 0 local dt          */
   int32_t rampa_cfl_iter;   /* CFL ramp length in iterations (0: off), Mod_dt.f90:45           */
   int32_t precision;        /* MOSE_PREC_*                                                     */
-  int32_t reserved[4];
+  int32_t muscl_sd;         /* [MOSE-Numerics] space-reconstruction = MUSCL-SD: limited slopes  */
+                            /* scaled by the shock-detector weight (Lib_Reconstruction.f90:26-27) */
+  int32_t reserved[3];
 } mose_cfg;
 
 /* One boundary-face-cell record of bc.txt (src/lib/io/IO_BC.f90:136-414).

@@ -10,14 +10,15 @@ import ctypes as C
 import numpy as np
 
 LIM_MINMOD, LIM_VANLEER, LIM_VANALBADA, LIM_FIRST_ORDER, LIM_MC, LIM_SUPERBEE = 0, 1, 2, 3, 4, 5
-RS_HLLC, RS_AUSMP, RS_HLLE, RS_SLAU, RS_SLAU2 = 0, 1, 2, 3, 4
+RS_HLLC, RS_AUSMP, RS_HLLE, RS_SLAU, RS_SLAU2, RS_HLLEPP, RS_HLLCP = 0, 1, 2, 3, 4, 5, 6
 PREC_FP64, PREC_FP32_FLUX = 0, 1
 
 E_OK, E_ARG, E_HIP, E_STATE, E_INTERFACE, E_METRIC, E_BC, E_NOGPU = range(8)
 
 LIMITERS = {"minmod": LIM_MINMOD, "vanleer": LIM_VANLEER, "vanalbada": LIM_VANALBADA,
             "first-order": LIM_FIRST_ORDER, "mc": LIM_MC, "superbee": LIM_SUPERBEE}
-RIEMANN = {"HLLC": RS_HLLC, "AUSM+": RS_AUSMP, "HLLE": RS_HLLE, "SLAU": RS_SLAU, "SLAU2": RS_SLAU2}
+RIEMANN = {"HLLC": RS_HLLC, "AUSM+": RS_AUSMP, "HLLE": RS_HLLE, "SLAU": RS_SLAU, "SLAU2": RS_SLAU2,
+           "HLLE++": RS_HLLEPP, "HLLC+": RS_HLLCP}
 
 NPRIM = 5
 GC = 2
@@ -27,7 +28,7 @@ class MoseCfg(C.Structure):
     _fields_ = [("gamma", C.c_double), ("Rgas", C.c_double), ("cfl", C.c_double),
                 ("limiter", C.c_int32), ("riemann", C.c_int32), ("n_rk", C.c_int32),
                 ("time_accurate", C.c_int32), ("rampa_cfl_iter", C.c_int32),
-                ("precision", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("precision", C.c_int32), ("muscl_sd", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class MoseBcEntry(C.Structure):
@@ -55,12 +56,13 @@ assert BC_DTYPE.itemsize == C.sizeof(MoseBcEntry) == 112
 
 
 def make_cfg(gamma=1.4, Rgas=1.0 / 1.4, cfl=0.5, limiter="minmod", riemann="HLLC", n_rk=3,
-             time_accurate=True, rampa_cfl_iter=0, precision=PREC_FP64):
-    """Build a MoseCfg from the reference's key values (flux-limiter, riemann-solver, ...)."""
+             time_accurate=True, rampa_cfl_iter=0, precision=PREC_FP64, muscl_sd=False):
+    """Build a MoseCfg from the reference's key values (flux-limiter, riemann-solver, ...);
+    muscl_sd: space-reconstruction = MUSCL-SD."""
     lim = LIMITERS[limiter] if isinstance(limiter, str) else int(limiter)
     rs = RIEMANN[riemann] if isinstance(riemann, str) else int(riemann)
     return MoseCfg(gamma, Rgas, cfl, lim, rs, n_rk, int(bool(time_accurate)), rampa_cfl_iter,
-                   precision)
+                   precision, int(bool(muscl_sd)))
 
 
 def padded_shape(dims):

@@ -11,6 +11,8 @@
 //   AUSM+               src/lib/numerics/fluxes/riemann/Lib_Riemann_AUSM.f90:150-238
 //   HLLE                src/lib/numerics/fluxes/riemann/Lib_Riemann_HLL.f90:261-320
 //   SLAU, SLAU2         src/lib/numerics/fluxes/riemann/Lib_Riemann_SLAU.f90:19-140
+//   HLLE++, HLLC+       src/lib/numerics/fluxes/riemann/Lib_Riemann_HLL.f90:321-534
+//   shock detector      src/lib/numerics/fluxes/Lib_Shock_Detector.f90:9-38
 //   interface flux      src/lib/numerics/fluxes/Lib_Convective.f90:10-85
 // Gas: single species, calorically perfect (the FLINT tables of the reference are absent;
 // with tables linear in T the reference's Roe averages reduce to the forms used here).
@@ -21,7 +23,9 @@
 namespace mose {
 
 enum { LIM_MINMOD = 0, LIM_VANLEER = 1, LIM_VANALBADA = 2, LIM_FIRST = 3, LIM_MC = 4, LIM_SUPERBEE = 5 };
-enum { RS_HLLC = 0, RS_AUSMP = 1, RS_HLLE = 2, RS_SLAU = 3, RS_SLAU2 = 4 };
+enum { RS_HLLC = 0, RS_AUSMP = 1, RS_HLLE = 2, RS_SLAU = 3, RS_SLAU2 = 4, RS_HLLEPP = 5, RS_HLLCP = 6 };
+// Riemann solvers that take the shock detector's weight beta (obj_riemann%SD, Mod_Riemann.f90:74-81)
+constexpr bool rs_takes_beta(int rs) { return rs == RS_HLLEPP || rs == RS_HLLCP; }
 
 template <typename T>
 struct GasT {
@@ -296,9 +300,94 @@ __device__ __forceinline__ void slau(const T* L, const T* R, T nx, T ny, T nz, c
   F[4] = Fr * h0;
 }
 
+// HLLE++ (Tramel), riemann_HLLEpp (Lib_Riemann_HLL.f90:321-440): the central flux minus the
+// Roe wave decomposition (contact, two shear, two acoustic waves) with eigenvalue magnitudes
+// beta |lambda|_Roe (Harten-Hyman fix, eps = 0.3 (|u_n| + a)) + (1 - beta) |lambda|_HLLE,
+// |lambda|_HLLE = |((S4 + S1) lambda - 2 S4 S1) / (S4 - S1)| with Batten's S1, S4.
+template <typename T>
+__device__ __forceinline__ T harten_hyman(T v, T e) {
+  const T av = fabs_(v);
+  return av <= e ? (v * v + e * e) * rcp_(T(2) * e) : av;
+}
+
+template <typename T>
+__device__ __forceinline__ void hllepp(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T beta, T* F) {
+  const T rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
+  const T rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
+  const T irl = rcp_(rl), irr = rcp_(rr);
+  const T al = sqrtp_(g.gam * pl * irl), ar = sqrtp_(g.gam * pr * irr);
+  const T unl = ul * nx + vl * ny + wl * nz;
+  const T unr = ur * nx + vr * ny + wr * nz;
+  const T kgl = g.gam * g.rgm1;
+  const T h0l = kgl * pl * irl + T(0.5) * (ul * ul + vl * vl + wl * wl);
+  const T h0r = kgl * pr * irr + T(0.5) * (ur * ur + vr * vr + wr * wr);
+  const T srl = sqrtp_(rl), srr = sqrtp_(rr);
+  const T isr = rcp_(srl + srr);
+  const T rho_roe = srl * srr;
+  const T u_roe = (srr * ur + srl * ul) * isr;
+  const T v_roe = (srr * vr + srl * vl) * isr;
+  const T w_roe = (srr * wr + srl * wl) * isr;
+  const T h0_roe = (srr * h0r + srl * h0l) * isr;
+  const T q2 = u_roe * u_roe + v_roe * v_roe + w_roe * w_roe;
+  const T a_roe = sqrtp_(g.gm1 * (h0_roe - T(0.5) * q2));
+  const T un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+  const T Sm = min_(T(0), min_(unl - al, un_roe - a_roe));
+  const T Sp = max_(T(0), max_(unr + ar, un_roe + a_roe));
+  const T inv = rcp_(Sp - Sm);
+  const T c1 = (Sp + Sm) * inv, c0 = T(2) * Sp * Sm * inv;
+  const T l1h = fabs_(c1 * un_roe - c0), l2h = fabs_(c1 * (un_roe - a_roe) - c0), l3h = fabs_(c1 * (un_roe + a_roe) - c0);
+  const T eps = T(0.3) * (fabs_(un_roe) + a_roe);
+  const T l1 = beta * fabs_(un_roe) + (T(1) - beta) * l1h;
+  const T l2 = beta * harten_hyman(un_roe - a_roe, eps) + (T(1) - beta) * l2h;
+  const T l3 = beta * harten_hyman(un_roe + a_roe, eps) + (T(1) - beta) * l3h;
+  // tangential basis
+  T t1x, t1y, t1z;
+  if (fabs_(nx) < T(0.9)) { t1x = T(0); t1y = -nz; t1z = ny; }
+  else { t1x = -nz; t1y = T(0); t1z = nx; }
+  const T it = rcp_(sqrt_(t1x * t1x + t1y * t1y + t1z * t1z));
+  t1x *= it; t1y *= it; t1z *= it;
+  const T t2x = ny * t1z - nz * t1y, t2y = nz * t1x - nx * t1z, t2z = nx * t1y - ny * t1x;
+  // jumps and wave strengths
+  const T dr = rr - rl, dp = pr - pl, du = ur - ul, dv = vr - vl, dw = wr - wl, dun = unr - unl;
+  const T ia2 = rcp_(a_roe * a_roe);
+  const T ra = rho_roe * a_roe;
+  const T am = (dp - ra * dun) * T(0.5) * ia2, ap = (dp + ra * dun) * T(0.5) * ia2;
+  const T ac = dr - dp * ia2;
+  const T as1 = rho_roe * (du * t1x + dv * t1y + dw * t1z), as2 = rho_roe * (du * t2x + dv * t2y + dw * t2z);
+  const T FL = rl * unl, FR = rr * unr;
+  const T wm = am * l2, wp = ap * l3, wc = ac * l1;
+  F[0] = T(0.5) * (FL + FR) - T(0.5) * (wc + wm + wp);
+  F[1] = T(0.5) * (FL * ul + pl * nx + FR * ur + pr * nx) -
+         T(0.5) * (wc * u_roe + l1 * (as1 * t1x + as2 * t2x) + wm * (u_roe - a_roe * nx) + wp * (u_roe + a_roe * nx));
+  F[2] = T(0.5) * (FL * vl + pl * ny + FR * vr + pr * ny) -
+         T(0.5) * (wc * v_roe + l1 * (as1 * t1y + as2 * t2y) + wm * (v_roe - a_roe * ny) + wp * (v_roe + a_roe * ny));
+  F[3] = T(0.5) * (FL * wl + pl * nz + FR * wr + pr * nz) -
+         T(0.5) * (wc * w_roe + l1 * (as1 * t1z + as2 * t2z) + wm * (w_roe - a_roe * nz) + wp * (w_roe + a_roe * nz));
+  F[4] = T(0.5) * (FL * h0l + FR * h0r) -
+         T(0.5) * (wc * T(0.5) * q2 + l1 * (as1 * (u_roe * t1x + v_roe * t1y + w_roe * t1z) +
+                                            as2 * (u_roe * t2x + v_roe * t2y + w_roe * t2z)) +
+                   wm * (h0_roe - a_roe * un_roe) + wp * (h0_roe + a_roe * un_roe));
+}
+
+// beta: the shock detector's weight of the face, read by HLLE++ and HLLC+ only.  HLLC+ is
+// riemann_HLLCSD (Lib_Riemann_HLL.f90:486-534): HLLE at beta = 0, HLLC at beta >= 1, else
+// beta F_HLLC + (1 - beta) F_HLLE.
 template <int RS, typename T>
-__device__ __forceinline__ void riemann(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F) {
-  if constexpr (RS == RS_AUSMP) ausmp(L, R, nx, ny, nz, g, F);
+__device__ __forceinline__ void riemann(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F,
+                                        T beta = T(1)) {
+  if constexpr (RS == RS_HLLEPP) hllepp(L, R, nx, ny, nz, g, beta, F);
+  else if constexpr (RS == RS_HLLCP) {
+    if (beta == T(0)) hlle(L, R, nx, ny, nz, g, F);
+    else if (beta >= T(1)) hllc(L, R, nx, ny, nz, g, F);
+    else {
+      T Fe[5];
+      hlle(L, R, nx, ny, nz, g, Fe);
+      hllc(L, R, nx, ny, nz, g, F);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) F[v] = beta * F[v] + (T(1) - beta) * Fe[v];
+    }
+  }
+  else if constexpr (RS == RS_AUSMP) ausmp(L, R, nx, ny, nz, g, F);
   else if constexpr (RS == RS_HLLE) hlle(L, R, nx, ny, nz, g, F);
   else if constexpr (RS == RS_SLAU) slau<false>(L, R, nx, ny, nz, g, F);
   else if constexpr (RS == RS_SLAU2) slau<true>(L, R, nx, ny, nz, g, F);
@@ -472,6 +561,22 @@ __device__ __forceinline__ void cons2prim(const double* U, double gm1, double* P
 __device__ __forceinline__ void prim2cons(const double* P, double rgm1, double* U) {
   U[0] = P[0]; U[1] = P[0] * P[1]; U[2] = P[0] * P[2]; U[3] = P[0] * P[3];
   U[4] = P[4] * rgm1 + 0.5 * P[0] * (P[1] * P[1] + P[2] * P[2] + P[3] * P[3]);
+}
+
+// SD_Tramel (Lib_Shock_Detector.f90:9-38, DELTA = 20): Jameson-type pressure sensor of a cell
+// from its own pressure and its six face neighbours' (p1/p2 along i, p3/p4 along j, p5/p6
+// along k); 1 - tanh(10 (sw/DELTA)^3) below sw = 1/DELTA, else 0.
+__device__ __forceinline__ double sd_tramel(double p, double p1, double p2, double p3, double p4, double p5,
+                                            double p6) {
+  const double kp1 = fabs((p2 - 2.0 * p + p1) / (p2 + 2.0 * p + p1));
+  const double kp2 = fabs((p4 - 2.0 * p + p3) / (p4 + 2.0 * p + p3));
+  const double kp3 = fabs((p6 - 2.0 * p + p5) / (p6 + 2.0 * p + p5));
+  const double sw = fmax(kp1, fmax(kp2, kp3));
+  if (sw < 1.0 / 20.0) {
+    const double phi = fmax(sw / 20.0, 0.0);
+    return 1.0 - tanh(10.0 * phi * phi * phi);
+  }
+  return 0.0;
 }
 
 }  // namespace mose

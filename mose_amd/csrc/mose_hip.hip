@@ -119,6 +119,44 @@ __global__ void k_dtreset(unsigned long long* dtmin) {
   *dtmin = (unsigned long long)__double_as_longlong(1e5);
 }
 
+// Shock-detector weight of every interior cell (Fluxes_blk, Mod_Fluxes.f90:78-85): SD_Tramel of
+// the pressures of the cell and its six face neighbours, ghost layers filled.  Ghost entries
+// keep the 1 they were initialised with.
+struct SdArgs {
+  const double* U;
+  double* beta;
+  const Tile* tiles;
+  const BlockDev* blocks;
+  int64_t npool;
+  int ntiles, tx;
+  double gam;
+};
+
+__global__ void __launch_bounds__(256) k_sd(SdArgs a) {
+  const int tx = threadIdx.x % a.tx, ty = threadIdx.x / a.tx;
+  const double gm1 = a.gam - 1.0;
+  const int64_t N = a.npool;
+  auto pres = [&](int64_t c) {
+    double u[5], P[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = a.U[v * N + c];
+    cons2prim(u, gm1, P);
+    return P[4];
+  };
+  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const Tile T = a.tiles[t];
+    const BlockDev B = a.blocks[T.b];
+    const int i = T.i0 + tx, j = T.j0 + ty;
+    if (i > B.ni || j > B.nj) continue;
+    const int64_t sy = B.sy, sz = B.sz;
+    const int64_t cbase = B.off + (int64_t)(i + 1) + sy * (j + 1);
+    for (int k = T.k0; k < T.k1; ++k) {
+      const int64_t c = cbase + sz * (int64_t)(k + 1);
+      a.beta[c] = sd_tramel(pres(c), pres(c - 1), pres(c + 1), pres(c - sy), pres(c + sy), pres(c - sz), pres(c + sz));
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_normfin(const double* partials, int ntiles, double* scal) {
   __shared__ double red[256][5];
   const int tid = threadIdx.x;
@@ -142,16 +180,21 @@ __global__ void __launch_bounds__(256) k_normfin(const double* partials, int nti
 // Copies (101/201 Ghost_Connection, zero-gradient) move conserved values, which equal the
 // primitive copies of the reference exactly; extrapolation (Ghost_Extrapolate) and the
 // symmetry mirror (Ghost_Symmetry) act on primitive variables as in the reference.
+// The records of one launch are independent: a record that reads a ghost cell written by an
+// earlier record runs in a later launch (phase), and one that reads a ghost cell written by a
+// later record reads it from arc, the values at the end of the previous fill (narc cells).
 __global__ void __launch_bounds__(256) k_ghost(const GhostOp* __restrict__ ops, int nops, double* U,
-                                               int64_t N, double gam) {
+                                               int64_t N, double gam, const double* __restrict__ arc,
+                                               int64_t narc) {
   const double gm1 = gam - 1.0, rgm1 = 1.0 / gm1;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nops; q += gridDim.x * blockDim.x) {
     const GhostOp op = ops[q];
     const int64_t d1 = op.b - op.s, d2 = op.b - 2 * op.s;
+    auto rd = [&](int v, int64_t cell, int slot) { return slot >= 0 ? arc[v * narc + slot] : U[v * N + cell]; };
     if (op.type == G_COPY) {
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
-        const double a1 = U[v * N + op.src1], a2 = U[v * N + op.src2];
+        const double a1 = rd(v, op.src1, op.a1), a2 = rd(v, op.src2, op.a2);
         U[v * N + d1] = a1;
         U[v * N + d2] = a2;
       }
@@ -168,10 +211,10 @@ __global__ void __launch_bounds__(256) k_ghost(const GhostOp* __restrict__ ops, 
       for (int v = 0; v < 5; ++v) u[v] = U[v * N + op.b];
       cons2prim(u, gm1, Pb);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) u[v] = U[v * N + op.b + op.s];
+      for (int v = 0; v < 5; ++v) u[v] = rd(v, op.b + op.s, op.a1);
       cons2prim(u, gm1, P1);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) u[v] = U[v * N + op.b + 2 * op.s];
+      for (int v = 0; v < 5; ++v) u[v] = rd(v, op.b + 2 * op.s, op.a2);
       cons2prim(u, gm1, P2);
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
@@ -201,6 +244,14 @@ __global__ void __launch_bounds__(256) k_ghost(const GhostOp* __restrict__ ops, 
 }
 
 // blk%P (AoS primitive, padded, Fortran order) <-> conserved SoA pool.
+// Values of the archived ghost cells at the end of a fill (or after a state upload).
+__global__ void k_garc_save(const double* __restrict__ U, int64_t N, const int64_t* __restrict__ cells, int64_t narc,
+                            double* __restrict__ arc) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < narc; s += (int64_t)gridDim.x * blockDim.x)
+#pragma unroll
+    for (int v = 0; v < 5; ++v) arc[v * narc + s] = U[v * N + cells[s]];
+}
+
 __global__ void k_put(const double* __restrict__ Paos, double* U, int64_t N, int64_t off, int64_t ncell,
                       double gam) {
   const double rgm1 = 1.0 / (gam - 1.0);
@@ -306,7 +357,12 @@ struct mose_ctx {
   LineEnt* d_lines = nullptr;
   double* d_geo = nullptr;
   Tile* d_tiles = nullptr;
-  GhostOp* d_ghost = nullptr;
+  GhostOp* d_ghost = nullptr;                 // ghost records in bc order (the adjoint's transpose)
+  GhostOp* d_ghostp = nullptr;                // the same grouped by fill phase
+  std::vector<int> gphase;                    // phase p: records [gphase[p], gphase[p+1]) of d_ghostp
+  int64_t* d_garc_cell = nullptr;             // archived ghost cells (previous-fill values)
+  double* d_garc = nullptr;
+  int64_t narc = 0;
   BcFace* d_bcfaces = nullptr;
   double* d_bcbuf = nullptr;
   ViscFace* d_vfaces = nullptr;
@@ -315,6 +371,7 @@ struct mose_ctx {
   double* d_scal = nullptr;
   double* d_partials = nullptr;
   double* d_dtloc = nullptr;
+  double* d_beta = nullptr;                  // shock-detector weights (pool-sized), when used
   unsigned long long* d_dtmin = nullptr;
   int* d_err = nullptr;
   int* d_iter = nullptr;
@@ -359,11 +416,24 @@ static int fail(mose_ctx* c, int code, int b = 0, int i = 0, int j = 0, int k = 
 
 static int launch_blocks(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 65536); }
 
+// Ghost layers of the state U, then (schemes that use it) the shock detector on the filled state.
+static void enqueue_garc_save(mose_ctx* c, const double* U) {
+  if (c->narc > 0)
+    hipLaunchKernelGGL(k_garc_save, dim3(launch_blocks(c->narc)), dim3(256), 0, c->stream, U, c->npool, c->d_garc_cell,
+                       c->narc, c->d_garc);
+}
+
 static int enqueue_ghost(mose_ctx* c, double* U) {
-  if (c->lay.ghost.empty()) return 0;
-  const int n = (int)c->lay.ghost.size();
-  hipLaunchKernelGGL(k_ghost, dim3(launch_blocks(n)), dim3(256), 0, c->stream, c->d_ghost, n, U, c->npool,
-                     c->cfg.gamma);
+  for (size_t p = 0; p + 1 < c->gphase.size(); ++p) {
+    const int n = c->gphase[p + 1] - c->gphase[p];
+    hipLaunchKernelGGL(k_ghost, dim3(launch_blocks(n)), dim3(256), 0, c->stream, c->d_ghostp + c->gphase[p], n, U,
+                       c->npool, c->cfg.gamma, c->d_garc, c->narc);
+  }
+  enqueue_garc_save(c, U);
+  if (c->d_beta) {
+    SdArgs sa{U, c->d_beta, c->d_tiles, c->d_blocks, c->npool, (int)c->lay.tiles.size(), c->tx, c->cfg.gamma};
+    hipLaunchKernelGGL(k_sd, dim3(sa.ntiles), dim3(c->tx * c->ty), 0, c->stream, sa);
+  }
   return 0;
 }
 
@@ -375,6 +445,7 @@ static StageArgs stage_args(mose_ctx* c, const double* Uin, const double* Un, do
   a.scal = c->d_scal; a.dtloc = c->d_dtloc; a.partials = c->d_partials; a.err = c->d_err;
   a.bcbuf = c->d_bcbuf;
   a.vbuf = c->d_vbuf;
+  a.beta = c->d_beta;
   a.dtmin = c->d_dtmin; a.dtnext = c->d_dtloc; a.iter = c->d_iter;
   a.ndir = c->ndir; a.rampa = c->cfg.rampa_cfl_iter; a.cfl = c->cfg.cfl;
   a.npool = c->npool; a.ntiles = (int)c->lay.tiles.size(); a.local_dt = c->cfg.time_accurate ? 0 : 1;
@@ -409,6 +480,7 @@ static void enqueue_bcfaces(mose_ctx* c, const StageArgs& a) {
 
 static StageFn stage_fn(mose_ctx* c, int mode) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
+  if (c->cfg.muscl_sd) return ktable().s[c->lay.curvi ? 1 : 0][c->cfg.limiter][c->cfg.riemann][mode];
   if (c->lay.curvi) return ktable().g[0][p][c->cfg.limiter][c->cfg.riemann][mode];
   if (c->lay.visc) return ktable().g[1][p][c->cfg.limiter][c->cfg.riemann][mode];
   return ktable().f[p][c->cfg.limiter][c->cfg.riemann][c->shape][mode];
@@ -519,7 +591,7 @@ static int ensure_stage(mose_ctx* c, int64_t n) {
 
 static bool shape_compiled(const mose_ctx* c, int shape) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
-  if (c->lay.curvi || c->lay.visc) return shape == 0;
+  if (c->lay.curvi || c->lay.visc || c->cfg.muscl_sd) return shape == 0;
   return shape >= 0 && shape < NSHAPE && ktable().f[p][c->cfg.limiter][c->cfg.riemann][shape][0] != nullptr;
 }
 
@@ -554,6 +626,8 @@ static int retile(mose_ctx* c, int shape, int kchunk, int xcd) {
 
 // Discrete adjoint set-up: face work segments, the transposed ghost-fill map, buffers.
 static int ensure_adjoint(mose_ctx* c) {
+  // the adjoint of the shock-detector schemes (HLLE++, HLLC+, MUSCL-SD) is not built
+  if (c->d_beta) return fail(c, MOSE_E_ARG);
   if (c->adj_ready) return 0;
   std::vector<AdjSeg> segs;
   int64_t start = 0;
@@ -704,14 +778,79 @@ static void enqueue_residual(mose_ctx* c) {
   launch_stage(c, stage_fn(c, 2), a);
 }
 
+// Fill phases and previous-fill archive of the ghost records.  The reference fills them one
+// record at a time in bc order (Fill_Ghost_Cell, Lib_Ghost.f90:11-137), so across a block one or
+// two cells thick an extrapolation (or a copy from such a donor) reads ghost cells of the
+// opposite face: those an earlier record wrote in this fill, or, for a later record's, the
+// value of the previous fill.  Here record q runs in a phase after every earlier record whose
+// ghosts it reads, and reads a later record's ghosts from an archive saved after each fill.
+static int plan_ghost_fill(mose_ctx* c) {
+  const std::vector<GhostOp>& G = c->lay.ghost;
+  const int n = (int)G.size();
+  std::map<int64_t, int> writer;
+  for (int q = 0; q < n; ++q) {
+    writer[G[q].b - G[q].s] = q;
+    if (G[q].type != G_SYM) writer[G[q].b - 2 * G[q].s] = q;
+  }
+  std::vector<GhostOp> ops(G);
+  std::vector<int> phase(n, 0);
+  std::vector<int64_t> cells;
+  std::map<int64_t, int> slot;
+  int np = n > 0 ? 1 : 0;
+  for (int q = 0; q < n; ++q) {
+    GhostOp& op = ops[q];
+    op.a1 = op.a2 = -1;
+    if (op.type != G_COPY && op.type != G_EXTRAP) continue;   // zero-gradient, symmetry: the boundary cell only
+    const int64_t rc[2] = {op.type == G_COPY ? op.src1 : op.b + op.s, op.type == G_COPY ? op.src2 : op.b + 2 * op.s};
+    for (int r = 0; r < 2; ++r) {
+      auto it = writer.find(rc[r]);
+      if (it == writer.end() || it->second == q) continue;
+      if (it->second < q) {
+        phase[q] = std::max(phase[q], phase[it->second] + 1);
+      } else {
+        auto sit = slot.find(rc[r]);
+        int sl = (int)cells.size();
+        if (sit == slot.end()) {
+          slot[rc[r]] = sl;
+          cells.push_back(rc[r]);
+        } else {
+          sl = sit->second;
+        }
+        (r == 0 ? op.a1 : op.a2) = sl;
+      }
+    }
+    np = std::max(np, phase[q] + 1);
+  }
+  std::vector<GhostOp> sorted;
+  sorted.reserve(n);
+  c->gphase.assign(1, 0);
+  for (int p = 0; p < np; ++p) {
+    for (int q = 0; q < n; ++q)
+      if (phase[q] == p) sorted.push_back(ops[q]);
+    c->gphase.push_back((int)sorted.size());
+  }
+  if (upload(c, &c->d_ghostp, sorted)) return c->err.code;
+  c->narc = (int64_t)cells.size();
+  if (c->narc > 0) {
+    if (upload(c, &c->d_garc_cell, cells)) return c->err.code;
+    HIPCHK(c, hipMalloc((void**)&c->d_garc, 5 * c->narc * sizeof(double)));
+    HIPCHK(c, hipMemset(c->d_garc, 0, 5 * c->narc * sizeof(double)));
+  }
+  return 0;
+}
+
 // ============================================================================= C ABI
 extern "C" {
 
 int mose_create(const mose_cfg* cfg, mose_ctx** out) {
   if (!cfg || !out) return MOSE_E_ARG;
   *out = nullptr;
-  if (cfg->limiter < 0 || cfg->limiter > 5 || cfg->riemann < 0 || cfg->riemann > 4 || cfg->n_rk < 1 ||
+  if (cfg->limiter < 0 || cfg->limiter > 5 || cfg->riemann < 0 || cfg->riemann > 6 || cfg->n_rk < 1 ||
       cfg->n_rk > 3 || !(cfg->gamma > 1.0) || !(cfg->Rgas > 0.0) || cfg->precision < 0 || cfg->precision > 1)
+    return MOSE_E_ARG;
+  // MUSCL-SD: FP64 kernels of the limited reconstructions only
+  if (cfg->muscl_sd != 0 && (cfg->muscl_sd != 1 || cfg->limiter == MOSE_LIM_FIRST_ORDER ||
+                             cfg->precision != MOSE_PREC_FP64))
     return MOSE_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MOSE_E_NOGPU;
@@ -785,7 +924,7 @@ int mose_finalize(mose_ctx* c) {
   if (upload(c, &c->d_lines, c->lay.lines)) return c->err.code;
   if (c->lay.curvi && upload(c, &c->d_geo, c->lay.geo)) return c->err.code;
   std::vector<double>().swap(c->lay.geo);   // host copy no longer needed
-  if (upload(c, &c->d_ghost, c->lay.ghost)) return c->err.code;
+  if (plan_ghost_fill(c)) return c->err.code;
   if (upload(c, &c->d_facebc, c->lay.facebc)) return c->err.code;
   if (upload(c, &c->d_bcfaces, c->lay.bcfaces)) return c->err.code;
   if (upload(c, &c->d_vfaces, c->lay.vfaces)) return c->err.code;
@@ -803,6 +942,12 @@ int mose_finalize(mose_ctx* c) {
   if (!c->cfg.time_accurate) {
     HIPCHK(c, hipMalloc((void**)&c->d_dtloc, c->npool * sizeof(double)));
     HIPCHK(c, hipMemset(c->d_dtloc, 0, c->npool * sizeof(double)));
+  }
+  if (c->cfg.muscl_sd || rs_takes_beta(c->cfg.riemann)) {
+    if (c->cfg.muscl_sd && c->lay.visc) return fail(c, MOSE_E_ARG);   // no viscous MUSCL-SD kernels
+    HIPCHK(c, hipMalloc((void**)&c->d_beta, c->npool * sizeof(double)));
+    hipLaunchKernelGGL(k_fill, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, c->d_beta, c->npool, 1.0);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   // tiling: default shape 0, 32-plane chunks, XCD-aware order; MOSE_TILE_SHAPE,
   // MOSE_KCHUNK and MOSE_TILE_XCD override (mose_autotune picks by measurement)
@@ -823,6 +968,7 @@ int mose_put_state(mose_ctx* c, int32_t b, const double* P) {
   HIPCHK(c, hipMemcpyAsync(c->stage, P, 5 * ncell * sizeof(double), hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_put, dim3(launch_blocks(ncell)), dim3(256), 0, c->stream, c->stage, c->U[0], c->npool,
                      B.off, ncell, c->cfg.gamma);
+  enqueue_garc_save(c, c->U[0]);      // the uploaded ghost values are the "previous fill" of the next
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->dt_primed = false;
   return 0;
@@ -1219,7 +1365,8 @@ void mose_destroy(mose_ctx* c) {
   void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_geo, c->d_tiles,
                   c->d_ghost, c->d_bcfaces, c->d_vfaces, c->d_vbuf, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
                   c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
-                  c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB};
+                  c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_ghostp,
+                  c->d_garc_cell, c->d_garc};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

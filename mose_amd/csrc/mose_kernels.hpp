@@ -84,6 +84,11 @@ struct GhostOp {
   int64_t src1, src2;
   int type, pad;
   double n[3];
+  // Archive slots of the op's second and third reads (copy: src1 / src2; extrapolation:
+  // b + s / b + 2s), or -1: a read of a ghost cell that a later record of the same fill
+  // writes takes that cell's value from the previous fill, as the reference's serial fill
+  // does (blocks one or two cells thick).
+  int a1, a2;
 };
 
 struct StageArgs {
@@ -101,6 +106,7 @@ struct StageArgs {
   double* __restrict__ partials;
   double* __restrict__ bcbuf;
   double* __restrict__ vbuf;    // viscous flux of connection faces (per unit area), viscous path
+  const double* __restrict__ beta;   // shock-detector weight per pool cell (HLLE++ / HLLC+ / MUSCL-SD)
   int* err;
   unsigned long long* dtmin;   // last stage: CFL time step of the next step (min as bits)
   double* dtnext;              // last stage, local time stepping: dtlocal of the next step

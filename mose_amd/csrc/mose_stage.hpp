@@ -35,6 +35,27 @@ __device__ __forceinline__ void cell_states(const FT* Pm, const FT* P0, const FT
   }
 }
 
+// The same with MUSCL-SD (Lib_Reconstruction.f90:26-27): the slope of the low face state is
+// scaled by bm, that of the high face state by bp -- the shock detector's weights of the two
+// faces (the weight of a face is that of its low cell, or of the block's own boundary cell on a
+// block face).
+template <int LIM, bool SDR, typename FT>
+__device__ __forceinline__ void cell_states_b(const FT* Pm, const FT* P0, const FT* Pp, FT rlm, FT rlp, FT hdl,
+                                              FT bm, FT bp, FT* Sm, FT* Sp) {
+  if constexpr (!SDR) {
+    cell_states<LIM>(Pm, P0, Pp, rlm, rlp, hdl, Sm, Sp);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const FT sm = (P0[v] - Pm[v]) * rlm;
+      const FT sp = (Pp[v] - P0[v]) * rlp;
+      const FT sg = limiter<LIM>(sp, sm) * hdl;
+      Sm[v] = P0[v] - bm * sg;
+      Sp[v] = P0[v] + bp * sg;
+    }
+  }
+}
+
 template <typename FT>
 __device__ __forceinline__ void prim_at(const double* __restrict__ U, unsigned N, unsigned c, double gm1, FT* P) {
   double u[5], p[5];
@@ -219,7 +240,7 @@ __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const 
                                            const BlockDev* __restrict__ Bp, int bnum, int ni, int nj, int nk,
                                            unsigned off, unsigned sy, unsigned sz, const double* __restrict__ bcbuf,
                                            const double* __restrict__ geo, unsigned N, int* err,
-                                           const GasT<FT>& g) {
+                                           const GasT<FT>& g, const double* __restrict__ beta) {
   int idx, nd, t1, t2, n1;
   if (d == 0) { idx = fi; nd = ni; t1 = fj; t2 = fk; n1 = nj; }
   else if (d == 1) { idx = fj; nd = nj; t1 = fk; t2 = fi; n1 = nk; }
@@ -240,7 +261,13 @@ __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const 
       ny = FT(geo[(size_t)(GN + 4 * d + 1) * N + pf]);
       nz = FT(geo[(size_t)(GN + 4 * d + 2) * N + pf]);
     }
-    riemann<RS>(L, R, nx, ny, nz, g, F);
+    if constexpr (rs_takes_beta(RS)) {
+      // the low cell's weight; on the block's low face, that of its own boundary cell
+      const unsigned st = d == 0 ? 1u : (d == 1 ? sy : sz);
+      riemann<RS>(L, R, nx, ny, nz, g, F, FT(beta[idx == 1 ? pf : pf - st]));
+    } else {
+      riemann<RS>(L, R, nx, ny, nz, g, F);
+    }
   }
   if constexpr (GEO == 1) {
     const FT area = FT(geo[(size_t)(GN + 4 * d + 3) * N + pf]);
@@ -261,7 +288,7 @@ __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const 
 // and the k face above it; the TX + TY faces on the high edges of the patch are solved by
 // TX + TY threads whose choice rotates with the plane so no wave always carries them.  The
 // k faces need only the thread's own registers.
-template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc, int GEO>
+template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc, int GEO, bool SDR>
 __global__ void __launch_bounds__(TXc* TYc)
     __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 && TXc * TYc == 256 && GEO != 2 ? 3 : 1)))
     k_stage(StageArgs a) {
@@ -298,6 +325,7 @@ __global__ void __launch_bounds__(TXc* TYc)
   // struct would force a private copy of it)
   const double* __restrict__ geo = a.geo;
   const double* __restrict__ bcbuf = a.bcbuf;
+  const double* __restrict__ beta = a.beta;
   int* const err = a.err;
   // geometry array g at pool cell c (general metrics)
   auto GA = [geo, N](int g, unsigned c) -> double { return geo[(size_t)g * N + c]; };
@@ -355,11 +383,14 @@ __global__ void __launch_bounds__(TXc* TYc)
       prim_at(Uin, N, cbase + sz * (unsigned)(k0), gm1, Q0);         // P(k0-1)
       prim_at(Uin, N, cbase + sz * (unsigned)(k0 + 1), gm1, Q1);     // P(k0)
       prim_at(Uin, N, cbase + sz * (unsigned)(k0 + 2), gm1, Q2);     // P(k0+1)
+      const unsigned cm = cbase + sz * (unsigned)(k0);            // cell k0-1
+      FT bp = FT(1);
+      if constexpr (SDR) bp = FT(beta[k0 >= 2 ? cm : cm + sz]);
       if constexpr (GEO != 1) {
-        cell_states<LIM>(Qm, Q0, Q1, FT(Lz[k0].rl1), FT(Lz[k0 + 1].rl1), FT(Lz[k0].lp), Sm_, Skp);   // P+(k0-1)
+        cell_states_b<LIM, SDR>(Qm, Q0, Q1, FT(Lz[k0].rl1), FT(Lz[k0 + 1].rl1), FT(Lz[k0].lp), FT(1), bp, Sm_, Skp);   // P+(k0-1)
       } else {
-        const unsigned cm = cbase + sz * (unsigned)(k0);
-        cell_states<LIM>(Qm, Q0, Q1, FT(GA(GRLO + 2, cm)), FT(GA(GRLO + 2, cm + sz)), FT(0.5 * GA(GDL + 2, cm)), Sm_, Skp);
+        cell_states_b<LIM, SDR>(Qm, Q0, Q1, FT(GA(GRLO + 2, cm)), FT(GA(GRLO + 2, cm + sz)), FT(0.5 * GA(GDL + 2, cm)),
+                                FT(1), bp, Sm_, Skp);
       }
     }
     if (hval) {
@@ -450,15 +481,25 @@ __global__ void __launch_bounds__(TXc* TYc)
               c[v] = sP[v][ly + 2][lx + 2];
               p[v] = sP[v][ly + 2 + sy][lx + 2 + sx];
             }
+            FT bm = FT(1), bp = FT(1);
+            if constexpr (SDR) {
+              const int cc = d == 0 ? T.i0 + lx : T.j0 + ly;
+              const unsigned pc = off + (unsigned)(T.i0 + lx + 1) + (unsigned)sy_ * (unsigned)(T.j0 + ly + 1) +
+                                  sz * (unsigned)(k + 1);
+              const unsigned st = d == 0 ? 1u : (unsigned)sy_;
+              bp = FT(beta[cc >= 1 ? pc : pc + st]);
+              bm = FT(beta[cc >= 2 ? pc - st : pc]);
+            }
             if constexpr (GEO != 1) {
               const LineEnt* L = d == 0 ? Lx : Ly;
               const int cc = d == 0 ? T.i0 + lx : T.j0 + ly;
-              cell_states<LIM>(m, c, p, FT(L[cc + 1].rl1), FT(L[cc + 2].rl1), FT(L[cc + 1].lp), Sm_, Sp_);
+              cell_states_b<LIM, SDR>(m, c, p, FT(L[cc + 1].rl1), FT(L[cc + 2].rl1), FT(L[cc + 1].lp), bm, bp, Sm_, Sp_);
             } else {
               const unsigned pc = off + (unsigned)(T.i0 + lx + 1) + (unsigned)sy_ * (unsigned)(T.j0 + ly + 1) +
                                   sz * (unsigned)(k + 1);
               const unsigned st = d == 0 ? 1u : (unsigned)sy_;
-              cell_states<LIM>(m, c, p, FT(GA(GRLO + d, pc)), FT(GA(GRLO + d, pc + st)), FT(0.5 * GA(GDL + d, pc)), Sm_, Sp_);
+              cell_states_b<LIM, SDR>(m, c, p, FT(GA(GRLO + d, pc)), FT(GA(GRLO + d, pc + st)), FT(0.5 * GA(GDL + d, pc)),
+                                      bm, bp, Sm_, Sp_);
             }
             if (pass == 0) {
 #pragma unroll
@@ -514,7 +555,7 @@ __global__ void __launch_bounds__(TXc* TYc)
             for (int v = 0; v < 5; ++v) { Lq[v] = sSy[v][TYc][x]; Rq[v] = sMy[v][x]; }
           }
           if (!ok) continue;
-          stage_face<RS, FT, GEO>(d, fi, fj, k, Lq, Rq, F, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g);
+          stage_face<RS, FT, GEO>(d, fi, fj, k, Lq, Rq, F, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g, beta);
           if constexpr (VIS) {
             if (Bp->visc) {
               // cell on the low side of the face, patch coordinates
@@ -565,13 +606,19 @@ __global__ void __launch_bounds__(TXc* TYc)
         FT Fk[5];
         if (active) {
           FT Sm_[5], Sp_[5];
-          if constexpr (GEO != 1) {
-            cell_states<LIM>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), Sm_, Sp_);
-          } else {
-            const unsigned c1 = cbase + sz * (unsigned)(k + 2);
-            cell_states<LIM>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)), Sm_, Sp_);
+          const unsigned c1 = cbase + sz * (unsigned)(k + 2);      // cell k+1
+          FT bm = FT(1), bp = FT(1);
+          if constexpr (SDR) {
+            bp = FT(beta[c1]);
+            bm = FT(beta[k + 1 >= 2 ? c1 - sz : c1]);
           }
-          stage_face<RS, FT, GEO>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g);
+          if constexpr (GEO != 1) {
+            cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), bm, bp, Sm_, Sp_);
+          } else {
+            cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)),
+                                    bm, bp, Sm_, Sp_);
+          }
+          stage_face<RS, FT, GEO>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g, beta);
           if constexpr (VIS) {
             if (Bp->visc)
               visc_kface<FT, TXc, TYc>(Bp, i, j, k, ni, nk, tx, ty, sV, Q0, Q1, Lx, Ly, Lz, a.vbuf, Rg, vis, Fk);
@@ -655,13 +702,19 @@ __global__ void __launch_bounds__(TXc* TYc)
         FT Fk[5];
         if (active) {
           FT Sm_[5], Sp_[5];
-          if constexpr (GEO != 1) {
-            cell_states<LIM>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), Sm_, Sp_);
-          } else {
-            const unsigned c1 = cbase + sz * (unsigned)(k + 2);
-            cell_states<LIM>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)), Sm_, Sp_);
+          const unsigned c1 = cbase + sz * (unsigned)(k + 2);      // cell k+1
+          FT bm = FT(1), bp = FT(1);
+          if constexpr (SDR) {
+            bp = FT(beta[c1]);
+            bm = FT(beta[k + 1 >= 2 ? c1 - sz : c1]);
           }
-          stage_face<RS, FT, GEO>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g);
+          if constexpr (GEO != 1) {
+            cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), bm, bp, Sm_, Sp_);
+          } else {
+            cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)),
+                                    bm, bp, Sm_, Sp_);
+          }
+          stage_face<RS, FT, GEO>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g, beta);
           if constexpr (VIS) {
             if (Bp->visc)
               visc_kface<FT, TXc, TYc>(Bp, i, j, k, ni, nk, tx, ty, sV, Q0, Q1, Lx, Ly, Lz, a.vbuf, Rg, vis, Fk);

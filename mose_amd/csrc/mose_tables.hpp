@@ -11,7 +11,7 @@ typedef void (*BcFn)(mose::StageArgs, const mose::BcFace*, int);
 typedef void (*AdjFn)(AdjArgs);
 
 constexpr int NLIM = 6;     // minmod, van Leer, van Albada, first order, MC, superbee
-constexpr int NRS = 5;      // HLLC, AUSM+, HLLE, SLAU, SLAU2
+constexpr int NRS = 7;      // HLLC, AUSM+, HLLE, SLAU, SLAU2, HLLE++, HLLC+
 // Patch shapes (TX x TY threads) of the stage kernel per precision; shape 0 is the default,
 // the others are compiled for the van Leer limiter and selected by the tile tuner.
 constexpr int NSHAPE = 3;
@@ -21,6 +21,7 @@ constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 16}, {8, 16}}, {{32, 8}, {1
 struct KTable {
   StageFn f[2][NLIM][NRS][NSHAPE][3] = {};
   StageFn g[2][2][NLIM][NRS][3] = {};     // [geo - 1][prec][lim][rs][mode]
+  StageFn s[2][NLIM][NRS][3] = {};        // MUSCL-SD, FP64, tile shape 0: [geo 0/1][lim][rs][mode]
   BcFn bc[2][NLIM][NRS] = {};
 };
 

@@ -14,7 +14,8 @@ module mose_hip
                                            MOSE_LIM_VANALBADA = 2, MOSE_LIM_FIRST_ORDER = 3, &
                                            MOSE_LIM_MC = 4, MOSE_LIM_SUPERBEE = 5
   integer(c_int32_t), parameter, public :: MOSE_RS_HLLC = 0, MOSE_RS_AUSMP = 1, MOSE_RS_HLLE = 2, &
-                                           MOSE_RS_SLAU = 3, MOSE_RS_SLAU2 = 4
+                                           MOSE_RS_SLAU = 3, MOSE_RS_SLAU2 = 4, MOSE_RS_HLLEPP = 5, &
+                                           MOSE_RS_HLLCP = 6
   integer(c_int32_t), parameter, public :: MOSE_PREC_FP64 = 0, MOSE_PREC_FP32_FLUX = 1
   integer(c_int32_t), parameter, public :: MOSE_OK = 0, MOSE_E_ARG = 1, MOSE_E_HIP = 2, &
                                            MOSE_E_STATE = 3, MOSE_E_INTERFACE = 4, &
@@ -30,7 +31,8 @@ module mose_hip
     integer(c_int32_t) :: time_accurate = 1
     integer(c_int32_t) :: rampa_cfl_iter = 0
     integer(c_int32_t) :: precision = 0
-    integer(c_int32_t) :: reserved(4) = 0
+    integer(c_int32_t) :: muscl_sd = 0
+    integer(c_int32_t) :: reserved(3) = 0
   end type mose_cfg
 
   type, bind(C), public :: mose_bc_entry

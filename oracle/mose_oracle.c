@@ -47,6 +47,7 @@ typedef struct {
   double *fn[3], *fa[3];        /* dir(d)%f(i,j,k)%n / %a, face between cell i and i+1 ... */
   double *dl, *M, *vol;         /* (3)/(9)/(1) per padded cell                              */
   double* dt;                   /* dtlocal per padded cell                                  */
+  double* beta;                 /* shock-detector weight blk%beta per padded cell (1 outside) */
   int visc;                     /* laminar viscous terms on                                 */
   double mu_ref, T_ref, S_suth, Pr, vnn;
 } OBlk;
@@ -127,21 +128,38 @@ double ob_limiter(int32_t kind, double x, double y) {
   }
 }
 
-/* Reconstruction, src/lib/numerics/space/Lib_Reconstruction.f90:10-29 (beta = 1) */
+/* Reconstruction, src/lib/numerics/space/Lib_Reconstruction.f90:10-29; beta is the shock
+ * detector's weight with MUSCL-SD, else 1 */
 static inline void reconstruct(int kind, double prev, double local, double next, double next2,
                                double dl0, double dl1, double dl2, double dll, double dlr,
-                               double* priml, double* primr) {
+                               double beta, double* priml, double* primr) {
   double slope0 = (local - prev) / dl0;
   double slope1 = (next - local) / dl1;
   double slope2 = (next2 - next) / dl2;
   double slopel = ob_limiter(kind, slope1, slope0);
   double sloper = ob_limiter(kind, slope2, slope1);
-  *priml = local + 1.0 * slopel * dll;
-  *primr = next - 1.0 * sloper * dlr;
+  *priml = local + beta * slopel * dll;
+  *primr = next - beta * sloper * dlr;
 }
 void ob_reconstruct(int32_t kind, const double q[4], double l0, double l1, double l2,
                     double lm, double lp, double* ql, double* qr) {
-  reconstruct(kind, q[0], q[1], q[2], q[3], l0, l1, l2, lm, lp, ql, qr);
+  reconstruct(kind, q[0], q[1], q[2], q[3], l0, l1, l2, lm, lp, 1.0, ql, qr);
+}
+
+/* SD_Tramel, src/lib/numerics/fluxes/Lib_Shock_Detector.f90:9-38 (DELTA = 20, :5): Jameson-type
+ * pressure sensor over the cell and its six face neighbours (p1/p2 along i, p3/p4 along j,
+ * p5/p6 along k) */
+double ob_sd_tramel(double p, double p1, double p2, double p3, double p4, double p5, double p6) {
+  const double DELTA = 20.0;
+  double kp1 = fabs((p2 - 2.0 * p + p1) / (p2 + 2.0 * p + p1));
+  double kp2 = fabs((p4 - 2.0 * p + p3) / (p4 + 2.0 * p + p3));
+  double kp3 = fabs((p6 - 2.0 * p + p5) / (p6 + 2.0 * p + p5));
+  double sw = fmax(kp1, fmax(kp2, kp3));
+  if (sw < 1.0 / DELTA) {
+    double phi = fmax(sw / DELTA, 0.0);
+    return 1.0 - tanh(10.0 * phi * phi * phi);
+  }
+  return 0.0;
 }
 
 /* -------------------------------- Riemann solvers ----------------------------------- */
@@ -330,10 +348,109 @@ static void riemann_slau(const ob_domain* d, int v2, double ul, double vl, doubl
   }
 }
 
+/* the Riemann solvers that take the shock detector's weight (obj_riemann%SD, Mod_Riemann.f90:74-81) */
+static int sd_riemann(const ob_domain* d) {
+  return d->cfg.riemann == MOSE_RS_HLLEPP || d->cfg.riemann == MOSE_RS_HLLCP;
+}
+
+/* hartenHyman, Lib_Riemann_HLL.f90:536-546 */
+static double harten_hyman(double v, double e) {
+  double al = fabs(v), l = al;
+  if (al <= e) l = (v * v + e * e) / (2.0 * e);
+  return l;
+}
+
+/* riemann_HLLEpp (Tramel's HLLE++), Lib_Riemann_HLL.f90:321-440: central flux minus the Roe
+ * wave decomposition with eigenvalue magnitudes blended between Roe (Harten-Hyman fix) and
+ * HLLE by the shock-detector weight beta. */
+static void riemann_hllepp(const ob_domain* d, double dl, double ul, double vl, double wl, double pl,
+                           double al, double dltot, double dr, double ur, double vr, double wr,
+                           double pr, double ar, double drtot, double beta, double nx, double ny,
+                           double nz, double* F) {
+  double rho_roe, u_roe, v_roe, w_roe, a_roe, h0_roe, h0l, h0r;
+  double unl = ul * nx + vl * ny + wl * nz;
+  double unr = ur * nx + vr * ny + wr * nz;
+  roe_averages(d, dl, dr, dltot, drtot, ul, vl, wl, ur, vr, wr, pl, pr, &h0l, &h0r,
+               &rho_roe, &u_roe, &v_roe, &w_roe, &a_roe, &h0_roe);
+  double un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+  /* eigenvalues: HLLE */
+  double Sm = fmin(0.0, fmin(unl - al, un_roe - a_roe));
+  double Sp = fmax(0.0, fmax(unr + ar, un_roe + a_roe));
+  double inv_denom = 1.0 / (Sp - Sm);
+  double lam1_hlle = fabs((Sp + Sm) * un_roe * inv_denom - 2.0 * Sp * Sm * inv_denom);
+  double lam2_hlle = fabs((Sp + Sm) * (un_roe - a_roe) * inv_denom - 2.0 * Sp * Sm * inv_denom);
+  double lam3_hlle = fabs((Sp + Sm) * (un_roe + a_roe) * inv_denom - 2.0 * Sp * Sm * inv_denom);
+  /* Roe */
+  double eps = 0.3 * (fabs(un_roe) + a_roe);
+  double lam1_roe = fabs(un_roe);
+  double lam2_roe = harten_hyman(un_roe - a_roe, eps);
+  double lam3_roe = harten_hyman(un_roe + a_roe, eps);
+  /* HLLE++ */
+  double lam1_pp = beta * lam1_roe + (1.0 - beta) * lam1_hlle;
+  double lam2_pp = beta * lam2_roe + (1.0 - beta) * lam2_hlle;
+  double lam3_pp = beta * lam3_roe + (1.0 - beta) * lam3_hlle;
+  /* physical fluxes (fluxes, :651-669) and their average */
+  double Frl = dltot * unl, Frr = drtot * unr;
+  double Ful = Frl * ul + pl * nx, Fvl = Frl * vl + pl * ny, Fwl = Frl * wl + pl * nz;
+  double Fur = Frr * ur + pr * nx, Fvr = Frr * vr + pr * ny, Fwr = Frr * wr + pr * nz;
+  double Fel = Frl * H0(d, pl, dltot, sqrt(ul * ul + vl * vl + wl * wl));
+  double Fer = Frr * H0(d, pr, drtot, sqrt(ur * ur + vr * vr + wr * wr));
+  double F_r = 0.5 * (Frl + Frr), F_u = 0.5 * (Ful + Fur), F_v = 0.5 * (Fvl + Fvr);
+  double F_w = 0.5 * (Fwl + Fwr), F_E = 0.5 * (Fel + Fer);
+  /* jumps */
+  double drho = drtot - dltot, dp = pr - pl, du = ur - ul, dv = vr - vl, dw = wr - wl, dun = unr - unl;
+  /* tangential basis */
+  double t1x, t1y, t1z;
+  if (fabs(nx) < 0.9) { t1x = 0.0; t1y = -nz; t1z = ny; }
+  else { t1x = -nz; t1y = 0.0; t1z = nx; }
+  double normt = sqrt(t1x * t1x + t1y * t1y + t1z * t1z);
+  t1x = t1x / normt; t1y = t1y / normt; t1z = t1z / normt;
+  double t2x = ny * t1z - nz * t1y, t2y = nz * t1x - nx * t1z, t2z = nx * t1y - ny * t1x;
+  /* wave strengths */
+  double alpha_m = (dp - rho_roe * a_roe * dun) / (2.0 * a_roe * a_roe);
+  double alpha_c = drho - dp / (a_roe * a_roe);
+  double alpha_s1 = rho_roe * (du * t1x + dv * t1y + dw * t1z);
+  double alpha_s2 = rho_roe * (du * t2x + dv * t2y + dw * t2z);
+  double alpha_p = (dp + rho_roe * a_roe * dun) / (2.0 * a_roe * a_roe);
+  F_r = F_r - 0.5 * (alpha_c * lam1_pp + alpha_m * lam2_pp + alpha_p * lam3_pp);
+  F_u = F_u - 0.5 * (alpha_c * lam1_pp * u_roe);
+  F_v = F_v - 0.5 * (alpha_c * lam1_pp * v_roe);
+  F_w = F_w - 0.5 * (alpha_c * lam1_pp * w_roe);
+  F_u = F_u - 0.5 * lam1_pp * (alpha_s1 * t1x + alpha_s2 * t2x);
+  F_v = F_v - 0.5 * lam1_pp * (alpha_s1 * t1y + alpha_s2 * t2y);
+  F_w = F_w - 0.5 * lam1_pp * (alpha_s1 * t1z + alpha_s2 * t2z);
+  F_u = F_u - 0.5 * (alpha_m * lam2_pp * (u_roe - a_roe * nx) + alpha_p * lam3_pp * (u_roe + a_roe * nx));
+  F_v = F_v - 0.5 * (alpha_m * lam2_pp * (v_roe - a_roe * ny) + alpha_p * lam3_pp * (v_roe + a_roe * ny));
+  F_w = F_w - 0.5 * (alpha_m * lam2_pp * (w_roe - a_roe * nz) + alpha_p * lam3_pp * (w_roe + a_roe * nz));
+  F_E = F_E - 0.5 * lam1_pp * (alpha_c * 0.5 * (u_roe * u_roe + v_roe * v_roe + w_roe * w_roe));
+  F_E = F_E - 0.5 * lam1_pp * (alpha_s1 * (u_roe * t1x + v_roe * t1y + w_roe * t1z) +
+                               alpha_s2 * (u_roe * t2x + v_roe * t2y + w_roe * t2z));
+  F_E = F_E - 0.5 * (alpha_m * lam2_pp * (h0_roe - a_roe * un_roe) + alpha_p * lam3_pp * (h0_roe + a_roe * un_roe));
+  F[0] = F_r; F[1] = F_u; F[2] = F_v; F[3] = F_w; F[4] = F_E;
+}
+
 /* The `Riemann` procedure pointer (Mod_Riemann.f90:9-28,32-114) */
 static void riemann(const ob_domain* d, const double* PL, double aL, double rL,
-                    const double* PR, double aR, double rR, const double* n, double* F) {
-  if (d->cfg.riemann == MOSE_RS_AUSMP)
+                    const double* PR, double aR, double rR, double beta, const double* n, double* F) {
+  if (d->cfg.riemann == MOSE_RS_HLLCP) {        /* riemann_HLLCSD, Lib_Riemann_HLL.f90:486-534 */
+    if (beta == 0.0) {
+      riemann_hlle(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
+                   n[0], n[1], n[2], F);
+    } else if (beta >= 1.0) {
+      riemann_hllc(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
+                   n[0], n[1], n[2], F);
+    } else {
+      double FE[NV], FC[NV];
+      riemann_hlle(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
+                   n[0], n[1], n[2], FE);
+      riemann_hllc(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
+                   n[0], n[1], n[2], FC);
+      for (int v = 0; v < NV; ++v) F[v] = beta * FC[v] + (1.0 - beta) * FE[v];
+    }
+  } else if (d->cfg.riemann == MOSE_RS_HLLEPP)
+    riemann_hllepp(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
+                   beta, n[0], n[1], n[2], F);
+  else if (d->cfg.riemann == MOSE_RS_AUSMP)
     riemann_ausmp(d, PL[0], PL[1], PL[2], PL[3], PL[4], aL, rL, PR[0], PR[1], PR[2], PR[3], PR[4], aR, rR,
                   n[0], n[1], n[2], F);
   else if (d->cfg.riemann == MOSE_RS_HLLE)
@@ -348,9 +465,13 @@ static void riemann(const ob_domain* d, const double* PL, double aL, double rL,
 }
 
 void ob_riemann(const mose_cfg* cfg, const double PL[5], const double PR[5], const double n[3], double F[5]) {
+  ob_riemann_beta(cfg, PL, PR, n, 1.0, F);
+}
+void ob_riemann_beta(const mose_cfg* cfg, const double PL[5], const double PR[5], const double n[3], double beta,
+                     double F[5]) {
   ob_domain d; memset(&d, 0, sizeof d);
   d.cfg = *cfg; d.gam = cfg->gamma; d.Rg = cfg->Rgas; d.cp = d.gam * d.Rg / (d.gam - 1.0); d.cv = d.cp - d.Rg;
-  riemann(&d, PL, f_ss(&d, PL[4], PL[0]), PL[0], PR, f_ss(&d, PR[4], PR[0]), PR[0], n, F);
+  riemann(&d, PL, f_ss(&d, PL[4], PL[0]), PL[0], PR, f_ss(&d, PR[4], PR[0]), PR[0], beta, n, F);
 }
 static void init_gas(ob_domain* d) {
   d->gam = d->cfg.gamma; d->Rg = d->cfg.Rgas; d->cp = d->gam * d->Rg / (d->gam - 1.0); d->cv = d->cp - d->Rg;
@@ -369,7 +490,8 @@ static inline int gas_bad(const double* P) { return !(P[0] > 0.0) || !(P[4] > 0.
  * src/lib/numerics/fluxes/Lib_Convective.f90:10-85; returns nonzero on interface error */
 static int convective_flux(const ob_domain* d, const double dl[4], const double* normal, double area,
                            const double* Pm1, const double* P0, const double* P1, const double* P2,
-                           double* Res0, double* Res1) {
+                           double beta, double* Res0, double* Res1) {
+  const double beta_ = d->cfg.muscl_sd ? beta : 1.0;      /* :37-41 */
   double l0 = 0.5 * (dl[0] + dl[1]);
   double l1 = 0.5 * (dl[1] + dl[2]);
   double l2 = 0.5 * (dl[2] + dl[3]);
@@ -377,11 +499,11 @@ static int convective_flux(const ob_domain* d, const double dl[4], const double*
   double lp = 0.5 * dl[2];
   double P1s[NV], P4s[NV], F[NV], Flux[NV];
   for (int v = 0; v < NV; ++v)
-    reconstruct(d->cfg.limiter, Pm1[v], P0[v], P1[v], P2[v], l0, l1, l2, lm, lp, &P1s[v], &P4s[v]);
+    reconstruct(d->cfg.limiter, Pm1[v], P0[v], P1[v], P2[v], l0, l1, l2, lm, lp, beta_, &P1s[v], &P4s[v]);
   if (gas_bad(P1s) || gas_bad(P4s)) return MOSE_E_INTERFACE;
   double rotot1 = P1s[0], rotot4 = P4s[0];
   double a1 = f_ss(d, P1s[4], rotot1), a4 = f_ss(d, P4s[4], rotot4);
-  riemann(d, P1s, a1, rotot1, P4s, a4, rotot4, normal, F);
+  riemann(d, P1s, a1, rotot1, P4s, a4, rotot4, beta, normal, F);
   double su = copysign(0.5, F[0]);
   double sp = 0.5 + su, sm = su - 0.5;
   Flux[0] = F[0] * area * (sp * (P1s[0] / rotot1) - sm * (P4s[0] / rotot4));
@@ -473,6 +595,19 @@ static int fluxes_blk(ob_domain* d, int b) {
   OBlk* B = &d->blk[b];
   memset(B->R, 0, sizeof(double) * NV * B->npad);
   int ni = B->ni, nj = B->nj, nk = B->nk, err = 0;
+  /* shock detector, Mod_Fluxes.f90:70-85: beta = 1, or SD_Tramel of the pressures of the cell
+   * and its six face neighbours (ghost layers filled) */
+  if (d->cfg.muscl_sd || sd_riemann(d)) {
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= nk; ++k)
+      for (int j = 1; j <= nj; ++j)
+        for (int i = 1; i <= ni; ++i) {
+#define PAT(a, b_, c) B->P[NV * cidx(B, a, b_, c) + 4]
+          B->beta[cidx(B, i, j, k)] = ob_sd_tramel(PAT(i, j, k), PAT(i - 1, j, k), PAT(i + 1, j, k), PAT(i, j - 1, k),
+                                                   PAT(i, j + 1, k), PAT(i, j, k - 1), PAT(i, j, k + 1));
+#undef PAT
+        }
+  }
   first_err fe = {INT64_MAX, 0, 0, 0, 0};
   /* i sweep, :89-99 */
 #pragma omp parallel for collapse(2) schedule(static)
@@ -485,7 +620,7 @@ static int fluxes_blk(ob_domain* d, int b) {
         int e = convective_flux(d, dl, &B->fn[0][3 * f], B->fa[0][f],
                                 &B->P[NV * cidx(B, i - 1, j, k)], &B->P[NV * cidx(B, i, j, k)],
                                 &B->P[NV * cidx(B, i + 1, j, k)], &B->P[NV * cidx(B, i + 2, j, k)],
-                                &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i + 1, j, k)]);
+                                B->beta[f], &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i + 1, j, k)]);
         if (e) first_err_note(&fe, ((int64_t)k * (nj + 2) + j) * (ni + 2) + i, e, i, j, k);
       }
   first_err_flush(d, &fe, b, &err);
@@ -516,7 +651,7 @@ static int fluxes_blk(ob_domain* d, int b) {
         int e = convective_flux(d, dl, &B->fn[1][3 * f], B->fa[1][f],
                                 &B->P[NV * cidx(B, i, j - 1, k)], &B->P[NV * cidx(B, i, j, k)],
                                 &B->P[NV * cidx(B, i, j + 1, k)], &B->P[NV * cidx(B, i, j + 2, k)],
-                                &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i, j + 1, k)]);
+                                B->beta[f], &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i, j + 1, k)]);
         if (e) first_err_note(&fe, ((int64_t)k * (ni + 2) + i) * (nj + 2) + j, e, i, j, k);
       }
   first_err_flush(d, &fe, b, &err);
@@ -547,7 +682,7 @@ static int fluxes_blk(ob_domain* d, int b) {
         int e = convective_flux(d, dl, &B->fn[2][3 * f], B->fa[2][f],
                                 &B->P[NV * cidx(B, i, j, k - 1)], &B->P[NV * cidx(B, i, j, k)],
                                 &B->P[NV * cidx(B, i, j, k + 1)], &B->P[NV * cidx(B, i, j, k + 2)],
-                                &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i, j, k + 1)]);
+                                B->beta[f], &B->R[NV * cidx(B, i, j, k)], &B->R[NV * cidx(B, i, j, k + 1)]);
         if (e) first_err_note(&fe, ((int64_t)j * (ni + 2) + i) * (nk + 2) + k, e, i, j, k);
       }
   first_err_flush(d, &fe, b, &err);
@@ -609,11 +744,14 @@ static int bc_connection(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm) 
   double Dl4 = B->dl[3 * cidx(B, I4, J4, K4) + dir] * 0.5;
   double Dist23 = Dl2 + Dl3, Dist12 = Dl1 + Dl2, Dist34 = Dl3 + Dl4;
   double PL[NV], PR[NV], F[NV], Flux[NV];
+  /* the shock-detector weight of the boundary cell (:58-62, :85-89) */
+  const double beta_m = B->beta[cidx(B, Im, Jm, Km)];
   for (int v = 0; v < NV; ++v)
-    reconstruct(d->cfg.limiter, Pr1[v], Pr2[v], Pr3[v], Pr4[v], Dist12, Dist23, Dist34, Dl2, Dl3, &PL[v], &PR[v]);
+    reconstruct(d->cfg.limiter, Pr1[v], Pr2[v], Pr3[v], Pr4[v], Dist12, Dist23, Dist34, Dl2, Dl3,
+                d->cfg.muscl_sd ? beta_m : 1.0, &PL[v], &PR[v]);
   double rho_L = PL[0], rho_R = PR[0];
   double a_L = f_ss(d, PL[4], rho_L), a_R = f_ss(d, PR[4], rho_R);
-  riemann(d, PL, a_L, rho_L, PR, a_R, rho_R, normal, F);
+  riemann(d, PL, a_L, rho_L, PR, a_R, rho_R, sd_riemann(d) ? beta_m : 1.0, normal, F);
   double su = copysign(0.5, F[0]);
   double sel_L = 0.5 + su, sel_R = su - 0.5;
   double Fmass = F[0] * area;
@@ -740,7 +878,7 @@ static void bc_symmetry(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm) {
   double PL[NV] = {Face[0], U1[0], U1[1], U1[2], Face[4]};
   double PR[NV] = {Face[0], U4[0], U4[1], U4[2], Face[4]};
   double F[NV];
-  riemann(d, PL, Sound, rotot, PR, Sound, rotot, n, F);
+  riemann(d, PL, Sound, rotot, PR, Sound, rotot, 1.0, n, F);
   double* R = &B->R[NV * cidx(B, Im, Jm, Km)];
   R[1] += modfm2 * (F[1] * area);
   R[2] += modfm2 * (F[2] * area);
@@ -795,7 +933,7 @@ static int bc_inflow405(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm, c
     PR[v] = modfm * Face[v] + modfm1 * Sup[v];
   }
   double rho_L = PL[0], rho_R = PR[0];
-  riemann(d, PL, f_ss(d, PL[4], rho_L), rho_L, PR, f_ss(d, PR[4], rho_R), rho_R, n, F);
+  riemann(d, PL, f_ss(d, PL[4], rho_L), rho_L, PR, f_ss(d, PR[4], rho_R), rho_R, 1.0, n, F);
   double su = copysign(1.0, F[0]);
   double Sel_L = (1.0 + su) / 2.0, Sel_R = (su - 1.0) / 2.0;
   double Fmass = F[0] * area;
@@ -1289,6 +1427,8 @@ int ob_add_block(ob_domain* d, int32_t b, const int32_t dims[3], const double* n
   B->M = (double*)calloc((size_t)(9 * B->npad), sizeof(double));
   B->vol = (double*)calloc((size_t)B->npad, sizeof(double));
   B->dt = (double*)calloc((size_t)B->npad, sizeof(double));
+  B->beta = (double*)malloc(sizeof(double) * (size_t)B->npad);
+  for (int64_t c = 0; c < B->npad; ++c) B->beta[c] = 1.0;
   return 0;
 }
 int ob_set_viscous(ob_domain* d, int32_t b, const double params[5]) {
@@ -1353,7 +1493,7 @@ void ob_destroy(ob_domain* d) {
     OBlk* B = &d->blk[b];
     free(B->node); free(B->P); free(B->PO); free(B->R);
     for (int dd = 0; dd < 3; ++dd) { free(B->fn[dd]); free(B->fa[dd]); }
-    free(B->dl); free(B->M); free(B->vol); free(B->dt);
+    free(B->dl); free(B->M); free(B->vol); free(B->dt); free(B->beta);
   }
   free(d->blk); free(d->bc); free(d);
 }

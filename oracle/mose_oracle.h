@@ -47,6 +47,10 @@ void   ob_reconstruct(int32_t kind, const double q[4], double l0, double l1, dou
                       double lm, double lp, double* ql, double* qr);
 void   ob_riemann(const mose_cfg* cfg, const double PL[5], const double PR[5],
                   const double n[3], double F[5]);
+/* as ob_riemann with the shock-detector weight beta (HLLE++ / HLLC+ use it) */
+void   ob_riemann_beta(const mose_cfg* cfg, const double PL[5], const double PR[5],
+                       const double n[3], double beta, double F[5]);
+double ob_sd_tramel(double p, double p1, double p2, double p3, double p4, double p5, double p6);
 void   ob_prim2cons(const mose_cfg* cfg, const double P[5], double U[5]);
 void   ob_cons2prim(const mose_cfg* cfg, const double U[5], double P[5]);
 

@@ -47,6 +47,10 @@ def lib():
         L.ob_limiter.argtypes = [C.c_int32, C.c_double, C.c_double]
         L.ob_reconstruct.argtypes = [C.c_int32, P(C.c_double)] + [C.c_double] * 5 + [P(C.c_double)] * 2
         L.ob_riemann.argtypes = [P(abi.MoseCfg), P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.ob_riemann_beta.argtypes = [P(abi.MoseCfg), P(C.c_double), P(C.c_double), P(C.c_double), C.c_double,
+                                      P(C.c_double)]
+        L.ob_sd_tramel.restype = C.c_double
+        L.ob_sd_tramel.argtypes = [C.c_double] * 7
         L.ob_prim2cons.argtypes = [P(abi.MoseCfg), P(C.c_double), P(C.c_double)]
         L.ob_cons2prim.argtypes = [P(abi.MoseCfg), P(C.c_double), P(C.c_double)]
         _LIB = L
@@ -158,12 +162,17 @@ def limiter(kind, x, y):
     return lib().ob_limiter(kind, x, y)
 
 
-def riemann(cfg, PL, PR, n):
+def riemann(cfg, PL, PR, n, beta=1.0):
+    """Face flux per unit area; beta is the shock-detector weight the HLLE++ / HLLC+ solvers take."""
     PL = np.ascontiguousarray(PL, dtype=np.float64); PR = np.ascontiguousarray(PR, dtype=np.float64)
     n = np.ascontiguousarray(n, dtype=np.float64)
     F = np.zeros(5)
-    lib().ob_riemann(C.byref(cfg), _dp(PL), _dp(PR), _dp(n), _dp(F))
+    lib().ob_riemann_beta(C.byref(cfg), _dp(PL), _dp(PR), _dp(n), float(beta), _dp(F))
     return F
+
+
+def sd_tramel(p, p1, p2, p3, p4, p5, p6):
+    return lib().ob_sd_tramel(p, p1, p2, p3, p4, p5, p6)
 
 
 def reconstruct(kind, q, l0, l1, l2, lm, lp):

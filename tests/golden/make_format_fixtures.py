@@ -4,6 +4,9 @@ directory (data fixtures; gzip for the larger ones):
   test/2D/euler/prandtl-meyer/INPUT/ic.tec, bc.txt   -> prandtl_meyer_ic.tec.gz, prandtl_meyer_bc.txt.gz
   test/1D/Ein91/INPUT/ic_x8.tec, bc_x8.txt           -> ein91_ic_x8.tec.gz, ein91_bc_x8.txt.gz
   test/1D/Noh87/INPUT/ic_x8.tec, bc_x8.txt           -> noh87_ic_x8.tec.gz, noh87_bc_x8.txt.gz
+  test/1D/Tor99/INPUT/ic_x8.tec, bc_x8.txt           -> tor99_ic_x8.tec.gz, tor99_bc_x8.txt.gz
+  test/2D/euler/ramp-channel/INPUT/ic.tec, bc.txt    -> ramp_channel_ic.tec.gz, ramp_channel_bc.txt.gz
+  test/2D/euler/oblique-shock/INPUT/ic.tec, bc.txt   -> oblique_shock_ic.tec.gz, oblique_shock_bc.txt.gz
 Run in a container that holds the reference at /root/reference."""
 import gzip
 import os
@@ -25,9 +28,12 @@ def main():
     shutil.copyfile(os.path.join(REF, "1D/Sod79/INPUT/bc_x1.txt"), os.path.join(HERE, "sod79_bc_x1.txt"))
     for src, dst in (("ic.tec", "prandtl_meyer_ic.tec.gz"), ("bc.txt", "prandtl_meyer_bc.txt.gz")):
         _gzip(os.path.join(REF, "2D/euler/prandtl-meyer/INPUT", src), dst)
-    for case in ("Ein91", "Noh87"):
+    for case in ("Ein91", "Noh87", "Tor99"):
         for src in ("ic_x8.tec", "bc_x8.txt"):
             _gzip(os.path.join(REF, "1D", case, "INPUT", src), "%s_%s.gz" % (case.lower(), src))
+    for case in ("ramp-channel", "oblique-shock"):
+        for src in ("ic.tec", "bc.txt"):
+            _gzip(os.path.join(REF, "2D/euler", case, "INPUT", src), "%s_%s.gz" % (case.replace("-", "_"), src))
 
 
 if __name__ == "__main__":

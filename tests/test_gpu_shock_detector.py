@@ -1,0 +1,204 @@
+"""GPU parity of the shock-detector schemes against the oracle: HLLE++ and HLLC+ (the Riemann
+solvers that take the per-face weight SD_Tramel) and MUSCL-SD (limited slopes scaled by it),
+on states rough enough that the weight is 0 in part of the domain and ~1 elsewhere; the
+reference's own Tor99, ramp-channel and oblique-shock inputs; the refusals."""
+import copy
+import os
+
+import numpy as np
+import pytest
+
+from mose_amd import abi, cases, formats
+from mose_amd.lib import GpuSolver, MoseError
+from oracle import exact_riemann as er
+from oracle.oracle import OracleSolver
+
+from helpers import flux_scale, interior, perturb, rel_l2, skewed_nodes
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+LIMS = ["minmod", "vanleer", "vanalbada", "mc", "superbee"]
+
+
+def _with(case, **kw):
+    c = copy.deepcopy(case)
+    f = {n: getattr(case.cfg, n) for n, _ in abi.MoseCfg._fields_ if n != "reserved"}
+    f.update(kw)
+    c.cfg = abi.MoseCfg(*[f[n] for n, _ in abi.MoseCfg._fields_ if n != "reserved"])
+    return c
+
+
+def _switch_fraction(case):
+    """Fraction of interior cells whose pressure second difference along some axis exceeds the
+    detector's 1/20 threshold (weight 0 there): the test states must have both kinds."""
+    fr = []
+    for blk in case.blocks:
+        p = blk.P[..., 4]
+        sw = np.zeros(p[2:-2, 2:-2, 2:-2].shape)
+        for ax in range(3):
+            sl = [slice(2, -2)] * 3
+            lo, hi = list(sl), list(sl)
+            lo[ax] = slice(1, -3)
+            hi[ax] = slice(3, -1)
+            c = p[2:-2, 2:-2, 2:-2]
+            a, b = p[tuple(lo)], p[tuple(hi)]
+            sw = np.maximum(sw, np.abs((b - 2 * c + a) / (b + 2 * c + a)))
+        fr.append(np.mean(sw >= 0.05))
+    return float(np.mean(fr))
+
+
+def _residual_check(case, tol=1e-12):
+    g, o = GpuSolver(case), OracleSolver(case)
+    for b in range(1, len(case.blocks) + 1):
+        Rg, Ro = g.residual(b), o.residual(b)
+        assert rel_l2(Rg, Ro) < tol, "block %d: rel L2 %.3e" % (b, rel_l2(Rg, Ro))
+        assert np.max(np.abs(interior(Rg) - interior(Ro))) < 1e-13 * flux_scale(case.blocks[b - 1].P)
+    g.close()
+
+
+def _steps_check(case, nsteps=5, tol=1e-12):
+    g, o = GpuSolver(case), OracleSolver(case)
+    for _ in range(nsteps):
+        dg, tg, _ = g.step()
+        do, to, _ = o.step()
+        assert abs(dg - do) <= 1e-13 * do
+    for b in range(1, len(case.blocks) + 1):
+        assert rel_l2(g.get_state(b), o.get_state(b)) < tol
+    g.close()
+
+
+def _rough_vortex(lim, rs, muscl_sd=False, dims=(20, 12, 9)):
+    case = perturb(cases.vortex(dims=dims, limiter=lim, riemann=rs), amp=0.12, seed=21)
+    case = _with(case, muscl_sd=int(muscl_sd))
+    frac = _switch_fraction(case)
+    assert 0.05 < frac < 0.95, frac
+    return case
+
+
+@pytest.mark.parametrize("lim", LIMS)
+@pytest.mark.parametrize("rs", ["HLLE++", "HLLC+"])
+def test_sd_riemann_residual(lim, rs):
+    _residual_check(_rough_vortex(lim, rs))
+
+
+@pytest.mark.parametrize("lim", LIMS)
+@pytest.mark.parametrize("rs", ["HLLC", "HLLE", "HLLE++", "AUSM+"])
+def test_muscl_sd_residual(lim, rs):
+    _residual_check(_rough_vortex(lim, rs, muscl_sd=True))
+
+
+@pytest.mark.parametrize("rs", ["HLLE++", "HLLC+", "HLLE"])
+def test_sd_steps_periodic(rs):
+    case = _rough_vortex("vanleer", rs, muscl_sd=rs == "HLLE", dims=(16, 16, 8))
+    _steps_check(case, nsteps=6)
+
+
+@pytest.mark.parametrize("muscl_sd", [False, True])
+def test_sd_multiblock_connections(muscl_sd):
+    """Connection faces take the weight of each block's own boundary cell (the two sides of a
+    connection can differ, as in the reference's BC_Connection_Eul)."""
+    case = cases.multiblock_chain(nblocks=5, seed=5, lo=6, hi=12, riemann="HLLE++", precision=abi.PREC_FP64)
+    case = _with(perturb(case, amp=0.12, seed=8), muscl_sd=int(muscl_sd))
+    _residual_check(case)
+    _steps_check(case, nsteps=3)
+
+
+@pytest.mark.parametrize("rs", ["HLLE++", "HLLE"])
+def test_sd_curvilinear(rs):
+    dims = (14, 10, 7)
+    prim = np.zeros(tuple(dims[::-1]) + (5,))
+    prim[..., 0], prim[..., 1], prim[..., 2], prim[..., 4] = 1.0, 1.5, 0.2, 1.0 / 1.4
+    blk = cases.Block(dims, skewed_nodes(dims), cases.padded_state(dims, prim))
+    inflow = {"type": 405, "mach": 1.8, "T": 1.05, "p": 0.7, "alpha": 0.1, "beta": -0.05, "rel_fac": 0.8}
+    bc = cases.bc_records([{"dims": dims, "faces": [inflow, 400, 300, 300, 300, 300]}])
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.5, "vanalbada", rs, 3, False, muscl_sd=rs == "HLLE")
+    case = perturb(cases.Case("skewed_sd", cfg, [blk], bc), amp=0.12, seed=9)
+    _residual_check(case)
+    _steps_check(case, nsteps=3)
+
+
+def test_sd_fp32_flux_within_1e5():
+    case = _rough_vortex("vanleer", "HLLE++")
+    g64 = GpuSolver(case)
+    g32 = GpuSolver(_with(case, precision=abi.PREC_FP32_FLUX))
+    R64, R32 = g64.residual(1), g32.residual(1)
+    assert rel_l2(R32, R64) < 1e-5
+    g64.close()
+    g32.close()
+
+
+def test_tor99_shipped_configuration():
+    """test/1D/Tor99 with its input.ini settings (HLLE++ + van Leer, RK3, CFL 0.9) on its 800-cell
+    input: the GPU matches the oracle after 100 steps and passes Tor99's probe rule at t = 0.2."""
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.9, "vanleer", "HLLE++", 3, True)
+    case = formats.load_case(os.path.join(GOLD, "tor99_ic_x8.tec.gz"), os.path.join(GOLD, "tor99_bc_x8.txt.gz"), cfg)
+    g, o = GpuSolver(case), OracleSolver(case)
+    for _ in range(100):
+        g.step()
+        o.step()
+    assert rel_l2(g.get_state(1), o.get_state(1)) < 1e-12
+    t = g.time()[0]
+    while t < 0.2:
+        _, t, _ = g.step()
+    P = interior(g.get_state(1))[0, 0]
+    g.close()
+    xc = (np.arange(800) + 0.5) / 800
+    rho, u, p = er.sample(xc, 0.2, (1.0, 0.75, 1.0), (0.125, 0.0, 0.1), 0.3)
+    cfd = {"rho": P[:, 0], "u": P[:, 1], "p": P[:, 4], "sie": P[:, 4] / (0.4 * P[:, 0])}
+    exact = {"rho": rho, "u": u, "p": p, "sie": p / (0.4 * rho)}
+    assert not er.probe_fails((80, 240, 400, 560, 720), cfd, exact, {"p": 1.0})
+
+
+@pytest.mark.parametrize("name,lim", [("ramp_channel", "minmod"), ("oblique_shock", "vanalbada")])
+def test_shipped_2d_muscl_sd_inputs(name, lim):
+    """test/2D/euler/ramp-channel and oblique-shock inputs with their input.ini numerics (MUSCL-SD,
+    HLLE, RK3, local time step, CFL 0.9; without the residual smoothing, irs-beta = 0.5, that
+    those runs add): residual and five steps against the oracle, on the given state and on a
+    rough one where the detector switches."""
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.9, lim, "HLLE", 3, False, muscl_sd=True)
+    case = formats.load_case(os.path.join(GOLD, name + "_ic.tec.gz"), os.path.join(GOLD, name + "_bc.txt.gz"), cfg)
+    _residual_check(case)
+    _steps_check(case, nsteps=5)
+    rough = perturb(copy.deepcopy(case), amp=0.1, seed=12)
+    assert _switch_fraction(rough) > 0.05
+    _residual_check(rough)
+
+
+def test_refusals():
+    case = cases.vortex(dims=(8, 8, 8), riemann="HLLE++")
+    g = GpuSolver(case)
+    with pytest.raises(MoseError) as ei:
+        g.residual_vjp([np.ones(abi.padded_shape((8, 8, 8)))])
+    assert ei.value.code == abi.E_ARG
+    g.close()
+    for kw in ({"muscl_sd": 1, "precision": abi.PREC_FP32_FLUX}, {"muscl_sd": 1, "limiter": abi.LIM_FIRST_ORDER},
+               {"muscl_sd": 2}):
+        with pytest.raises(MoseError) as ei:
+            GpuSolver(_with(cases.vortex(dims=(8, 8, 8)), **kw))
+        assert ei.value.code == abi.E_ARG
+
+
+def test_thin_block_ghost_fill_follows_serial_order():
+    """A block one cell thick with extrapolation (400) on both faces of that direction (the
+    reference's 1-D inputs): each face's extrapolation reads the opposite face's ghost layers,
+    as the previous fill left them or as an earlier record of this fill wrote them.  The GPU
+    fill reproduces the reference's serial order: after steps and fills, the ghost layers and
+    the shock detector's input agree with the oracle's, and repeated runs agree bit for bit."""
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.9, "vanleer", "HLLE++", 3, True)
+    case = formats.load_case(os.path.join(GOLD, "tor99_ic_x8.tec.gz"), os.path.join(GOLD, "tor99_bc_x8.txt.gz"), cfg)
+    runs = []
+    for _ in range(2):
+        g, o = GpuSolver(case), OracleSolver(case)
+        for _ in range(4):
+            g.step()
+            o.step()
+        g.residual(1)
+        o.residual(1)
+        Pg, Po = g.get_state(1), o.get_state(1)
+        # j ghost layers (k interior) of every column
+        jg = [0, 1, 3, 4]
+        assert np.allclose(Pg[2][jg][:, 2:-2], Po[2][jg][:, 2:-2], rtol=1e-12, atol=1e-14)
+        runs.append(Pg)
+        g.close()
+    assert np.array_equal(runs[0], runs[1])

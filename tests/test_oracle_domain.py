@@ -86,6 +86,27 @@ def test_sod79_shipped_configuration_passes_check_py(riemann):
     assert not er.probe_check(xc, P[:, 0], P[:, 1], P[:, 4], P[:, 4] / (0.4 * P[:, 0]), table, 1.0)
 
 
+def test_tor99_shipped_configuration_probe_rule():
+    """test/1D/Tor99 (Toro's test 1: the Sod tube with u_L = 0.75, interface at 0.3) with its
+    input.ini settings -- HLLE++ (the shock detector's weight per face) + van Leer, RK3, CFL
+    0.9 -- on its 800-cell input passes its probe rule (verify.py:141-215: rho, u, p, sie within
+    1 % at five probes) against the exact solution at t = 0.2."""
+    from mose_amd import formats
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.9, "vanleer", "HLLE++", 3, True)
+    case = formats.load_case(os.path.join(GOLDEN, "tor99_ic_x8.tec.gz"), os.path.join(GOLDEN, "tor99_bc_x8.txt.gz"),
+                             cfg)
+    o = OracleSolver(case)
+    t = 0.0
+    while t < 0.2:
+        _, t, _ = o.step()
+    P = interior(o.get_state(1))[0, 0]
+    xc = cases.cell_centres(case.blocks[0].nodes)[0, 0, :, 0]
+    rho, u, p = er.sample(xc, 0.2, (1.0, 0.75, 1.0), (0.125, 0.0, 0.1), 0.3)
+    cfd = {"rho": P[:, 0], "u": P[:, 1], "p": P[:, 4], "sie": P[:, 4] / (0.4 * P[:, 0])}
+    exact = {"rho": rho, "u": u, "p": p, "sie": p / (0.4 * rho)}
+    assert not er.probe_fails((80, 240, 400, 560, 720), cfd, exact, {"p": 1.0})
+
+
 def _reference_1d_input(name, limiter, cfl):
     """A case from the reference's own 1-D input files (test/1D/<name>/INPUT, 800 cells),
     with the numerics of its input.ini."""

@@ -156,14 +156,14 @@ def rand_normal(rng):
     return n / np.linalg.norm(n)
 
 
-@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE, abi.RS_SLAU, abi.RS_SLAU2])
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE, abi.RS_SLAU, abi.RS_SLAU2, abi.RS_HLLEPP, abi.RS_HLLCP])
 def test_riemann_consistency(rs):
     cfg = abi.make_cfg(riemann=rs)
     rng = np.random.default_rng(1)
     for _ in range(200):
         P = rand_state(rng, mach=rng.uniform(0, 3))
         n = rand_normal(rng)
-        F = O.riemann(cfg, P, P, n)
+        F = O.riemann(cfg, P, P, n, beta=rng.uniform())
         assert np.allclose(F, physical_flux(P, n), rtol=1e-12, atol=1e-12)
 
 
@@ -197,15 +197,16 @@ def test_riemann_stationary_contact(rs):
         assert np.allclose(F, [0, p * n[0], p * n[1], p * n[2], 0], atol=1e-13)
 
 
-@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE, abi.RS_SLAU, abi.RS_SLAU2])
+@pytest.mark.parametrize("rs", [abi.RS_HLLC, abi.RS_AUSMP, abi.RS_HLLE, abi.RS_SLAU, abi.RS_SLAU2, abi.RS_HLLEPP, abi.RS_HLLCP])
 def test_riemann_mirror_symmetry(rs):
     """F(L, R, n) = -F(R, L, -n) componentwise (relabelling the two sides)."""
     cfg = abi.make_cfg(riemann=rs)
     rng = np.random.default_rng(4)
     for _ in range(200):
         L, R, n = rand_state(rng, rng.uniform(0, 2)), rand_state(rng, rng.uniform(0, 2)), rand_normal(rng)
-        F1 = O.riemann(cfg, L, R, n)
-        F2 = O.riemann(cfg, R, L, -n)
+        beta = rng.uniform()
+        F1 = O.riemann(cfg, L, R, n, beta)
+        F2 = O.riemann(cfg, R, L, -n, beta)
         assert np.allclose(F1, -F2, rtol=1e-11, atol=1e-12)
 
 
@@ -246,6 +247,69 @@ def test_slau_pressure_jump_at_rest(rs):
         F = O.riemann(cfg, L, R, n)
         expect = [m, *(0.5 * (L[4] + R[4]) * n), m * h]
         assert np.allclose(F, expect, rtol=1e-13, atol=1e-14)
+
+
+def test_hllepp_is_hlle_at_beta_zero():
+    """HLLE++ written as the central flux minus a Roe wave decomposition with HLLE's eigenvalue
+    magnitudes |((S4 + S1) l - 2 S4 S1)/(S4 - S1)| is algebraically the HLLE flux (Roe's property
+    F(R) - F(L) = A_roe (U(R) - U(L)) and S1 <= l <= S4 for every Roe eigenvalue), so at beta = 0
+    the two agree to round-off: a wrong eigenvector, wave strength or tangent basis fails this."""
+    rng = np.random.default_rng(13)
+    cpp, ce = abi.make_cfg(riemann=abi.RS_HLLEPP), abi.make_cfg(riemann=abi.RS_HLLE)
+    for _ in range(300):
+        L, R, n = rand_state(rng, rng.uniform(0, 3)), rand_state(rng, rng.uniform(0, 3)), rand_normal(rng)
+        Fe = O.riemann(ce, L, R, n)
+        assert np.allclose(O.riemann(cpp, L, R, n, 0.0), Fe, rtol=1e-12, atol=1e-12 * (1 + np.abs(Fe).max()))
+
+
+def test_hllepp_roe_limit_resolves_contacts_and_shear():
+    """At beta = 1 (Roe eigenvalues): a stationary contact with a shear jump (u_n = 0, equal p,
+    different rho and tangential velocity) gives F = (0, p n, 0): the contact and shear waves
+    travel at u_n = 0 and the acoustic waves carry no jump."""
+    cfg = abi.make_cfg(riemann=abi.RS_HLLEPP)
+    rng = np.random.default_rng(14)
+    for _ in range(50):
+        n = rand_normal(rng)
+        p = rng.uniform(0.2, 2)
+        L = np.array([rng.uniform(0.1, 3), *np.cross(n, rand_normal(rng)), p])
+        R = np.array([rng.uniform(0.1, 3), *np.cross(n, rand_normal(rng)), p])
+        F = O.riemann(cfg, L, R, n, 1.0)
+        assert np.allclose(F, [0, p * n[0], p * n[1], p * n[2], 0], atol=1e-13)
+
+
+def test_hllcp_blends_hllc_and_hlle():
+    """HLLC+ (riemann_HLLCSD): HLLE at beta = 0, HLLC at beta >= 1, the convex blend between."""
+    rng = np.random.default_rng(15)
+    cp, cc, ce = (abi.make_cfg(riemann=r) for r in (abi.RS_HLLCP, abi.RS_HLLC, abi.RS_HLLE))
+    for _ in range(100):
+        L, R, n = rand_state(rng, rng.uniform(0, 2)), rand_state(rng, rng.uniform(0, 2)), rand_normal(rng)
+        Fc, Fe = O.riemann(cc, L, R, n), O.riemann(ce, L, R, n)
+        assert np.array_equal(O.riemann(cp, L, R, n, 0.0), Fe)
+        assert np.array_equal(O.riemann(cp, L, R, n, 1.0), Fc)
+        b = rng.uniform()
+        assert np.allclose(O.riemann(cp, L, R, n, b), b * Fc + (1 - b) * Fe, rtol=1e-14, atol=1e-14)
+
+
+def test_shock_detector_pins():
+    """SD_Tramel: 1 where the pressure is linear along every axis (no second difference); 0 at
+    a jump; invariant under scaling p; switches at a normalised second difference of 1/20
+    (1 - tanh(10 (1/400)^3) just below, 0 at and above); takes the worst of the three axes."""
+    rng = np.random.default_rng(16)
+    for _ in range(50):
+        p, gx, gy, gz = rng.uniform(1, 2), rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3)
+        assert O.sd_tramel(p, p - gx, p + gx, p - gy, p + gy, p - gz, p + gz) == pytest.approx(1.0, abs=1e-15)
+    assert O.sd_tramel(1.0, 1.0, 10.0, 1.0, 1.0, 1.0, 1.0) == 0.0
+    # second difference e = p2 - 2p + p1 with |e / (p2 + 2p + p1)| = r
+    def sw_case(r, axis):
+        q = [1.0] * 6
+        e = r * 4.0 / (1.0 - r)          # p = p1 = 1, p2 = 1 + e: e / (4 + e) = r
+        q[2 * axis + 1] = 1.0 + e
+        return q
+    for axis in range(3):
+        assert O.sd_tramel(1.0, *sw_case(0.049, axis)) == pytest.approx(1.0 - np.tanh(10 * (0.049 / 20) ** 3), rel=1e-12)
+        assert O.sd_tramel(1.0, *sw_case(0.0501, axis)) == 0.0
+        q = sw_case(0.03, axis)
+        assert O.sd_tramel(3.0, *[3.0 * x for x in q]) == pytest.approx(O.sd_tramel(1.0, *q), rel=1e-14)
 
 
 def cons(P):
