@@ -129,7 +129,12 @@ int  mose_step(mose_ctx* ctx, double* dt_out, double* time_out, double* norms);
  * captured hipGraph).  norms (if not NULL) receives the last step's norms. */
 int  mose_advance(mose_ctx* ctx, int32_t nsteps, double* time_out, double* norms);
 
-/* One Fill_Ghost_Cell + Fluxes + BC_Fluxes on the current state, no update: the residual
+/* Ghost fill (every entry point that evaluates fluxes starts with it): the records run in
+ * the reference's bc order semantics -- across a block one or two cells thick a record reads
+ * the opposite face's ghost cells as an earlier record of the same fill wrote them, or as the
+ * previous fill (or mose_put_state) left them -- so such blocks give deterministic results.
+ *
+ * One Fill_Ghost_Cell + Fluxes + BC_Fluxes on the current state, no update: the residual
  * R(nprim, 1-gc:ni+gc, ...) of block b in the blk%R layout (interior cells filled, ghosts 0).
  * This is the value the reference holds in blk%R right before RK_Newstate. */
 int  mose_residual(mose_ctx* ctx, int32_t b, double* R);
@@ -140,8 +145,9 @@ int  mose_residual(mose_ctx* ctx, int32_t b, double* R);
  * (Rbar in the blk%R layout; ghost entries ignored), mose_residual_vjp, then
  * mose_get_state_bar for every block: Pbar = (dR/dP)^T Rbar in the blk%P layout (ghost
  * entries 0), R being the residual of the current state including its ghost fill.
- * Limitation: ghost cells filled from other ghost cells (extrapolation across a block
- * thinner than three cells) are treated as constants. */
+ * Limitations: ghost cells filled from other ghost cells (extrapolation across a block
+ * thinner than three cells) are treated as constants; the schemes that use the shock
+ * detector (MOSE_RS_HLLEPP, MOSE_RS_HLLCP, muscl_sd) have no adjoint (MOSE_E_ARG). */
 int  mose_put_residual_bar(mose_ctx* ctx, int32_t b, const double* Rbar);
 int  mose_residual_vjp(mose_ctx* ctx);
 int  mose_get_state_bar(mose_ctx* ctx, int32_t b, double* Pbar);
