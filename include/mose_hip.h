@@ -73,6 +73,8 @@ typedef struct {
   int32_t muscl_sd;         /* [MOSE-Numerics] space-reconstruction = MUSCL-SD: limited slopes  */
                             /* scaled by the shock-detector weight (Lib_Reconstruction.f90:26-27) */
   int32_t reserved[3];
+  double  irs_beta;         /* [MOSE-Numerics] irs-beta: implicit residual smoothing of every   */
+                            /* stage's residual when > 0 (Lib_Newstate.f90:41-75, Lib_IRS.f90)    */
 } mose_cfg;
 
 /* One boundary-face-cell record of bc.txt (src/lib/io/IO_BC.f90:136-414).

@@ -119,6 +119,169 @@ __global__ void k_dtreset(unsigned long long* dtmin) {
   *dtmin = (unsigned long long)__double_as_longlong(1e5);
 }
 
+// ---------------------------------------------------------------------------- IRS
+// Implicit residual smoothing (Newstate_Conservative with IRS, Lib_Newstate.f90:41-75, and
+// Residual_Smoothing, Lib_IRS.f90:14-191): the stage kernel writes the residual R, k_irs_scale
+// turns it into R* = -R dt / vol, each direction d < ndir gets the one-layer residual ghosts
+// (k_irs_ghost: the donor cell for 101 connections, the boundary cell for every other type)
+// and two Jacobi sweeps R*_i <- (R_i + beta (R*_{i-1} + R*_{i+1})) / (1 + 2 beta) with the ghost
+// values held, and k_irs_update takes the RK stage with the smoothed R*.
+struct IrsArgs {
+  double* R;
+  const double* src;
+  double* dst;
+  const Tile* tiles;
+  const BlockDev* blocks;
+  const LineEnt* lines;
+  const double* geo;
+  const double* scal;
+  const double* dtloc;
+  int64_t npool;
+  int ntiles, tx, local_dt, d;
+  double beta;
+};
+
+template <int GEO>
+__global__ void __launch_bounds__(256) k_irs_scale(IrsArgs a) {
+  const int tx = threadIdx.x % a.tx, ty = threadIdx.x / a.tx;
+  const int64_t N = a.npool;
+  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const Tile T = a.tiles[t];
+    const BlockDev B = a.blocks[T.b];
+    const int i = T.i0 + tx, j = T.j0 + ty;
+    if (i > B.ni || j > B.nj) continue;
+    const LineEnt* Lx = a.lines + B.line[0];
+    const LineEnt* Ly = a.lines + B.line[1];
+    const LineEnt* Lz = a.lines + B.line[2];
+    const int64_t cbase = B.off + (int64_t)(i + 1) + (int64_t)B.sy * (j + 1);
+    for (int k = T.k0; k < T.k1; ++k) {
+      const int64_t c = cbase + B.sz * (int64_t)(k + 1);
+      const double vol = GEO != 1 ? Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta : a.geo[(size_t)GVOL * N + c];
+      const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) a.R[v * N + c] = -a.R[v * N + c] / vol * dt;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_irs_ghost(const int64_t* __restrict__ pairs, int n, double* R, int64_t N) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int64_t dst = pairs[2 * q], src = pairs[2 * q + 1];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) R[v * N + dst] = R[v * N + src];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_irs_jacobi(IrsArgs a) {
+  const int tx = threadIdx.x % a.tx, ty = threadIdx.x / a.tx;
+  const int64_t N = a.npool;
+  const double den = 1.0 + 2.0 * a.beta;
+  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const Tile T = a.tiles[t];
+    const BlockDev B = a.blocks[T.b];
+    const int i = T.i0 + tx, j = T.j0 + ty;
+    if (i > B.ni || j > B.nj) continue;
+    const int64_t st = a.d == 0 ? 1 : (a.d == 1 ? (int64_t)B.sy : B.sz);
+    const int64_t cbase = B.off + (int64_t)(i + 1) + (int64_t)B.sy * (j + 1);
+    for (int k = T.k0; k < T.k1; ++k) {
+      const int64_t c = cbase + B.sz * (int64_t)(k + 1);
+#pragma unroll
+      for (int v = 0; v < 5; ++v)
+        a.dst[v * N + c] = (a.R[v * N + c] + a.beta * (a.src[v * N + c - st] + a.src[v * N + c + st])) / den;
+    }
+  }
+}
+
+// RK stage of every interior cell with the smoothed R* in a.Rout (MODE 0 / 1 of k_stage
+// without the fluxes): U = Un + rkc (Uin - Un + R*), the state check and density floor, and in
+// the last stage the CFL time step of the new state and the residual-norm partials per tile.
+template <int GEO, bool LAST>
+__global__ void __launch_bounds__(256) k_irs_update(StageArgs a, int tx_) {
+  const int tid = threadIdx.x, tx = tid % tx_, ty = tid / tx_;
+  const double gm1 = a.gam - 1.0;
+  const int64_t N = a.npool;
+  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const Tile T = a.tiles[t];
+    const BlockDev* __restrict__ Bp = a.blocks + T.b;
+    const int i = T.i0 + tx, j = T.j0 + ty;
+    double nsum[5] = {0, 0, 0, 0, 0};
+    double dtmin_t = 1e300;
+    if (i <= Bp->ni && j <= Bp->nj) {
+      const LineEnt* Lx = a.lines + Bp->line[0];
+      const LineEnt* Ly = a.lines + Bp->line[1];
+      const LineEnt* Lz = a.lines + Bp->line[2];
+      const int64_t cbase = Bp->off + (int64_t)(i + 1) + (int64_t)Bp->sy * (j + 1);
+      for (int k = T.k0; k < T.k1; ++k) {
+        const int64_t c = cbase + Bp->sz * (int64_t)(k + 1);
+        double un[5], ui[5], unew[5], P[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) { un[v] = a.Un[v * N + c]; ui[v] = a.Uin[v * N + c]; }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) unew[v] = un[v] + a.rkc * (ui[v] - un[v] + a.Rout[v * N + c]);
+        cons2prim(unew, gm1, P);
+        bool bad = !(P[4] >= 0.0);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) bad = bad || (P[v] != P[v]);
+        if (bad) set_error(a.err, MOSE_E_STATE, T.b + 1, i, j, k);
+        if (P[0] < 1e-20) { P[0] = 1e-20; prim2cons(P, 1.0 / gm1, unew); }
+        if constexpr (LAST) {
+          const double snd = sqrt(a.gam * P[4] / P[0]);
+          double dtc = 1e8;
+          if constexpr (GEO != 1) {
+            dtc = fmin(Lx[i + 1].dl / (fabs(P[1]) + snd) * a.cfl, dtc);
+            if (a.ndir > 1) dtc = fmin(Ly[j + 1].dl / (fabs(P[2]) + snd) * a.cfl, dtc);
+            if (a.ndir > 2) dtc = fmin(Lz[k + 1].dl / (fabs(P[3]) + snd) * a.cfl, dtc);
+            if (GEO == 2 && Bp->visc) {   // VNN condition (Mod_dt.f90:88-103)
+              const double mi = sutherland(P[4] / (P[0] * a.R), Bp->vis[0], Bp->vis[1], Bp->vis[2]);
+              const double dls[3] = {Lx[i + 1].dl, Ly[j + 1].dl, Lz[k + 1].dl};
+              for (int dd = 0; dd < a.ndir; ++dd) dtc = fmin(P[0] * dls[dd] * dls[dd] * Bp->vis[4] / mi, dtc);
+            }
+          } else {
+            for (int dd = 0; dd < a.ndir; ++dd) {
+              const double vel = fabs(P[1] * a.geo[(size_t)(GVS + 3 * dd) * N + c] +
+                                      P[2] * a.geo[(size_t)(GVS + 3 * dd + 1) * N + c] +
+                                      P[3] * a.geo[(size_t)(GVS + 3 * dd + 2) * N + c]);
+              dtc = fmin(a.geo[(size_t)(GDL + dd) * N + c] / (vel + snd) * a.cfl, dtc);
+            }
+          }
+          const int itn = *a.iter + 1;
+          if (itn < a.rampa) dtc = dtc * itn / a.rampa;
+          if (a.local_dt) a.dtnext[c] = dtc;
+          dtmin_t = fmin(dtmin_t, dtc);
+          double pn[5];
+          cons2prim(un, gm1, pn);
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            const double dd = P[v] - pn[v];
+            nsum[v] += dd * dd;
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) a.Uout[v * N + c] = unew[v];
+      }
+    }
+    if constexpr (LAST) {   // fixed-order reduction over the workgroup
+      __shared__ double sW[4][5];
+      const int lane = tid & 63, wave = tid >> 6, nw = (int)(blockDim.x / 64);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        double x = nsum[v];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) sW[wave][v] = x;
+      }
+      __syncthreads();
+      if (tid < 5) {
+        double tot = 0.0;
+        for (int w = 0; w < nw; ++w) tot += sW[w][tid];
+        a.partials[5 * (int64_t)t + tid] = tot;
+      }
+      for (int o = 32; o > 0; o >>= 1) dtmin_t = fmin(dtmin_t, __shfl_xor(dtmin_t, o, 64));
+      if (lane == 0 && dtmin_t < 1e300) atomicMin(a.dtmin, (unsigned long long)__double_as_longlong(dtmin_t));
+      __syncthreads();
+    }
+  }
+}
+
 // Shock-detector weight of every interior cell (Fluxes_blk, Mod_Fluxes.f90:78-85): SD_Tramel of
 // the pressures of the cell and its six face neighbours, ghost layers filled.  Ghost entries
 // keep the 1 they were initialised with.
@@ -363,6 +526,10 @@ struct mose_ctx {
   int64_t* d_garc_cell = nullptr;             // archived ghost cells (previous-fill values)
   double* d_garc = nullptr;
   int64_t narc = 0;
+  double* d_rs1 = nullptr;                    // IRS iterates (pool-sized), when irs_beta > 0
+  double* d_rs2 = nullptr;
+  int64_t* d_irsg = nullptr;                  // IRS residual ghosts: (ghost, source) per bc record
+  int nirsg = 0;
   BcFace* d_bcfaces = nullptr;
   double* d_bcbuf = nullptr;
   ViscFace* d_vfaces = nullptr;
@@ -511,6 +678,48 @@ static void enqueue_prime_dt(mose_ctx* c) {
   launch_dt(c, da);
 }
 
+static IrsArgs irs_args(mose_ctx* c) {
+  IrsArgs ia{};
+  ia.R = c->Rbuf; ia.tiles = c->d_tiles; ia.blocks = c->d_blocks; ia.lines = c->d_lines; ia.geo = c->d_geo;
+  ia.scal = c->d_scal; ia.dtloc = c->d_dtloc; ia.npool = c->npool; ia.ntiles = (int)c->lay.tiles.size();
+  ia.tx = c->tx; ia.local_dt = c->cfg.time_accurate ? 0 : 1; ia.beta = c->cfg.irs_beta;
+  return ia;
+}
+
+// The smoothing and the RK update of one stage whose residual is in Rbuf.
+static void enqueue_irs(mose_ctx* c, const StageArgs& a, bool last) {
+  IrsArgs ia = irs_args(c);
+  const dim3 grid(ia.ntiles), block(c->tx * c->ty);
+  if (c->lay.curvi) hipLaunchKernelGGL(k_irs_scale<1>, grid, block, 0, c->stream, ia);
+  else hipLaunchKernelGGL(k_irs_scale<0>, grid, block, 0, c->stream, ia);
+  const int64_t n5 = 5 * c->npool;
+  for (int d = 0; d < c->ndir; ++d) {
+    if (c->nirsg > 0)
+      hipLaunchKernelGGL(k_irs_ghost, dim3(launch_blocks(c->nirsg)), dim3(256), 0, c->stream, c->d_irsg, c->nirsg,
+                         c->Rbuf, c->npool);
+    hipLaunchKernelGGL(k_copy, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->Rbuf, c->d_rs1, n5);
+    hipLaunchKernelGGL(k_copy, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->Rbuf, c->d_rs2, n5);
+    ia.d = d;
+    ia.src = c->d_rs1; ia.dst = c->d_rs2;
+    hipLaunchKernelGGL(k_irs_jacobi, grid, block, 0, c->stream, ia);
+    ia.src = c->d_rs2; ia.dst = c->d_rs1;
+    hipLaunchKernelGGL(k_irs_jacobi, grid, block, 0, c->stream, ia);
+    hipLaunchKernelGGL(k_copy, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->d_rs1, c->Rbuf, n5);
+  }
+  StageArgs u = a;
+  u.Rout = c->Rbuf;
+  const int tx = c->tx;
+  const int geo = c->lay.curvi ? 1 : (c->lay.visc ? 2 : 0);
+  if (last) {
+    if (geo == 1) hipLaunchKernelGGL((k_irs_update<1, true>), grid, block, 0, c->stream, u, tx);
+    else if (geo == 2) hipLaunchKernelGGL((k_irs_update<2, true>), grid, block, 0, c->stream, u, tx);
+    else hipLaunchKernelGGL((k_irs_update<0, true>), grid, block, 0, c->stream, u, tx);
+  } else {
+    if (geo == 1) hipLaunchKernelGGL((k_irs_update<1, false>), grid, block, 0, c->stream, u, tx);
+    else hipLaunchKernelGGL((k_irs_update<0, false>), grid, block, 0, c->stream, u, tx);
+  }
+}
+
 // One explicit step enqueued on the context stream (no host synchronisation).  The CFL
 // time step of the next step is produced by the last RK stage from the state it writes.
 static int enqueue_step(mose_ctx* c) {
@@ -524,7 +733,12 @@ static int enqueue_step(mose_ctx* c) {
     enqueue_ghost(c, in);
     StageArgs a = stage_args(c, in, c->U[0], out, kRK[nrk - 1][s - 1]);
     enqueue_bcfaces(c, a);
-    launch_stage(c, stage_fn(c, s == nrk ? 1 : 0), a);
+    if (c->cfg.irs_beta > 0.0) {
+      launch_stage(c, stage_fn(c, 2), a);        // residual into Rbuf
+      enqueue_irs(c, a, s == nrk);
+    } else {
+      launch_stage(c, stage_fn(c, s == nrk ? 1 : 0), a);
+    }
   }
   if (nrk == 1) {
     const int64_t n = 5 * c->npool;
@@ -848,6 +1062,7 @@ int mose_create(const mose_cfg* cfg, mose_ctx** out) {
   if (cfg->limiter < 0 || cfg->limiter > 5 || cfg->riemann < 0 || cfg->riemann > 6 || cfg->n_rk < 1 ||
       cfg->n_rk > 3 || !(cfg->gamma > 1.0) || !(cfg->Rgas > 0.0) || cfg->precision < 0 || cfg->precision > 1)
     return MOSE_E_ARG;
+  if (!(cfg->irs_beta >= 0.0) || !(cfg->irs_beta < 1e300)) return MOSE_E_ARG;
   // MUSCL-SD: FP64 kernels of the limited reconstructions only
   if (cfg->muscl_sd != 0 && (cfg->muscl_sd != 1 || cfg->limiter == MOSE_LIM_FIRST_ORDER ||
                              cfg->precision != MOSE_PREC_FP64))
@@ -942,6 +1157,22 @@ int mose_finalize(mose_ctx* c) {
   if (!c->cfg.time_accurate) {
     HIPCHK(c, hipMalloc((void**)&c->d_dtloc, c->npool * sizeof(double)));
     HIPCHK(c, hipMemset(c->d_dtloc, 0, c->npool * sizeof(double)));
+  }
+  if (c->cfg.irs_beta > 0.0) {
+    HIPCHK(c, hipMalloc((void**)&c->d_rs1, 5 * c->npool * sizeof(double)));
+    HIPCHK(c, hipMalloc((void**)&c->d_rs2, 5 * c->npool * sizeof(double)));
+    HIPCHK(c, hipMemset(c->d_rs1, 0, 5 * c->npool * sizeof(double)));
+    HIPCHK(c, hipMemset(c->d_rs2, 0, 5 * c->npool * sizeof(double)));
+    // residual ghosts (Ghost_Residual_Extrapolation, Lib_IRS.f90:154-189): the first ghost
+    // layer of every bc record from the donor cell (101) or the boundary cell (all others)
+    std::vector<int64_t> pairs;
+    for (size_t q = 0; q < c->bc.size(); ++q) {
+      const GhostOp& op = c->lay.ghost[q];
+      pairs.push_back(op.b - op.s);
+      pairs.push_back(c->bc[q].type == 101 ? op.src1 : op.b);
+    }
+    c->nirsg = (int)c->bc.size();
+    if (c->nirsg > 0 && upload(c, &c->d_irsg, pairs)) return c->err.code;
   }
   if (c->cfg.muscl_sd || rs_takes_beta(c->cfg.riemann)) {
     if (c->cfg.muscl_sd && c->lay.visc) return fail(c, MOSE_E_ARG);   // no viscous MUSCL-SD kernels
@@ -1366,7 +1597,7 @@ void mose_destroy(mose_ctx* c) {
                   c->d_ghost, c->d_bcfaces, c->d_vfaces, c->d_vbuf, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
                   c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
                   c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_ghostp,
-                  c->d_garc_cell, c->d_garc};
+                  c->d_garc_cell, c->d_garc, c->d_rs1, c->d_rs2, c->d_irsg};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

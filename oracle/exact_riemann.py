@@ -161,3 +161,24 @@ def ein91(x, t=0.15, gamma=1.4):
     rho, u, p = sample(x, t, EIN91_LEFT, EIN91_RIGHT, 0.5, gamma)
     return rho, u, p, p / ((gamma - 1.0) * rho)
 
+
+def prandtl_meyer_nu(M, gamma=1.4):
+    """Prandtl-Meyer function nu(M) in radians (textbook closed form)."""
+    g = (gamma + 1.0) / (gamma - 1.0)
+    return np.sqrt(g) * np.arctan(np.sqrt((M * M - 1.0) / g)) - np.arctan(np.sqrt(M * M - 1.0))
+
+
+def prandtl_meyer_mach(M1, theta_deg, gamma=1.4):
+    """Mach number after an isentropic expansion of a Mach M1 flow by theta degrees: the root of
+    nu(M2) = nu(M1) + theta by bisection on [1, 30] (the reference's check uses the same
+    relation, test/2D/euler/prandtl-meyer/check.py)."""
+    target = prandtl_meyer_nu(M1, gamma) + theta_deg * np.pi / 180.0
+    lo, hi = 1.0, 30.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if prandtl_meyer_nu(mid, gamma) < target:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+

@@ -1292,8 +1292,102 @@ int ob_compute_dt(ob_domain* d, int32_t iter, double* dtglobal_out) {
  * Newstate_Conservative / Compute_Cons_NoIRS (Lib_Newstate.f90:13-93,134-160), RK_stage
  * with RKcoeff (Lib_RK.f90:8-27), Check_And_Fix_State (Lib_Newstate.f90:323-356) */
 static const double RKcoeff[3][3] = {{1.0, 0.0, 0.0}, {1.0, 0.5, 0.0}, {1.0, 0.25, 2.0 / 3.0}};
+
+/* Ghost_Residual_Extrapolation, src/lib/numerics/time/explicit/Lib_IRS.f90:154-189: the first
+ * ghost layer of every bc record, from the donor cell for 101 connections, from the boundary
+ * cell for every other type */
+static void ghost_residual_extrapolation(ob_domain* d) {
+  for (int64_t q = 0; q < d->nbc; ++q) {
+    const mose_bc_entry* e = &d->bc[q];
+    OBlk* B = &d->blk[e->b - 1];
+    const int* g = guide[e->f - 1];
+    double* dst = &B->R[NV * cidx(B, e->i - g[0], e->j - g[1], e->k - g[2])];
+    const double* src;
+    if (e->type == 101) {
+      OBlk* Bs = &d->blk[e->bs - 1];
+      src = &Bs->R[NV * cidx(Bs, e->is, e->js, e->ks)];
+    } else {
+      src = &B->R[NV * cidx(B, e->i, e->j, e->k)];
+    }
+    for (int v = 0; v < NV; ++v) dst[v] = src[v];
+  }
+}
+
+/* Residual_Smoothing / Residual_Smoothing_Blk, Lib_IRS.f90:14-120 (njacobi = 2): per direction,
+ * R* = R on cells 0..n+1, two Jacobi sweeps R*_new = (R + beta (R*(-1) + R*(+1))) / (1 + 2 beta)
+ * over the interior, then R = R* */
+static void residual_smoothing(ob_domain* d, double beta) {
+  const int ndir = ob_ndir(d);
+  for (int dir = 0; dir < ndir; ++dir) {
+    ghost_residual_extrapolation(d);
+    for (int b = 0; b < d->nb; ++b) {
+      OBlk* B = &d->blk[b];
+      double* Rs = (double*)malloc(sizeof(double) * NV * (size_t)B->npad);
+      double* Rn = (double*)malloc(sizeof(double) * NV * (size_t)B->npad);
+      memcpy(Rs, B->R, sizeof(double) * NV * (size_t)B->npad);
+      memcpy(Rn, B->R, sizeof(double) * NV * (size_t)B->npad);
+      const int64_t st = cidx(B, 1 + (dir == 0), 1 + (dir == 1), 1 + (dir == 2)) - cidx(B, 1, 1, 1);
+      for (int jac = 0; jac < 2; ++jac) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 1; k <= B->nk; ++k)
+          for (int j = 1; j <= B->nj; ++j)
+            for (int i = 1; i <= B->ni; ++i) {
+              const int64_t q = cidx(B, i, j, k);
+              for (int v = 0; v < NV; ++v)
+                Rn[NV * q + v] = (B->R[NV * q + v] + beta * (Rs[NV * (q - st) + v] + Rs[NV * (q + st) + v])) /
+                                 (1.0 + 2.0 * beta);
+            }
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 1; k <= B->nk; ++k)
+          for (int j = 1; j <= B->nj; ++j)
+            for (int i = 1; i <= B->ni; ++i) {
+              const int64_t q = cidx(B, i, j, k);
+              for (int v = 0; v < NV; ++v) Rs[NV * q + v] = Rn[NV * q + v];
+            }
+      }
+      for (int k = 1; k <= B->nk; ++k)
+        for (int j = 1; j <= B->nj; ++j)
+          for (int i = 1; i <= B->ni; ++i) {
+            const int64_t q = cidx(B, i, j, k);
+            for (int v = 0; v < NV; ++v) B->R[NV * q + v] = Rs[NV * q + v];
+          }
+      free(Rs);
+      free(Rn);
+    }
+  }
+}
+
+/* Test entry: residual_smoothing of a given residual field of block b (every other block's
+ * residual zero), the field in the blk%R layout, smoothed in place. */
+int ob_irs_smooth(ob_domain* d, int32_t b, double beta, double* R) {
+  if (b < 1 || b > d->nb || !(beta >= 0.0)) return set_err(d, MOSE_E_ARG, b, 0, 0, 0);
+  for (int q = 0; q < d->nb; ++q) memset(d->blk[q].R, 0, sizeof(double) * NV * d->blk[q].npad);
+  OBlk* B = &d->blk[b - 1];
+  memcpy(B->R, R, sizeof(double) * NV * B->npad);
+  residual_smoothing(d, beta);
+  memcpy(R, B->R, sizeof(double) * NV * B->npad);
+  return 0;
+}
+
 static int newstate(ob_domain* d, int irk) {
   double c = RKcoeff[d->cfg.n_rk - 1][irk - 1];
+  const int irs = d->cfg.irs_beta > 0.0;        /* obj_irs%enabled, Assign_Setup.f90:55 */
+  if (irs) {
+    /* Newstate_Conservative with IRS (Lib_Newstate.f90:41-75): scale every residual, smooth,
+     * then update */
+    for (int b = 0; b < d->nb; ++b) {
+      OBlk* B = &d->blk[b];
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int k = 1; k <= B->nk; ++k)
+        for (int j = 1; j <= B->nj; ++j)
+          for (int i = 1; i <= B->ni; ++i) {
+            int64_t q = cidx(B, i, j, k);
+            double* R = &B->R[NV * q];
+            for (int v = 0; v < NV; ++v) R[v] = -R[v] / B->vol[q] * B->dt[q] * 1.0;
+          }
+    }
+    residual_smoothing(d, d->cfg.irs_beta);
+  }
   for (int b = 0; b < d->nb; ++b) {
     OBlk* B = &d->blk[b];
     first_err fe = {INT64_MAX, 0, 0, 0, 0};
@@ -1306,7 +1400,8 @@ static int newstate(ob_domain* d, int irk) {
           double* R = &B->R[NV * q];
           const double* PO = &B->PO[NV * q];
           double cons[NV], consold[NV];
-          for (int v = 0; v < NV; ++v) R[v] = -R[v] / B->vol[q] * B->dt[q] * 1.0;
+          if (!irs)
+            for (int v = 0; v < NV; ++v) R[v] = -R[v] / B->vol[q] * B->dt[q] * 1.0;
           prim2cons_d(d, PO, consold);
           prim2cons_d(d, P, cons);
           for (int v = 0; v < NV; ++v) {

@@ -51,6 +51,8 @@ void   ob_riemann(const mose_cfg* cfg, const double PL[5], const double PR[5],
 void   ob_riemann_beta(const mose_cfg* cfg, const double PL[5], const double PR[5],
                        const double n[3], double beta, double F[5]);
 double ob_sd_tramel(double p, double p1, double p2, double p3, double p4, double p5, double p6);
+/* implicit residual smoothing of a residual field R (blk%R layout) of block b, in place */
+int    ob_irs_smooth(ob_domain* d, int32_t b, double beta, double* R);
 void   ob_prim2cons(const mose_cfg* cfg, const double P[5], double U[5]);
 void   ob_cons2prim(const mose_cfg* cfg, const double U[5], double P[5]);
 

@@ -49,6 +49,7 @@ def lib():
         L.ob_riemann.argtypes = [P(abi.MoseCfg), P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]
         L.ob_riemann_beta.argtypes = [P(abi.MoseCfg), P(C.c_double), P(C.c_double), P(C.c_double), C.c_double,
                                       P(C.c_double)]
+        L.ob_irs_smooth.argtypes = [C.c_void_p, C.c_int32, C.c_double, P(C.c_double)]
         L.ob_sd_tramel.restype = C.c_double
         L.ob_sd_tramel.argtypes = [C.c_double] * 7
         L.ob_prim2cons.argtypes = [P(abi.MoseCfg), P(C.c_double), P(C.c_double)]
@@ -118,6 +119,13 @@ class OracleSolver:
     def residual(self, b):
         R = np.zeros(abi.padded_shape(self.dims[b - 1]))
         self._check(lib().ob_residual(self.h, b, _dp(R)))
+        return R
+
+    def irs_smooth(self, b, R, beta):
+        """Residual_Smoothing of the field R (padded blk%R layout) of block b, the other
+        blocks' residuals zero; returns the smoothed field."""
+        R = np.ascontiguousarray(R, dtype=np.float64).copy()
+        self._check(lib().ob_irs_smooth(self.h, b, float(beta), _dp(R)))
         return R
 
     def residuals(self):

@@ -42,7 +42,7 @@ def test_fortran_binding_covers_the_abi():
 def test_struct_layouts_match_header():
     # mose_bc_entry: 16 int32 + 6 double = 112 bytes; mose_cfg: 3 double + 6 int32 + 4 int32 = 64
     assert C.sizeof(abi.MoseBcEntry) == 112
-    assert C.sizeof(abi.MoseCfg) == 64
+    assert C.sizeof(abi.MoseCfg) == 72
     assert C.sizeof(abi.MoseErr) == 20
     assert abi.BC_DTYPE.itemsize == 112
 

@@ -107,6 +107,61 @@ def test_tor99_shipped_configuration_probe_rule():
     assert not er.probe_fails((80, 240, 400, 560, 720), cfd, exact, {"p": 1.0})
 
 
+def test_irs_smoothing_pins():
+    """Residual_Smoothing (two Jacobi sweeps per direction of R* = (R + b (R*_-1 + R*_+1)) /
+    (1 + 2b), ghosts held): a unit impulse spreads to b^2/d^2, b/d^2, (1 + 2b^2/d)/d, ... with
+    d = 1 + 2b (the two-sweep response); constant and linear fields away from the ends are
+    unchanged; in 2-D the directions act one after the other, so an impulse's response is the
+    product of the 1-D ones; beta = 0 is the identity."""
+    b, d = 0.5, 2.0
+    case = cases.sod(n=30)
+    o = OracleSolver(case)
+    R = np.zeros(abi.padded_shape((30, 1, 1)))
+    R[2, 2, 2 + 14, :] = 1.0
+    Rs = o.irs_smooth(1, R, b)[2, 2, 2:-2, 0]
+    resp = [b * b / d / d, b / d / d, (1 + b * 2 * b / d) / d, b / d / d, b * b / d / d]
+    assert np.allclose(Rs[12:17], resp, rtol=1e-15)
+    assert np.all(Rs[:12] == 0) and np.all(Rs[17:] == 0)
+    assert np.array_equal(o.irs_smooth(1, R, 0.0)[2, 2, 2:-2], R[2, 2, 2:-2])
+    x = np.arange(30, dtype=float)
+    R = np.zeros(abi.padded_shape((30, 1, 1)))
+    R[2, 2, 2:-2, :] = (0.3 + 0.01 * x)[:, None]
+    Rs = o.irs_smooth(1, R, 0.7)[2, 2, 2:-2, 0]
+    assert np.allclose(Rs[3:-3], 0.3 + 0.01 * x[3:-3], rtol=1e-14)
+    R[2, 2, 2:-2, :] = 1.7
+    assert np.allclose(o.irs_smooth(1, R, 0.7)[2, 2, 2:-2, :], 1.7, rtol=1e-15)
+    # 2-D: product of the 1-D responses (impulse far from the block ends)
+    nodes = cases.cartesian_nodes((16, 16, 1), (0, 0, 0), (1, 1, 0.1))
+    prim = np.zeros((1, 16, 16, 5))
+    prim[..., 0], prim[..., 4] = 1.0, 1.0
+    case2 = cases.Case("box2", abi.make_cfg(), [cases.Block((16, 16, 1), nodes, cases.padded_state((16, 16, 1), prim))],
+                       cases.bc_records([{"dims": (16, 16, 1), "faces": [400, 400, 400, 400, 0, 0]}]))
+    o2 = OracleSolver(case2)
+    R = np.zeros(abi.padded_shape((16, 16, 1)))
+    R[2, 2 + 7, 2 + 8, :] = 1.0
+    Rs = o2.irs_smooth(1, R, b)[2, 2:-2, 2:-2, 0]
+    assert np.allclose(Rs[7 - 2:7 + 3, 8 - 2:8 + 3], np.outer(resp, resp), rtol=1e-14)
+
+
+def test_prandtl_meyer_shipped_configuration():
+    """test/2D/euler/prandtl-meyer with its input.ini settings (HLLE + van Leer, RK3, local time
+    step, CFL 1.0, implicit residual smoothing irs-beta = 0.5, 2500 iterations) on its input
+    (dimensional air, R = p / (rho T) of the input) passes its check.py: the Mach number at the
+    probe (block 1, cell 50, 33, 1) within 0.5 % of the Prandtl-Meyer expansion of Mach 2 by
+    34.971 degrees."""
+    from mose_amd import formats
+    ic = os.path.join(GOLDEN, "prandtl_meyer_ic.tec.gz")
+    (b,) = formats.read_tecplot(ic)
+    R = float(b["prim"][0, 0, 0, 4] / b["prim"][0, 0, 0, 0])     # T = 1 in the input
+    cfg = abi.make_cfg(1.4, R, 1.0, "vanleer", "HLLE", 3, False, irs_beta=0.5)
+    o = OracleSolver(formats.load_case(ic, os.path.join(GOLDEN, "prandtl_meyer_bc.txt.gz"), cfg))
+    for _ in range(2500):
+        o.step()
+    p = interior(o.get_state(1))[0, 32, 49]
+    mach = np.sqrt(np.sum(p[1:4] ** 2)) / np.sqrt(1.4 * p[4] / p[0])
+    assert abs(mach - er.prandtl_meyer_mach(2.0, 34.971)) / er.prandtl_meyer_mach(2.0, 34.971) * 100 <= 0.5
+
+
 def _reference_1d_input(name, limiter, cfl):
     """A case from the reference's own 1-D input files (test/1D/<name>/INPUT, 800 cells),
     with the numerics of its input.ini."""
