@@ -92,6 +92,8 @@ def bc_records(block_faces):
 
     block_faces: list over blocks of a list of 6 face specs.  A face spec is either an
     int BC type (0, 300, 400), a dict {"type": 405, mach, T, p, alpha, beta, rel_fac},
+    {"type": 401, T (the stagnation T0), p (p0), alpha, beta, rel_fac}, {"type": 406, p (the
+    back pressure), rel_fac},
     or a dict {"type": 101|201, "bs": donor block, "fs": donor face} for an aligned
     1-to-1 connection (orientation d11,d12,d21,d22 = 1,0,0,1)."""
     parts = []
@@ -120,6 +122,11 @@ def bc_records(block_faces):
             elif spec["type"] == 405:
                 for key in ("mach", "T", "p", "alpha", "beta", "rel_fac"):
                     arr[key] = spec[key]
+            elif spec["type"] == 401:
+                for key in ("T", "p", "alpha", "beta", "rel_fac"):
+                    arr[key] = spec[key]
+            elif spec["type"] == 406:
+                arr["p"], arr["rel_fac"] = spec["p"], spec.get("rel_fac", 1.0)
             parts.append(arr)
     return np.concatenate(parts)
 

@@ -505,6 +505,97 @@ __device__ __forceinline__ bool inflow405_flux(const T* Pb, const T* Pi, bool hi
   return true;
 }
 
+// Subsonic inlet with stagnation conditions, BC 401 (BC_Inlet_StagnCond + Stagn_Inflow,
+// Lib_BC_Fluxes_Inflow.f90): from the outgoing Riemann invariant of the boundary cell and the
+// imposed T0, p0 and flow direction, the face state on the isentrope through (T0, p0); when the
+// quadratic for the face normal velocity has no real root the imposed p0 is relaxed towards the
+// cell pressure (rel_fac) until it has one (at most 1000 times).  Returns false when the
+// boundary-cell flow leaves the domain (the caller then uses the symmetry flux).  FP64;
+// bc = {T0, p0, alpha, beta, rel_fac, -, -, flags}, t the unit face diagonal.
+__device__ inline bool inlet401_flux(const double* Pb, bool hi, const double* n, const double* t, const double* bc,
+                                     double gam, double Rg, double* F) {
+  const double sign = hi ? 1.0 : -1.0;
+  const double un = Pb[1] * n[0] + Pb[2] * n[1] + Pb[3] * n[2];
+  if (sign * un >= 0.0) return false;
+  const double snd = sqrt(gam * Pb[4] / Pb[0]);
+  const double riem = snd + sign * 0.5 * (gam - 1.0) * un;
+  const int flags = (int)bc[7];
+  const double alpha = (flags & 1) ? atan(n[1] / (n[0] + 1e-20)) : bc[2];
+  const double beta = (flags & 2) ? atan(n[2] / (n[0] + 1e-20)) : bc[3];
+  // b = n x t; velocity components along t and b per unit normal velocity, E1 / E2
+  double b[3] = {n[1] * t[2] - n[2] * t[1], n[2] * t[0] - n[0] * t[2], n[0] * t[1] - n[1] * t[0]};
+  const double bm = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  b[0] /= bm; b[1] /= bm; b[2] /= bm;
+  const double ta = tan(alpha), tb = tan(beta);
+  const double XA = n[1] - n[0] * ta, XB = t[1] - t[0] * ta, XC = b[1] - b[0] * ta;
+  const double XD = n[2] - n[0] * tb, XE = t[2] - t[0] * tb, XF = b[2] - b[0] * tb;
+  const double check = XC * XE - XF * XB;
+  const double E1 = (XF * XA - XC * XD) / check, E2 = (XD * XB - XE * XA) / check;
+  const double T0 = bc[0], rel = bc[4];
+  double p0 = bc[1];
+  const double snd0 = sqrt(gam * Rg * T0);
+  const double cD = (1.0 + E1 * E1 + E2 * E2) * (gam - 1.0) / 2.0;
+  double YA = 0.0, YB = 0.0, disc = -1.0;
+  for (int it = 1; disc < 0.0; ++it) {
+    double cC = pow(p0 / Pb[4], (gam - 1.0) / gam);
+    cC = cC * ((snd / snd0) * (snd / snd0));
+    YA = ((gam - 1.0) / 2.0) * ((gam - 1.0) / 2.0) + cC * cD;
+    YB = sign * riem * ((gam - 1.0) / 2.0);
+    const double YC = riem * riem - cC * snd0 * snd0;
+    disc = YB * YB - YA * YC;
+    if (disc < 0.0 && fabs(disc) < 1e-2) disc = 0.0;
+    if (disc < 0.0 && fabs(disc) > 1e-2) p0 = rel * p0 + (1.0 - rel) * Pb[4];
+    if (it > 1000) { disc = 0.0; break; }
+  }
+  const double un3 = (YB + (un >= 0.0 ? 1.0 : -1.0) * sqrt(disc)) / YA;
+  const double snd3 = sqrt(snd0 * snd0 - cD * un3 * un3);
+  const double p3 = p0 * pow(snd3 / snd0, 2.0 * gam / (gam - 1.0));
+  const double rho = gam * p3 / (snd3 * snd3);
+  double u[3];
+  for (int q = 0; q < 3; ++q) u[q] = un3 * n[q] + un3 * E1 * t[q] + un3 * E2 * b[q];
+  const double h = gam / (gam - 1.0) * p3 / rho;
+  const double m = rho * un3;
+  F[0] = m;
+  for (int q = 0; q < 3; ++q) F[1 + q] = m * u[q] + p3 * n[q];
+  F[4] = m * (0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) + h);
+  return true;
+}
+
+// Outlet, BC 406 (BC_Outflow + Outflow, Lib_BC_Fluxes_Outflow.f90:12-130): subsonic normal Mach
+// and a back pressure p_exit > 0: the face pressure is max(p_exit, p*) (p* the cell's critical
+// pressure), the face state follows the isentrope from the cell and its outgoing Riemann
+// invariant, tangential velocity extrapolated; otherwise the cell state.  Returns false when the
+// boundary-cell flow enters the domain (symmetry flux instead).  FP64; bc = {p_exit, rel_fac}.
+__device__ inline bool outflow406_flux(const double* Pb, bool hi, const double* n, const double* bc, double gam,
+                                       double* F) {
+  const double sign = hi ? 1.0 : -1.0;
+  const double un = Pb[1] * n[0] + Pb[2] * n[1] + Pb[3] * n[2];
+  if (sign * un < 0.0) return false;
+  const double snd = sqrt(gam * Pb[4] / Pb[0]);
+  const double riem = snd + sign * 0.5 * (gam - 1.0) * un;
+  const double vt[3] = {Pb[1] - un * n[0], Pb[2] - un * n[1], Pb[3] - un * n[2]};
+  const double machn = fabs(un / snd);
+  double rho = Pb[0], p = Pb[4], ufn = un, u[3] = {Pb[1], Pb[2], Pb[3]};
+  double pe = bc[0];
+  if (machn < 1.0 && pe > 0.0) {
+    const double delta = 0.5 * (gam - 1.0);
+    const double den = 1.0 / pow(1.0 + delta, gam / (gam - 1.0));
+    const double pstar = Pb[4] * den * pow(1.0 + delta * machn * machn, gam / (gam - 1.0));
+    if (pstar >= pe) pe = pstar;
+    const double sndf = snd * pow(pe / Pb[4], (gam - 1.0) / (2.0 * gam));
+    ufn = (riem - sndf) / (sign * 0.5 * (gam - 1.0));
+    p = pe;
+    rho = gam * p / (sndf * sndf);
+    for (int q = 0; q < 3; ++q) u[q] = vt[q] + ufn * n[q];
+  }
+  const double h = gam / (gam - 1.0) * p / rho;
+  const double m = rho * ufn;
+  F[0] = m;
+  for (int q = 0; q < 3; ++q) F[1 + q] = u[q] * m + p * n[q];
+  F[4] = m * (0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) + h);
+  return true;
+}
+
 // ---------------------------------------------------------------------------- viscous
 // Laminar viscosity by Sutherland's law, mu = mu_ref (T/T_ref)^1.5 (T_ref + S)/(T + S).
 template <typename T>

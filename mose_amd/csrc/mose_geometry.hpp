@@ -85,6 +85,8 @@ static int face_kind(int type) {
     case 400: return mose::F_EXTRAP;
     case 300: return mose::F_SYM;
     case 405: return mose::F_INFLOW;
+    case 401: return mose::F_INLET;
+    case 406: return mose::F_OUTFLOW;
     default: return -1;
   }
 }
@@ -294,7 +296,7 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
     if (geo::face_kind(e.type) < 0) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
     if (ftype[slot] == -9999) { ftype[slot] = e.type; fentry[slot] = (int)q; }
     else if (ftype[slot] != e.type) return geo::set_err(err, MOSE_E_BC, e.b, e.i, e.j, e.k);
-    if (e.type == 405) {   // one inflow record per block face: every cell of the face carries the same values
+    if (e.type == 405 || e.type == 401 || e.type == 406) {   // one record set per block face: every cell of the face carries the same values
       const mose_bc_entry& e0 = bc[fentry[slot]];
       if (e.mach != e0.mach || e.T != e0.T || e.p != e0.p || e.alpha != e0.alpha || e.beta != e0.beta ||
           e.rel_fac != e0.rel_fac)
@@ -479,6 +481,20 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
           v[7] = (double)flags;
         }
         L.facebc.insert(L.facebc.end(), v, v + 8);
+      } else if (ftype[6 * b + f] == 401 || ftype[6 * b + f] == 406) {
+        const mose_bc_entry& e = bc[fentry[6 * b + f]];
+        D.bcd[f] = (int)(L.facebc.size() / 8);
+        double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (ftype[6 * b + f] == 401) {
+          // {T0, p0, alpha, beta, rel_fac, -, -, flags}: flag bits mark the 'normal' token
+          // (angles resolved per face cell from its normal, Compute_Inflow_Direction)
+          const int flags = (e.alpha >= 0.5 * DBL_MAX ? 1 : 0) | (e.beta >= 0.5 * DBL_MAX ? 2 : 0);
+          v[0] = e.T; v[1] = e.p; v[2] = (flags & 1) ? 0.0 : e.alpha; v[3] = (flags & 2) ? 0.0 : e.beta;
+          v[4] = e.rel_fac; v[7] = (double)flags;
+        } else {
+          v[0] = e.p; v[1] = e.rel_fac;     // {p_exit, rel_fac}
+        }
+        L.facebc.insert(L.facebc.end(), v, v + 8);
       }
     }
   }
@@ -557,6 +573,18 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
             e.idl = 0.5 * LE(b, d, m3[d]).dl;
             e.n[d] = 1.0;
             e.area = LE(b, d1, c3[d1]).delta * LE(b, d2, c3[d2]).delta;
+          }
+          {   // face diagonal (Setup_Inflow_Geometry, Lib_BC_Fluxes_Inflow.f90: t_Vec = node of the
+              // face's high corner minus node of the corner opposite along the face)
+            int hiN[3] = {c3[0], c3[1], c3[2]};
+            hiN[d] = hi ? n[d] : 0;
+            int loN[3] = {hiN[0] - 1, hiN[1] - 1, hiN[2] - 1};
+            loN[d] = hiN[d];
+            const double* p1 = B.node(hiN[0], hiN[1], hiN[2]);
+            const double* p0 = B.node(loN[0], loN[1], loN[2]);
+            const double tv[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+            const double tm = std::sqrt(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
+            for (int q = 0; q < 3; ++q) e.t[q] = tv[q] / tm;
           }
           L.bcfaces.push_back(e);
         }

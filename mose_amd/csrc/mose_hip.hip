@@ -840,8 +840,12 @@ static int retile(mose_ctx* c, int shape, int kchunk, int xcd) {
 
 // Discrete adjoint set-up: face work segments, the transposed ghost-fill map, buffers.
 static int ensure_adjoint(mose_ctx* c) {
-  // the adjoint of the shock-detector schemes (HLLE++, HLLC+, MUSCL-SD) is not built
+  // the adjoint of the shock-detector schemes (HLLE++, HLLC+, MUSCL-SD) and of the subsonic
+  // inlet / outlet faces (401, 406) is not built
   if (c->d_beta) return fail(c, MOSE_E_ARG);
+  for (const BlockDev& B : c->lay.blocks)
+    for (int f = 0; f < 6; ++f)
+      if (B.bct[f] == F_INLET || B.bct[f] == F_OUTFLOW) return fail(c, MOSE_E_ARG);
   if (c->adj_ready) return 0;
   std::vector<AdjSeg> segs;
   int64_t start = 0;

@@ -25,7 +25,7 @@ namespace mose {
 // Ghost-op kinds.
 enum { G_COPY = 0, G_ZG = 1, G_EXTRAP = 2, G_SYM = 3 };
 // Face BC kinds used inside the stage kernel (bctype of a block face).
-enum { F_CONN = 0, F_NONE = 1, F_EXTRAP = 2, F_SYM = 3, F_INFLOW = 4 };
+enum { F_CONN = 0, F_NONE = 1, F_EXTRAP = 2, F_SYM = 3, F_INFLOW = 4, F_INLET = 5, F_OUTFLOW = 6 };
 // Scalars kept on the device.
 enum { S_DT = 0, S_TIME = 1, S_NORM = 2, S_COUNT = 8 };
 // Curvilinear geometry arrays (index g: array g * npool + cell).
@@ -72,6 +72,7 @@ struct BcFace {
   double bdl, idl;   // half lengths of the two cells along the face normal
   double n[3];       // unit normal of the boundary face (increasing-index orientation)
   double area;       // face area
+  double t[3];       // unit face diagonal (t_Vec of Setup_Inflow_Geometry, subsonic inlets)
 };
 
 struct Tile {

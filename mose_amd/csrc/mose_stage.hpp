@@ -87,7 +87,19 @@ __global__ void __launch_bounds__(256) k_bcface(StageArgs a, const BcFace* __res
         physical_flux(Pb, nx, ny, nz, g, F);
       } else {
         bool done = false;
-        if (bct == F_INFLOW) done = inflow405_flux<LIM, RS>(Pb, Pi, hi, nx, ny, nz, a.facebc + 8 * Bp->bcd[e.f], g, F);
+        if (bct == F_INFLOW) {
+          done = inflow405_flux<LIM, RS>(Pb, Pi, hi, nx, ny, nz, a.facebc + 8 * Bp->bcd[e.f], g, F);
+        } else if (bct == F_INLET || bct == F_OUTFLOW) {   // FP64 boundary state
+          double P64[5], F64[5];
+          prim_at(a.Uin, (unsigned)a.npool, (unsigned)e.cb, gm1, P64);
+          const double* rec = a.facebc + 8 * Bp->bcd[e.f];
+          done = bct == F_INLET ? inlet401_flux(P64, hi, e.n, e.t, rec, a.gam, a.R, F64)
+                                : outflow406_flux(P64, hi, e.n, rec, a.gam, F64);
+          if (done) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) F[v] = FT(F64[v]);
+          }
+        }
         if (!done) symmetry_flux<LIM, RS>(Pb, Pi, FT(e.bdl), FT(e.idl), hi, nx, ny, nz, g, F);
       }
     }

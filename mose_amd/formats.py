@@ -9,8 +9,9 @@
 * bc.txt (Read_BCfile, IO_BC.f90:136-414): per boundary face cell a line
   "b i j k f type" and, for the types that carry properties, a second line: the donor
   (bs is js ks fs d11 d12 d21 d22) for 101/201, (mach T p alpha beta rel_fac [massf...])
-  for 405, where an angle may be the token 'normal' (parse_dir_tok, IO_BC.f90:443-456,
-  stored as the largest double).  The number of records must match the blocks
+  for 405, (T0 p0 alpha beta rel_fac [massf...]) for 401 (stored in T and p), (p_back
+  rel_fac) for 406 (p_back in p), where an angle may be the token 'normal' (parse_dir_tok,
+  IO_BC.f90:443-456, stored as the largest double).  The number of records must match the blocks
   (Check_BC, IO_BC.f90:77-134).
 """
 import gzip
@@ -25,7 +26,7 @@ from . import abi, cases
 NORMAL = sys.float_info.max          # the 'normal' direction token, like huge(1.0_R8)
 # types without a property line (IO_BC.f90:107-119) and the ones this path runs
 _NO_PROPS = {0, 200, 300, 400, 409}
-_SUPPORTED = {0, 101, 201, 300, 400, 405}
+_SUPPORTED = {0, 101, 201, 300, 400, 401, 405, 406}
 
 
 def _open_text(path):
@@ -172,6 +173,12 @@ def read_bc(path, dims_list=None):
                 r["mach"], r["T"], r["p"] = _num(p[0]), _num(p[1]), _num(p[2])
                 r["alpha"], r["beta"] = _dir_tok(p[3]), _dir_tok(p[4])
                 r["rel_fac"] = _num(p[5])
+            elif typ == 401:
+                r["T"], r["p"] = _num(p[0]), _num(p[1])
+                r["alpha"], r["beta"] = _dir_tok(p[2]), _dir_tok(p[3])
+                r["rel_fac"] = _num(p[4])
+            elif typ == 406:
+                r["p"], r["rel_fac"] = _num(p[0]), _num(p[1])
         if typ not in _SUPPORTED:
             raise ValueError("bc record %d (block %d face %d): type %d is not run by this path"
                              % (len(recs) + 1, b, f, typ))
@@ -193,10 +200,15 @@ def write_bc(path, bc):
         if r["type"] in (101, 201):
             out.append("%8d%8d%8d%8d%8d%8d%8d%8d%8d" % (r["bs"], r["is"], r["js"], r["ks"], r["fs"], r["d11"],
                                                          r["d12"], r["d21"], r["d22"]))
-        elif r["type"] == 405:
+        elif r["type"] in (401, 405):
             ang = ["%15s" % "normal" if x >= 0.5 * NORMAL else "%15.6E" % x for x in (r["alpha"], r["beta"])]
-            out.append("%15.6E,%15.6E,%15.6E,%s,%s,%15.6E,%15.6E," % (r["mach"], r["T"], r["p"], ang[0], ang[1],
-                                                                     r["rel_fac"], 1.0))
+            if r["type"] == 405:
+                out.append("%15.6E,%15.6E,%15.6E,%s,%s,%15.6E,%15.6E," % (r["mach"], r["T"], r["p"], ang[0], ang[1],
+                                                                         r["rel_fac"], 1.0))
+            else:
+                out.append("%15.6E,%15.6E,%s,%s,%15.6E,%15.6E," % (r["T"], r["p"], ang[0], ang[1], r["rel_fac"], 1.0))
+        elif r["type"] == 406:
+            out.append("%15.6E,%15.6E," % (r["p"], r["rel_fac"]))
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
 

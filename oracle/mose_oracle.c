@@ -888,6 +888,129 @@ static void bc_symmetry(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm) {
 /* BC_Inlet_Supersonic_Static + Supersonic_Inflow, Lib_BC_Fluxes_Inflow.f90:244-382.
  * Returns 1 (error) when the flow leaves through the face; the caller then applies the
  * symmetry flux (Mod_BC_Fluxes.f90:159-168). */
+/* Setup_Inflow_Geometry's face diagonal t_Vec (Lib_BC_Fluxes_Inflow.f90), normalised */
+static void face_tvec(const OBlk* B, int Fm, int fi, int fj, int fk, double* t) {
+  const double *a, *b;
+  if (Fm <= 2) { a = node_at(B, fi, fj, fk); b = node_at(B, fi, fj - 1, fk - 1); }
+  else if (Fm <= 4) { a = node_at(B, fi, fj, fk); b = node_at(B, fi - 1, fj, fk - 1); }
+  else { a = node_at(B, fi, fj, fk); b = node_at(B, fi - 1, fj - 1, fk); }
+  double m = 0.0;
+  for (int c = 0; c < 3; ++c) { t[c] = a[c] - b[c]; m += t[c] * t[c]; }
+  m = sqrt(m);
+  for (int c = 0; c < 3; ++c) t[c] /= m;
+}
+
+/* BC_Inlet_StagnCond + Stagn_Inflow, Lib_BC_Fluxes_Inflow.f90 (BC 401: stagnation T0, p0, flow
+ * direction); returns 1 when the flow leaves the domain (the caller falls back to symmetry) */
+static int bc_inlet401(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm, const mose_bc_entry* e) {
+  int modfm = Fm % 2, modfm2 = 1 - 2 * modfm;
+  int dir, fi, fj, fk;
+  face_index(Fm, Im, Jm, Km, &dir, &fi, &fj, &fk);
+  int64_t f = cidx(B, fi, fj, fk);
+  const double* Normal = &B->fn[dir][3 * f];
+  double Area = B->fa[dir][f];
+  double t[3];
+  face_tvec(B, Fm, fi, fj, fk, t);
+  const double* Pb = &B->P[NV * cidx(B, Im, Jm, Km)];
+  double BC_Sign = (double)modfm2;
+  double Un = Pb[1] * Normal[0] + Pb[2] * Normal[1] + Pb[3] * Normal[2];
+  if (BC_Sign * Un >= 0.0) return 1;
+  double Gamma = d->gam;
+  double Sound = f_ss(d, Pb[4], Pb[0]);
+  double Riem = Sound + BC_Sign * 0.5 * (Gamma - 1.0) * Un;
+  double alpha = e->alpha >= 0.5 * DBL_MAX ? atan(Normal[1] / (Normal[0] + 1e-20)) : e->alpha;
+  double beta = e->beta >= 0.5 * DBL_MAX ? atan(Normal[2] / (Normal[0] + 1e-20)) : e->beta;
+  double b[3] = {Normal[1] * t[2] - Normal[2] * t[1], Normal[2] * t[0] - Normal[0] * t[2],
+                 Normal[0] * t[1] - Normal[1] * t[0]};
+  double bm = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  for (int c = 0; c < 3; ++c) b[c] /= bm;
+  double XA = Normal[1] - Normal[0] * tan(alpha), XB = t[1] - t[0] * tan(alpha), XC = b[1] - b[0] * tan(alpha);
+  double XD = Normal[2] - Normal[0] * tan(beta), XE = t[2] - t[0] * tan(beta), XF = b[2] - b[0] * tan(beta);
+  double check = XC * XE - XF * XB;
+  double E1 = (XF * XA - XC * XD) / check, E2 = (XD * XB - XE * XA) / check;
+  double discr = -1.0, p0_try = e->p, YA = 0.0, YB = 0.0, Coef_D = 0.0, Sound0 = 0.0;
+  int iter_discr = 0;
+  while (discr < 0) {
+    iter_discr += 1;
+    Sound0 = sqrt(Gamma * d->Rg * e->T);
+    Coef_D = (1.0 + E1 * E1 + E2 * E2) * (Gamma - 1.0) / 2.0;
+    double Coef_C = pow(p0_try / Pb[4], (Gamma - 1.0) / Gamma);
+    Coef_C = Coef_C * ((Sound / Sound0) * (Sound / Sound0));
+    YA = ((Gamma - 1.0) / 2.0) * ((Gamma - 1.0) / 2.0) + Coef_C * Coef_D;
+    YB = BC_Sign * Riem * ((Gamma - 1.0) / 2.0);
+    double YC = Riem * Riem - Coef_C * Sound0 * Sound0;
+    discr = YB * YB - YA * YC;
+    if (discr < 0.0 && fabs(discr) < 1e-2) discr = 0.0;
+    if (discr < 0.0 && fabs(discr) > 1e-2) p0_try = e->rel_fac * p0_try + (1.0 - e->rel_fac) * Pb[4];
+    if (iter_discr > 1000) { discr = 0.0; break; }
+  }
+  double Un_Sign = copysign(1.0, Un);
+  double Un3 = (YB + Un_Sign * sqrt(discr)) / YA;
+  double Sound3 = sqrt(Sound0 * Sound0 - Coef_D * Un3 * Un3);
+  double p3 = p0_try * pow(Sound3 / Sound0, 2.0 * Gamma / (Gamma - 1.0));
+  double Face_rho = Gamma * p3 / (Sound3 * Sound3);
+  double u[3];
+  for (int c = 0; c < 3; ++c) u[c] = Un3 * Normal[c] + Un3 * E1 * t[c] + Un3 * E2 * b[c];
+  double Face_T = p3 / (d->Rg * Face_rho);
+  double Fmass = Face_rho * Un3 * Area;
+  double* R = &B->R[NV * cidx(B, Im, Jm, Km)];
+  double Flux[NV];
+  Flux[0] = Face_rho * Un3 * Area;
+  for (int c = 0; c < 3; ++c) Flux[1 + c] = Fmass * u[c] + p3 * Area * Normal[c];
+  Flux[4] = Fmass * (0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) + d->cp * Face_T);
+  for (int v = 0; v < NV; ++v) R[v] += modfm2 * Flux[v];
+  return 0;
+}
+
+/* BC_Outflow + Outflow, Lib_BC_Fluxes_Outflow.f90:12-130 (BC 406: back pressure); returns 1
+ * when the flow enters the domain (the caller falls back to symmetry) */
+static int bc_outflow406(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm, const mose_bc_entry* e) {
+  int modfm = Fm % 2, modfm2 = 1 - 2 * modfm;
+  int dir, fi, fj, fk;
+  face_index(Fm, Im, Jm, Km, &dir, &fi, &fj, &fk);
+  int64_t f = cidx(B, fi, fj, fk);
+  const double* Normal = &B->fn[dir][3 * f];
+  double Area = B->fa[dir][f];
+  const double* Pb = &B->P[NV * cidx(B, Im, Jm, Km)];
+  double BC_Sign = (double)modfm2;
+  double Un = Pb[1] * Normal[0] + Pb[2] * Normal[1] + Pb[3] * Normal[2];
+  if (BC_Sign * Un < 0.0) return 1;
+  double Gamma = d->gam, rho = Pb[0];
+  double Sound = f_ss(d, Pb[4], rho);
+  double Riem = Sound + BC_Sign * 0.5 * (Gamma - 1.0) * Un;
+  double VelT[3];
+  for (int c = 0; c < 3; ++c) VelT[c] = Pb[1 + c] - Un * Normal[c];
+  double Mach_N = fabs(Un / Sound);
+  double Face[NV], Face_Un, Face_rho;
+  double p_exit = e->p;
+  if (Mach_N < 1.0 && p_exit > 0.0) {
+    double delta = 0.5 * (Gamma - 1.0);
+    double den = 1.0 / pow(1 + delta, Gamma / (Gamma - 1.0));
+    double pstar = Pb[4] * den * pow(1 + delta * Mach_N * Mach_N, Gamma / (Gamma - 1.0));
+    if (pstar >= p_exit) p_exit = pstar;
+    double Isentropic = (Gamma - 1.0) / (2.0 * Gamma);
+    double Face_Sound = Sound * pow(p_exit / Pb[4], Isentropic);
+    Face_Un = (Riem - Face_Sound) / (BC_Sign * 0.5 * (Gamma - 1.0));
+    Face[4] = p_exit;
+    Face_rho = Gamma * Face[4] / (Face_Sound * Face_Sound);
+    Face[0] = Pb[0] / rho * Face_rho;
+    for (int c = 0; c < 3; ++c) Face[1 + c] = VelT[c] + Face_Un * Normal[c];
+  } else {
+    for (int v = 0; v < NV; ++v) Face[v] = Pb[v];
+    Face_Un = Un;
+    Face_rho = rho;
+  }
+  double Face_T = Face[4] / (d->Rg * Face_rho);
+  double mdot = Face_rho * Area * Face_Un;
+  double* R = &B->R[NV * cidx(B, Im, Jm, Km)];
+  double Flux[NV];
+  Flux[0] = Face[0] * Area * Face_Un;
+  for (int c = 0; c < 3; ++c) Flux[1 + c] = Face[1 + c] * mdot + Face[4] * Area * Normal[c];
+  Flux[4] = mdot * (0.5 * (Face[1] * Face[1] + Face[2] * Face[2] + Face[3] * Face[3]) + d->cp * Face_T);
+  for (int v = 0; v < NV; ++v) R[v] += modfm2 * Flux[v];
+  return 0;
+}
+
 static int bc_inflow405(ob_domain* d, OBlk* B, int Im, int Jm, int Km, int Fm, const mose_bc_entry* e) {
   int modfm = Fm % 2, modfm1 = 1 - modfm, modfm2 = 1 - 2 * modfm;
   const int* g = guide[Fm - 1];
@@ -966,6 +1089,14 @@ static int bc_fluxes(ob_domain* d) {
       case 405:
         if (B->visc) return set_err(d, MOSE_E_BC, e->b, e->i, e->j, e->k);
         if (bc_inflow405(d, B, e->i, e->j, e->k, e->f, e)) bc_symmetry(d, B, e->i, e->j, e->k, e->f);
+        break;
+      case 401:
+        if (B->visc) return set_err(d, MOSE_E_BC, e->b, e->i, e->j, e->k);
+        if (bc_inlet401(d, B, e->i, e->j, e->k, e->f, e)) bc_symmetry(d, B, e->i, e->j, e->k, e->f);
+        break;
+      case 406:
+        if (B->visc) return set_err(d, MOSE_E_BC, e->b, e->i, e->j, e->k);
+        if (bc_outflow406(d, B, e->i, e->j, e->k, e->f, e)) bc_symmetry(d, B, e->i, e->j, e->k, e->f);
         break;
       case 0: break;
       default: return set_err(d, MOSE_E_BC, e->b, e->i, e->j, e->k);
