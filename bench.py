@@ -170,6 +170,40 @@ def taylor_green_batch(steps, warmup, n=192, nbox=8):
     return out
 
 
+# The other scheme rows on the same 256^3 vortex, FP64: (label, limiter, Riemann solver,
+# MUSCL-SD, irs-beta).  IRS runs the residual-only stage kernel, the scaling, per direction the
+# residual ghosts and one fused two-sweep Jacobi pass, and the update kernel.
+SCHEME_LEGS = [("vanleer+HLLE", "vanleer", "HLLE", False, 0.0),
+               ("vanleer+SLAU2", "vanleer", "SLAU2", False, 0.0),
+               ("mc+SLAU", "mc", "SLAU", False, 0.0),
+               ("vanleer+HLLE++", "vanleer", "HLLE++", False, 0.0),
+               ("vanleer+HLLC+", "vanleer", "HLLC+", False, 0.0),
+               ("minmod+HLLE MUSCL-SD", "minmod", "HLLE", True, 0.0),
+               ("vanleer+HLLC IRS 0.5", "vanleer", "HLLC", False, 0.5)]
+
+
+def scheme_legs(n, steps, warmup, ref_ms_per_step):
+    from mose_amd import abi, cases
+    out = {}
+    for label, lim, rs, sd, irs in SCHEME_LEGS:
+        case = cases.vortex(n=n, riemann=rs, limiter=lim, precision=abi.PREC_FP64)
+        c = case.cfg
+        case.cfg = abi.make_cfg(c.gamma, c.Rgas, c.cfl, lim, rs, c.n_rk, bool(c.time_accurate),
+                                precision=abi.PREC_FP64, muscl_sd=sd, irs_beta=irs)
+        g, secs, t_sim, ms = run_gpu(case, steps, warmup, label)
+        leg = {"value": n ** 3 * steps / secs, "steps": steps, "ms_per_step": 1e3 * secs / steps,
+               "step_time_vs_vanleer_hllc": 1e3 * secs / steps / ref_ms_per_step,
+               "tiling": g.tiling, "l2_rho_vs_exact": l2_vs_exact(g, case, t_sim)}
+        if irs == 0.0:   # the stage kernel this scheme steps with
+            leg["stage_kernel_ms"] = ms[2]
+            leg["stage_frac"] = STAGE_BYTES_PER_CELL * n ** 3 / (ms[2] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        if sd:
+            leg["ghost_and_detector_ms"] = ms[1]
+        out[label] = leg
+        g.close()
+    return out
+
+
 def cpu_baseline(n=96, steps=20):
     """The C oracle (no fast-math; OpenMP over lines of cells like the reference's
     collapse(2) loops, OMP_NUM_THREADS threads) on the same workload at n^3."""
@@ -204,6 +238,8 @@ def main():
     ap.add_argument("--tgv-steps", type=int, default=20)
     ap.add_argument("--no-ausmp", action="store_true")
     ap.add_argument("--ausmp-steps", type=int, default=50)
+    ap.add_argument("--no-schemes", action="store_true")
+    ap.add_argument("--scheme-steps", type=int, default=30)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; sharding blocks over several GPUs is not "
@@ -279,6 +315,8 @@ def main():
             "stage_frac": STAGE_BYTES_PER_CELL * ncell / (msa[2] * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "l2_rho_vs_exact": l2_vs_exact(ga, ca, ta)}
         ga.close()
+    if not args.no_schemes:
+        result["schemes"] = scheme_legs(n, args.scheme_steps, min(args.warmup, 2), result["ms_per_step"])
     if adj_vortex is not None:
         result["adjoint"] = {"vortex": adj_vortex, "bump_512x256": bump_adjoint(),
                              "note": "device ms of one mose_residual_vjp / one mose_residual, FP64; "

@@ -81,9 +81,13 @@ typedef struct {
  * type 101/201 use (bs,is,js,ks,fs,d11,d12,d21,d22); type 405 uses (mach,T,p,alpha,beta,rel_fac)
  * with alpha/beta in radians, or an angle >= DBL_MAX/2 for bc.txt's 'normal' token (the
  * reference's huge() sentinel, parse_dir_tok IO_BC.f90:443-456): the angle then follows the
- * boundary face normal (Compute_Inflow_Direction, Lib_BC_Fluxes_Inflow.f90:479-496).  All
- * records of one block face share one type, and for 405 one set of values.
- * Supported types: 0, 101, 201, 300, 400, 405. */
+ * boundary face normal (Compute_Inflow_Direction, Lib_BC_Fluxes_Inflow.f90:479-496).
+ * Type 401 (subsonic inlet, BC_Inlet_StagnCond, Lib_BC_Fluxes_Inflow.f90:19-75) uses T = T0,
+ * p = p0, alpha, beta (angles as for 405) and rel_fac (the relaxation of p0 towards the cell
+ * pressure); type 406 (outlet, BC_Outflow, Lib_BC_Fluxes_Outflow.f90:12-59) uses p = the back
+ * pressure (0: extrapolation).  All records of one block face share one type, and for
+ * 405/401/406 one set of values.  401/406 are refused on viscous blocks (MOSE_E_BC).
+ * Supported types: 0, 101, 201, 300, 400, 401, 405, 406. */
 typedef struct {
   int32_t b, i, j, k, f, type;
   int32_t bs, is, js, ks, fs;
@@ -149,7 +153,8 @@ int  mose_residual(mose_ctx* ctx, int32_t b, double* R);
  * entries 0), R being the residual of the current state including its ghost fill.
  * Limitations: ghost cells filled from other ghost cells (extrapolation across a block
  * thinner than three cells) are treated as constants; the schemes that use the shock
- * detector (MOSE_RS_HLLEPP, MOSE_RS_HLLCP, muscl_sd) have no adjoint (MOSE_E_ARG). */
+ * detector (MOSE_RS_HLLEPP, MOSE_RS_HLLCP, muscl_sd) and the 401/406 boundary fluxes have no
+ * adjoint (MOSE_E_ARG). */
 int  mose_put_residual_bar(mose_ctx* ctx, int32_t b, const double* Rbar);
 int  mose_residual_vjp(mose_ctx* ctx);
 int  mose_get_state_bar(mose_ctx* ctx, int32_t b, double* Pbar);

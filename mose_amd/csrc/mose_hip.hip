@@ -125,10 +125,10 @@ __global__ void k_dtreset(unsigned long long* dtmin) {
 // turns it into R* = -R dt / vol, each direction d < ndir gets the one-layer residual ghosts
 // (k_irs_ghost: the donor cell for 101 connections, the boundary cell for every other type)
 // and two Jacobi sweeps R*_i <- (R_i + beta (R*_{i-1} + R*_{i+1})) / (1 + 2 beta) with the ghost
-// values held, and k_irs_update takes the RK stage with the smoothed R*.
+// values held (k_irs_jacobi: both sweeps in one pass from one buffer into the other), and
+// k_irs_update takes the RK stage with the smoothed R*.
 struct IrsArgs {
   double* R;
-  const double* src;
   double* dst;
   const Tile* tiles;
   const BlockDev* blocks;
@@ -172,10 +172,14 @@ __global__ void __launch_bounds__(256) k_irs_ghost(const int64_t* __restrict__ p
   }
 }
 
+// Both Jacobi sweeps of direction d for every interior cell: the first sweep's values at the
+// two neighbours along d are recomputed from R (a neighbour that is a ghost keeps its R), so
+// one pass reads R (the residual ghosts filled) and writes the second sweep into dst.
 __global__ void __launch_bounds__(256) k_irs_jacobi(IrsArgs a) {
   const int tx = threadIdx.x % a.tx, ty = threadIdx.x / a.tx;
   const int64_t N = a.npool;
   const double den = 1.0 + 2.0 * a.beta;
+  const double* __restrict__ R = a.R;
   for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
     const Tile T = a.tiles[t];
     const BlockDev B = a.blocks[T.b];
@@ -183,11 +187,18 @@ __global__ void __launch_bounds__(256) k_irs_jacobi(IrsArgs a) {
     if (i > B.ni || j > B.nj) continue;
     const int64_t st = a.d == 0 ? 1 : (a.d == 1 ? (int64_t)B.sy : B.sz);
     const int64_t cbase = B.off + (int64_t)(i + 1) + (int64_t)B.sy * (j + 1);
+    const int n = a.d == 0 ? B.ni : (a.d == 1 ? B.nj : B.nk);
     for (int k = T.k0; k < T.k1; ++k) {
       const int64_t c = cbase + B.sz * (int64_t)(k + 1);
+      const int q = a.d == 0 ? i : (a.d == 1 ? j : k);     // 1-based index along d
 #pragma unroll
-      for (int v = 0; v < 5; ++v)
-        a.dst[v * N + c] = (a.R[v * N + c] + a.beta * (a.src[v * N + c - st] + a.src[v * N + c + st])) / den;
+      for (int v = 0; v < 5; ++v) {
+        const double* r = R + v * N;
+        const double r0 = r[c], rm = r[c - st], rp = r[c + st];
+        const double sm = q > 1 ? (rm + a.beta * (r[c - 2 * st] + r0)) / den : rm;
+        const double sp = q < n ? (rp + a.beta * (r0 + r[c + 2 * st])) / den : rp;
+        a.dst[v * N + c] = (r0 + a.beta * (sm + sp)) / den;
+      }
     }
   }
 }
@@ -526,8 +537,7 @@ struct mose_ctx {
   int64_t* d_garc_cell = nullptr;             // archived ghost cells (previous-fill values)
   double* d_garc = nullptr;
   int64_t narc = 0;
-  double* d_rs1 = nullptr;                    // IRS iterates (pool-sized), when irs_beta > 0
-  double* d_rs2 = nullptr;
+  double* d_rs1 = nullptr;                    // IRS second buffer (pool-sized), when irs_beta > 0
   int64_t* d_irsg = nullptr;                  // IRS residual ghosts: (ghost, source) per bc record
   int nirsg = 0;
   BcFace* d_bcfaces = nullptr;
@@ -692,22 +702,20 @@ static void enqueue_irs(mose_ctx* c, const StageArgs& a, bool last) {
   const dim3 grid(ia.ntiles), block(c->tx * c->ty);
   if (c->lay.curvi) hipLaunchKernelGGL(k_irs_scale<1>, grid, block, 0, c->stream, ia);
   else hipLaunchKernelGGL(k_irs_scale<0>, grid, block, 0, c->stream, ia);
-  const int64_t n5 = 5 * c->npool;
+  double* cur = c->Rbuf;     // the directions alternate between Rbuf and d_rs1
+  double* nxt = c->d_rs1;
   for (int d = 0; d < c->ndir; ++d) {
     if (c->nirsg > 0)
       hipLaunchKernelGGL(k_irs_ghost, dim3(launch_blocks(c->nirsg)), dim3(256), 0, c->stream, c->d_irsg, c->nirsg,
-                         c->Rbuf, c->npool);
-    hipLaunchKernelGGL(k_copy, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->Rbuf, c->d_rs1, n5);
-    hipLaunchKernelGGL(k_copy, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->Rbuf, c->d_rs2, n5);
+                         cur, c->npool);
     ia.d = d;
-    ia.src = c->d_rs1; ia.dst = c->d_rs2;
+    ia.R = cur;
+    ia.dst = nxt;
     hipLaunchKernelGGL(k_irs_jacobi, grid, block, 0, c->stream, ia);
-    ia.src = c->d_rs2; ia.dst = c->d_rs1;
-    hipLaunchKernelGGL(k_irs_jacobi, grid, block, 0, c->stream, ia);
-    hipLaunchKernelGGL(k_copy, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->d_rs1, c->Rbuf, n5);
+    std::swap(cur, nxt);
   }
   StageArgs u = a;
-  u.Rout = c->Rbuf;
+  u.Rout = cur;
   const int tx = c->tx;
   const int geo = c->lay.curvi ? 1 : (c->lay.visc ? 2 : 0);
   if (last) {
@@ -1164,9 +1172,7 @@ int mose_finalize(mose_ctx* c) {
   }
   if (c->cfg.irs_beta > 0.0) {
     HIPCHK(c, hipMalloc((void**)&c->d_rs1, 5 * c->npool * sizeof(double)));
-    HIPCHK(c, hipMalloc((void**)&c->d_rs2, 5 * c->npool * sizeof(double)));
     HIPCHK(c, hipMemset(c->d_rs1, 0, 5 * c->npool * sizeof(double)));
-    HIPCHK(c, hipMemset(c->d_rs2, 0, 5 * c->npool * sizeof(double)));
     // residual ghosts (Ghost_Residual_Extrapolation, Lib_IRS.f90:154-189): the first ghost
     // layer of every bc record from the donor cell (101) or the boundary cell (all others)
     std::vector<int64_t> pairs;
@@ -1601,7 +1607,7 @@ void mose_destroy(mose_ctx* c) {
                   c->d_ghost, c->d_bcfaces, c->d_vfaces, c->d_vbuf, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
                   c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
                   c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_ghostp,
-                  c->d_garc_cell, c->d_garc, c->d_rs1, c->d_rs2, c->d_irsg};
+                  c->d_garc_cell, c->d_garc, c->d_rs1, c->d_irsg};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
