@@ -125,8 +125,9 @@ __global__ void k_dtreset(unsigned long long* dtmin) {
 // turns it into R* = -R dt / vol, each direction d < ndir gets the one-layer residual ghosts
 // (k_irs_ghost: the donor cell for 101 connections, the boundary cell for every other type)
 // and two Jacobi sweeps R*_i <- (R_i + beta (R*_{i-1} + R*_{i+1})) / (1 + 2 beta) with the ghost
-// values held (k_irs_jacobi: both sweeps in one pass from one buffer into the other), and
-// k_irs_update takes the RK stage with the smoothed R*.
+// values held (k_irs_jacobi: both sweeps in one pass from one buffer into the other; those of
+// the last direction are formed inside k_irs_update, which takes the RK stage with the
+// smoothed R*).
 struct IrsArgs {
   double* R;
   double* dst;
@@ -203,11 +204,13 @@ __global__ void __launch_bounds__(256) k_irs_jacobi(IrsArgs a) {
   }
 }
 
-// RK stage of every interior cell with the smoothed R* in a.Rout (MODE 0 / 1 of k_stage
-// without the fluxes): U = Un + rkc (Uin - Un + R*), the state check and density floor, and in
-// the last stage the CFL time step of the new state and the residual-norm partials per tile.
+// RK stage of every interior cell with the smoothed R* (MODE 0 / 1 of k_stage without the
+// fluxes): U = Un + rkc (Uin - Un + R*), the state check and density floor, and in the last
+// stage the CFL time step of the new state and the residual-norm partials per tile.  R* is
+// a.Rout after the smoothing of the directions before d; the two Jacobi sweeps of direction d
+// are formed here, as k_irs_jacobi forms them (residual ghosts of a.Rout filled).
 template <int GEO, bool LAST>
-__global__ void __launch_bounds__(256) k_irs_update(StageArgs a, int tx_) {
+__global__ void __launch_bounds__(256) k_irs_update(StageArgs a, int tx_, int d, double beta) {
   const int tid = threadIdx.x, tx = tid % tx_, ty = tid / tx_;
   const double gm1 = a.gam - 1.0;
   const int64_t N = a.npool;
@@ -224,11 +227,25 @@ __global__ void __launch_bounds__(256) k_irs_update(StageArgs a, int tx_) {
       const int64_t cbase = Bp->off + (int64_t)(i + 1) + (int64_t)Bp->sy * (j + 1);
       for (int k = T.k0; k < T.k1; ++k) {
         const int64_t c = cbase + Bp->sz * (int64_t)(k + 1);
-        double un[5], ui[5], unew[5], P[5];
+        double un[5], ui[5], unew[5], P[5], rs[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) { un[v] = a.Un[v * N + c]; ui[v] = a.Uin[v * N + c]; }
+        {
+          const int64_t st = d == 0 ? 1 : (d == 1 ? (int64_t)Bp->sy : Bp->sz);
+          const int n = d == 0 ? Bp->ni : (d == 1 ? Bp->nj : Bp->nk);
+          const int q = d == 0 ? i : (d == 1 ? j : k);
+          const double den = 1.0 + 2.0 * beta;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) unew[v] = un[v] + a.rkc * (ui[v] - un[v] + a.Rout[v * N + c]);
+          for (int v = 0; v < 5; ++v) {
+            const double* r = a.Rout + v * N;
+            const double r0 = r[c], rm = r[c - st], rp = r[c + st];
+            const double sm = q > 1 ? (rm + beta * (r[c - 2 * st] + r0)) / den : rm;
+            const double sp = q < n ? (rp + beta * (r0 + r[c + 2 * st])) / den : rp;
+            rs[v] = (r0 + beta * (sm + sp)) / den;
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; ++v) unew[v] = un[v] + a.rkc * (ui[v] - un[v] + rs[v]);
         cons2prim(unew, gm1, P);
         bool bad = !(P[4] >= 0.0);
 #pragma unroll
@@ -297,7 +314,7 @@ __global__ void __launch_bounds__(256) k_irs_update(StageArgs a, int tx_) {
 // the pressures of the cell and its six face neighbours, ghost layers filled.  Ghost entries
 // keep the 1 they were initialised with.
 struct SdArgs {
-  const double* U;
+  const double* pres;
   double* beta;
   const Tile* tiles;
   const BlockDev* blocks;
@@ -306,17 +323,23 @@ struct SdArgs {
   double gam;
 };
 
-__global__ void __launch_bounds__(256) k_sd(SdArgs a) {
-  const int tx = threadIdx.x % a.tx, ty = threadIdx.x / a.tx;
-  const double gm1 = a.gam - 1.0;
-  const int64_t N = a.npool;
-  auto pres = [&](int64_t c) {
+// Pressure of every pool cell (the detector's input, one cons2prim per cell); entries of
+// pool cells no fill writes (ghost edges and corners) are computed but never read.
+__global__ void __launch_bounds__(256) k_pres(const double* __restrict__ U, int64_t N, double gam,
+                                              double* __restrict__ pres) {
+  const double gm1 = gam - 1.0;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
     double u[5], P[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) u[v] = a.U[v * N + c];
+    for (int v = 0; v < 5; ++v) u[v] = U[v * N + c];
     cons2prim(u, gm1, P);
-    return P[4];
-  };
+    pres[c] = P[4];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sd(SdArgs a) {
+  const int tx = threadIdx.x % a.tx, ty = threadIdx.x / a.tx;
+  const double* __restrict__ p = a.pres;
   for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
     const Tile T = a.tiles[t];
     const BlockDev B = a.blocks[T.b];
@@ -326,7 +349,7 @@ __global__ void __launch_bounds__(256) k_sd(SdArgs a) {
     const int64_t cbase = B.off + (int64_t)(i + 1) + sy * (j + 1);
     for (int k = T.k0; k < T.k1; ++k) {
       const int64_t c = cbase + sz * (int64_t)(k + 1);
-      a.beta[c] = sd_tramel(pres(c), pres(c - 1), pres(c + 1), pres(c - sy), pres(c + sy), pres(c - sz), pres(c + sz));
+      a.beta[c] = sd_tramel(p[c], p[c - 1], p[c + 1], p[c - sy], p[c + sy], p[c - sz], p[c + sz]);
     }
   }
 }
@@ -549,6 +572,7 @@ struct mose_ctx {
   double* d_partials = nullptr;
   double* d_dtloc = nullptr;
   double* d_beta = nullptr;                  // shock-detector weights (pool-sized), when used
+  double* d_pres = nullptr;                  // the detector's pressures (pool-sized)
   unsigned long long* d_dtmin = nullptr;
   int* d_err = nullptr;
   int* d_iter = nullptr;
@@ -608,7 +632,9 @@ static int enqueue_ghost(mose_ctx* c, double* U) {
   }
   enqueue_garc_save(c, U);
   if (c->d_beta) {
-    SdArgs sa{U, c->d_beta, c->d_tiles, c->d_blocks, c->npool, (int)c->lay.tiles.size(), c->tx, c->cfg.gamma};
+    hipLaunchKernelGGL(k_pres, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, U, c->npool, c->cfg.gamma,
+                       c->d_pres);
+    SdArgs sa{c->d_pres, c->d_beta, c->d_tiles, c->d_blocks, c->npool, (int)c->lay.tiles.size(), c->tx, c->cfg.gamma};
     hipLaunchKernelGGL(k_sd, dim3(sa.ntiles), dim3(c->tx * c->ty), 0, c->stream, sa);
   }
   return 0;
@@ -708,6 +734,7 @@ static void enqueue_irs(mose_ctx* c, const StageArgs& a, bool last) {
     if (c->nirsg > 0)
       hipLaunchKernelGGL(k_irs_ghost, dim3(launch_blocks(c->nirsg)), dim3(256), 0, c->stream, c->d_irsg, c->nirsg,
                          cur, c->npool);
+    if (d + 1 == c->ndir) break;          // the last direction is smoothed inside the update
     ia.d = d;
     ia.R = cur;
     ia.dst = nxt;
@@ -716,15 +743,16 @@ static void enqueue_irs(mose_ctx* c, const StageArgs& a, bool last) {
   }
   StageArgs u = a;
   u.Rout = cur;
-  const int tx = c->tx;
+  const int tx = c->tx, dl = c->ndir - 1;
+  const double b = c->cfg.irs_beta;
   const int geo = c->lay.curvi ? 1 : (c->lay.visc ? 2 : 0);
   if (last) {
-    if (geo == 1) hipLaunchKernelGGL((k_irs_update<1, true>), grid, block, 0, c->stream, u, tx);
-    else if (geo == 2) hipLaunchKernelGGL((k_irs_update<2, true>), grid, block, 0, c->stream, u, tx);
-    else hipLaunchKernelGGL((k_irs_update<0, true>), grid, block, 0, c->stream, u, tx);
+    if (geo == 1) hipLaunchKernelGGL((k_irs_update<1, true>), grid, block, 0, c->stream, u, tx, dl, b);
+    else if (geo == 2) hipLaunchKernelGGL((k_irs_update<2, true>), grid, block, 0, c->stream, u, tx, dl, b);
+    else hipLaunchKernelGGL((k_irs_update<0, true>), grid, block, 0, c->stream, u, tx, dl, b);
   } else {
-    if (geo == 1) hipLaunchKernelGGL((k_irs_update<1, false>), grid, block, 0, c->stream, u, tx);
-    else hipLaunchKernelGGL((k_irs_update<0, false>), grid, block, 0, c->stream, u, tx);
+    if (geo == 1) hipLaunchKernelGGL((k_irs_update<1, false>), grid, block, 0, c->stream, u, tx, dl, b);
+    else hipLaunchKernelGGL((k_irs_update<0, false>), grid, block, 0, c->stream, u, tx, dl, b);
   }
 }
 
@@ -1187,6 +1215,7 @@ int mose_finalize(mose_ctx* c) {
   if (c->cfg.muscl_sd || rs_takes_beta(c->cfg.riemann)) {
     if (c->cfg.muscl_sd && c->lay.visc) return fail(c, MOSE_E_ARG);   // no viscous MUSCL-SD kernels
     HIPCHK(c, hipMalloc((void**)&c->d_beta, c->npool * sizeof(double)));
+    HIPCHK(c, hipMalloc((void**)&c->d_pres, c->npool * sizeof(double)));
     hipLaunchKernelGGL(k_fill, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, c->d_beta, c->npool, 1.0);
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
@@ -1606,7 +1635,7 @@ void mose_destroy(mose_ctx* c) {
   void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_geo, c->d_tiles,
                   c->d_ghost, c->d_bcfaces, c->d_vfaces, c->d_vbuf, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
                   c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
-                  c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_ghostp,
+                  c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_pres, c->d_ghostp,
                   c->d_garc_cell, c->d_garc, c->d_rs1, c->d_irsg};
   for (void* p : bufs)
     if (p) (void)hipFree(p);

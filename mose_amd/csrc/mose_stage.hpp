@@ -300,9 +300,12 @@ __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const 
 // and the k face above it; the TX + TY faces on the high edges of the patch are solved by
 // TX + TY threads whose choice rotates with the plane so no wave always carries them.  The
 // k faces need only the thread's own registers.
+// Occupancy floor: FP32 flux 3 waves per SIMD; FP64 2 (256 registers: without the floor some
+// variants take a few AGPRs beyond 256 and fall to one wave per SIMD); the viscous kernels,
+// whose LDS allows one workgroup per CU, none.
 template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc, int GEO, bool SDR>
 __global__ void __launch_bounds__(TXc* TYc)
-    __attribute__((amdgpu_waves_per_eu(sizeof(FT) == 4 && TXc * TYc == 256 && GEO != 2 ? 3 : 1)))
+    __attribute__((amdgpu_waves_per_eu(GEO == 2 ? 1 : (sizeof(FT) == 4 ? (TXc * TYc == 256 ? 3 : 1) : 2))))
     k_stage(StageArgs a) {
   constexpr int NTc = TXc * TYc;
   constexpr int NW = NTc / 64;                   // waves per workgroup
