@@ -121,8 +121,8 @@ __global__ void k_dtreset(unsigned long long* dtmin) {
 
 // ---------------------------------------------------------------------------- IRS
 // Implicit residual smoothing (Newstate_Conservative with IRS, Lib_Newstate.f90:41-75, and
-// Residual_Smoothing, Lib_IRS.f90:14-191): the stage kernel writes the residual R, k_irs_scale
-// turns it into R* = -R dt / vol, each direction d < ndir gets the one-layer residual ghosts
+// Residual_Smoothing, Lib_IRS.f90:14-191): the residual-only stage kernel writes the scaled
+// residual R* = -R dt / vol, each direction d < ndir gets the one-layer residual ghosts
 // (k_irs_ghost: the donor cell for 101 connections, the boundary cell for every other type)
 // and two Jacobi sweeps R*_i <- (R_i + beta (R*_{i-1} + R*_{i+1})) / (1 + 2 beta) with the ghost
 // values held (k_irs_jacobi: both sweeps in one pass from one buffer into the other; those of
@@ -141,29 +141,6 @@ struct IrsArgs {
   int ntiles, tx, local_dt, d;
   double beta;
 };
-
-template <int GEO>
-__global__ void __launch_bounds__(256) k_irs_scale(IrsArgs a) {
-  const int tx = threadIdx.x % a.tx, ty = threadIdx.x / a.tx;
-  const int64_t N = a.npool;
-  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
-    const Tile T = a.tiles[t];
-    const BlockDev B = a.blocks[T.b];
-    const int i = T.i0 + tx, j = T.j0 + ty;
-    if (i > B.ni || j > B.nj) continue;
-    const LineEnt* Lx = a.lines + B.line[0];
-    const LineEnt* Ly = a.lines + B.line[1];
-    const LineEnt* Lz = a.lines + B.line[2];
-    const int64_t cbase = B.off + (int64_t)(i + 1) + (int64_t)B.sy * (j + 1);
-    for (int k = T.k0; k < T.k1; ++k) {
-      const int64_t c = cbase + B.sz * (int64_t)(k + 1);
-      const double vol = GEO != 1 ? Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta : a.geo[(size_t)GVOL * N + c];
-      const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
-#pragma unroll
-      for (int v = 0; v < 5; ++v) a.R[v * N + c] = -a.R[v * N + c] / vol * dt;
-    }
-  }
-}
 
 __global__ void __launch_bounds__(256) k_irs_ghost(const int64_t* __restrict__ pairs, int n, double* R, int64_t N) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
@@ -722,12 +699,10 @@ static IrsArgs irs_args(mose_ctx* c) {
   return ia;
 }
 
-// The smoothing and the RK update of one stage whose residual is in Rbuf.
+// The smoothing and the RK update of one stage whose scaled residual R* is in Rbuf.
 static void enqueue_irs(mose_ctx* c, const StageArgs& a, bool last) {
   IrsArgs ia = irs_args(c);
   const dim3 grid(ia.ntiles), block(c->tx * c->ty);
-  if (c->lay.curvi) hipLaunchKernelGGL(k_irs_scale<1>, grid, block, 0, c->stream, ia);
-  else hipLaunchKernelGGL(k_irs_scale<0>, grid, block, 0, c->stream, ia);
   double* cur = c->Rbuf;     // the directions alternate between Rbuf and d_rs1
   double* nxt = c->d_rs1;
   for (int d = 0; d < c->ndir; ++d) {
@@ -770,7 +745,9 @@ static int enqueue_step(mose_ctx* c) {
     StageArgs a = stage_args(c, in, c->U[0], out, kRK[nrk - 1][s - 1]);
     enqueue_bcfaces(c, a);
     if (c->cfg.irs_beta > 0.0) {
-      launch_stage(c, stage_fn(c, 2), a);        // residual into Rbuf
+      StageArgs r = a;
+      r.irs_scale = 1;
+      launch_stage(c, stage_fn(c, 2), r);        // R* = -R dt / vol into Rbuf
       enqueue_irs(c, a, s == nrk);
     } else {
       launch_stage(c, stage_fn(c, s == nrk ? 1 : 0), a);

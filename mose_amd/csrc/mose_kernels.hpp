@@ -116,6 +116,7 @@ struct StageArgs {
   int ntiles;
   int local_dt;
   int ndir, rampa;
+  int irs_scale;               // MODE 2: write R* = -R dt / vol (residual smoothing) instead of R
   double rkc, gam, R, cfl;
 };
 

@@ -655,8 +655,15 @@ __global__ void __launch_bounds__(TXc* TYc)
                       Ay * ((double)sFj[v][ty + 1][tx] - (double)sFj[v][ty][tx]);
           const unsigned c = cbase + sz * (unsigned)(k + 1);
           if constexpr (MODE == 2) {
+            if (a.irs_scale) {
+              const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
+              const double vol = GEO != 1 ? Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta : GA(GVOL, c);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) a.Rout[v * N + c] = Racc[v];
+              for (int v = 0; v < 5; ++v) a.Rout[v * N + c] = -Racc[v] / vol * dt;
+            } else {
+#pragma unroll
+              for (int v = 0; v < 5; ++v) a.Rout[v * N + c] = Racc[v];
+            }
           } else {
             double un[5], ui[5], unew[5], P[5];
 #pragma unroll
