@@ -72,7 +72,9 @@ typedef struct {
   int32_t precision;        /* MOSE_PREC_*                                                     */
   int32_t muscl_sd;         /* [MOSE-Numerics] space-reconstruction = MUSCL-SD: limited slopes  */
                             /* scaled by the shock-detector weight (Lib_Reconstruction.f90:26-27) */
-  int32_t reserved[3];
+  int32_t prim_vars;        /* [MOSE-Numerics] integration-variables = prim: the RK stage on the   */
+                            /* primitive variables (Newstate_Primitive, Lib_Newstate.f90:168-317)  */
+  int32_t reserved[2];
   double  irs_beta;         /* [MOSE-Numerics] irs-beta: implicit residual smoothing of every   */
                             /* stage's residual when > 0 (Lib_Newstate.f90:41-75, Lib_IRS.f90)    */
 } mose_cfg;

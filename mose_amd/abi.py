@@ -28,7 +28,8 @@ class MoseCfg(C.Structure):
     _fields_ = [("gamma", C.c_double), ("Rgas", C.c_double), ("cfl", C.c_double),
                 ("limiter", C.c_int32), ("riemann", C.c_int32), ("n_rk", C.c_int32),
                 ("time_accurate", C.c_int32), ("rampa_cfl_iter", C.c_int32),
-                ("precision", C.c_int32), ("muscl_sd", C.c_int32), ("reserved", C.c_int32 * 3),
+                ("precision", C.c_int32), ("muscl_sd", C.c_int32), ("prim_vars", C.c_int32),
+                ("reserved", C.c_int32 * 2),
                 ("irs_beta", C.c_double)]
 
 
@@ -57,14 +58,17 @@ assert BC_DTYPE.itemsize == C.sizeof(MoseBcEntry) == 112
 
 
 def make_cfg(gamma=1.4, Rgas=1.0 / 1.4, cfl=0.5, limiter="minmod", riemann="HLLC", n_rk=3,
-             time_accurate=True, rampa_cfl_iter=0, precision=PREC_FP64, muscl_sd=False, irs_beta=0.0):
+             time_accurate=True, rampa_cfl_iter=0, precision=PREC_FP64, muscl_sd=False, irs_beta=0.0,
+             prim_vars=False):
     """Build a MoseCfg from the reference's key values (flux-limiter, riemann-solver, ...);
-    muscl_sd: space-reconstruction = MUSCL-SD; irs_beta: implicit residual smoothing."""
+    muscl_sd: space-reconstruction = MUSCL-SD; irs_beta: implicit residual smoothing;
+    prim_vars: integration-variables = prim."""
     lim = LIMITERS[limiter] if isinstance(limiter, str) else int(limiter)
     rs = RIEMANN[riemann] if isinstance(riemann, str) else int(riemann)
     cfg = MoseCfg(gamma, Rgas, cfl, lim, rs, n_rk, int(bool(time_accurate)), rampa_cfl_iter,
                   precision, int(bool(muscl_sd)))
     cfg.irs_beta = float(irs_beta)
+    cfg.prim_vars = int(bool(prim_vars))
     return cfg
 
 

@@ -186,7 +186,64 @@ __global__ void __launch_bounds__(256) k_irs_jacobi(IrsArgs a) {
 // stage the CFL time step of the new state and the residual-norm partials per tile.  R* is
 // a.Rout after the smoothing of the directions before d; the two Jacobi sweeps of direction d
 // are formed here, as k_irs_jacobi forms them (residual ghosts of a.Rout filled).
-template <int GEO, bool LAST>
+//
+// PRIM (integration-variables = prim, Newstate_Primitive, Lib_Newstate.f90:168-247): the stage
+// is taken on the primitive variables, P = Pn + rkc (Pin - Pn + R*), then stored as conserved;
+// R* is the primitive-variable residual (prim_residual), formed here from the flux residual in
+// a.Rout when d < 0 (no smoothing), else converted before the smoothing (k_prim_resid).
+//
+// Compute_Prim_Residual (Lib_Newstate.f90:252-299) for one species of a calorically perfect gas
+// (h(T) - R T = cv T, cp - R = cv): flux residual r of a cell with primitive state P -> the
+// primitive-variable residual times dt.
+__device__ inline void prim_residual(const double* P, double* r, double dt, double vol, double Rg, double cp) {
+  const double rho = P[0], T = P[4] / (rho * Rg);
+  const double inv_volume = 1.0 / vol, inv_rhovolume = inv_volume / rho;
+  r[0] = -r[0] * inv_volume;
+  const double dwrotot = r[0];
+  for (int c = 1; c <= 3; ++c) r[c] = -r[c] * inv_rhovolume - P[c] / rho * dwrotot;
+  const double dwE = -r[4] * inv_volume;
+  const double dwroe = dwE - rho * (P[1] * r[1] + P[2] * r[2] + P[3] * r[3]) -
+                       0.5 * dwrotot * (P[1] * P[1] + P[2] * P[2] + P[3] * P[3]);
+  const double eiroi = (cp * T - Rg * T) * r[0];
+  const double cv = P[0] * (cp - Rg);
+  const double dwTemp = (dwroe - eiroi) / cv;
+  r[4] = Rg * (T * r[0] + P[0] * dwTemp);
+#pragma unroll
+  for (int v = 0; v < 5; ++v) r[v] = r[v] * dt;
+}
+
+// The flux residual in a.Rout of every interior cell -> its primitive-variable residual (the
+// smoothing of integration-variables = prim acts on it).
+template <int GEO>
+__global__ void __launch_bounds__(256) k_prim_resid(StageArgs a, int tx_) {
+  const int tid = threadIdx.x, tx = tid % tx_, ty = tid / tx_;
+  const double gm1 = a.gam - 1.0, cp = a.gam * a.R / gm1;
+  const int64_t N = a.npool;
+  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+    const Tile T = a.tiles[t];
+    const BlockDev* __restrict__ Bp = a.blocks + T.b;
+    const int i = T.i0 + tx, j = T.j0 + ty;
+    if (i > Bp->ni || j > Bp->nj) continue;
+    const LineEnt* Lx = a.lines + Bp->line[0];
+    const LineEnt* Ly = a.lines + Bp->line[1];
+    const LineEnt* Lz = a.lines + Bp->line[2];
+    const int64_t cbase = Bp->off + (int64_t)(i + 1) + (int64_t)Bp->sy * (j + 1);
+    for (int k = T.k0; k < T.k1; ++k) {
+      const int64_t c = cbase + Bp->sz * (int64_t)(k + 1);
+      double u[5], P[5], r[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) { u[v] = a.Uin[v * N + c]; r[v] = a.Rout[v * N + c]; }
+      cons2prim(u, gm1, P);
+      const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
+      const double vol = GEO != 1 ? Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta : a.geo[(size_t)GVOL * N + c];
+      prim_residual(P, r, dt, vol, a.R, cp);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) a.Rout[v * N + c] = r[v];
+    }
+  }
+}
+
+template <int GEO, bool LAST, bool PRIM>
 __global__ void __launch_bounds__(256) k_irs_update(StageArgs a, int tx_, int d, double beta) {
   const int tid = threadIdx.x, tx = tid % tx_, ty = tid / tx_;
   const double gm1 = a.gam - 1.0;
@@ -207,7 +264,10 @@ __global__ void __launch_bounds__(256) k_irs_update(StageArgs a, int tx_, int d,
         double un[5], ui[5], unew[5], P[5], rs[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) { un[v] = a.Un[v * N + c]; ui[v] = a.Uin[v * N + c]; }
-        {
+        if (d < 0) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) rs[v] = a.Rout[v * N + c];
+        } else {
           const int64_t st = d == 0 ? 1 : (d == 1 ? (int64_t)Bp->sy : Bp->sz);
           const int n = d == 0 ? Bp->ni : (d == 1 ? Bp->nj : Bp->nk);
           const int q = d == 0 ? i : (d == 1 ? j : k);
@@ -221,14 +281,33 @@ __global__ void __launch_bounds__(256) k_irs_update(StageArgs a, int tx_, int d,
             rs[v] = (r0 + beta * (sm + sp)) / den;
           }
         }
+        if constexpr (PRIM) {
+          double Pi[5], Pn[5];
+          cons2prim(ui, gm1, Pi);
+          cons2prim(un, gm1, Pn);
+          if (d < 0) {
+            const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
+            const double vol =
+                GEO != 1 ? Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta : a.geo[(size_t)GVOL * N + c];
+            prim_residual(Pi, rs, dt, vol, a.R, a.gam * a.R / gm1);
+          }
 #pragma unroll
-        for (int v = 0; v < 5; ++v) unew[v] = un[v] + a.rkc * (ui[v] - un[v] + rs[v]);
-        cons2prim(unew, gm1, P);
+          for (int v = 0; v < 5; ++v) P[v] = Pn[v] + a.rkc * (Pi[v] - Pn[v] + rs[v]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) unew[v] = un[v] + a.rkc * (ui[v] - un[v] + rs[v]);
+          cons2prim(unew, gm1, P);
+        }
         bool bad = !(P[4] >= 0.0);
 #pragma unroll
         for (int v = 0; v < 5; ++v) bad = bad || (P[v] != P[v]);
         if (bad) set_error(a.err, MOSE_E_STATE, T.b + 1, i, j, k);
-        if (P[0] < 1e-20) { P[0] = 1e-20; prim2cons(P, 1.0 / gm1, unew); }
+        if constexpr (PRIM) {
+          if (P[0] < 1e-20) P[0] = 1e-20;
+          prim2cons(P, 1.0 / gm1, unew);
+        } else {
+          if (P[0] < 1e-20) { P[0] = 1e-20; prim2cons(P, 1.0 / gm1, unew); }
+        }
         if constexpr (LAST) {
           const double snd = sqrt(a.gam * P[4] / P[0]);
           double dtc = 1e8;
@@ -699,36 +778,61 @@ static IrsArgs irs_args(mose_ctx* c) {
   return ia;
 }
 
-// The smoothing and the RK update of one stage whose scaled residual R* is in Rbuf.
+// The stage update after the residual-only stage kernel: with IRS, the smoothing of the
+// residual in Rbuf (R* = -R dt / vol, or the primitive-variable residual for
+// integration-variables = prim) and the RK stage; without IRS (prim only), the RK stage from
+// the flux residual.
 static void enqueue_irs(mose_ctx* c, const StageArgs& a, bool last) {
   IrsArgs ia = irs_args(c);
   const dim3 grid(ia.ntiles), block(c->tx * c->ty);
+  const bool prim = c->cfg.prim_vars != 0, irs = c->cfg.irs_beta > 0.0;
+  const int geo = c->lay.curvi ? 1 : (c->lay.visc ? 2 : 0);
+  const int tx = c->tx;
   double* cur = c->Rbuf;     // the directions alternate between Rbuf and d_rs1
   double* nxt = c->d_rs1;
-  for (int d = 0; d < c->ndir; ++d) {
-    if (c->nirsg > 0)
-      hipLaunchKernelGGL(k_irs_ghost, dim3(launch_blocks(c->nirsg)), dim3(256), 0, c->stream, c->d_irsg, c->nirsg,
-                         cur, c->npool);
-    if (d + 1 == c->ndir) break;          // the last direction is smoothed inside the update
-    ia.d = d;
-    ia.R = cur;
-    ia.dst = nxt;
-    hipLaunchKernelGGL(k_irs_jacobi, grid, block, 0, c->stream, ia);
-    std::swap(cur, nxt);
+  int dl = -1;
+  if (irs) {
+    if (prim) {
+      if (geo == 1) hipLaunchKernelGGL(k_prim_resid<1>, grid, block, 0, c->stream, a, tx);
+      else hipLaunchKernelGGL(k_prim_resid<0>, grid, block, 0, c->stream, a, tx);
+    }
+    for (int d = 0; d < c->ndir; ++d) {
+      if (c->nirsg > 0)
+        hipLaunchKernelGGL(k_irs_ghost, dim3(launch_blocks(c->nirsg)), dim3(256), 0, c->stream, c->d_irsg,
+                           c->nirsg, cur, c->npool);
+      if (d + 1 == c->ndir) break;          // the last direction is smoothed inside the update
+      ia.d = d;
+      ia.R = cur;
+      ia.dst = nxt;
+      hipLaunchKernelGGL(k_irs_jacobi, grid, block, 0, c->stream, ia);
+      std::swap(cur, nxt);
+    }
+    dl = c->ndir - 1;
   }
   StageArgs u = a;
   u.Rout = cur;
-  const int tx = c->tx, dl = c->ndir - 1;
   const double b = c->cfg.irs_beta;
-  const int geo = c->lay.curvi ? 1 : (c->lay.visc ? 2 : 0);
-  if (last) {
-    if (geo == 1) hipLaunchKernelGGL((k_irs_update<1, true>), grid, block, 0, c->stream, u, tx, dl, b);
-    else if (geo == 2) hipLaunchKernelGGL((k_irs_update<2, true>), grid, block, 0, c->stream, u, tx, dl, b);
-    else hipLaunchKernelGGL((k_irs_update<0, true>), grid, block, 0, c->stream, u, tx, dl, b);
+#define MOSE_UPD(G, L, P) hipLaunchKernelGGL((k_irs_update<G, L, P>), grid, block, 0, c->stream, u, tx, dl, b)
+  if (prim) {
+    if (last) {
+      if (geo == 1) MOSE_UPD(1, true, true);
+      else if (geo == 2) MOSE_UPD(2, true, true);
+      else MOSE_UPD(0, true, true);
+    } else {
+      if (geo == 1) MOSE_UPD(1, false, true);
+      else MOSE_UPD(0, false, true);
+    }
   } else {
-    if (geo == 1) hipLaunchKernelGGL((k_irs_update<1, false>), grid, block, 0, c->stream, u, tx, dl, b);
-    else hipLaunchKernelGGL((k_irs_update<0, false>), grid, block, 0, c->stream, u, tx, dl, b);
+    if (last) {
+      if (geo == 1) MOSE_UPD(1, true, false);
+      else if (geo == 2) MOSE_UPD(2, true, false);
+      else MOSE_UPD(0, true, false);
+    } else {
+      if (geo == 1) MOSE_UPD(1, false, false);
+      else MOSE_UPD(0, false, false);
+    }
   }
+#undef MOSE_UPD
 }
 
 // One explicit step enqueued on the context stream (no host synchronisation).  The CFL
@@ -744,10 +848,10 @@ static int enqueue_step(mose_ctx* c) {
     enqueue_ghost(c, in);
     StageArgs a = stage_args(c, in, c->U[0], out, kRK[nrk - 1][s - 1]);
     enqueue_bcfaces(c, a);
-    if (c->cfg.irs_beta > 0.0) {
+    if (c->cfg.irs_beta > 0.0 || c->cfg.prim_vars) {
       StageArgs r = a;
-      r.irs_scale = 1;
-      launch_stage(c, stage_fn(c, 2), r);        // R* = -R dt / vol into Rbuf
+      r.irs_scale = c->cfg.prim_vars ? 0 : 1;
+      launch_stage(c, stage_fn(c, 2), r);        // R (prim) or R* = -R dt / vol into Rbuf
       enqueue_irs(c, a, s == nrk);
     } else {
       launch_stage(c, stage_fn(c, s == nrk ? 1 : 0), a);
@@ -1080,6 +1184,7 @@ int mose_create(const mose_cfg* cfg, mose_ctx** out) {
       cfg->n_rk > 3 || !(cfg->gamma > 1.0) || !(cfg->Rgas > 0.0) || cfg->precision < 0 || cfg->precision > 1)
     return MOSE_E_ARG;
   if (!(cfg->irs_beta >= 0.0) || !(cfg->irs_beta < 1e300)) return MOSE_E_ARG;
+  if (cfg->prim_vars != 0 && cfg->prim_vars != 1) return MOSE_E_ARG;
   // MUSCL-SD: FP64 kernels of the limited reconstructions only
   if (cfg->muscl_sd != 0 && (cfg->muscl_sd != 1 || cfg->limiter == MOSE_LIM_FIRST_ORDER ||
                              cfg->precision != MOSE_PREC_FP64))

@@ -32,7 +32,8 @@ module mose_hip
     integer(c_int32_t) :: rampa_cfl_iter = 0
     integer(c_int32_t) :: precision = 0
     integer(c_int32_t) :: muscl_sd = 0
-    integer(c_int32_t) :: reserved(3) = 0
+    integer(c_int32_t) :: prim_vars = 0
+    integer(c_int32_t) :: reserved(2) = 0
     real(c_double)     :: irs_beta = 0.0_c_double
   end type mose_cfg
 

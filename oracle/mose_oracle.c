@@ -1500,7 +1500,72 @@ int ob_irs_smooth(ob_domain* d, int32_t b, double beta, double* R) {
   return 0;
 }
 
+/* Compute_Prim_Residual (Lib_Newstate.f90:252-299), one species, calorically perfect gas
+ * (h(T) - R T = cv T, cp_tab - R = cv): the flux residual r of a cell of primitive state P
+ * becomes the primitive-variable residual, times dt. */
+static void prim_residual(const ob_domain* d, const double* P, double* r, double dt, double vol) {
+  const double rho = P[0], Rg = d->Rg, T = P[4] / (rho * Rg);
+  const double inv_volume = 1.0 / vol, inv_rhovolume = inv_volume / rho;
+  r[0] = -r[0] * inv_volume;
+  const double dwrotot = r[0];
+  for (int c = 1; c <= 3; ++c) r[c] = -r[c] * inv_rhovolume - P[c] / rho * dwrotot;
+  const double dwE = -r[4] * inv_volume;
+  const double dwroe = dwE - rho * (P[1] * r[1] + P[2] * r[2] + P[3] * r[3]) -
+                       0.5 * dwrotot * (P[1] * P[1] + P[2] * P[2] + P[3] * P[3]);
+  const double eiroi = (d->cp * T - Rg * T) * r[0];
+  const double cv = P[0] * (d->cp - Rg);
+  const double dwTemp = (dwroe - eiroi) / cv;
+  r[4] = Rg * (T * r[0] + P[0] * dwTemp);
+  for (int v = 0; v < NV; ++v) r[v] = r[v] * dt * 1.0;
+}
+
+/* Newstate_Primitive (Lib_Newstate.f90:168-247): the RK stage on the primitive variables,
+ * RK_stage(P, PO, R*) then Check_And_Fix_State; with IRS every cell's residual is converted
+ * first, then smoothed, then the stage is taken. */
+static int newstate_prim(ob_domain* d, int irk) {
+  const double c = RKcoeff[d->cfg.n_rk - 1][irk - 1];
+  const int irs = d->cfg.irs_beta > 0.0;
+  for (int b = 0; b < d->nb; ++b) {
+    OBlk* B = &d->blk[b];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= B->nk; ++k)
+      for (int j = 1; j <= B->nj; ++j)
+        for (int i = 1; i <= B->ni; ++i) {
+          const int64_t q = cidx(B, i, j, k);
+          prim_residual(d, &B->P[NV * q], &B->R[NV * q], B->dt[q], B->vol[q]);
+        }
+  }
+  if (irs) residual_smoothing(d, d->cfg.irs_beta);
+  for (int b = 0; b < d->nb; ++b) {
+    OBlk* B = &d->blk[b];
+    first_err fe = {INT64_MAX, 0, 0, 0, 0};
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= B->nk; ++k)
+      for (int j = 1; j <= B->nj; ++j)
+        for (int i = 1; i <= B->ni; ++i) {
+          const int64_t q = cidx(B, i, j, k);
+          double* P = &B->P[NV * q];
+          const double* R = &B->R[NV * q];
+          const double* PO = &B->PO[NV * q];
+          for (int v = 0; v < NV; ++v) {
+            const double res_ = (P[v] - PO[v] + R[v]) * c;
+            P[v] = PO[v] + res_;
+          }
+          int bad = 0;
+          for (int v = 0; v < NV; ++v) if (isnan(P[v])) bad = 1;
+          if (bad || P[4] < 0.0) {
+            first_err_note(&fe, ((int64_t)k * (B->nj + 2) + j) * (B->ni + 2) + i, MOSE_E_STATE, i, j, k);
+            continue;
+          }
+          if (P[0] < 1e-20) P[0] = 1e-20;
+        }
+    if (fe.code) return set_err(d, MOSE_E_STATE, b + 1, fe.i, fe.j, fe.k);
+  }
+  return 0;
+}
+
 static int newstate(ob_domain* d, int irk) {
+  if (d->cfg.prim_vars) return newstate_prim(d, irk);
   double c = RKcoeff[d->cfg.n_rk - 1][irk - 1];
   const int irs = d->cfg.irs_beta > 0.0;        /* obj_irs%enabled, Assign_Setup.f90:55 */
   if (irs) {

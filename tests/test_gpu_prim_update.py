@@ -1,0 +1,95 @@
+"""GPU parity of the primitive-variable stage update (integration-variables = prim,
+Newstate_Primitive, Lib_Newstate.f90:168-317) against the oracle: with and without residual
+smoothing, global and local dt, RK1/2/3, periodic, multi-block, curvilinear with inflow,
+inlet/outlet and viscous cases; the argument check.  The GPU keeps the conserved state and
+converts around the primitive stage, the oracle keeps the primitive state as the reference
+does: they agree to round-off."""
+import copy
+
+import numpy as np
+import pytest
+
+from mose_amd import abi, cases
+from mose_amd.lib import GpuSolver, MoseError
+from oracle.oracle import OracleSolver
+
+from helpers import perturb, rel_l2, skewed_nodes
+
+pytestmark = pytest.mark.gpu
+
+
+def _with(case, **kw):
+    c = copy.deepcopy(case)
+    f = {n: getattr(case.cfg, n) for n, _ in abi.MoseCfg._fields_ if n != "reserved"}
+    f.update(kw)
+    c.cfg = abi.MoseCfg(**f)
+    return c
+
+
+def _steps_check(case, nsteps=8, tol=1e-12):
+    g, o = GpuSolver(case), OracleSolver(case)
+    for _ in range(nsteps):
+        dg, _, ng = g.step()
+        do, _, no = o.step()
+        assert abs(dg - do) <= 1e-13 * do
+        assert np.allclose(ng, no, rtol=1e-9, atol=1e-12 * np.max(no))
+    for b in range(1, len(case.blocks) + 1):
+        err = rel_l2(g.get_state(b), o.get_state(b))
+        assert err < tol, "block %d: %.3e" % (b, err)
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("irs", [0.0, 0.5])
+@pytest.mark.parametrize("time_accurate", [1, 0])
+def test_prim_periodic(irs, time_accurate):
+    case = perturb(cases.vortex(dims=(16, 12, 8), limiter="vanleer", riemann="HLLC"), seed=3)
+    _steps_check(_with(case, prim_vars=1, irs_beta=irs, time_accurate=time_accurate))
+
+
+@pytest.mark.parametrize("nrk", [1, 2])
+def test_prim_rk1_rk2(nrk):
+    case = perturb(cases.vortex(dims=(12, 12, 6), riemann="AUSM+"), seed=5)
+    _steps_check(_with(case, prim_vars=1, n_rk=nrk), nsteps=6)
+
+
+@pytest.mark.parametrize("irs", [0.0, 0.4])
+def test_prim_multiblock(irs):
+    case = cases.multiblock_chain(nblocks=5, seed=9, lo=6, hi=12, precision=abi.PREC_FP64)
+    _steps_check(_with(perturb(case, seed=4), prim_vars=1, irs_beta=irs, time_accurate=0), nsteps=6)
+
+
+@pytest.mark.parametrize("irs", [0.0, 0.5])
+def test_prim_curvilinear_inflow(irs):
+    dims = (14, 10, 7)
+    prim = np.zeros(tuple(dims[::-1]) + (5,))
+    prim[..., 0], prim[..., 1], prim[..., 2], prim[..., 4] = 1.0, 1.5, 0.2, 1.0 / 1.4
+    blk = cases.Block(dims, skewed_nodes(dims), cases.padded_state(dims, prim))
+    inflow = {"type": 405, "mach": 1.8, "T": 1.05, "p": 0.7, "alpha": 0.1, "beta": -0.05, "rel_fac": 0.8}
+    bc = cases.bc_records([{"dims": dims, "faces": [inflow, 400, 300, 300, 300, 300]}])
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.5, "vanalbada", "HLLE", 3, False, irs_beta=irs, prim_vars=True)
+    _steps_check(perturb(cases.Case("skewed_prim", cfg, [blk], bc), amp=0.03, seed=9), nsteps=6)
+
+
+def test_prim_inlet_outlet_with_shock_detector():
+    dims = (16, 10, 6)
+    prim = np.zeros(tuple(dims[::-1]) + (5,))
+    prim[..., 0], prim[..., 1], prim[..., 4] = 1.0, 0.45, 1.0 / 1.4
+    blk = cases.Block(dims, cases.cartesian_nodes(dims, (0, 0, 0), (1.0, 0.7, 0.4)), cases.padded_state(dims, prim))
+    inlet = {"type": 401, "T": 1.04, "p": 0.86, "alpha": 0.1, "beta": -0.05, "rel_fac": 0.8}
+    outlet = {"type": 406, "p": 0.7, "rel_fac": 1.0}
+    bc = cases.bc_records([{"dims": dims, "faces": [inlet, outlet, 300, 300, 400, 400]}])
+    cfg = abi.make_cfg(1.4, 1.0 / 1.4, 0.5, "minmod", "HLLE", 3, False, muscl_sd=True, irs_beta=0.5, prim_vars=True)
+    _steps_check(perturb(cases.Case("inout_prim", cfg, [blk], bc), amp=0.05, seed=2), nsteps=8)
+
+
+def test_prim_viscous():
+    case = cases.taylor_green(n=16, nbox=2, riemann="AUSM+", precision=abi.PREC_FP64)
+    _steps_check(_with(case, prim_vars=1), nsteps=4)
+    _steps_check(_with(case, prim_vars=1, irs_beta=0.5), nsteps=4)
+
+
+def test_prim_vars_argument_check():
+    with pytest.raises(MoseError) as ei:
+        GpuSolver(_with(cases.vortex(dims=(8, 8, 8)), prim_vars=2))
+    assert ei.value.code == abi.E_ARG
