@@ -13,7 +13,7 @@ from mose_amd.lib import GpuSolver
 from oracle import exact_riemann as er
 from oracle.oracle import OracleSolver
 
-from helpers import interior, perturb, rel_l2, skewed_nodes
+from helpers import interior, oblique_shock_verify, perturb, rel_l2, skewed_nodes
 
 pytestmark = pytest.mark.gpu
 
@@ -112,3 +112,11 @@ def test_shipped_2d_inputs_with_irs(name, lim):
     cfg = abi.make_cfg(1.4, R, 0.9, lim, "HLLE", 3, False, muscl_sd=True, irs_beta=0.5)
     case = formats.load_case(ic, os.path.join(GOLD, name + "_bc.txt.gz"), cfg)
     _steps_check(case, nsteps=20)
+
+
+
+@pytest.mark.parametrize("rs", ["HLLE", "HLLC", "HLLC+", "SLAU"])
+def test_oblique_shock_shipped_configuration_passes_verify(rs):
+    """test/2D/euler/oblique-shock with its input.ini, for the four Riemann solvers its MOSE.sh
+    runs, through its verify.py rule (see helpers.oblique_shock_verify)."""
+    oblique_shock_verify(GpuSolver, rs)

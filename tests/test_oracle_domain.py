@@ -441,3 +441,11 @@ def test_oracle_results_do_not_depend_on_thread_count():
         assert r.returncode == 0, r.stderr
         out.append(r.stdout.strip().splitlines()[-1])
     assert out[0] == out[1]
+
+
+def test_oblique_shock_shipped_configuration_passes_verify():
+    """The reference's oblique-shock input with all of its input.ini settings (MUSCL-SD, van
+    Albada, HLLE, IRS 0.5, CFL 0.9 ramped over 10 iterations, 3000 iterations) passes its
+    verify.py rule on the oracle (about half a minute on 8 threads)."""
+    from helpers import oblique_shock_verify
+    oblique_shock_verify(OracleSolver, "HLLE")
