@@ -76,6 +76,8 @@ def build_fortran():
     out = os.path.join(fdir, "mose_run")
     run([FC, "-O2", "-o", out, "mose_hip_iface.f90", "mose_explicit.f90", "mose_run.f90",
          "-L" + HERE, "-lmose_hip", "-Wl,-rpath,$ORIGIN/.."], cwd=fdir)
+    run([FC, "-O2", "-o", os.path.join(fdir, "mose_case"), "mose_hip_iface.f90", "mose_explicit.f90",
+         "mose_case.f90", "-L" + HERE, "-lmose_hip", "-Wl,-rpath,$ORIGIN/.."], cwd=fdir)
     return out
 
 
