@@ -155,8 +155,8 @@ int  mose_residual(mose_ctx* ctx, int32_t b, double* R);
  * entries 0), R being the residual of the current state including its ghost fill.
  * Limitations: ghost cells filled from other ghost cells (extrapolation across a block
  * thinner than three cells) are treated as constants; the schemes that use the shock
- * detector (MOSE_RS_HLLEPP, MOSE_RS_HLLCP, muscl_sd) and the 401/406 boundary fluxes have no
- * adjoint (MOSE_E_ARG). */
+ * detector (MOSE_RS_HLLEPP, MOSE_RS_HLLCP, muscl_sd) and the 401 inlet flux have no adjoint
+ * (MOSE_E_ARG); the 406 outlet flux has one (state and face normal, not p_exit). */
 int  mose_put_residual_bar(mose_ctx* ctx, int32_t b, const double* Rbar);
 int  mose_residual_vjp(mose_ctx* ctx);
 int  mose_get_state_bar(mose_ctx* ctx, int32_t b, double* Pbar);

@@ -683,6 +683,107 @@ __device__ void symmetry_flux_adj(const double* Pb, const double* Pi, double bdl
   for (int c = 0; c < 3; ++c) nb[c] += -2.0 * (Pb[1 + c] * gdn + un * Gpb[1 + c]);
 }
 
+// Reverse of outflow406_flux (BC 406, Lib_BC_Fluxes_Outflow.f90:12-130) for the flux adjoint Fb
+// (per unit area): accumulates into Pbb (boundary-cell primitives) and nb (face normal).
+// The branch (subsonic with p_exit > 0, and whether p* replaces p_exit) is taken as the
+// forward pass takes it; returns false, touching nothing, where the forward falls back to
+// the symmetry flux.
+__device__ bool outflow406_adj(const double* Pb, bool hi, const double* n, const double* bc, double gam,
+                               const double* Fb, double* Pbb, double* nb) {
+  const double sgn = hi ? 1.0 : -1.0;
+  const double un = Pb[1] * n[0] + Pb[2] * n[1] + Pb[3] * n[2];
+  if (sgn * un < 0.0) return false;
+  const double dl = 0.5 * (gam - 1.0), E = gam / (gam - 1.0), X = (gam - 1.0) / (2.0 * gam);
+  const double snd = sqrt(gam * Pb[4] / Pb[0]);
+  const double riem = snd + sgn * dl * un;
+  const double vt[3] = {Pb[1] - un * n[0], Pb[2] - un * n[1], Pb[3] - un * n[2]};
+  const double machn = fabs(un / snd);
+  const bool sub = machn < 1.0 && bc[0] > 0.0;
+  double rho = Pb[0], p = Pb[4], ufn = un, u[3] = {Pb[1], Pb[2], Pb[3]};
+  double pe = bc[0], sndf = 0.0, ratio = 0.0, base = 0.0, den = 0.0;
+  bool clip = false;
+  if (sub) {
+    den = 1.0 / pow(1.0 + dl, E);
+    base = 1.0 + dl * machn * machn;
+    const double pstar = Pb[4] * den * pow(base, E);
+    clip = pstar >= pe;
+    if (clip) pe = pstar;
+    ratio = pow(pe / Pb[4], X);
+    sndf = snd * ratio;
+    ufn = (riem - sndf) / (sgn * dl);
+    p = pe;
+    rho = gam * p / (sndf * sndf);
+    for (int q = 0; q < 3; ++q) u[q] = vt[q] + ufn * n[q];
+  }
+  const double h = E * p / rho;
+  const double m = rho * ufn;
+  const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  // reverse of F0 = m, F(1+q) = m u_q + p n_q, F4 = m (q2/2 + h)
+  const double mb = Fb[0] + Fb[1] * u[0] + Fb[2] * u[1] + Fb[3] * u[2] + Fb[4] * (0.5 * q2 + h);
+  double ub[3], pb = 0.0, rhob = 0.0, ufnb = 0.0, unb = 0.0, sndb = 0.0;
+  for (int q = 0; q < 3; ++q) {
+    ub[q] = Fb[1 + q] * m + Fb[4] * m * u[q];
+    pb += Fb[1 + q] * n[q];
+    nb[q] += Fb[1 + q] * p;
+  }
+  const double hb = Fb[4] * m;
+  pb += hb * E / rho;
+  rhob -= hb * E * p / (rho * rho);
+  rhob += mb * ufn;
+  ufnb += mb * rho;
+  if (sub) {
+    double vtb[3];
+    for (int q = 0; q < 3; ++q) {
+      vtb[q] = ub[q];
+      ufnb += ub[q] * n[q];
+      nb[q] += ub[q] * ufn;
+    }
+    // rho = gam pe / sndf^2, p = pe
+    double peb = pb + rhob * gam / (sndf * sndf);
+    double sndfb = -2.0 * rhob * gam * p / (sndf * sndf * sndf);
+    // ufn = (riem - sndf) / (sgn dl)
+    const double riemb = ufnb / (sgn * dl);
+    sndfb -= ufnb / (sgn * dl);
+    // sndf = snd (pe / p_cell)^X
+    sndb += sndfb * ratio;
+    peb += sndfb * sndf * X / pe;
+    double pcb = -sndfb * sndf * X / Pb[4];
+    double machnb = 0.0;
+    if (clip) {   // pe = p* = p_cell den base^E, base = 1 + dl machn^2
+      pcb += peb * den * pow(base, E);
+      machnb = peb * Pb[4] * den * E * pow(base, E - 1.0) * 2.0 * dl * machn;
+    }
+    // machn = |un / snd|
+    const double sr = (un / snd) >= 0.0 ? 1.0 : -1.0;
+    unb += machnb * sr / snd;
+    sndb -= machnb * sr * un / (snd * snd);
+    // vt_q = P(1+q) - un n_q
+    for (int q = 0; q < 3; ++q) {
+      Pbb[1 + q] += vtb[q];
+      unb -= vtb[q] * n[q];
+      nb[q] -= vtb[q] * un;
+    }
+    // riem = snd + sgn dl un
+    sndb += riemb;
+    unb += riemb * sgn * dl;
+    Pbb[4] += pcb;
+  } else {
+    Pbb[0] += rhob;
+    Pbb[4] += pb;
+    unb += ufnb;
+    for (int q = 0; q < 3; ++q) Pbb[1 + q] += ub[q];
+  }
+  // snd = sqrt(gam p / rho)
+  Pbb[4] += sndb * gam / (2.0 * snd * Pb[0]);
+  Pbb[0] -= sndb * snd / (2.0 * Pb[0]);
+  // un = P . n
+  for (int q = 0; q < 3; ++q) {
+    Pbb[1 + q] += unb * n[q];
+    nb[q] += unb * Pb[1 + q];
+  }
+  return true;
+}
+
 // Supersonic inflow face (inflow405_flux); returns false when the symmetry flux applies.
 // bcb (6) receives the adjoints of the imposed Mach number, temperature, pressure and the
 // three components of the velocity direction.
@@ -1082,12 +1183,15 @@ __global__ void __launch_bounds__(256) k_adj_bcface(AdjArgs a) {
         bool done = false;
         if (bct == F_INFLOW)
           done = inflow405_adj<LIM, RS>(Pb, Pi, hi, e.n[0], e.n[1], e.n[2], bc, g, Fb, Pbb, Pib, nb, bcb);
+        else if (bct == F_OUTFLOW)
+          done = outflow406_adj(Pb, hi, e.n, bc, a.gam, Fb, Pbb, nb);
         if (!done)
           symmetry_flux_adj<LIM, RS>(Pb, Pi, e.bdl, e.idl, hi, e.n[0], e.n[1], e.n[2], g, Fb, Pbb, Pib, nb, &bdlb,
                                      &idlb);
         if (want_geo) {
           bool fdone = false;
           if (bct == F_INFLOW) fdone = inflow405_flux<LIM, RS>(Pb, Pi, hi, e.n[0], e.n[1], e.n[2], bc, g, F);
+          else if (bct == F_OUTFLOW) fdone = outflow406_flux(Pb, hi, e.n, bc, a.gam, F);
           if (!fdone) symmetry_flux<LIM, RS>(Pb, Pi, e.bdl, e.idl, hi, e.n[0], e.n[1], e.n[2], g, F);
         }
       }

@@ -962,7 +962,7 @@ static int ensure_adjoint(mose_ctx* c) {
   if (c->d_beta) return fail(c, MOSE_E_ARG);
   for (const BlockDev& B : c->lay.blocks)
     for (int f = 0; f < 6; ++f)
-      if (B.bct[f] == F_INLET || B.bct[f] == F_OUTFLOW) return fail(c, MOSE_E_ARG);
+      if (B.bct[f] == F_INLET) return fail(c, MOSE_E_ARG);
   if (c->adj_ready) return 0;
   std::vector<AdjSeg> segs;
   int64_t start = 0;

@@ -12,7 +12,7 @@ import torch
 
 from mose_amd import abi, cases
 from mose_amd.autograd import residual as torch_residual
-from mose_amd.lib import GpuSolver
+from mose_amd.lib import GpuSolver, MoseError
 from oracle.oracle import OracleSolver
 
 from helpers import interior, perturb, skewed_nodes
@@ -113,3 +113,33 @@ def test_torch_gradcheck_small_case():
     g = GpuSolver(case)
     P = torch.from_numpy(case.blocks[0].P.copy()).requires_grad_(True)
     assert torch.autograd.gradcheck(lambda x: torch_residual(g, x), (P,), eps=1e-6, atol=1e-6, rtol=1e-4)
+
+
+@pytest.mark.parametrize("pe,mach", [(0.62, 0.5), (0.05, 0.5), (0.0, 0.5), (0.7, 1.5)])
+def test_vjp_outflow406(pe, mach):
+    """The 406 outlet flux (back pressure p_exit: subsonic, below the critical pressure, none,
+    supersonic exit) in the state adjoint, against central differences of the GPU and of the
+    oracle residual."""
+    outlet = {"type": 406, "p": pe, "rel_fac": 1.0}
+    case = _box((9, 7, 5), [400, outlet, 300, 300, 300, 0], mach=mach)
+    _directional_check(case)
+    _directional_check(case, solver_cls=OracleSolver)
+
+
+def test_vjp_outflow406_mixed_directions():
+    """Cells of the 406 face that flow in take the symmetry flux and its adjoint."""
+    outlet = {"type": 406, "p": 0.62, "rel_fac": 1.0}
+    case = _box((9, 7, 5), [400, outlet, 300, 400, 300, 0], mach=0.02, seed=11)
+    un = interior(case.blocks[0].P)[:, :, -1, 1]
+    assert np.any(un > 0) and np.any(un < 0)
+    _directional_check(case)
+
+
+def test_vjp_refuses_inlet401():
+    inlet = {"type": 401, "T": 1.04, "p": 0.86, "alpha": 0.1, "beta": -0.05, "rel_fac": 0.8}
+    case = _box((8, 6, 5), [inlet, 400, 300, 300, 300, 0], mach=0.4)
+    g = GpuSolver(case)
+    with pytest.raises(MoseError) as ei:
+        g.residual_vjp([np.ones(abi.padded_shape((8, 6, 5)))])
+    assert ei.value.code == abi.E_ARG
+    g.close()

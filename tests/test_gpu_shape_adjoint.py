@@ -164,3 +164,12 @@ def test_geometry_adjoint_needs_general_metrics():
     with pytest.raises(Exception):
         g.residual_vjp_geometry(Rbar)
     g.close()
+
+
+def test_node_adjoint_with_outflow406():
+    """Node adjoint of a skewed box with a subsonic 406 outlet (its flux depends on the face
+    normal) against central differences of the GPU and oracle residuals."""
+    outlet = {"type": 406, "p": 0.62, "rel_fac": 1.0}
+    case = _box((9, 7, 5), [400, outlet, 300, 300, 300, 300], mach=0.5)
+    _shape_check(case)
+    _shape_check(case, solver_cls=OracleSolver)
