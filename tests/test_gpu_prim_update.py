@@ -93,3 +93,17 @@ def test_prim_vars_argument_check():
     with pytest.raises(MoseError) as ei:
         GpuSolver(_with(cases.vortex(dims=(8, 8, 8)), prim_vars=2))
     assert ei.value.code == abi.E_ARG
+
+
+@pytest.mark.parametrize("prim", [0, 1])
+def test_irs_and_prim_with_fp32_flux_within_1e5(prim):
+    """The FP32-flux kernels under residual smoothing and the primitive-variable update: the
+    state after five steps within 1e-5 (relative L2) of the FP64 path."""
+    case = _with(perturb(cases.vortex(dims=(16, 12, 8)), seed=6), irs_beta=0.5, prim_vars=prim, time_accurate=0)
+    out = []
+    for prec in (abi.PREC_FP64, abi.PREC_FP32_FLUX):
+        g = GpuSolver(_with(case, precision=prec))
+        g.advance(5)
+        out.append(g.get_state(1))
+        g.close()
+    assert rel_l2(out[1], out[0]) < 1e-5
