@@ -204,6 +204,10 @@ int  mose_time_kernels(mose_ctx* ctx, int32_t nrep, double ms[4]);
  * OpenMP schedules). */
 int  mose_autotune(mose_ctx* ctx, int32_t nrep, int32_t choice[3], double* best_ms);
 
+/* The tiling in use, in mose_autotune's encoding {TX*1000+TY, k-chunk, xcd order}.  No
+ * counterpart in the reference. */
+int  mose_get_tiling(const mose_ctx* ctx, int32_t choice[3]);
+
 /* Total interior cells over all blocks. */
 int64_t mose_num_cells(const mose_ctx* ctx);
 

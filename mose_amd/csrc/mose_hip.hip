@@ -54,7 +54,7 @@ struct DtArgs {
 // minimum.  GEO 0: rectilinear metrics (versor = axis, dl = cell length); GEO 1: the unit
 // rows of M and dl from the geometry arrays.
 template <int GEO>
-__global__ void __launch_bounds__(256) k_dt(DtArgs a) {
+__global__ void __launch_bounds__(kMaxTileThreads) k_dt(DtArgs a) {
   const int tid = threadIdx.x, tx = tid % a.tx, ty = tid / a.tx;
   const double gm1 = a.gam - 1.0;
   const int it = *a.iter + a.iter_add;
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) k_irs_ghost(const int64_t* __restrict__ p
 // Both Jacobi sweeps of direction d for every interior cell: the first sweep's values at the
 // two neighbours along d are recomputed from R (a neighbour that is a ghost keeps its R), so
 // one pass reads R (the residual ghosts filled) and writes the second sweep into dst.
-__global__ void __launch_bounds__(256) k_irs_jacobi(IrsArgs a) {
+__global__ void __launch_bounds__(kMaxTileThreads) k_irs_jacobi(IrsArgs a) {
   const int tx = threadIdx.x % a.tx, ty = threadIdx.x / a.tx;
   const int64_t N = a.npool;
   const double den = 1.0 + 2.0 * a.beta;
@@ -215,7 +215,7 @@ __device__ inline void prim_residual(const double* P, double* r, double dt, doub
 // The flux residual in a.Rout of every interior cell -> its primitive-variable residual (the
 // smoothing of integration-variables = prim acts on it).
 template <int GEO>
-__global__ void __launch_bounds__(256) k_prim_resid(StageArgs a, int tx_) {
+__global__ void __launch_bounds__(kMaxTileThreads) k_prim_resid(StageArgs a, int tx_) {
   const int tid = threadIdx.x, tx = tid % tx_, ty = tid / tx_;
   const double gm1 = a.gam - 1.0, cp = a.gam * a.R / gm1;
   const int64_t N = a.npool;
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(256) k_prim_resid(StageArgs a, int tx_) {
 }
 
 template <int GEO, bool LAST, bool PRIM>
-__global__ void __launch_bounds__(256) k_irs_update(StageArgs a, int tx_, int d, double beta) {
+__global__ void __launch_bounds__(kMaxTileThreads) k_irs_update(StageArgs a, int tx_, int d, double beta) {
   const int tid = threadIdx.x, tx = tid % tx_, ty = tid / tx_;
   const double gm1 = a.gam - 1.0;
   const int64_t N = a.npool;
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(256) k_irs_update(StageArgs a, int tx_, int d,
       }
     }
     if constexpr (LAST) {   // fixed-order reduction over the workgroup
-      __shared__ double sW[4][5];
+      __shared__ double sW[kMaxTileThreads / 64][5];
       const int lane = tid & 63, wave = tid >> 6, nw = (int)(blockDim.x / 64);
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
@@ -393,7 +393,7 @@ __global__ void __launch_bounds__(256) k_pres(const double* __restrict__ U, int6
   }
 }
 
-__global__ void __launch_bounds__(256) k_sd(SdArgs a) {
+__global__ void __launch_bounds__(kMaxTileThreads) k_sd(SdArgs a) {
   const int tx = threadIdx.x % a.tx, ty = threadIdx.x / a.tx;
   const double* __restrict__ p = a.pres;
   for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
@@ -923,7 +923,32 @@ static int ensure_stage(mose_ctx* c, int64_t n) {
 static bool shape_compiled(const mose_ctx* c, int shape) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
   if (c->lay.curvi || c->lay.visc || c->cfg.muscl_sd) return shape == 0;
-  return shape >= 0 && shape < NSHAPE && ktable().f[p][c->cfg.limiter][c->cfg.riemann][shape][0] != nullptr;
+  if (shape < 0 || shape >= NSHAPE) return false;
+  // 512-thread patches only when MOSE_TILE_WIDE=1
+  if (kShape[p][shape][0] * kShape[p][shape][1] > 256) {
+    const char* e = getenv("MOSE_TILE_WIDE");
+    if (!e || atoi(e) != 1) return false;
+  }
+  return ktable().f[p][c->cfg.limiter][c->cfg.riemann][shape][0] != nullptr;
+}
+
+// Every kernel launched with one thread per patch column takes tx * ty threads: refuse a
+// shape that exceeds the launch bound a kernel was compiled with (such a launch fails).
+static int tile_launch_fits(mose_ctx* c) {
+  const int nt = c->tx * c->ty;
+  if (nt > kMaxTileThreads || nt % 64 != 0) return fail(c, MOSE_E_ARG);
+  const void* fns[] = {(const void*)stage_fn(c, 0), (const void*)stage_fn(c, 1), (const void*)stage_fn(c, 2),
+                       (const void*)k_dt<0>, (const void*)k_dt<1>, (const void*)k_sd, (const void*)k_irs_jacobi,
+                       (const void*)k_prim_resid<0>, (const void*)k_prim_resid<1>,
+                       (const void*)k_irs_update<0, false, false>, (const void*)k_irs_update<0, true, false>,
+                       (const void*)k_irs_update<1, true, true>, (const void*)k_irs_update<2, true, true>};
+  for (const void* f : fns) {
+    if (!f) continue;
+    hipFuncAttributes at{};
+    if (hipFuncGetAttributes(&at, f) != hipSuccess) continue;     // no device: nothing to check against
+    if (at.maxThreadsPerBlock < nt) return fail(c, MOSE_E_ARG);
+  }
+  return 0;
 }
 
 // Rebuild the work list for another tile shape / k chunk / order (buffers sized to the
@@ -934,6 +959,7 @@ static int retile(mose_ctx* c, int shape, int kchunk, int xcd) {
   c->shape = shape;
   c->tx = kShape[prec][shape][0];
   c->ty = kShape[prec][shape][1];
+  if (tile_launch_fits(c)) return c->err.code;
   c->kchunk = kchunk;
   c->xcd = xcd;
   const size_t old = c->lay.tiles.size();
@@ -958,7 +984,7 @@ static int retile(mose_ctx* c, int shape, int kchunk, int xcd) {
 // Discrete adjoint set-up: face work segments, the transposed ghost-fill map, buffers.
 static int ensure_adjoint(mose_ctx* c) {
   // the adjoint of the shock-detector schemes (HLLE++, HLLC+, MUSCL-SD) and of the subsonic
-  // inlet / outlet faces (401, 406) is not built
+  // inlet faces (401) is not built
   if (c->d_beta) return fail(c, MOSE_E_ARG);
   for (const BlockDev& B : c->lay.blocks)
     for (int f = 0; f < 6; ++f)
@@ -1495,6 +1521,14 @@ int mose_autotune(mose_ctx* c, int32_t nrep, int32_t choice[3], double* best_ms)
   }
   if (best_ms) *best_ms = best;
   return sync_and_check(c);
+}
+
+int mose_get_tiling(const mose_ctx* c, int32_t choice[3]) {
+  if (!c || !c->finalized || !choice) return MOSE_E_ARG;
+  choice[0] = c->tx * 1000 + c->ty;
+  choice[1] = c->kchunk;
+  choice[2] = c->xcd;
+  return 0;
 }
 
 int mose_put_residual_bar(mose_ctx* c, int32_t b, const double* Rbar) {

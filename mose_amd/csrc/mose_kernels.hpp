@@ -79,6 +79,12 @@ struct Tile {
   int b, i0, j0, k0, k1, pad0, pad1, pad2;
 };
 
+// Largest patch (TX x TY threads) of any tile shape.  Every kernel launched with one thread
+// per patch column (the stage kernels and k_dt, k_sd, k_prim_resid, k_irs_jacobi,
+// k_irs_update) declares a launch bound of at least this, and the host refuses a shape above
+// it (a launch with more threads than a kernel's bound fails).
+constexpr int kMaxTileThreads = 512;
+
 struct GhostOp {
   int64_t b;         // boundary cell (pool index)
   int64_t s;         // inward pool stride

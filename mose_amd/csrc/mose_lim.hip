@@ -42,6 +42,8 @@ void solver_entries(KTable& k, ATable& a) {
     stage_modes<LIM, RS, double, kShape[0][2][0], kShape[0][2][1]>(k.f[0][LIM][RS][2]);
     stage_modes<LIM, RS, float, kShape[1][1][0], kShape[1][1][1]>(k.f[1][LIM][RS][1]);
     stage_modes<LIM, RS, float, kShape[1][2][0], kShape[1][2][1]>(k.f[1][LIM][RS][2]);
+    stage_modes<LIM, RS, double, kShape[0][3][0], kShape[0][3][1]>(k.f[0][LIM][RS][3]);
+    stage_modes<LIM, RS, float, kShape[1][3][0], kShape[1][3][1]>(k.f[1][LIM][RS][3]);
   }
   k.bc[0][LIM][RS] = k_bcface<LIM, RS, double>;
   k.bc[1][LIM][RS] = k_bcface<LIM, RS, float>;

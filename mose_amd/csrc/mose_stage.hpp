@@ -300,12 +300,13 @@ __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const 
 // and the k face above it; the TX + TY faces on the high edges of the patch are solved by
 // TX + TY threads whose choice rotates with the plane so no wave always carries them.  The
 // k faces need only the thread's own registers.
-// Occupancy floor: FP32 flux 3 waves per SIMD; FP64 2 (256 registers: without the floor some
-// variants take a few AGPRs beyond 256 and fall to one wave per SIMD); the viscous kernels,
-// whose LDS allows one workgroup per CU, none.
+// Occupancy floor: FP32 flux 3 waves per SIMD (2 for 512-thread patches, whose workgroup
+// brings two waves to every SIMD); FP64 2 (256 registers: without the floor some variants take
+// a few AGPRs beyond 256 and fall to one wave per SIMD); the viscous kernels, whose LDS allows
+// one workgroup per CU, none.
 template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc, int GEO, bool SDR>
 __global__ void __launch_bounds__(TXc* TYc)
-    __attribute__((amdgpu_waves_per_eu(GEO == 2 ? 1 : (sizeof(FT) == 4 ? (TXc * TYc == 256 ? 3 : 1) : 2))))
+    __attribute__((amdgpu_waves_per_eu(GEO == 2 ? 1 : (sizeof(FT) == 4 ? (TXc * TYc == 256 ? 3 : (TXc * TYc == 512 ? 2 : 1)) : 2))))
     k_stage(StageArgs a) {
   constexpr int NTc = TXc * TYc;
   constexpr int NW = NTc / 64;                   // waves per workgroup
@@ -327,9 +328,7 @@ __global__ void __launch_bounds__(TXc* TYc)
   // (LDS-bound); reduced in a fixed order either way
   constexpr bool NLDS = MODE == 1 && sizeof(FT) == 4;
   __shared__ double sN[5][NLDS ? NTc : 1];
-  // 512-thread tiles (32 x 16) failed at launch ("unspecified launch failure") with the cause
-  // not identified; only tiles of at most 256 threads are compiled
-  static_assert(NTc <= 256, "tile shapes above 256 threads are not supported");
+  static_assert(NTc <= kMaxTileThreads, "tile shape above the helper kernels' launch bound");
 
   const int tid = threadIdx.x, tx = tid % TXc, ty = tid / TXc;
   const double gm1 = a.gam - 1.0;

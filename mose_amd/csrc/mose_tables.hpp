@@ -14,8 +14,17 @@ constexpr int NLIM = 6;     // minmod, van Leer, van Albada, first order, MC, su
 constexpr int NRS = 7;      // HLLC, AUSM+, HLLE, SLAU, SLAU2, HLLE++, HLLC+
 // Patch shapes (TX x TY threads) of the stage kernel per precision; shape 0 is the default,
 // the others are compiled for the van Leer limiter and selected by the tile tuner.
-constexpr int NSHAPE = 3;
-constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 16}, {8, 16}}, {{32, 8}, {16, 16}, {16, 8}}};
+constexpr int NSHAPE = 4;
+constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 16}, {8, 16}, {32, 16}}, {{32, 8}, {16, 16}, {16, 8}, {32, 16}}};
+constexpr bool shapes_within_bound() {
+  for (int p = 0; p < 2; ++p)
+    for (int s = 0; s < NSHAPE; ++s) {
+      const int n = kShape[p][s][0] * kShape[p][s][1];
+      if (n > mose::kMaxTileThreads || n % 64 != 0) return false;
+    }
+  return true;
+}
+static_assert(shapes_within_bound(), "every tile shape fits the tile kernels' launch bound in whole waves");
 
 // null where a variant is not compiled
 struct KTable {

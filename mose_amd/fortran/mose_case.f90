@@ -6,7 +6,9 @@
 !> time-threshold, and writes the final blocks in the same Tecplot layout.
 !>
 !> Usage: mose_case input.ini ic.tec bc.txt field.tec [gamma [Rgas]]
-!> gamma defaults to 1.4; Rgas to p / rho of the first cell (the shipped inputs start at T = 1).
+!> gamma defaults to 1.4.  Rgas (the gas constant of the case's thermo) must be given when
+!> bc.txt holds inflow records (401, 405), whose imposed temperature sets a density through it;
+!> otherwise only p / rho enters the path and Rgas defaults to p / rho of the first cell.
 program mose_case
   use iso_c_binding
   use mose_hip
@@ -43,9 +45,13 @@ program mose_case
   call scan_tecplot(trim(fic), nb, dims, nsp)
   call Domain_Allocate(domain, cfg, dims)
   call load_tecplot(trim(fic), domain, nsp)
-  if (rgas <= 0) rgas = domain%blk(1)%P(5, 1, 1, 1) / domain%blk(1)%P(1, 1, 1, 1)
-  domain%cfg%Rgas = rgas
   call read_bc(trim(fbc), bc)
+  if (rgas <= 0) then
+    if (any(bc%type == 401 .or. bc%type == 405)) &
+      call fail('inflow records (401/405) need the gas constant: pass gamma and Rgas')
+    rgas = domain%blk(1)%P(5, 1, 1, 1) / domain%blk(1)%P(1, 1, 1, 1)
+  endif
+  domain%cfg%Rgas = rgas
   call Domain_Setup(domain, bc)
 
   if (time_max > 0) then
@@ -209,6 +215,7 @@ contains
           if (index(lower(line), '"rho' // trim(itoa(q)) // '"') > 0) nsp = q
         enddo
       else if (index(lower(adjustl(line)), 'zone') == 1) then
+        if (nb >= size(tmp, 2)) call fail('ic file with more zones than scan_tecplot holds (4096)')
         h = line
         do
           read(u, '(A)', iostat=ios) line

@@ -53,7 +53,7 @@ module mose_hip
             mose_get_state, mose_compute_dt, mose_step, mose_advance, mose_residual, &
             mose_put_residual_bar, mose_residual_vjp, mose_residual_vjp_geometry, mose_get_viscous_bar, &
             mose_get_state_bar, mose_get_node_bar, mose_get_inflow_bar, mose_time_residual_vjp, &
-            mose_time_kernels, mose_autotune, &
+            mose_time_kernels, mose_autotune, mose_get_tiling, &
             mose_get_time, mose_set_time, mose_num_cells, mose_last_error, mose_destroy
 
   interface
@@ -178,6 +178,11 @@ module mose_hip
       integer(c_int32_t), value       :: nrep
       integer(c_int32_t), intent(out) :: choice(3)
       real(c_double), intent(out)     :: best_ms
+    end function
+    integer(c_int) function mose_get_tiling(ctx, choice) bind(C, name="mose_get_tiling")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value              :: ctx
+      integer(c_int32_t), intent(out) :: choice(3)
     end function
     integer(c_int) function mose_get_time(ctx, time, iter) bind(C, name="mose_get_time")
       import :: c_int, c_ptr, c_int32_t, c_double

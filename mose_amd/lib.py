@@ -19,7 +19,7 @@ _LIB = None
 
 EXPORTS = ["mose_create", "mose_add_block", "mose_set_viscous", "mose_set_bc", "mose_finalize", "mose_put_state",
            "mose_get_state", "mose_compute_dt", "mose_step", "mose_advance", "mose_residual",
-           "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_put_residual_bar",
+           "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_get_tiling", "mose_put_residual_bar",
            "mose_residual_vjp", "mose_get_state_bar", "mose_residual_vjp_geometry", "mose_get_viscous_bar", "mose_get_node_bar",
            "mose_get_inflow_bar", "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
            "mose_destroy"]
@@ -50,6 +50,7 @@ def load():
     L.mose_set_time.argtypes = [v, C.c_double, C.c_int32]
     L.mose_time_kernels.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_autotune.argtypes = [v, C.c_int32, P(C.c_int32), P(C.c_double)]
+    L.mose_get_tiling.argtypes = [v, P(C.c_int32)]
     L.mose_put_residual_bar.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_residual_vjp.argtypes = [v]
     L.mose_get_state_bar.argtypes = [v, C.c_int32, P(C.c_double)]
@@ -231,6 +232,12 @@ class GpuSolver:
         ms = np.zeros(4)
         self._check(load().mose_time_kernels(self.h, nrep, _dp(ms)))
         return ms
+
+    def tiling(self):
+        """The tiling in use: {"tile": (TX, TY), "kchunk": kc, "xcd_order": bool}."""
+        ch = np.zeros(3, dtype=np.int32)
+        self._check(load().mose_get_tiling(self.h, ch.ctypes.data_as(C.POINTER(C.c_int32))))
+        return {"tile": (int(ch[0]) // 1000, int(ch[0]) % 1000), "kchunk": int(ch[1]), "xcd_order": bool(ch[2])}
 
     def autotune(self, nrep=3):
         """Pick the fastest tile shape / k chunk / tile order by timing; returns

@@ -71,3 +71,46 @@ def test_bc_record_count_and_order_follow_read_bcfile():
     fwd = {(r["b"], r["i"], r["j"], r["k"], r["f"]): (r["bs"], r["is"], r["js"], r["ks"], r["fs"]) for r in conn}
     for src, dst in fwd.items():
         assert fwd[dst] == src
+
+
+CSRC = os.path.join(ROOT, "mose_amd", "csrc")
+
+
+def _kernel_bounds(text):
+    """name -> the __launch_bounds__ expression of every __global__ kernel in a source text."""
+    return dict((m.group(2), m.group(1)) for m in
+                re.finditer(r"__global__ void __launch_bounds__\(([^)]*)\)\s*(?:__attribute__\(\(.*?\)\)\s*)?(\w+)\(",
+                            text, re.S))
+
+
+def test_tile_shaped_launches_fit_their_kernels_bounds():
+    """Every launch that takes one thread per patch column (dim3(c->tx * c->ty) or the `block`
+    built from it) goes to a kernel whose launch bound is kMaxTileThreads (the stage kernel's is
+    its own TX * TY), and every tile shape of either precision is at most kMaxTileThreads
+    threads in whole waves: a 512-thread shape launched into a 256-thread bound fails."""
+    hip = open(os.path.join(CSRC, "mose_hip.hip")).read()
+    kern = open(os.path.join(CSRC, "mose_kernels.hpp")).read()
+    tables = open(os.path.join(CSRC, "mose_tables.hpp")).read()
+    stage = open(os.path.join(CSRC, "mose_stage.hpp")).read()
+    kmax = int(re.search(r"constexpr int kMaxTileThreads = (\d+);", kern).group(1))
+    shapes = re.search(r"kShape\[2\]\[NSHAPE\]\[2\] = \{(.*?)\};", tables, re.S).group(1)
+    pairs = [(int(a), int(b)) for a, b in re.findall(r"\{(\d+), (\d+)\}", shapes)]
+    nshape = int(re.search(r"constexpr int NSHAPE = (\d+);", tables).group(1))
+    assert len(pairs) == 2 * nshape
+    for tx, ty in pairs:
+        assert tx * ty <= kmax and (tx * ty) % 64 == 0, (tx, ty)
+    bounds = _kernel_bounds(hip)
+    # kernels launched with the tile's thread count
+    tile_launched = set()
+    for m in re.finditer(r"hipLaunchKernelGGL\(\(?(\w+)(?:<[^>]*>)?\)?,\s*([^,]+),\s*([^,]+),", hip):
+        name, block = m.group(1), m.group(3).strip()
+        if block in ("dim3(c->tx * c->ty)", "block"):
+            tile_launched.add(name)
+    assert {"k_dt", "k_sd", "k_irs_jacobi", "k_prim_resid", "k_irs_update"} <= tile_launched
+    for name in tile_launched:
+        assert bounds[name] == "kMaxTileThreads", (name, bounds.get(name))
+    # the stage kernels (launched through hipLaunchKernel with c->tx * c->ty) are bounded by their shape
+    assert re.search(r"__launch_bounds__\(TXc\* TYc\)", stage)
+    assert "static_assert(NTc <= kMaxTileThreads" in stage
+    # shared arrays indexed by wave are sized for the largest workgroup
+    assert "sW[kMaxTileThreads / 64][5]" in hip

@@ -92,10 +92,12 @@ flux-limiter = vanleer
 riemann-solver = HLLE ; the shipped solver
 integration-variables = cons
 """
-    ic, bc, out, _ = _run_case(tmp_path, "prandtl_meyer", ini)
-    (fb,) = formats.read_tecplot(out)
-    (b0,) = formats.read_tecplot(ic)
+    # the input starts at T = 1 in its gas: R = p / rho of any cell, passed with gamma (its
+    # 405 inflow records need the gas constant)
+    (b0,) = formats.read_tecplot(os.path.join(GOLD, "prandtl_meyer_ic.tec.gz"))
     R = float(b0["prim"][0, 0, 0, 4] / b0["prim"][0, 0, 0, 0])
+    ic, bc, out, _ = _run_case(tmp_path, "prandtl_meyer", ini, extra=("1.4", repr(R)))
+    (fb,) = formats.read_tecplot(out)
     cfg = abi.make_cfg(1.4, R, 1.0, "vanleer", "HLLE", 3, False, irs_beta=0.5)
     g = GpuSolver(formats.load_case(ic, bc, cfg))
     g.advance(300)

@@ -200,3 +200,85 @@ def test_duct_converges_to_isentropic_flow():
     assert np.allclose(u * u / (GAM * p / rho), M2, rtol=1e-7)
     assert np.allclose(p / (rho * RG) * (1.0 + DELTA * M2), T0, rtol=1e-8)
     assert np.max(np.abs(P[..., 2:4])) < 1e-10
+
+
+# ---- 405 supersonic-static inflow face state (BC_Inlet_Supersonic_Static + Supersonic_Inflow,
+# Lib_BC_Fluxes_Inflow.f90:244-382).  The boundary cell's face state is its value extrapolated
+# half a cell towards the face with the limited *undivided* difference of the imposed state, the
+# boundary cell and its interior neighbour (:341-350), and the face flux is the Riemann flux
+# between that state and the imposed one.  On a 1-D block whose other faces carry no flux (type
+# 0) the interior face fluxes telescope, so the sum of the residual over all cells is the 405
+# face flux times its area (sign: -1 on a low face, +1 on a high face).
+def _vanleer(x, y):
+    return 2 * x * y / (x + y) if x * y > 0 else 0.0
+
+
+def _minmod(x, y):
+    return 0.0 if x * y < 0 else (x if abs(x) <= abs(y) else y)
+
+
+def _inflow_state(mach, T, p, alpha, beta):
+    """Imposed primitive state: static p and T, speed mach * a along the direction of the
+    azimuth alpha and elevation beta (cos a cos b, sin a cos b, sin b)."""
+    rho = p / (RG * T)
+    a = np.sqrt(GAM * p / rho)
+    q = mach * a
+    return np.array([rho, q * np.cos(alpha) * np.cos(beta), q * np.sin(alpha) * np.cos(beta), q * np.sin(beta), p])
+
+
+@pytest.mark.parametrize("face", [1, 2])
+@pytest.mark.parametrize("lim", ["vanleer", "minmod"])
+@pytest.mark.parametrize("kind", ["linear", "curved", "relaxed"])
+def test_inflow405_face_state_is_half_a_limited_undivided_difference(face, lim, kind):
+    """kind linear: the cells lie on a line through the imposed state (one cell spacing
+    beyond the block), so the face state is the midpoint of the boundary cell and the
+    imposed state; curved: the limited slope is the limiter of the two unequal differences;
+    relaxed: rel_fac = 0.7 blends the imposed Mach number and pressure with the boundary
+    cell's (Lib_BC_Fluxes_Inflow.f90:300-303)."""
+    from oracle import oracle as O
+    n = 6
+    dims = (n, 1, 1)
+    mach, T, p = 0.55, 1.1, 0.8                 # subsonic: the face state reaches the flux
+    alpha = 0.3 if face == 1 else np.pi - 0.3  # into the block
+    beta = -0.2
+    rel = 0.7 if kind == "relaxed" else 1.0
+    sup0 = _inflow_state(mach, T, p, alpha, beta)
+    delta = np.array([0.12, 0.08, 0.04, -0.03, 0.1])
+    curv = np.zeros(5) if kind == "linear" else np.array([0.02, -0.015, 0.01, 0.012, 0.03])
+    # cell k (1..n) at distance s = k (low face) or n + 1 - k (high face) from the imposed state
+    s = np.arange(1, n + 1, dtype=float) if face == 1 else np.arange(n, 0, -1, dtype=float)
+    cells = sup0 + s[:, None] * delta + (s * (s - 1))[:, None] * curv
+    prim = cells.reshape(1, 1, n, 5)
+    faces = [0] * 6
+    faces[face - 1] = {"type": 405, "mach": mach, "T": T, "p": p, "alpha": alpha, "beta": beta, "rel_fac": rel}
+    nodes = cases.cartesian_nodes(dims, (0, 0, 0), (6.0, 0.5, 0.25))
+    area = 0.5 * 0.25
+    case = _case(dims, faces, prim, nodes=nodes, limiter=lim)
+    o = OracleSolver(case)
+    R = interior(o.residual(1)).reshape(n, 5)
+    o.close()
+    phi = _vanleer if lim == "vanleer" else _minmod
+    b, nb = (cells[0], cells[1]) if face == 1 else (cells[-1], cells[-2])
+    # relaxation towards the boundary cell's Mach number and pressure at the imposed T
+    Mb = np.linalg.norm(b[1:4]) / np.sqrt(GAM * b[4] / b[0])
+    sup = _inflow_state(rel * mach + (1 - rel) * Mb, T, rel * p + (1 - rel) * b[4], alpha, beta)
+    if rel == 1.0:
+        assert np.array_equal(sup, sup0)
+    nx = np.array([1.0, 0.0, 0.0])
+    if face == 1:    # stencil (imposed | cell 1 | cell 2): x = P3 - P2, y = P2 - P1
+        slope = np.array([phi(nb[v] - b[v], b[v] - sup[v]) for v in range(5)])
+        fs = b - 0.5 * slope
+        F = O.riemann(case.cfg, sup, fs, nx)
+        got = -R.sum(axis=0) / area
+    else:            # stencil (cell n-1 | cell n | imposed)
+        slope = np.array([phi(sup[v] - b[v], b[v] - nb[v]) for v in range(5)])
+        fs = b + 0.5 * slope
+        F = O.riemann(case.cfg, fs, sup, nx)
+        got = R.sum(axis=0) / area
+    assert np.allclose(got, F, rtol=1e-12, atol=1e-13)
+    if kind == "linear":   # the half-difference face state is the midpoint of cell and imposed state
+        assert np.allclose(fs, 0.5 * (b + sup), rtol=1e-14, atol=1e-15)
+        # the face state matters: the imposed state's own flux (a full-difference face state
+        # on linear data) is far from it
+        Fsup = O.riemann(case.cfg, sup, sup, nx)
+        assert np.max(np.abs(F - Fsup)) > 1e-3

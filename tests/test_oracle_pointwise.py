@@ -362,3 +362,135 @@ def test_prim_cons_round_trip():
         U = O.prim2cons(CFG, P)
         assert U[4] == pytest.approx(P[4] / 0.4 + 0.5 * P[0] * np.sum(P[1:4] ** 2), rel=1e-14)
         assert np.allclose(O.cons2prim(CFG, U), P, rtol=1e-13, atol=1e-14)
+
+
+# ---- closed forms of the splitting polynomials at subsonic states with ML != MR.  AUSM+
+# (Liou, J. Comput. Phys. 129, 1996) with the reference's interface sound speed
+# a_half = sqrt(aL aR) (Lib_Riemann_AUSM.f90:196):
+#   M+-(M) = +-(M +- 1)^2/4 +- beta (M^2 - 1)^2,  P+-(M) = (M +- 1)^2 (2 -+ M)/4 +- alpha M (M^2 - 1)^2,
+#   alpha = 3/16, beta = 1/8 (:160).  The expected values below are those polynomials evaluated
+# by hand at M = 0 and M = +-1/2 and reduced to rationals.
+AUSM_ALPHA, AUSM_BETA = 3.0 / 16.0, 1.0 / 8.0
+
+
+def test_ausmp_colliding_streams():
+    """uL = -uR = sqrt(aL aR)/2 along n, equal pressures, unequal densities (aL != aR): ML = 1/2,
+    MR = -1/2, so M+(1/2) + M-(-1/2) = 0 (no mass, momentum-advection or energy flux) and
+    p_half = p (P+(1/2) + P-(-1/2)) = p (27/16 + 9 alpha/16) = 459/256 p.  An arithmetic-mean
+    a_half moves ML off 1/2; alpha = 0 gives 27/16."""
+    cfg = abi.make_cfg(riemann=abi.RS_AUSMP)
+    rng = np.random.default_rng(21)
+    coef = 27.0 / 16.0 + 9.0 * AUSM_ALPHA / 16.0
+    assert coef == 459.0 / 256.0
+    for _ in range(100):
+        n = rand_normal(rng)
+        p = rng.uniform(0.3, 3.0)
+        rl, rr = rng.uniform(0.15, 0.5), rng.uniform(1.5, 4.0)
+        aL, aR = np.sqrt(G * p / rl), np.sqrt(G * p / rr)
+        un = 0.5 * np.sqrt(aL * aR)
+        tl, tr = np.cross(n, rand_normal(rng)), np.cross(n, rand_normal(rng))   # any tangential velocities
+        L = np.array([rl, *(un * n + tl), p])
+        R = np.array([rr, *(-un * n + tr), p])
+        F = O.riemann(cfg, L, R, n)
+        expect = np.array([0.0, *(coef * p * n), 0.0])
+        assert np.allclose(F, expect, rtol=1e-13, atol=1e-13 * p)
+
+
+def test_ausmp_left_at_rest_right_half_sonic():
+    """L at rest, R moving along n at MR = 1/2 (unR = a_half / 2): M_half = M+(0) + M-(1/2) =
+    (1/4 + beta) - (1/16 + 9 beta/16) = 3/16 + 7 beta/16 = 31/128 > 0, so the mass flux is
+    rhoL a_half 31/128 and carries the left state (zero velocity, total enthalpy
+    gamma pL / ((gamma - 1) rhoL)); p_half = P+(0) pL + P-(1/2) pR = pL/2 + (5/32 - 9 alpha/32) pR
+    = pL/2 + 53/512 pR."""
+    cfg = abi.make_cfg(riemann=abi.RS_AUSMP)
+    rng = np.random.default_rng(22)
+    mh = 3.0 / 16.0 + 7.0 * AUSM_BETA / 16.0
+    pr_w = 5.0 / 32.0 - 9.0 * AUSM_ALPHA / 32.0
+    assert (mh, pr_w) == (31.0 / 128.0, 53.0 / 512.0)
+    for _ in range(100):
+        n = rand_normal(rng)
+        rl, pl = rng.uniform(0.2, 3.0), rng.uniform(0.2, 3.0)
+        rr, pr = rng.uniform(0.2, 3.0), rng.uniform(0.2, 3.0)
+        aL, aR = np.sqrt(G * pl / rl), np.sqrt(G * pr / rr)
+        ah = np.sqrt(aL * aR)
+        tr = np.cross(n, rand_normal(rng))
+        L = np.array([rl, 0.0, 0.0, 0.0, pl])
+        R = np.array([rr, *(0.5 * ah * n + tr), pr])
+        F = O.riemann(cfg, L, R, n)
+        mdot = rl * ah * mh
+        ph = 0.5 * pl + pr_w * pr
+        expect = np.array([mdot, *(ph * n), mdot * G / (G - 1.0) * pl / rl])
+        assert np.allclose(F, expect, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("rs", [abi.RS_SLAU, abi.RS_SLAU2])
+def test_slau_pressure_jump_in_uniform_low_mach_flow(rs):
+    """Equal densities and one velocity vector on both sides, a pressure jump, |u| < aF =
+    (aL + aR)/2 (Shima & Kitamura, AIAA J. 49, 2011; Lib_Riemann_SLAU.f90:19-140): |Vn|bar = |un|
+    and f_rho = 0, so the mass flux is rho un - chi (pR - pL) / (2 aF) with chi = (1 - M~)^2,
+    M~ = |u| / aF.  With mL = mR = m the weights satisfy beta+(m) + beta-(m) = 1 (so SLAU's
+    chi term and SLAU2's sqrt term of p_half vanish) and beta+(m) - beta-(m) = (3m - m^3)/2,
+    so p_half = (pL + pR)/2 + (3m - m^3)(pL - pR)/4.  Momentum and energy carry the flux's
+    upwind side.  chi = 1 - M~ (unsquared) fails this."""
+    cfg = abi.make_cfg(riemann=rs)
+    rng = np.random.default_rng(23)
+    for _ in range(100):
+        n = rand_normal(rng)
+        rho = rng.uniform(0.3, 2.5)
+        pl, pr = rng.uniform(0.3, 3.0), rng.uniform(0.3, 3.0)
+        aL, aR = np.sqrt(G * pl / rho), np.sqrt(G * pr / rho)
+        aF = 0.5 * (aL + aR)
+        un = rng.uniform(-0.5, 0.5) * aF
+        t = np.cross(n, rand_normal(rng))
+        t *= rng.uniform(0.0, 0.6) * aF / max(np.linalg.norm(t), 1e-300)
+        vel = un * n + t
+        Mt = np.linalg.norm(vel) / aF
+        assert 0.0 < Mt < 1.0
+        chi = (1.0 - Mt) ** 2
+        m = un / aF
+        mass = rho * un - chi * (pr - pl) / (2.0 * aF)
+        ph = 0.5 * (pl + pr) + 0.25 * (3.0 * m - m ** 3) * (pl - pr)
+        pu = pl if mass >= 0.0 else pr
+        H = G / (G - 1.0) * pu / rho + 0.5 * vel @ vel
+        L = np.array([rho, *vel, pl])
+        R = np.array([rho, *vel, pr])
+        F = O.riemann(cfg, L, R, n)
+        expect = np.array([mass, *(mass * vel + ph * n), mass * H])
+        assert np.allclose(F, expect, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("rs", [abi.RS_SLAU, abi.RS_SLAU2])
+def test_slau_left_at_rest(rs):
+    """L at rest, R moving along n at mR = m = unR / aF in (0, 1): mL = 0 gives beta+ = 1/2 and
+    f_rho = 0, the R side's speed enters |Vn|bar = rhoR |unR| / (rhoL + rhoR) and M~ = m / sqrt(2);
+    beta-(m) = (2 + m)(m - 1)^2 / 4 != 1/2, so the extra pressure terms differ: SLAU's
+    (1 - chi)(beta- - 1/2)(pL + pR)/2 and SLAU2's sqrt((|uL|^2 + |uR|^2)/2) (beta- - 1/2) aF
+    (rhoL + rhoR)/2 (Kitamura & Shima, J. Comput. Phys. 245, 2013)."""
+    cfg = abi.make_cfg(riemann=rs)
+    rng = np.random.default_rng(24)
+    for _ in range(100):
+        n = rand_normal(rng)
+        rl, pl = rng.uniform(0.3, 2.5), rng.uniform(0.3, 3.0)
+        rr, pr = rng.uniform(0.3, 2.5), rng.uniform(0.3, 3.0)
+        aL, aR = np.sqrt(G * pl / rl), np.sqrt(G * pr / rr)
+        aF = 0.5 * (aL + aR)
+        m = rng.uniform(0.05, 0.95)
+        unr = m * aF
+        Mt = m / np.sqrt(2.0)
+        chi = (1.0 - Mt) ** 2
+        vn = rr * unr / (rl + rr)
+        mass = 0.5 * (rl * vn + rr * (unr - vn) - chi * (pr - pl) / aF)
+        bm = 0.25 * (2.0 + m) * (m - 1.0) ** 2
+        ph = 0.5 * (pl + pr) + 0.5 * (0.5 - bm) * (pl - pr)
+        if rs == abi.RS_SLAU:
+            ph += (1.0 - chi) * (bm - 0.5) * 0.5 * (pl + pr)
+        else:
+            ph += Mt * aF * (bm - 0.5) * aF * 0.5 * (rl + rr)
+        L = np.array([rl, 0.0, 0.0, 0.0, pl])
+        R = np.array([rr, *(unr * n), pr])
+        F = O.riemann(cfg, L, R, n)
+        if mass >= 0.0:
+            expect = [mass, *(ph * n), mass * G / (G - 1.0) * pl / rl]
+        else:
+            expect = [mass, *(mass * unr * n + ph * n), mass * (G / (G - 1.0) * pr / rr + 0.5 * unr * unr)]
+        assert np.allclose(F, expect, rtol=1e-12, atol=1e-13)
