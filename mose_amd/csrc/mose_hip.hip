@@ -924,11 +924,6 @@ static bool shape_compiled(const mose_ctx* c, int shape) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
   if (c->lay.curvi || c->lay.visc || c->cfg.muscl_sd) return shape == 0;
   if (shape < 0 || shape >= NSHAPE) return false;
-  // 512-thread patches only when MOSE_TILE_WIDE=1
-  if (kShape[p][shape][0] * kShape[p][shape][1] > 256) {
-    const char* e = getenv("MOSE_TILE_WIDE");
-    if (!e || atoi(e) != 1) return false;
-  }
   return ktable().f[p][c->cfg.limiter][c->cfg.riemann][shape][0] != nullptr;
 }
 
