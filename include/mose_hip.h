@@ -57,7 +57,8 @@ enum {
   MOSE_E_INTERFACE = 4,    /* non-physical reconstructed face state: Lib_Convective.f90:45-50  */
   MOSE_E_METRIC = 5,       /* det=0 or non-positive volume: Lib_Metrics.f90:286-288,353-356    */
   MOSE_E_BC = 6,           /* boundary list inconsistent with the blocks: IO_BC.f90:123-127    */
-  MOSE_E_NOGPU = 7         /* no HIP device                                                    */
+  MOSE_E_NOGPU = 7,        /* no HIP device                                                    */
+  MOSE_E_SIZE = 8          /* input beyond a fixed maximum (mose_get_state_noalloc)            */
 };
 
 typedef struct {
@@ -123,6 +124,14 @@ int  mose_finalize(mose_ctx* ctx);
 /* blk%P in / out (primitive, with ghost layers; ghosts are ignored on input). */
 int  mose_put_state(mose_ctx* ctx, int32_t b, const double* P);
 int  mose_get_state(mose_ctx* ctx, int32_t b, double* P);
+/* mose_get_state without any device allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight coupling loops,
+ * harnesses that forbid hipMalloc).  The conversion stages through a fixed 32 MiB device array
+ * per GPU instead of the buffer mose_get_state allocates on first use, so blocks of more than
+ * 838,860 cells counting the two ghost layers on every side ((ni+4)(nj+4)(nk+4)) are refused
+ * with MOSE_E_SIZE.  The array is shared by every context on a GPU: calls that overlap on one
+ * GPU race.  Call from the thread and device that ran mose_finalize (else MOSE_E_ARG). */
+int  mose_get_state_noalloc(mose_ctx* ctx, int32_t b, double* P);
 
 /* Compute_dt (Mod_dt.f90:10-107) preceded by dtglobal=1e5 (Mod_Explicit.f90:44), using
  * iteration number `iter` for the CFL ramp; writes the global minimum dt. */

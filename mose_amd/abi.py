@@ -13,7 +13,7 @@ LIM_MINMOD, LIM_VANLEER, LIM_VANALBADA, LIM_FIRST_ORDER, LIM_MC, LIM_SUPERBEE = 
 RS_HLLC, RS_AUSMP, RS_HLLE, RS_SLAU, RS_SLAU2, RS_HLLEPP, RS_HLLCP = 0, 1, 2, 3, 4, 5, 6
 PREC_FP64, PREC_FP32_FLUX = 0, 1
 
-E_OK, E_ARG, E_HIP, E_STATE, E_INTERFACE, E_METRIC, E_BC, E_NOGPU = range(8)
+E_OK, E_ARG, E_HIP, E_STATE, E_INTERFACE, E_METRIC, E_BC, E_NOGPU, E_SIZE = range(9)
 
 LIMITERS = {"minmod": LIM_MINMOD, "vanleer": LIM_VANLEER, "vanalbada": LIM_VANALBADA,
             "first-order": LIM_FIRST_ORDER, "mc": LIM_MC, "superbee": LIM_SUPERBEE}

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/mose_hip.h"
@@ -559,6 +560,18 @@ __global__ void k_copy(const double* __restrict__ s, double* __restrict__ d, int
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) d[c] = s[c];
 }
 
+// Staging area of mose_get_state_noalloc, which must not allocate: a file-scope device array of
+// kNoAllocDoubles doubles (32 MiB, blocks of up to 838,860 cells counting ghost layers).  Each
+// device holds its own copy at its own address; host code never passes the array's name to a
+// kernel or a copy (that is a host-side stand-in) but the address hipGetSymbolAddress gave for
+// the device, looked up once per device at mose_finalize.  Every call on a device shares that
+// device's array, so calls that overlap on one device race.
+constexpr int64_t kNoAllocDoubles = (int64_t)1 << 22;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
 // ============================================================================== host
 #include "mose_geometry.hpp"
 #include "mose_metrics_adj.hpp"
@@ -602,6 +615,8 @@ struct mose_ctx {
   int ndir = 3;
   // device
   hipStream_t stream = nullptr;
+  int dev = 0;                                // device current at mose_finalize
+  double* scratch_ptr = nullptr;              // that device's copy of `scratch`
   double* U[3] = {nullptr, nullptr, nullptr};
   double* Rbuf = nullptr;
   double* stage = nullptr;
@@ -909,6 +924,23 @@ static int upload(mose_ctx* c, V** dptr, const std::vector<V>& h) {
   const size_t n = std::max<size_t>(h.size(), 1);
   HIPCHK(c, hipMalloc((void**)dptr, n * sizeof(V)));
   if (!h.empty()) HIPCHK(c, hipMemcpy(*dptr, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Address of the current device's `scratch` (looked up once per device).
+static int setup_scratch(mose_ctx* c) {
+  int dev = 0;
+  HIPCHK(c, hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(c, MOSE_E_ARG);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(c, MOSE_E_HIP);
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  c->dev = dev;
+  c->scratch_ptr = g_scratch_ptr[dev];
   return 0;
 }
 
@@ -1272,6 +1304,7 @@ int mose_finalize(mose_ctx* c) {
   if (5 * c->lay.npool >= (int64_t)1 << 31) return fail(c, MOSE_E_ARG);   // 32-bit element offsets
   c->npool = c->lay.npool;
   c->ndir = c->lay.ndir;
+  if (setup_scratch(c)) return c->err.code;
   // device buffers
   for (int s = 0; s < 3; ++s) {
     HIPCHK(c, hipMalloc((void**)&c->U[s], 5 * c->npool * sizeof(double)));
@@ -1355,6 +1388,21 @@ int mose_get_state(mose_ctx* c, int32_t b, double* P) {
   hipLaunchKernelGGL(k_get, dim3(launch_blocks(ncell)), dim3(256), 0, c->stream, c->U[0], c->npool, B.off, ncell,
                      c->cfg.gamma, c->stage);
   HIPCHK(c, hipMemcpyAsync(P, c->stage, 5 * ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mose_get_state_noalloc(mose_ctx* c, int32_t b, double* P) {
+  if (!c || !c->finalized || b < 1 || b > c->nb || !P) return fail(c, MOSE_E_ARG, b);
+  const BlockDev& B = c->lay.blocks[b - 1];
+  const int64_t ncell = B.sz * (B.nk + 4);
+  if (5 * ncell > kNoAllocDoubles) return fail(c, MOSE_E_SIZE, b);
+  int dev = -1;
+  HIPCHK(c, hipGetDevice(&dev));
+  if (dev != c->dev || !c->scratch_ptr) return fail(c, MOSE_E_ARG, b);   // not the context's device
+  hipLaunchKernelGGL(k_get, dim3(launch_blocks(ncell)), dim3(256), 0, c->stream, c->U[0], c->npool, B.off, ncell,
+                     c->cfg.gamma, c->scratch_ptr);
+  HIPCHK(c, hipMemcpyAsync(P, c->scratch_ptr, 5 * ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1734,6 +1782,7 @@ const char* mose_strerror(int32_t code) {
     case MOSE_E_METRIC: return "metric tensor det=0 or non-positive volume";
     case MOSE_E_BC: return "boundary-condition list inconsistent or unsupported";
     case MOSE_E_NOGPU: return "no HIP device";
+    case MOSE_E_SIZE: return "block larger than the no-allocation staging area (838,860 cells with ghosts)";
     default: return "unknown error";
   }
 }

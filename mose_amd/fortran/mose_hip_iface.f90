@@ -19,7 +19,8 @@ module mose_hip
   integer(c_int32_t), parameter, public :: MOSE_PREC_FP64 = 0, MOSE_PREC_FP32_FLUX = 1
   integer(c_int32_t), parameter, public :: MOSE_OK = 0, MOSE_E_ARG = 1, MOSE_E_HIP = 2, &
                                            MOSE_E_STATE = 3, MOSE_E_INTERFACE = 4, &
-                                           MOSE_E_METRIC = 5, MOSE_E_BC = 6, MOSE_E_NOGPU = 7
+                                           MOSE_E_METRIC = 5, MOSE_E_BC = 6, MOSE_E_NOGPU = 7, &
+                                           MOSE_E_SIZE = 8
 
   type, bind(C), public :: mose_cfg
     real(c_double)     :: gamma = 1.4_c_double
@@ -50,7 +51,7 @@ module mose_hip
   end type mose_err
 
   public :: mose_create, mose_add_block, mose_set_viscous, mose_set_bc, mose_finalize, mose_put_state, &
-            mose_get_state, mose_compute_dt, mose_step, mose_advance, mose_residual, &
+            mose_get_state, mose_get_state_noalloc, mose_compute_dt, mose_step, mose_advance, mose_residual, &
             mose_put_residual_bar, mose_residual_vjp, mose_residual_vjp_geometry, mose_get_viscous_bar, &
             mose_get_state_bar, mose_get_node_bar, mose_get_inflow_bar, mose_time_residual_vjp, &
             mose_time_kernels, mose_autotune, mose_get_tiling, &
@@ -92,6 +93,12 @@ module mose_hip
       real(c_double), intent(in) :: P(*)
     end function
     integer(c_int) function mose_get_state(ctx, b, P) bind(C, name="mose_get_state")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value            :: ctx
+      integer(c_int32_t), value     :: b
+      real(c_double), intent(inout) :: P(*)
+    end function
+    integer(c_int) function mose_get_state_noalloc(ctx, b, P) bind(C, name="mose_get_state_noalloc")
       import :: c_int, c_ptr, c_int32_t, c_double
       type(c_ptr), value            :: ctx
       integer(c_int32_t), value     :: b

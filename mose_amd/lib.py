@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get("MOSE_HIP_LIB") or os.path.join(_HERE, "libmose_hip.so
 _LIB = None
 
 EXPORTS = ["mose_create", "mose_add_block", "mose_set_viscous", "mose_set_bc", "mose_finalize", "mose_put_state",
-           "mose_get_state", "mose_compute_dt", "mose_step", "mose_advance", "mose_residual",
+           "mose_get_state", "mose_get_state_noalloc", "mose_compute_dt", "mose_step", "mose_advance", "mose_residual",
            "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_get_tiling", "mose_put_residual_bar",
            "mose_residual_vjp", "mose_get_state_bar", "mose_residual_vjp_geometry", "mose_get_viscous_bar", "mose_get_node_bar",
            "mose_get_inflow_bar", "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
@@ -42,6 +42,7 @@ def load():
     L.mose_finalize.argtypes = [v]
     L.mose_put_state.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_get_state.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_get_state_noalloc.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_compute_dt.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_step.argtypes = [v, P(C.c_double), P(C.c_double), P(C.c_double)]
     L.mose_advance.argtypes = [v, C.c_int32, P(C.c_double), P(C.c_double)]
@@ -135,6 +136,12 @@ class GpuSolver:
     def get_state(self, b):
         P = np.zeros(abi.padded_shape(self.dims[b - 1]))
         self._check(load().mose_get_state(self.h, b, _dp(P)))
+        return P
+
+    def get_state_noalloc(self, b):
+        """mose_get_state through the fixed device staging array (no device allocation)."""
+        P = np.zeros(abi.padded_shape(self.dims[b - 1]))
+        self._check(load().mose_get_state_noalloc(self.h, b, _dp(P)))
         return P
 
     def residual(self, b):
