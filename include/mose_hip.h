@@ -212,6 +212,9 @@ int  mose_time_kernels(mose_ctx* ctx, int32_t nrep, double ms[4]);
  * summed over the three launches.  No counterpart in the reference (its loops have fixed
  * OpenMP schedules). */
 int  mose_autotune(mose_ctx* ctx, int32_t nrep, int32_t choice[3], double* best_ms);
+/* choice[2] bits: 1 = XCD-aware tile order (tiles of one contiguous range per XCD queue),
+ * 2 = one workgroup per tile instead of the persistent launch (as many workgroups as the
+ * device holds, taking tiles from the queues). */
 
 /* The tiling in use, in mose_autotune's encoding {TX*1000+TY, k-chunk, xcd order}.  No
  * counterpart in the reference. */

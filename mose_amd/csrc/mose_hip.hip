@@ -667,10 +667,13 @@ struct mose_ctx {
   std::vector<std::vector<double>> xbar;
   bool dt_primed = false;   // d_dtmin holds the CFL step of the current state
   int grid_stage = 0;
+  int* d_tq = nullptr;                        // tile queues of the persistent stage launch
+  int ncu = 0;                                // compute units of the device
+  std::map<const void*, int> resident;        // resident workgroups per CU of each stage kernel
   int shape = 0;       // index into kShape of the stage kernel in use
   int tx = 16, ty = 8;
   int kchunk = 32;     // k-planes per tile
-  int xcd = 1;         // XCD-aware tile order
+  int xcd = 1;         // bit 0: XCD-aware tile order; bit 1: one workgroup per tile (else persistent)
 };
 
 static int fail(mose_ctx* c, int code, int b = 0, int i = 0, int j = 0, int k = 0) {
@@ -723,13 +726,29 @@ static StageArgs stage_args(mose_ctx* c, const double* Uin, const double* Un, do
   a.dtmin = c->d_dtmin; a.dtnext = c->d_dtloc; a.iter = c->d_iter;
   a.ndir = c->ndir; a.rampa = c->cfg.rampa_cfl_iter; a.cfl = c->cfg.cfl;
   a.npool = c->npool; a.ntiles = (int)c->lay.tiles.size(); a.local_dt = c->cfg.time_accurate ? 0 : 1;
+  a.tq = c->d_tq; a.nq = (c->xcd & 1) ? kTileQueues : 1;
   a.rkc = rkc; a.gam = c->cfg.gamma; a.R = c->cfg.Rgas;
   return a;
 }
 
+// Persistent stage launch: as many workgroups as the device holds at once (never more than
+// there are tiles); they take tiles from the queues (claim_tile) until all are done.
+static int resident_per_cu(mose_ctx* c, StageFn fn) {
+  auto it = c->resident.find((const void*)fn);
+  if (it != c->resident.end()) return it->second;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)fn, c->tx * c->ty, 0) != hipSuccess)
+    per_cu = 0;
+  c->resident[(const void*)fn] = per_cu;
+  return per_cu;
+}
+
 static void launch_stage(mose_ctx* c, StageFn fn, StageArgs a) {
   void* args[] = {&a};
-  (void)hipLaunchKernel((const void*)fn, dim3(c->grid_stage), dim3(c->tx * c->ty), args, 0, c->stream);
+  const int per_cu = resident_per_cu(c, fn);
+  const int64_t cap = (int64_t)std::max(per_cu, 1) * std::max(c->ncu, 1);
+  const int grid = (c->xcd & 2) ? c->grid_stage : (int)std::min<int64_t>(c->grid_stage, cap);
+  (void)hipLaunchKernel((const void*)fn, dim3(std::max(grid, 1)), dim3(c->tx * c->ty), args, 0, c->stream);
 }
 
 // Boundary-face fluxes of the stage input (the BC_Fluxes pass), before the stage kernel.
@@ -990,7 +1009,7 @@ static int retile(mose_ctx* c, int shape, int kchunk, int xcd) {
   c->kchunk = kchunk;
   c->xcd = xcd;
   const size_t old = c->lay.tiles.size();
-  build_tiles(c->lay, c->tx, c->ty, kchunk, xcd);
+  build_tiles(c->lay, c->tx, c->ty, kchunk, xcd & 1);
   const size_t nt = c->lay.tiles.size();
   if (c->d_tiles) HIPCHK(c, hipFree(c->d_tiles));
   c->d_tiles = nullptr;
@@ -1000,6 +1019,8 @@ static int retile(mose_ctx* c, int shape, int kchunk, int xcd) {
     HIPCHK(c, hipMalloc((void**)&c->d_partials, 5 * std::max<size_t>(nt, 1) * sizeof(double)));
   }
   c->grid_stage = (int)nt;
+  for (int m = 0; m < 3; ++m)     // occupancy queried here, never while a graph is captured
+    if (StageFn fn = stage_fn(c, m)) (void)resident_per_cu(c, fn);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   c->gexec = nullptr;
@@ -1326,6 +1347,13 @@ int mose_finalize(mose_ctx* c) {
   HIPCHK(c, hipMalloc((void**)&c->d_scal, S_COUNT * sizeof(double)));
   HIPCHK(c, hipMemset(c->d_scal, 0, S_COUNT * sizeof(double)));
   HIPCHK(c, hipMalloc((void**)&c->d_dtmin, sizeof(unsigned long long)));
+  HIPCHK(c, hipMalloc((void**)&c->d_tq, (kTileQueues + 1) * sizeof(int)));
+  HIPCHK(c, hipMemset(c->d_tq, 0, (kTileQueues + 1) * sizeof(int)));
+  {
+    hipDeviceProp_t prop{};
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->dev));
+    c->ncu = prop.multiProcessorCount;
+  }
   HIPCHK(c, hipMalloc((void**)&c->d_err, 8 * sizeof(int)));
   HIPCHK(c, hipMemset(c->d_err, 0, 8 * sizeof(int)));
   HIPCHK(c, hipMalloc((void**)&c->d_iter, sizeof(int)));
@@ -1355,12 +1383,13 @@ int mose_finalize(mose_ctx* c) {
     hipLaunchKernelGGL(k_fill, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, c->d_beta, c->npool, 1.0);
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
-  // tiling: default shape 0, 32-plane chunks, XCD-aware order; MOSE_TILE_SHAPE,
-  // MOSE_KCHUNK and MOSE_TILE_XCD override (mose_autotune picks by measurement)
+  // tiling: default shape 0, 32-plane chunks, XCD-aware order, persistent launch;
+  // MOSE_TILE_SHAPE, MOSE_KCHUNK and MOSE_TILE_XCD (bit 0 order, bit 1 one workgroup per
+  // tile) override (mose_autotune picks by measurement)
   int shape = 0, kc = 32, xcd = 1;
   if (const char* e = getenv("MOSE_TILE_SHAPE")) shape = shape_compiled(c, atoi(e)) ? atoi(e) : 0;
   if (const char* e = getenv("MOSE_KCHUNK")) kc = std::max(1, atoi(e));
-  if (const char* e = getenv("MOSE_TILE_XCD")) xcd = atoi(e) ? 1 : 0;
+  if (const char* e = getenv("MOSE_TILE_XCD")) xcd = atoi(e) & 3;
   if (retile(c, shape, kc, xcd)) return c->err.code;
   c->finalized = true;
   return 0;
@@ -1527,7 +1556,7 @@ int mose_autotune(mose_ctx* c, int32_t nrep, int32_t choice[3], double* best_ms)
   for (int shape = 0; shape < NSHAPE; ++shape) {
     if (!shape_compiled(c, shape)) continue;
     for (int kc : kcs)
-      for (int xcd = 0; xcd < 2; ++xcd) {
+      for (int xcd = 0; xcd < 4; ++xcd) {
         if (retile(c, shape, kc, xcd)) return c->err.code;
         double t = 0.0;
         for (int pass = 0; pass < 2; ++pass) {   // pass 0 warms up
@@ -1794,7 +1823,7 @@ void mose_destroy(mose_ctx* c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_geo, c->d_tiles,
                   c->d_ghost, c->d_bcfaces, c->d_vfaces, c->d_vbuf, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
-                  c->d_iter, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
+                  c->d_iter, c->d_tq, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
                   c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_pres, c->d_ghostp,
                   c->d_garc_cell, c->d_garc, c->d_rs1, c->d_irsg};
   for (void* p : bufs)

@@ -118,13 +118,44 @@ struct StageArgs {
   unsigned long long* dtmin;   // last stage: CFL time step of the next step (min as bits)
   double* dtnext;              // last stage, local time stepping: dtlocal of the next step
   const int* iter;
+  int* tq;                     // tile queues of the persistent stage launch (kTileQueues claim
+                               // counters and a finished-workgroup counter), zero between launches
   int64_t npool;
   int ntiles;
+  int nq;                      // queues in use: kTileQueues with the XCD-aware tile order, else 1
   int local_dt;
   int ndir, rampa;
   int irs_scale;               // MODE 2: write R* = -R dt / vol (residual smoothing) instead of R
   double rkc, gam, R, cfl;
 };
+
+// Persistent stage launch: the grid is the number of workgroups resident on the device and each
+// workgroup claims tiles from a queue until every queue is empty.  Queue x (x < nq) holds the tile
+// positions x, x + nq, x + 2 nq, ...: with the XCD-aware order those are the tiles of one
+// contiguous range of blocks, and workgroup w (dispatched to XCD w mod 8) starts on queue w mod nq,
+// so neighbouring tiles share an L2.  A workgroup whose queue is empty takes tiles from the next
+// queues.  Every claim is one atomicAdd, so each tile is taken exactly once; the last workgroup to
+// finish sets the counters back to zero for the next launch.
+constexpr int kTileQueues = 8;
+
+__device__ __forceinline__ int claim_tile(int* tq, int nq, int ntiles) {
+  const int x0 = (int)(blockIdx.x % (unsigned)nq);
+  for (int s = 0; s < nq; ++s) {
+    const int x = x0 + s < nq ? x0 + s : x0 + s - nq;
+    const int len = (ntiles - x + nq - 1) / nq;
+    if (__hip_atomic_load(tq + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len) continue;
+    const int q = atomicAdd(tq + x, 1);
+    if (q < len) return x + q * nq;
+  }
+  return -1;
+}
+
+// End of a workgroup's persistent loop (thread 0): the last workgroup resets the queues.
+__device__ __forceinline__ void finish_tiles(int* tq) {
+  if (atomicAdd(tq + kTileQueues, 1) == (int)gridDim.x - 1) {
+    for (int x = 0; x <= kTileQueues; ++x) atomicExch(tq + x, 0);
+  }
+}
 
 // First error of a launch wins (code, block, i, j, k); later ones are dropped.
 __device__ __forceinline__ void set_error(int* err, int code, int b, int i, int j, int k) {

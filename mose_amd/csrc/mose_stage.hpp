@@ -358,7 +358,10 @@ __global__ void __launch_bounds__(TXc* TYc)
   else if (tid < 2 * TYc + TXc) { xd = 1; xlx = tid - 2 * TYc; xly = -1; xplus = true; }
   else if (tid < NX) { xd = 1; xlx = tid - 2 * TYc - TXc; xly = TYc; }
 
-  for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+  __shared__ int sTile;
+  if (tid == 0) sTile = claim_tile(a.tq, a.nq, a.ntiles);
+  __syncthreads();
+  for (int t = sTile; t >= 0;) {
     const Tile T = a.tiles[t];
     const BlockDev* __restrict__ Bp = a.blocks + T.b;
     const int ni = Bp->ni, nj = Bp->nj, nk = Bp->nk, sy_ = Bp->sy;
@@ -788,7 +791,12 @@ __global__ void __launch_bounds__(TXc* TYc)
         atomicMin(a.dtmin, (unsigned long long)__double_as_longlong(dtmin_t));
       __syncthreads();
     }
+    // every thread has read sTile before the barriers above; claim the next tile
+    if (tid == 0) sTile = claim_tile(a.tq, a.nq, a.ntiles);
+    __syncthreads();
+    t = sTile;
   }
+  if (tid == 0) finish_tiles(a.tq);
 }
 
 }  // namespace mose

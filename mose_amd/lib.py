@@ -244,7 +244,8 @@ class GpuSolver:
         """The tiling in use: {"tile": (TX, TY), "kchunk": kc, "xcd_order": bool}."""
         ch = np.zeros(3, dtype=np.int32)
         self._check(load().mose_get_tiling(self.h, ch.ctypes.data_as(C.POINTER(C.c_int32))))
-        return {"tile": (int(ch[0]) // 1000, int(ch[0]) % 1000), "kchunk": int(ch[1]), "xcd_order": bool(ch[2])}
+        return {"tile": (int(ch[0]) // 1000, int(ch[0]) % 1000), "kchunk": int(ch[1]), "xcd_order": bool(ch[2] & 1),
+                "persistent": not bool(ch[2] & 2)}
 
     def autotune(self, nrep=3):
         """Pick the fastest tile shape / k chunk / tile order by timing; returns
@@ -253,7 +254,7 @@ class GpuSolver:
         ms = np.zeros(1)
         self._check(load().mose_autotune(self.h, nrep, ch.ctypes.data_as(C.POINTER(C.c_int32)), _dp(ms)))
         return {"tile": (int(ch[0]) // 1000, int(ch[0]) % 1000), "kchunk": int(ch[1]),
-                "xcd_order": bool(ch[2])}, float(ms[0])
+                "xcd_order": bool(ch[2] & 1), "persistent": not bool(ch[2] & 2)}, float(ms[0])
 
     def num_cells(self):
         return load().mose_num_cells(self.h)

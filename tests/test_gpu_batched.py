@@ -71,3 +71,27 @@ def test_fp64_multiblock_against_oracle():
     print("FP64 vs oracle, %d blocks, 20 steps: rel L2 %.3e" % (NB, err))
     assert err < 1e-12, err
     np.testing.assert_allclose(ng, no, rtol=1e-8, atol=1e-12 * np.max(no))
+
+
+@pytest.mark.parametrize("prec", [abi.PREC_FP64, abi.PREC_FP32_FLUX])
+def test_persistent_launch_equals_one_workgroup_per_tile(monkeypatch, prec):
+    """The persistent stage launch (as many workgroups as the device holds, taking tiles from
+    the per-XCD queues, stealing from the others when theirs is empty) covers every tile of the
+    uneven blocks exactly once: the state and the norms are bit-identical to the launch with
+    one workgroup per tile, with and without the XCD-aware order, over several steps (the
+    norms, summed per tile in tile order, are bit-identical between launches of the same order)."""
+    c = _chain(prec)
+    runs = []
+    for mode in (1, 3, 0, 2):     # bit 0 XCD order, bit 1 one workgroup per tile
+        monkeypatch.setenv("MOSE_TILE_XCD", str(mode))
+        g = GpuSolver(c)
+        assert g.tiling()["persistent"] == (mode < 2) and g.tiling()["xcd_order"] == bool(mode & 1)
+        norms = [g.step()[2] for _ in range(4)]
+        runs.append(([g.get_state(b) for b in range(1, NB + 1)], norms))
+        g.close()
+    for states, _ in runs[1:]:
+        for a, r in zip(states, runs[0][0]):
+            assert np.array_equal(a.view(np.uint64), r.view(np.uint64))
+    nrm = [np.asarray(n) for _, n in runs]
+    assert np.array_equal(nrm[1], nrm[0]) and np.array_equal(nrm[3], nrm[2])   # same tile order
+    np.testing.assert_allclose(nrm[2], nrm[0], rtol=1e-13)                     # other order
