@@ -188,6 +188,26 @@ int  mose_residual_vjp_geometry(mose_ctx* ctx);
 int  mose_get_viscous_bar(mose_ctx* ctx, int32_t b, double* mu_ref_bar);
 int  mose_get_node_bar(mose_ctx* ctx, int32_t b, double* Xbar);
 int  mose_get_inflow_bar(mose_ctx* ctx, int32_t b, int32_t f, double out[5]);
+/* Steady adjoint solve (no counterpart in the reference, which has no adjoint; it serves
+ * shape and boundary-value gradients of a steady objective J(U*, X) at a converged state U*
+ * of the local-time-step iteration, Mod_Explicit.f90:45-85):
+ *   mose_put_objective_bar(b, gP): dJ/dP of block b in the blk%P layout (ghost entries
+ *     ignored);
+ *   mose_adjoint_solve: solves (dR/dP)^T psi = dJ/dP on the device by the transpose of the
+ *     primal stages (same SSP-RK coefficients, the local dt of the current state, no CFL
+ *     ramp): lambda <- (1-c) lambda^n + c (lambda + (dP/dU)^T (dJ/dP - (dR/dP)^T D lambda)),
+ *     D = dt/vol per cell, psi = D lambda; it converges whenever the primal iteration does.
+ *     It stops after max_iter iterations or when |Q| <= rtol |Q_0| (checked every
+ *     check_every iterations; Q the conserved-variable adjoint residual); iters and
+ *     res = {|Q_0|, |Q|} report it;
+ *   mose_get_residual_bar(b, psi): psi in the blk%R layout (ghost entries 0).
+ * Afterwards psi is the residual adjoint in place: mose_residual_vjp_geometry (without
+ * mose_put_residual_bar) gives psi^T dR/dX and psi^T dR/d(inflow), and
+ * dJ/dX = dJ/dX|_U - psi^T dR/dX. */
+int  mose_put_objective_bar(mose_ctx* ctx, int32_t b, const double* gP);
+int  mose_adjoint_solve(mose_ctx* ctx, int32_t max_iter, double rtol, int32_t check_every, int32_t* iters,
+                        double res[2]);
+int  mose_get_residual_bar(mose_ctx* ctx, int32_t b, double* psi);
 /* Average device ms of one residual evaluation (ms[0]), one adjoint evaluation (ms[1]) and,
  * with general metrics, one geometry adjoint evaluation (ms[2], device passes only; 0 on
  * rectilinear layouts) over nrep back-to-back runs on the current state and residual

@@ -339,12 +339,18 @@ def split_block_case(case, axis=0, parts=2):
                 meta=dict(case.meta))
 
 
+def bump_control_shape(xs, cx, width=0.03):
+    """Wall displacement of one bump control point of unit height at centre cx."""
+    return np.exp(-((xs - cx) / width) ** 2) * ((xs >= 0) & (xs <= 1))
+
+
 def bump_channel(ni=512, nj=256, height=0.04, mach=2.0, seed=7, ncontrol=64, limiter="vanalbada",
-                 riemann="HLLC", cfl=0.5, heights=None):
+                 riemann="HLLC", cfl=0.5, heights=None, alpha=0.0):
     """2-D supersonic channel (nk=1) over x in [-1.5, 2.5], y in [0, 2], with a circular-arc
     bump on 0<=x<=1 of height `height` (x bump length 1), body-fitted grid.  The wall shape
     is the arc plus a smooth perturbation parametrised by `ncontrol` control-point heights
-    (Gaussian bumps, seeded centres); faces: i-min 405 (Mach 2 static p=1/gamma, T=1),
+    (Gaussian bumps, seeded centres); faces: i-min 405 (Mach 2 static p=1/gamma, T=1, flow
+    angle alpha from the x axis),
     i-max 400, j-min / j-max 300, k faces 0; local time stepping (steady)."""
     rng = np.random.default_rng(seed)
     dims = (ni, nj, 1)
@@ -356,7 +362,7 @@ def bump_channel(ni=512, nj=256, height=0.04, mach=2.0, seed=7, ncontrol=64, lim
     yb = np.where((xs >= 0) & (xs <= 1), np.sqrt(np.maximum(r * r - (xs - 0.5) ** 2, 0.0)) - (r - height), 0.0)
     yb = np.maximum(yb, 0.0)
     for cx, hh in zip(ctrl_x, heights):
-        yb = yb + hh * np.exp(-((xs - cx) / 0.03) ** 2) * ((xs >= 0) & (xs <= 1))
+        yb = yb + hh * bump_control_shape(xs, cx)
     H = 2.0
     eta = np.linspace(0.0, 1.0, nj + 1)
     X = np.repeat(xs[None, :], nj + 1, axis=0)
@@ -372,11 +378,12 @@ def bump_channel(ni=512, nj=256, height=0.04, mach=2.0, seed=7, ncontrol=64, lim
     prim[..., 1] = mach * a_in
     prim[..., 4] = 1.0 / GAMMA
     blk = Block(dims, np.ascontiguousarray(nodes), padded_state(dims, prim))
-    inflow = {"type": 405, "mach": mach, "T": 1.0, "p": 1.0 / GAMMA, "alpha": 0.0, "beta": 0.0,
+    inflow = {"type": 405, "mach": mach, "T": 1.0, "p": 1.0 / GAMMA, "alpha": alpha, "beta": 0.0,
               "rel_fac": 1.0}
     bc = bc_records([{"dims": dims, "faces": [inflow, 400, 300, 300, 0, 0]}])
     cfg = abi.make_cfg(GAMMA, RGAS_NONDIM, cfl, limiter, riemann, 3, False)
-    return Case("bump", cfg, [blk], bc, meta={"ctrl_x": ctrl_x, "heights": heights})
+    return Case("bump", cfg, [blk], bc, meta={"ctrl_x": ctrl_x, "heights": np.asarray(heights, dtype=float),
+                                              "xs": xs, "eta": eta, "H": H, "width": 0.03})
 
 
 def taylor_green(n=192, nbox=8, mach=0.1, seed=11, limiter="vanleer", riemann="AUSM+", cfl=0.5,

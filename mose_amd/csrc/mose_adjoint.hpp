@@ -12,7 +12,10 @@ namespace mose {
 template <int LIM>
 __device__ __forceinline__ void limiter_adj(double x, double y, double gb, double& xb, double& yb) {
   if constexpr (LIM == LIM_MINMOD) {
-    if (x * y < 0.0) return;
+    // where a slope is exactly zero (uniform regions) minmod clips perturbations of either
+    // sign of the other argument to first order; the adjoint takes the zero subgradient there,
+    // as the van Leer and van Albada adjoints do
+    if (!(x * y > 0.0)) return;
     if (fabs(x) > fabs(y)) yb += gb;
     else xb += gb;
   } else if constexpr (LIM == LIM_VANLEER) {

@@ -33,6 +33,7 @@ using namespace mose;
 #include "mose_stage.hpp"
 #include "mose_adjoint.hpp"
 #include "mose_adjoint_kernels.hpp"
+#include "mose_adjsolve.hpp"
 #include "mose_tables.hpp"
 
 struct DtArgs {
@@ -651,6 +652,13 @@ struct mose_ctx {
   hipGraphExec_t gexec = nullptr;
   // discrete adjoint (allocated on first use)
   bool adj_ready = false;
+  // steady adjoint solve (allocated on first use)
+  double *d_gP = nullptr, *d_lam0 = nullptr, *d_lam1 = nullptr, *d_Dq = nullptr, *d_apart = nullptr;
+  double* d_ares = nullptr;
+  int64_t* d_icell = nullptr;                // interior pool cells
+  int64_t nicell = 0;
+  hipGraph_t agraph = nullptr;               // one pseudo-time iteration of the adjoint solve
+  hipGraphExec_t agexec = nullptr;
   double *d_Rbar = nullptr, *d_W = nullptr, *d_BB = nullptr, *d_Pbar = nullptr, *d_Pp = nullptr;
   AdjSeg* d_segs = nullptr;
   AdjGhostRow* d_grows = nullptr;
@@ -1025,6 +1033,10 @@ static int retile(mose_ctx* c, int shape, int kchunk, int xcd) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   c->gexec = nullptr;
   c->graph = nullptr;
+  if (c->agexec) (void)hipGraphExecDestroy(c->agexec);
+  if (c->agraph) (void)hipGraphDestroy(c->agraph);
+  c->agexec = nullptr;
+  c->agraph = nullptr;
   c->dt_primed = false;
   return 0;
 }
@@ -1757,6 +1769,151 @@ int mose_get_state_bar(mose_ctx* c, int32_t b, double* Pbar) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------- steady adjoint
+constexpr int kAdjParts = 512;   // workgroups of k_adjsolve_rk (fixed: the norm's sum order)
+
+static int ensure_adjsolve(mose_ctx* c) {
+  if (c->d_icell) return 0;
+  std::vector<int64_t> cells;
+  for (const BlockDev& B : c->lay.blocks)
+    for (int k = 1; k <= B.nk; ++k)
+      for (int j = 1; j <= B.nj; ++j)
+        for (int i = 1; i <= B.ni; ++i) cells.push_back(B.off + (i + 1) + (int64_t)B.sy * (j + 1) + B.sz * (k + 1));
+  c->nicell = (int64_t)cells.size();
+  if (upload(c, &c->d_icell, cells)) return c->err.code;
+  const size_t n5 = 5 * (size_t)c->npool;
+  for (double** p : {&c->d_gP, &c->d_lam0, &c->d_lam1}) {
+    HIPCHK(c, hipMalloc((void**)p, n5 * sizeof(double)));
+    HIPCHK(c, hipMemset(*p, 0, n5 * sizeof(double)));
+  }
+  HIPCHK(c, hipMalloc((void**)&c->d_Dq, std::max<int64_t>(c->nicell, 1) * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_apart, kAdjParts * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_ares, sizeof(double)));
+  return 0;
+}
+
+int mose_put_objective_bar(mose_ctx* c, int32_t b, const double* gP) {
+  if (!c || !c->finalized || b < 1 || b > c->nb || !gP) return fail(c, MOSE_E_ARG, b);
+  if (ensure_adjoint(c) || ensure_adjsolve(c)) return c->err.code;
+  const BlockDev& B = c->lay.blocks[b - 1];
+  const int64_t ncell = B.sz * (B.nk + 4);
+  if (ensure_stage(c, 5 * ncell)) return c->err.code;
+  HIPCHK(c, hipMemcpyAsync(c->stage, gP, 5 * ncell * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_put_raw, dim3(launch_blocks(ncell)), dim3(256), 0, c->stream, c->stage, c->d_gP, c->npool,
+                     B.off, ncell);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// D = dt / vol of every interior cell at the current state (local dt, or the global step of
+// a time-accurate context), without the CFL ramp.
+static int adjsolve_dt(mose_ctx* c) {
+  hipLaunchKernelGGL(k_dtreset, dim3(1), dim3(1), 0, c->stream, c->d_dtmin);
+  launch_dt(c, dt_args(c, c->cfg.rampa_cfl_iter + 1));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int64_t np = c->npool;
+  std::vector<double> dt;
+  double dtg = 0.0;
+  if (c->cfg.time_accurate) {
+    unsigned long long bits = 0;
+    HIPCHK(c, hipMemcpy(&bits, c->d_dtmin, sizeof bits, hipMemcpyDeviceToHost));
+    memcpy(&dtg, &bits, sizeof dtg);
+  } else {
+    dt.resize((size_t)np);
+    HIPCHK(c, hipMemcpy(dt.data(), c->d_dtloc, np * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  std::vector<double> vol;
+  if (c->lay.curvi) {
+    vol.resize((size_t)np);
+    HIPCHK(c, hipMemcpy(vol.data(), c->d_geo + (size_t)GVOL * np, np * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  std::vector<double> D;
+  D.reserve((size_t)c->nicell);
+  for (const BlockDev& B : c->lay.blocks) {
+    const LineEnt* Lx = c->lay.lines.data() + B.line[0];
+    const LineEnt* Ly = c->lay.lines.data() + B.line[1];
+    const LineEnt* Lz = c->lay.lines.data() + B.line[2];
+    for (int k = 1; k <= B.nk; ++k)
+      for (int j = 1; j <= B.nj; ++j)
+        for (int i = 1; i <= B.ni; ++i) {
+          const int64_t cc = B.off + (i + 1) + (int64_t)B.sy * (j + 1) + B.sz * (k + 1);
+          const double v = c->lay.curvi ? vol[(size_t)cc] : Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta;
+          D.push_back((c->cfg.time_accurate ? dtg : dt[(size_t)cc]) / v);
+        }
+  }
+  HIPCHK(c, hipMemcpy(c->d_Dq, D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// One pseudo-time iteration: n_rk stages, each a residual VJP of D lambda then the stage
+// update (k_adjsolve_rk); the first stage also forms |Q|^2 of the iteration's start.
+static void enqueue_adjsolve_iter(mose_ctx* c) {
+  const int nrk = c->cfg.n_rk;
+  const double cs3[3] = {1.0, 0.25, 2.0 / 3.0}, cs2[2] = {1.0, 0.5}, cs1[1] = {1.0};
+  const double* cs = nrk == 3 ? cs3 : (nrk == 2 ? cs2 : cs1);
+  const int grid = (int)std::min<int64_t>(kAdjParts, std::max<int64_t>((c->nicell + 255) / 256, 1));
+  for (int s = 0; s < nrk; ++s) {
+    enqueue_vjp(c);
+    const double* lin = s == 0 ? c->d_lam0 : c->d_lam1;
+    double* lout = s == nrk - 1 ? c->d_lam0 : c->d_lam1;
+    hipLaunchKernelGGL(k_adjsolve_rk, dim3(grid), dim3(256), 0, c->stream, c->d_icell, c->d_Dq, c->nicell, c->npool,
+                       c->U[0], c->d_gP, c->d_Pbar, c->d_lam0, lin, lout, c->d_Rbar, 1.0 - cs[s], cs[s],
+                       c->cfg.gamma, s == 0 ? c->d_apart : nullptr);
+    if (s == 0) hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, c->stream, c->d_apart, grid, c->d_ares);
+  }
+}
+
+int mose_adjoint_solve(mose_ctx* c, int32_t max_iter, double rtol, int32_t check_every, int32_t* iters,
+                       double res[2]) {
+  if (!c || !c->finalized || max_iter < 1 || check_every < 1 || !(rtol >= 0.0)) return fail(c, MOSE_E_ARG);
+  if (c->err.code) return c->err.code;
+  if (ensure_adjoint(c) || ensure_adjsolve(c)) return c->err.code;
+  if (adjsolve_dt(c)) return c->err.code;
+  const int64_t n5 = 5 * c->npool;
+  for (double* p : {c->d_lam0, c->d_lam1, c->d_Rbar})
+    hipLaunchKernelGGL(k_fill, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, p, n5, 0.0);
+  if (!c->agexec) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_adjsolve_iter(c);
+    HIPCHK(c, hipStreamEndCapture(c->stream, &c->agraph));
+    HIPCHK(c, hipGraphInstantiate(&c->agexec, c->agraph, nullptr, nullptr, 0));
+  }
+  c->vjp_valid = c->geo_valid = c->xbar_valid = false;
+  double r0 = 0.0, r = 0.0;
+  int it = 0;
+  while (it < max_iter) {
+    const int n = it == 0 ? 1 : std::min(check_every, max_iter - it);
+    for (int q = 0; q < n; ++q) HIPCHK(c, hipGraphLaunch(c->agexec, c->stream));
+    it += n;
+    if (const int rc = sync_and_check(c)) return rc;
+    double a = 0.0;
+    HIPCHK(c, hipMemcpy(&a, c->d_ares, sizeof a, hipMemcpyDeviceToHost));
+    r = std::sqrt(a);
+    if (it == 1) r0 = r;
+    if (!(r == r)) return fail(c, MOSE_E_STATE);
+    if (r <= rtol * r0) break;
+  }
+  if (iters) *iters = it;
+  if (res) {
+    res[0] = r0;
+    res[1] = r;
+  }
+  return 0;
+}
+
+int mose_get_residual_bar(mose_ctx* c, int32_t b, double* psi) {
+  if (!c || !c->finalized || b < 1 || b > c->nb || !psi || !c->d_Rbar) return fail(c, MOSE_E_ARG, b);
+  const BlockDev& B = c->lay.blocks[b - 1];
+  const int64_t ncell = B.sz * (B.nk + 4);
+  if (ensure_stage(c, 5 * ncell)) return c->err.code;
+  hipLaunchKernelGGL(k_get_interior, dim3(launch_blocks(ncell)), dim3(256), 0, c->stream, c->d_Rbar, c->npool,
+                     B.off, B.ni, B.nj, B.nk, c->stage);
+  HIPCHK(c, hipMemcpyAsync(psi, c->stage, 5 * ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int mose_time_residual_vjp(mose_ctx* c, int32_t nrep, double ms[3]) {
   if (!c || !c->finalized || nrep < 1 || !ms) return fail(c, MOSE_E_ARG);
   if (ensure_adjoint(c)) return c->err.code;
@@ -1821,11 +1978,14 @@ void mose_destroy(mose_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->agexec) (void)hipGraphExecDestroy(c->agexec);
+  if (c->agraph) (void)hipGraphDestroy(c->agraph);
   void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_geo, c->d_tiles,
                   c->d_ghost, c->d_bcfaces, c->d_vfaces, c->d_vbuf, c->d_bcbuf, c->d_facebc, c->d_scal, c->d_partials, c->d_dtloc, c->d_dtmin, c->d_err,
                   c->d_iter, c->d_tq, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
                   c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_pres, c->d_ghostp,
-                  c->d_garc_cell, c->d_garc, c->d_rs1, c->d_irsg};
+                  c->d_garc_cell, c->d_garc, c->d_rs1, c->d_irsg, c->d_gP, c->d_lam0, c->d_lam1, c->d_Dq,
+                  c->d_apart, c->d_ares, c->d_icell};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

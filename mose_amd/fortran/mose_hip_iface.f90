@@ -54,6 +54,7 @@ module mose_hip
             mose_get_state, mose_get_state_noalloc, mose_compute_dt, mose_step, mose_advance, mose_residual, &
             mose_put_residual_bar, mose_residual_vjp, mose_residual_vjp_geometry, mose_get_viscous_bar, &
             mose_get_state_bar, mose_get_node_bar, mose_get_inflow_bar, mose_time_residual_vjp, &
+            mose_put_objective_bar, mose_adjoint_solve, mose_get_residual_bar, &
             mose_time_kernels, mose_autotune, mose_get_tiling, &
             mose_get_time, mose_set_time, mose_num_cells, mose_last_error, mose_destroy
 
@@ -159,6 +160,27 @@ module mose_hip
       type(c_ptr), value          :: ctx
       integer(c_int32_t), value   :: b
       real(c_double), intent(out) :: Xbar(*)
+    end function
+    integer(c_int) function mose_put_objective_bar(ctx, b, gP) bind(C, name="mose_put_objective_bar")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value         :: ctx
+      integer(c_int32_t), value  :: b
+      real(c_double), intent(in) :: gP(*)
+    end function
+    integer(c_int) function mose_adjoint_solve(ctx, max_iter, rtol, check_every, iters, res) &
+        bind(C, name="mose_adjoint_solve")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value              :: ctx
+      integer(c_int32_t), value       :: max_iter, check_every
+      real(c_double), value           :: rtol
+      integer(c_int32_t), intent(out) :: iters
+      real(c_double), intent(out)     :: res(2)
+    end function
+    integer(c_int) function mose_get_residual_bar(ctx, b, psi) bind(C, name="mose_get_residual_bar")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value            :: ctx
+      integer(c_int32_t), value     :: b
+      real(c_double), intent(inout) :: psi(*)
     end function
     integer(c_int) function mose_get_inflow_bar(ctx, b, f, dinflow) bind(C, name="mose_get_inflow_bar")
       import :: c_int, c_ptr, c_int32_t, c_double

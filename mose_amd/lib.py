@@ -21,7 +21,8 @@ EXPORTS = ["mose_create", "mose_add_block", "mose_set_viscous", "mose_set_bc", "
            "mose_get_state", "mose_get_state_noalloc", "mose_compute_dt", "mose_step", "mose_advance", "mose_residual",
            "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_get_tiling", "mose_put_residual_bar",
            "mose_residual_vjp", "mose_get_state_bar", "mose_residual_vjp_geometry", "mose_get_viscous_bar", "mose_get_node_bar",
-           "mose_get_inflow_bar", "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
+           "mose_get_inflow_bar", "mose_put_objective_bar", "mose_adjoint_solve", "mose_get_residual_bar",
+           "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
            "mose_destroy"]
 
 
@@ -60,6 +61,9 @@ def load():
     L.mose_get_viscous_bar.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_get_node_bar.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_get_inflow_bar.argtypes = [v, C.c_int32, C.c_int32, P(C.c_double)]
+    L.mose_put_objective_bar.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_adjoint_solve.argtypes = [v, C.c_int32, C.c_double, C.c_int32, P(C.c_int32), P(C.c_double)]
+    L.mose_get_residual_bar.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_num_cells.argtypes = [v]
     L.mose_num_cells.restype = C.c_int64
     L.mose_last_error.argtypes = [v, P(abi.MoseErr)]
@@ -181,6 +185,10 @@ class GpuSolver:
             assert Rb.shape == abi.padded_shape(self.dims[b - 1])
             self._check(L.mose_put_residual_bar(self.h, b, _dp(Rb)))
         self._check(L.mose_residual_vjp_geometry(self.h))
+        return self._geometry_bars()
+
+    def _geometry_bars(self):
+        L = load()
         Pbars, Xbars = [], []
         for b in range(1, len(self.dims) + 1):
             Pb = np.zeros(abi.padded_shape(self.dims[b - 1]))
@@ -199,6 +207,32 @@ class GpuSolver:
                     self._check(L.mose_get_inflow_bar(self.h, key[0], key[1], _dp(out)))
                     inflow[key] = out
         return Pbars, Xbars, inflow
+
+    def adjoint_solve(self, gPs, max_iter=20000, rtol=1e-10, check_every=100):
+        """Steady adjoint: gPs (dJ/dP per block, blk%P layout) -> psi per block solving
+        (dR/dP)^T psi = dJ/dP at the current (converged) state; returns (psi list, iterations,
+        |Q_0|, |Q|).  psi stays in place as the residual adjoint for residual_vjp_geometry_inplace."""
+        L = load()
+        for b, gP in enumerate(gPs, start=1):
+            gP = np.ascontiguousarray(gP, dtype=np.float64)
+            assert gP.shape == abi.padded_shape(self.dims[b - 1])
+            self._check(L.mose_put_objective_bar(self.h, b, _dp(gP)))
+        it = C.c_int32()
+        res = np.zeros(2)
+        self._check(L.mose_adjoint_solve(self.h, max_iter, rtol, check_every, C.byref(it), _dp(res)))
+        psi = []
+        for b in range(1, len(self.dims) + 1):
+            x = np.zeros(abi.padded_shape(self.dims[b - 1]))
+            self._check(L.mose_get_residual_bar(self.h, b, _dp(x)))
+            psi.append(x)
+        return psi, it.value, float(res[0]), float(res[1])
+
+    def residual_vjp_geometry_inplace(self):
+        """residual_vjp_geometry with the residual adjoint already on the device (after
+        adjoint_solve): (Pbar, Xbar, inflow) as residual_vjp_geometry returns them."""
+        L = load()
+        self._check(L.mose_residual_vjp_geometry(self.h))
+        return self._geometry_bars()
 
     def time_residual_vjp(self, nrep=10, geometry=False):
         """Average device ms of one residual and one adjoint evaluation (and, with
