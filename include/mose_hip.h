@@ -208,6 +208,21 @@ int  mose_put_objective_bar(mose_ctx* ctx, int32_t b, const double* gP);
 int  mose_adjoint_solve(mose_ctx* ctx, int32_t max_iter, double rtol, int32_t check_every, int32_t* iters,
                         double res[2]);
 int  mose_get_residual_bar(mose_ctx* ctx, int32_t b, double* psi);
+/* Adjoint of a fixed-dt march for the kinetic energy (no counterpart in the reference, which
+ * has no adjoint; it serves Reynolds-number sensitivities of unsteady runs): nsteps explicit
+ * steps of the reference's stages (Lib_Newstate.f90:134-160, Lib_RK.f90:8-27) with dt held
+ * fixed from the current state, then ke[b] = sum over block b's cells of rho |u|^2 vol / 2 of
+ * the final state.  With mu_ref_bar non-NULL the reverse sweep follows on the device and
+ * mu_ref_bar[b] = dJ/dmu_ref of block b for J = sum_b ke[b] (d/dRe = -mu_ref^2 d/dmu_ref for
+ * mu_ref = 1/Re), and mose_get_state_bar(b) returns dJ/dP of the initial state.  Memory: the
+ * state at the start of every `stride`-th step is kept on the device and each segment's
+ * steps are recomputed from it, ceil(nsteps/stride) + stride states of 5 doubles per pool
+ * cell (stride <= 0: sqrt(nsteps)); MOSE_E_HIP if they do not fit.  The context's state,
+ * clock and time step are unchanged afterwards.  Time-accurate contexts without residual
+ * smoothing or the primitive update only (else MOSE_E_ARG); the reverse refuses what
+ * mose_residual_vjp refuses. */
+int  mose_unsteady_adjoint_ke(mose_ctx* ctx, int32_t nsteps, double dt, int32_t stride, double* ke,
+                              double* mu_ref_bar);
 /* Average device ms of one residual evaluation (ms[0]), one adjoint evaluation (ms[1]) and,
  * with general metrics, one geometry adjoint evaluation (ms[2], device passes only; 0 on
  * rectilinear layouts) over nrep back-to-back runs on the current state and residual

@@ -1027,9 +1027,14 @@ struct AdjArgs {
   // viscous blocks (null / 0 without)
   double* __restrict__ VW;              // 3 x 18 x npool: interior viscous face adjoints at the low cell
   double* __restrict__ VB;              // nvf: mu_ref adjoint of each connection viscous face
+  double* __restrict__ VBC;             // 50 x nvf: each connection face's primitive adjoints of
+                                        // its ten stencil cells, gathered by k_adj_vbc_gather
+  const int64_t* __restrict__ vbt;      // cells receiving connection-face adjoints
+  const int64_t* __restrict__ vbrow;    // their entries vbent[vbrow[r] .. vbrow[r+1]) (face*10 + role)
+  const int* __restrict__ vbent;
   const ViscFace* __restrict__ vfaces;
   int64_t npool, nwork, nslots;
-  int nseg, nbcf, ngrows, nblocks, nents, nvf;
+  int nseg, nbcf, ngrows, nblocks, nents, nvf, nvbt;
   double gam, R;
 };
 

@@ -64,9 +64,10 @@ __global__ void __launch_bounds__(256) k_adj_ghostT(AdjArgs a) {
 // adjoints of its index-space gradients Gb[4][3], of its face state Pmb[5] and of mu_ref at
 // the face's low cell; k_adj_vcell (one thread per cell, ghost layers included) gathers them
 // over the ten stencil roles the cell plays (low / high cell, the eight tangential
-// neighbours) and converts (u, v, w, T) adjoints to primitive ones.  Connection faces are few
-// (one per boundary face cell) and reach cells of the donor block: k_adj_vbc adds their
-// contributions with atomics, so their last bits may vary between runs.
+// neighbours) and converts (u, v, w, T) adjoints to primitive ones.  Connection faces (one per
+// boundary face cell) reach cells of the donor block: k_adj_vbc stores the primitive adjoints of
+// each face's ten stencil cells, and k_adj_vbc_gather adds them to every receiving cell in a
+// fixed order (face, then role), so the result does not depend on the schedule.
 
 // Stencil of the interior face on the low side of cell (i, j, k) in direction d: the two
 // tangential axes in increasing order.
@@ -275,8 +276,24 @@ __global__ void __launch_bounds__(256) k_adj_vbc(AdjArgs a) {
       }
       visc_vars_adj(P[m], Rg, Vb[m], Pb);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) atomicAdd(&a.Pbar[v * N + cell[m]], Pb[v]);
+      for (int v = 0; v < 5; ++v) a.VBC[(10 * (int64_t)q + m) * 5 + v] = Pb[v];
     }
     a.VB[q] = murefb;
+  }
+}
+
+// Connection-face adjoints into Pbar: one thread per receiving cell, its entries in order.
+__global__ void __launch_bounds__(256) k_adj_vbc_gather(AdjArgs a) {
+  const int64_t N = a.npool;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nvbt; r += gridDim.x * blockDim.x) {
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int64_t e = a.vbrow[r]; e < a.vbrow[r + 1]; ++e) {
+      const int64_t x = a.vbent[e];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) acc[v] += a.VBC[x * 5 + v];
+    }
+    const int64_t cell = a.vbt[r];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) a.Pbar[v * N + cell] += acc[v];
   }
 }

@@ -34,6 +34,7 @@ using namespace mose;
 #include "mose_adjoint.hpp"
 #include "mose_adjoint_kernels.hpp"
 #include "mose_adjsolve.hpp"
+#include "mose_unsteady.hpp"
 #include "mose_tables.hpp"
 
 struct DtArgs {
@@ -657,6 +658,11 @@ struct mose_ctx {
   double* d_ares = nullptr;
   int64_t* d_icell = nullptr;                // interior pool cells
   int64_t nicell = 0;
+  // adjoint of the fixed-dt march (allocated on first use)
+  std::vector<double> hvol;                  // volume of each interior cell (interior-list order)
+  double *d_volq = nullptr, *d_upart = nullptr, *d_ke = nullptr, *d_mu = nullptr;
+  int64_t *d_icbeg = nullptr, *d_vfbeg = nullptr;
+  int* d_vfi = nullptr;
   hipGraph_t agraph = nullptr;               // one pseudo-time iteration of the adjoint solve
   hipGraphExec_t agexec = nullptr;
   double *d_Rbar = nullptr, *d_W = nullptr, *d_BB = nullptr, *d_Pbar = nullptr, *d_Pp = nullptr;
@@ -669,6 +675,10 @@ struct mose_ctx {
   std::vector<int64_t> gent_face;   // per ghost-fill entry: 3 * (cell holding the face) + d, or -1
   double *d_GF = nullptr, *d_GL = nullptr, *d_BG = nullptr, *d_GE = nullptr, *d_IB = nullptr;
   double *d_VW = nullptr, *d_VB = nullptr;   // viscous adjoint buffers (viscous layouts only)
+  double* d_VBC = nullptr;                    // connection-face adjoints before their gather
+  int64_t *d_vbt = nullptr, *d_vbrow = nullptr;
+  int* d_vbent = nullptr;
+  int nvbt = 0;
   bool vjp_valid = false;                    // the last vjp left Pbar and the viscous buffers
   bool geo_valid = false;           // the last vjp wrote the geometry adjoints
   bool xbar_valid = false;
@@ -1126,21 +1136,46 @@ static int ensure_adjoint(mose_ctx* c) {
   HIPCHK(c, hipMalloc((void**)&c->d_Pp, 5 * c->npool * sizeof(double)));
   if (c->lay.visc) {
     HIPCHK(c, hipMalloc((void**)&c->d_VW, (size_t)3 * 18 * c->npool * sizeof(double)));
-    HIPCHK(c, hipMalloc((void**)&c->d_VB, std::max<size_t>(c->lay.vfaces.size(), 1) * sizeof(double)));
+    const size_t nvf = c->lay.vfaces.size();
+    HIPCHK(c, hipMalloc((void**)&c->d_VB, std::max<size_t>(nvf, 1) * sizeof(double)));
+    HIPCHK(c, hipMalloc((void**)&c->d_VBC, std::max<size_t>(50 * nvf, 1) * sizeof(double)));
+    // receiving cells of the connection faces (the ten stencil cells of each, k_adj_vbc's
+    // order), grouped per cell with the entries in (face, role) order
+    std::vector<std::pair<int64_t, int>> te;
+    te.reserve(10 * nvf);
+    for (size_t q = 0; q < nvf; ++q) {
+      const ViscFace& e = c->lay.vfaces[q];
+      const int64_t cell[10] = {e.loc, e.gh, e.tn[0], e.tn[1], e.tn[2], e.tn[3], e.dn[0], e.dn[1], e.dn[2], e.dn[3]};
+      for (int m = 0; m < 10; ++m) te.push_back({cell[m], (int)(10 * q + m)});
+    }
+    std::sort(te.begin(), te.end());
+    std::vector<int64_t> tgt, row;
+    std::vector<int> ent;
+    for (size_t x = 0; x < te.size(); ++x) {
+      if (x == 0 || te[x].first != te[x - 1].first) {
+        tgt.push_back(te[x].first);
+        row.push_back((int64_t)x);
+      }
+      ent.push_back(te[x].second);
+    }
+    row.push_back((int64_t)te.size());
+    c->nvbt = (int)tgt.size();
+    if (upload(c, &c->d_vbt, tgt) || upload(c, &c->d_vbrow, row) || upload(c, &c->d_vbent, ent)) return c->err.code;
   }
   c->adj_ready = true;
   return 0;
 }
 
-static AdjArgs adj_args(mose_ctx* c) {
+static AdjArgs adj_args(mose_ctx* c, const double* U = nullptr) {
   AdjArgs a{};
-  a.U = c->U[0]; a.Pp = c->d_Pp; a.Rbar = c->d_Rbar; a.W = c->d_W; a.BB = c->d_BB; a.Pbar = c->d_Pbar;
+  a.U = U ? U : c->U[0]; a.Pp = c->d_Pp; a.Rbar = c->d_Rbar; a.W = c->d_W; a.BB = c->d_BB; a.Pbar = c->d_Pbar;
   a.blocks = c->d_blocks; a.lines = c->d_lines; a.geo = c->d_geo; a.facebc = c->d_facebc;
   a.segs = c->d_segs; a.bcfaces = c->d_bcfaces; a.grows = c->d_grows; a.gents = c->d_gents;
   a.npool = c->npool; a.nwork = c->nwork; a.nslots = std::max<int64_t>(c->lay.nbcslots, 1);
   a.nseg = c->nseg; a.nbcf = (int)c->lay.bcfaces.size(); a.ngrows = c->ngrows; a.nblocks = c->nb;
   a.nents = std::max(c->nents, 1);
   a.VW = c->d_VW; a.VB = c->d_VB; a.vfaces = c->d_vfaces; a.nvf = (int)c->lay.vfaces.size();
+  a.VBC = c->d_VBC; a.vbt = c->d_vbt; a.vbrow = c->d_vbrow; a.vbent = c->d_vbent; a.nvbt = c->nvbt;
   a.gam = c->cfg.gamma; a.R = c->cfg.Rgas;
   return a;
 }
@@ -1164,11 +1199,12 @@ static int ensure_adjoint_geometry(mose_ctx* c) {
 // Pbar = (dR/dP)^T Rbar on the current state (the ghost layers are filled first); with
 // geometry, also the adjoints of the geometric quantities (face normals and areas, lengths,
 // mirror normals, inflow values).
-static void enqueue_vjp(mose_ctx* c, bool geometry = false) {
-  enqueue_ghost(c, c->U[0]);
-  hipLaunchKernelGGL(k_adj_prim, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, c->U[0], c->d_Pp, c->npool,
+static void enqueue_vjp(mose_ctx* c, bool geometry = false, double* U = nullptr) {
+  if (!U) U = c->U[0];
+  enqueue_ghost(c, U);
+  hipLaunchKernelGGL(k_adj_prim, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, U, c->d_Pp, c->npool,
                      c->cfg.gamma);
-  AdjArgs a = adj_args(c);
+  AdjArgs a = adj_args(c, U);
   if (geometry) {
     a.GF = c->d_GF; a.GL = c->d_GL; a.BG = c->d_BG; a.GE = c->d_GE; a.IB = c->d_IB;
   }
@@ -1183,7 +1219,10 @@ static void enqueue_vjp(mose_ctx* c, bool geometry = false) {
   if (c->lay.visc) {   // viscous terms: interior faces (gathered per cell), then connection faces
     hipLaunchKernelGGL(k_adj_vface, dim3(launch_blocks(c->nwork)), dim3(256), 0, c->stream, a);
     hipLaunchKernelGGL(k_adj_vcell, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, a);
-    if (a.nvf > 0) hipLaunchKernelGGL(k_adj_vbc, dim3(launch_blocks(a.nvf)), dim3(256), 0, c->stream, a);
+    if (a.nvf > 0) {
+      hipLaunchKernelGGL(k_adj_vbc, dim3(launch_blocks(a.nvf)), dim3(256), 0, c->stream, a);
+      hipLaunchKernelGGL(k_adj_vbc_gather, dim3(launch_blocks(a.nvbt)), dim3(256), 0, c->stream, a);
+    }
   }
   if (c->ngrows > 0)
     hipLaunchKernelGGL(k_adj_ghostT, dim3(launch_blocks(c->ngrows)), dim3(256), 0, c->stream, a);
@@ -1805,6 +1844,30 @@ int mose_put_objective_bar(mose_ctx* c, int32_t b, const double* gP) {
   return 0;
 }
 
+// Volume of every interior cell, in the order of the interior cell list (host copy, once).
+static int interior_volumes(mose_ctx* c) {
+  if (!c->hvol.empty()) return 0;
+  const int64_t np = c->npool;
+  std::vector<double> vol;
+  if (c->lay.curvi) {
+    vol.resize((size_t)np);
+    HIPCHK(c, hipMemcpy(vol.data(), c->d_geo + (size_t)GVOL * np, np * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  c->hvol.reserve((size_t)c->nicell);
+  for (const BlockDev& B : c->lay.blocks) {
+    const LineEnt* Lx = c->lay.lines.data() + B.line[0];
+    const LineEnt* Ly = c->lay.lines.data() + B.line[1];
+    const LineEnt* Lz = c->lay.lines.data() + B.line[2];
+    for (int k = 1; k <= B.nk; ++k)
+      for (int j = 1; j <= B.nj; ++j)
+        for (int i = 1; i <= B.ni; ++i) {
+          const int64_t cc = B.off + (i + 1) + (int64_t)B.sy * (j + 1) + B.sz * (k + 1);
+          c->hvol.push_back(c->lay.curvi ? vol[(size_t)cc] : Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta);
+        }
+  }
+  return 0;
+}
+
 // D = dt / vol of every interior cell at the current state (local dt, or the global step of
 // a time-accurate context), without the CFL ramp.
 static int adjsolve_dt(mose_ctx* c) {
@@ -1822,25 +1885,11 @@ static int adjsolve_dt(mose_ctx* c) {
     dt.resize((size_t)np);
     HIPCHK(c, hipMemcpy(dt.data(), c->d_dtloc, np * sizeof(double), hipMemcpyDeviceToHost));
   }
-  std::vector<double> vol;
-  if (c->lay.curvi) {
-    vol.resize((size_t)np);
-    HIPCHK(c, hipMemcpy(vol.data(), c->d_geo + (size_t)GVOL * np, np * sizeof(double), hipMemcpyDeviceToHost));
-  }
-  std::vector<double> D;
-  D.reserve((size_t)c->nicell);
-  for (const BlockDev& B : c->lay.blocks) {
-    const LineEnt* Lx = c->lay.lines.data() + B.line[0];
-    const LineEnt* Ly = c->lay.lines.data() + B.line[1];
-    const LineEnt* Lz = c->lay.lines.data() + B.line[2];
-    for (int k = 1; k <= B.nk; ++k)
-      for (int j = 1; j <= B.nj; ++j)
-        for (int i = 1; i <= B.ni; ++i) {
-          const int64_t cc = B.off + (i + 1) + (int64_t)B.sy * (j + 1) + B.sz * (k + 1);
-          const double v = c->lay.curvi ? vol[(size_t)cc] : Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta;
-          D.push_back((c->cfg.time_accurate ? dtg : dt[(size_t)cc]) / v);
-        }
-  }
+  if (interior_volumes(c)) return c->err.code;
+  std::vector<double> D((size_t)c->nicell);
+  std::vector<int64_t> cells((size_t)c->nicell);
+  HIPCHK(c, hipMemcpy(cells.data(), c->d_icell, cells.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < D.size(); ++q) D[q] = (c->cfg.time_accurate ? dtg : dt[(size_t)cells[q]]) / c->hvol[q];
   HIPCHK(c, hipMemcpy(c->d_Dq, D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
@@ -1900,6 +1949,151 @@ int mose_adjoint_solve(mose_ctx* c, int32_t max_iter, double rtol, int32_t check
     res[1] = r;
   }
   return 0;
+}
+
+// ---------------------------------------------------------------------------- unsteady adjoint
+constexpr int kBlockParts = 256;   // workgroups per block of the per-block reductions
+
+static int ensure_unsteady(mose_ctx* c) {
+  if (ensure_adjsolve(c) || interior_volumes(c)) return c->err.code;
+  if (c->d_volq) return 0;
+  if (upload(c, &c->d_volq, c->hvol)) return c->err.code;
+  std::vector<int64_t> beg(1, 0);
+  for (const BlockDev& B : c->lay.blocks) beg.push_back(beg.back() + (int64_t)B.ni * B.nj * B.nk);
+  std::vector<int> vfi;
+  std::vector<int64_t> vbeg(1, 0);
+  for (int b = 0; b < c->nb; ++b) {
+    for (size_t q = 0; q < c->lay.vfaces.size(); ++q)
+      if (c->lay.vfaces[q].b == b) vfi.push_back((int)q);
+    vbeg.push_back((int64_t)vfi.size());
+  }
+  if (upload(c, &c->d_icbeg, beg) || upload(c, &c->d_vfi, vfi) || upload(c, &c->d_vfbeg, vbeg)) return c->err.code;
+  HIPCHK(c, hipMalloc((void**)&c->d_upart, (size_t)kBlockParts * c->nb * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_ke, c->nb * sizeof(double)));
+  HIPCHK(c, hipMalloc((void**)&c->d_mu, c->nb * sizeof(double)));
+  return 0;
+}
+
+// One step of the fixed-dt march from Un into Unext (stages 1..upto only when upto > 0; stage
+// temporaries U[1], U[2]); the stage kernels take dt from the scalar slot.
+static void enqueue_fixed_step(mose_ctx* c, double* Un, double* Unext, int upto = 0) {
+  const int nrk = c->cfg.n_rk;
+  const int last = upto > 0 ? upto : nrk;
+  for (int s = 1; s <= last; ++s) {
+    double* in = s == 1 ? Un : c->U[s - 1];
+    double* out = s == nrk ? Unext : c->U[s];
+    enqueue_ghost(c, in);
+    StageArgs a = stage_args(c, in, Un, out, kRK[nrk - 1][s - 1]);
+    enqueue_bcfaces(c, a);
+    launch_stage(c, stage_fn(c, 0), a);
+  }
+}
+
+// The reverse of one step from its start state Un: lamS holds dJ/dU of the step's result on
+// entry and dJ/dU_n on return (lamN is the accumulator, zero on entry).
+static void enqueue_reverse_step(mose_ctx* c, double* Un) {
+  const int nrk = c->cfg.n_rk;
+  if (nrk > 1) enqueue_fixed_step(c, Un, nullptr, nrk - 1);     // stage inputs U_1 .. U_{nrk-1}
+  const int g = launch_blocks(c->nicell);
+  for (int s = nrk; s >= 1; --s) {
+    double* in = s == 1 ? Un : c->U[s - 1];
+    const double cs = kRK[nrk - 1][s - 1];
+    hipLaunchKernelGGL(k_unst_rbar, dim3(g), dim3(256), 0, c->stream, c->d_icell, c->d_Dq, c->nicell, c->npool,
+                       c->d_lam0, c->d_lam1, c->d_Rbar, cs);
+    enqueue_vjp(c, false, in);
+    if (c->lay.visc) {
+      hipLaunchKernelGGL(k_mubar, dim3(kBlockParts, c->nb), dim3(256), 0, c->stream, c->d_blocks, c->d_VW, c->d_VB,
+                         c->d_vfi, c->d_vfbeg, c->npool, c->d_upart);
+      hipLaunchKernelGGL(k_block_sums, dim3((c->nb + 63) / 64), dim3(64), 0, c->stream, c->d_upart, kBlockParts, c->nb,
+                         c->d_mu, 1);
+    }
+    hipLaunchKernelGGL(k_unst_back, dim3(g), dim3(256), 0, c->stream, c->d_icell, c->nicell, c->npool, in, c->d_Pbar,
+                       c->d_lam0, c->d_lam1, cs, c->cfg.gamma, s == 1 ? 1 : 0);
+  }
+  std::swap(c->d_lam0, c->d_lam1);       // dJ/dU_n is the next (earlier) step's result adjoint
+  const int64_t n5 = 5 * c->npool;
+  hipLaunchKernelGGL(k_fill, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->d_lam1, n5, 0.0);
+}
+
+int mose_unsteady_adjoint_ke(mose_ctx* c, int32_t nsteps, double dt, int32_t stride, double* ke, double* mu_bar) {
+  if (!c || !c->finalized || nsteps < 1 || !(dt > 0.0) || !(dt < 1e300) || !ke) return fail(c, MOSE_E_ARG);
+  if (c->err.code) return c->err.code;
+  if (!c->cfg.time_accurate || c->cfg.irs_beta > 0.0 || c->cfg.prim_vars) return fail(c, MOSE_E_ARG);
+  const bool reverse = mu_bar != nullptr;
+  if (reverse && ensure_adjoint(c)) return c->err.code;
+  if (ensure_unsteady(c)) return c->err.code;
+  {   // D = dt / vol
+    std::vector<double> D(c->hvol.size());
+    for (size_t q = 0; q < D.size(); ++q) D[q] = dt / c->hvol[q];
+    HIPCHK(c, hipMemcpy(c->d_Dq, D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (stride < 1) stride = (int32_t)std::ceil(std::sqrt((double)nsteps));
+  stride = std::min(stride, nsteps);
+  const int nck = (nsteps + stride - 1) / stride;
+  const int nseg = std::max(stride, 2);
+  const int nbuf = (reverse ? nck : 1) + nseg;
+  const size_t words = 5 * (size_t)c->npool;
+  double* pool = nullptr;
+  if (hipMalloc((void**)&pool, (size_t)nbuf * words * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    fprintf(stderr, "[mose_hip] march adjoint: %d device states of %.2f GB do not fit; use a larger stride\n", nbuf,
+            words * 8e-9);
+    return fail(c, MOSE_E_HIP);
+  }
+  auto ck = [&](int k) { return pool + (size_t)k * words; };
+  auto seg = [&](int j) { return pool + (size_t)((reverse ? nck : 1) + j) * words; };
+  double dt_old = 0.0;
+  HIPCHK(c, hipMemcpy(&dt_old, c->d_scal + S_DT, sizeof dt_old, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(c->d_scal + S_DT, &dt, sizeof dt, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_copy, dim3(launch_blocks((int64_t)words)), dim3(256), 0, c->stream, c->U[0], ck(0),
+                     (int64_t)words);
+  // forward: checkpoints at the start of every segment
+  double* cur = ck(0);
+  for (int m = 0; m < nsteps; ++m) {
+    double* nxt = (reverse && (m + 1) % stride == 0 && m + 1 < nsteps) ? ck((m + 1) / stride)
+                                                                       : (cur == seg(0) ? seg(1) : seg(0));
+    enqueue_fixed_step(c, cur, nxt);
+    cur = nxt;
+  }
+  hipLaunchKernelGGL(k_ke, dim3(kBlockParts, c->nb), dim3(256), 0, c->stream, c->d_icell, c->d_volq, c->d_icbeg,
+                     c->npool, cur, c->d_lam0, c->d_lam1, c->d_upart, reverse ? 1 : 0);
+  hipLaunchKernelGGL(k_block_sums, dim3((c->nb + 63) / 64), dim3(64), 0, c->stream, c->d_upart, kBlockParts, c->nb,
+                     c->d_ke, 0);
+  int rc = sync_and_check(c);
+  // MOSE_PROGRESS=1: one line per reversed segment on stderr (long sweeps)
+  const char* pe = getenv("MOSE_PROGRESS");
+  const bool progress = pe && atoi(pe) == 1;
+  if (progress) {
+    fprintf(stderr, "[mose_hip] march adjoint: %d steps marched, %d segments of %d steps to reverse\n", nsteps, nck,
+            stride);
+    fflush(stderr);
+  }
+  if (rc == 0 && reverse) {
+    const int64_t n5 = (int64_t)words;
+    hipLaunchKernelGGL(k_fill, dim3(launch_blocks(n5)), dim3(256), 0, c->stream, c->d_Rbar, n5, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->stream, c->d_mu, (int64_t)c->nb, 0.0);
+    for (int k = nck - 1; k >= 0 && rc == 0; --k) {
+      const int len = std::min(stride, nsteps - k * stride);
+      // the segment's step-start states, recomputed from its checkpoint
+      for (int j = 1; j < len; ++j) enqueue_fixed_step(c, j == 1 ? ck(k) : seg(j - 1), seg(j));
+      for (int j = len - 1; j >= 0; --j) enqueue_reverse_step(c, j == 0 ? ck(k) : seg(j));
+      rc = sync_and_check(c);
+      if (progress) {
+        fprintf(stderr, "[mose_hip] march adjoint: segment %d of %d reversed\n", nck - k, nck);
+        fflush(stderr);
+      }
+    }
+    if (rc == 0) {
+      hipLaunchKernelGGL(k_prim_bar, dim3(launch_blocks(c->nicell)), dim3(256), 0, c->stream, c->d_icell, c->nicell,
+                         c->npool, ck(0), c->d_lam0, c->d_Pbar, c->cfg.gamma);
+      HIPCHK(c, hipMemcpy(mu_bar, c->d_mu, c->nb * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    c->vjp_valid = c->geo_valid = c->xbar_valid = false;
+  }
+  if (rc == 0) HIPCHK(c, hipMemcpy(ke, c->d_ke, c->nb * sizeof(double), hipMemcpyDeviceToHost));
+  (void)hipMemcpy(c->d_scal + S_DT, &dt_old, sizeof dt_old, hipMemcpyHostToDevice);
+  (void)hipFree(pool);
+  return rc ? rc : sync_and_check(c);
 }
 
 int mose_get_residual_bar(mose_ctx* c, int32_t b, double* psi) {
@@ -1985,7 +2179,8 @@ void mose_destroy(mose_ctx* c) {
                   c->d_iter, c->d_tq, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
                   c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_pres, c->d_ghostp,
                   c->d_garc_cell, c->d_garc, c->d_rs1, c->d_irsg, c->d_gP, c->d_lam0, c->d_lam1, c->d_Dq,
-                  c->d_apart, c->d_ares, c->d_icell};
+                  c->d_apart, c->d_ares, c->d_icell, c->d_VBC, c->d_vbt, c->d_vbrow, c->d_vbent, c->d_volq,
+                  c->d_upart, c->d_ke, c->d_mu, c->d_icbeg, c->d_vfbeg, c->d_vfi};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -54,7 +54,7 @@ module mose_hip
             mose_get_state, mose_get_state_noalloc, mose_compute_dt, mose_step, mose_advance, mose_residual, &
             mose_put_residual_bar, mose_residual_vjp, mose_residual_vjp_geometry, mose_get_viscous_bar, &
             mose_get_state_bar, mose_get_node_bar, mose_get_inflow_bar, mose_time_residual_vjp, &
-            mose_put_objective_bar, mose_adjoint_solve, mose_get_residual_bar, &
+            mose_put_objective_bar, mose_adjoint_solve, mose_get_residual_bar, mose_unsteady_adjoint_ke, &
             mose_time_kernels, mose_autotune, mose_get_tiling, &
             mose_get_time, mose_set_time, mose_num_cells, mose_last_error, mose_destroy
 
@@ -181,6 +181,15 @@ module mose_hip
       type(c_ptr), value            :: ctx
       integer(c_int32_t), value     :: b
       real(c_double), intent(inout) :: psi(*)
+    end function
+    integer(c_int) function mose_unsteady_adjoint_ke(ctx, nsteps, dt, stride, ke, mu_ref_bar) &
+        bind(C, name="mose_unsteady_adjoint_ke")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value            :: ctx
+      integer(c_int32_t), value     :: nsteps, stride
+      real(c_double), value         :: dt
+      real(c_double), intent(out)   :: ke(*)
+      real(c_double), intent(out)   :: mu_ref_bar(*)
     end function
     integer(c_int) function mose_get_inflow_bar(ctx, b, f, dinflow) bind(C, name="mose_get_inflow_bar")
       import :: c_int, c_ptr, c_int32_t, c_double

@@ -22,7 +22,7 @@ EXPORTS = ["mose_create", "mose_add_block", "mose_set_viscous", "mose_set_bc", "
            "mose_get_time", "mose_set_time", "mose_time_kernels", "mose_autotune", "mose_get_tiling", "mose_put_residual_bar",
            "mose_residual_vjp", "mose_get_state_bar", "mose_residual_vjp_geometry", "mose_get_viscous_bar", "mose_get_node_bar",
            "mose_get_inflow_bar", "mose_put_objective_bar", "mose_adjoint_solve", "mose_get_residual_bar",
-           "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
+           "mose_unsteady_adjoint_ke", "mose_time_residual_vjp", "mose_num_cells", "mose_last_error", "mose_strerror",
            "mose_destroy"]
 
 
@@ -64,6 +64,7 @@ def load():
     L.mose_put_objective_bar.argtypes = [v, C.c_int32, P(C.c_double)]
     L.mose_adjoint_solve.argtypes = [v, C.c_int32, C.c_double, C.c_int32, P(C.c_int32), P(C.c_double)]
     L.mose_get_residual_bar.argtypes = [v, C.c_int32, P(C.c_double)]
+    L.mose_unsteady_adjoint_ke.argtypes = [v, C.c_int32, C.c_double, C.c_int32, P(C.c_double), P(C.c_double)]
     L.mose_num_cells.argtypes = [v]
     L.mose_num_cells.restype = C.c_int64
     L.mose_last_error.argtypes = [v, P(abi.MoseErr)]
@@ -226,6 +227,24 @@ class GpuSolver:
             self._check(L.mose_get_residual_bar(self.h, b, _dp(x)))
             psi.append(x)
         return psi, it.value, float(res[0]), float(res[1])
+
+    def march_kinetic_energy(self, nsteps, dt, stride=0, adjoint=True):
+        """nsteps fixed-dt steps on the device from the current state (left unchanged): the
+        final kinetic energy per block and, with adjoint=True, (dJ/dmu_ref per block, dJ/dP0
+        per block) for J = the total kinetic energy."""
+        nb = len(self.dims)
+        ke = np.zeros(nb)
+        mub = np.zeros(nb) if adjoint else None
+        self._check(load().mose_unsteady_adjoint_ke(self.h, nsteps, dt, stride, _dp(ke),
+                                                    _dp(mub) if adjoint else None))
+        if not adjoint:
+            return ke
+        P0b = []
+        for b in range(1, nb + 1):
+            x = np.zeros(abi.padded_shape(self.dims[b - 1]))
+            self._check(load().mose_get_state_bar(self.h, b, _dp(x)))
+            P0b.append(x)
+        return ke, mub, P0b
 
     def residual_vjp_geometry_inplace(self):
         """residual_vjp_geometry with the residual adjoint already on the device (after

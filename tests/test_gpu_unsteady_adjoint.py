@@ -69,3 +69,68 @@ def test_kinetic_energy_sensitivities():
           _march(case, dt, nsteps, [p - t * v for p, v in zip(P0, V)])) / (2 * t)
     scale = sum(float(np.sum(np.abs(pb * v))) for pb, v in zip(P0b, V))
     assert abs(adj - fd) <= 1e-6 * scale, (adj, fd, scale)
+
+
+def test_device_march_adjoint_equals_host_tape():
+    """mose_unsteady_adjoint_ke (the march, its checkpoints and the reverse sweep on the device)
+    against the host-taped sweep above: final kinetic energy, dKE/dmu_ref per block and dKE/dP0."""
+    case = _case()
+    nsteps = 8
+    P0 = [abi.interior(b.P).copy() for b in case.blocks]
+    g = GpuSolver(case)
+    dt = 0.5 * g.compute_dt()
+    ke_d, mub_d, P0b_d = g.march_kinetic_energy(nsteps, dt, stride=3)
+    m = FixedStepMarch(g, case, dt)
+    P = m.run(P0, nsteps)
+    ke_h = kinetic_energy(case, P)
+    P0b_h, mub_h = m.reverse(kinetic_energy_ubar(case, P, case.cfg.gamma))
+    g.close()
+    print("KE device %.15e host %.15e; mu_bar device %s host %s" % (np.sum(ke_d), ke_h, mub_d, mub_h))
+    assert abs(np.sum(ke_d) - ke_h) <= 1e-13 * ke_h
+    np.testing.assert_allclose(mub_d, mub_h, rtol=1e-11)
+    for a, r in zip(P0b_d, P0b_h):
+        assert np.linalg.norm(abi.interior(a) - r) <= 1e-11 * np.linalg.norm(r)
+
+
+def test_device_march_adjoint_checkpoint_stride_invariance():
+    """The segments recompute their steps from the checkpoints bit for bit: every stride gives
+    the same gradients, and the context's state is left as it was."""
+    case = _case()
+    g = GpuSolver(case)
+    dt = 0.5 * g.compute_dt()
+    before = [g.get_state(b) for b in (1, 2)]
+    runs = [g.march_kinetic_energy(7, dt, stride=s) for s in (1, 3, 7, 0)]
+    after = [g.get_state(b) for b in (1, 2)]
+    g.close()
+    for ke, mub, P0b in runs[1:]:
+        assert np.array_equal(ke, runs[0][0]) and np.array_equal(mub, runs[0][1])
+        for a, r in zip(P0b, runs[0][2]):
+            assert np.array_equal(a, r)
+    for a, r in zip(after, before):
+        assert np.array_equal(a, r)
+
+
+def test_device_march_adjoint_reynolds_against_central_differences():
+    """dKE/dmu_ref of each of two 12^3 boxes over 16 steps (checkpoint stride 5) against central
+    differences of the device march; d/dRe = -mu_ref^2 d/dmu_ref."""
+    case = cases.taylor_green(n=12, nbox=2, mach=0.2, precision=abi.PREC_FP64)
+    case.meta["viscous"] = [(b, [0.02 * b] + list(p[1:])) for b, p in case.meta["viscous"]]
+    case = perturb(case, amp=0.01, seed=7)
+    g = GpuSolver(case)
+    dt = 0.5 * g.compute_dt()
+    ke, mub, _ = g.march_kinetic_energy(16, dt, stride=5)
+    g.close()
+    for b in (1, 2):
+        mu = case.meta["viscous"][b - 1][1][0]
+        h = 1e-6 * mu
+        kes = []
+        for sgn in (1.0, -1.0):
+            c = copy.deepcopy(case)
+            c.meta["viscous"] = [(bb, [p[0] + (sgn * h if bb == b else 0.0)] + list(p[1:])) for bb, p in c.meta["viscous"]]
+            gg = GpuSolver(c)
+            kes.append(float(np.sum(gg.march_kinetic_energy(16, dt, adjoint=False))))
+            gg.close()
+        fd = (kes[0] - kes[1]) / (2 * h)
+        print("box %d: dKE/dmu_ref adjoint %.10e fd %.10e; dKE/dRe %.6e" % (b, mub[b - 1], fd, -mu * mu * mub[b - 1]))
+        assert mub[b - 1] < 0.0
+        assert abs(mub[b - 1] - fd) <= 1e-6 * abs(fd)

@@ -131,3 +131,21 @@ def test_inviscid_block_has_zero_mu_ref_adjoint():
     g.residual_vjp(Rbar)
     assert g.viscous_bar(2) == 0.0 and g.viscous_bar(1) != 0.0
     g.close()
+
+
+def test_viscous_adjoint_is_deterministic():
+    """The connection-face viscous adjoints are gathered per receiving cell in a fixed order
+    (no atomics): two evaluations give the same bits, mu_ref adjoints included."""
+    case = cases.taylor_green(n=10, nbox=2, mach=0.3, precision=abi.PREC_FP64)
+    case = perturb(case, amp=0.02, seed=12)
+    g = GpuSolver(case)
+    rng = np.random.default_rng(2)
+    Rbar = [rng.standard_normal(abi.padded_shape(b.dims)) for b in case.blocks]
+    a = g.residual_vjp(Rbar)
+    mua = [g.viscous_bar(b) for b in (1, 2)]
+    b_ = g.residual_vjp(Rbar)
+    mub = [g.viscous_bar(b) for b in (1, 2)]
+    g.close()
+    for x, y in zip(a, b_):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    assert mua == mub
