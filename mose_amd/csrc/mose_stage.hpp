@@ -299,7 +299,10 @@ __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const 
 // Face ownership: every thread solves the low i face and the low j face of its own cell
 // and the k face above it; the TX + TY faces on the high edges of the patch are solved by
 // TX + TY threads whose choice rotates with the plane so no wave always carries them.  The
-// k faces need only the thread's own registers.
+// k faces need only the thread's own registers.  The own cell's face states and faces are
+// formed with the direction fixed at compile time (no per-lane direction selects); the slopes
+// of the x-halo cells are formed by wave 0 and those of the y-halo cells by wave 1, each with
+// its direction fixed; the spacing data along x and y of a tile sit in LDS.
 // Occupancy floor: FP32 flux 3 waves per SIMD (2 for 512-thread patches, whose workgroup
 // brings two waves to every SIMD); FP64 2 (256 registers: without the floor some variants take
 // a few AGPRs beyond 256 and fall to one wave per SIMD); the viscous kernels, whose LDS allows
@@ -311,9 +314,8 @@ __global__ void __launch_bounds__(TXc* TYc)
   constexpr int NTc = TXc * TYc;
   constexpr int NW = NTc / 64;                   // waves per workgroup
   constexpr int NH = 4 * (TXc + TYc);            // halo cells of a plane
-  constexpr int NX = 2 * (TXc + TYc);            // cells just outside the patch needing a slope
   constexpr int NE = TXc + TYc;                  // faces on the high edges of the patch
-  static_assert(NH <= NTc && NX <= NTc && NTc % 64 == 0, "one halo / edge cell per thread at most");
+  static_assert(NH <= NTc && NTc % 64 == 0, "one halo / edge cell per thread at most");
   __shared__ FT sP[5][TYc + 4][TXc + 4];
   __shared__ FT sSx[5][TYc][TXc + 1];    // [y][x]: "+" i-state of cell x-1 (x = 0: the halo cell)
   __shared__ FT sSy[5][TYc + 1][TXc];    // [y][x]: "+" j-state of cell y-1
@@ -324,6 +326,8 @@ __global__ void __launch_bounds__(TXc* TYc)
   constexpr bool VIS = GEO == 2;
   constexpr int VR = VIS ? 3 : 1, VY = VIS ? TYc + 2 : 1, VX = VIS ? TXc + 2 : 1;
   __shared__ FT sV[VR][4][VY][VX];       // viscous variables, ring over planes k-1, k, k+1
+  constexpr int NL = (TXc > TYc ? TXc : TYc) + 3;
+  __shared__ FT sRL[2][NL], sLP[2][NL];  // per-tile spacing data along x and y (rectilinear)
   // residual-norm sums: LDS for the FP32-flux kernels (register-bound), registers for FP64
   // (LDS-bound); reduced in a fixed order either way
   constexpr bool NLDS = MODE == 1 && sizeof(FT) == 4;
@@ -350,13 +354,13 @@ __global__ void __launch_bounds__(TXc* TYc)
   else if (tid < 4 * TYc) { const int h = tid - 2 * TYc; hy = h >> 1; hx = TXc + (h & 1); }
   else if (tid < 4 * TYc + 2 * TXc) { const int h = tid - 4 * TYc; hx = h % TXc; hy = -2 + h / TXc; }
   else if (tid < NH) { const int h = tid - 4 * TYc - 2 * TXc; hx = h % TXc; hy = TYc + h / TXc; }
-  // the cell just outside the patch whose slope this thread computes (direction xd)
-  int xd = 0, xlx = 0, xly = 0;
-  bool xplus = false;
-  if (tid < TYc) { xd = 0; xly = tid; xlx = -1; xplus = true; }
-  else if (tid < 2 * TYc) { xd = 0; xly = tid - TYc; xlx = TXc; }
-  else if (tid < 2 * TYc + TXc) { xd = 1; xlx = tid - 2 * TYc; xly = -1; xplus = true; }
-  else if (tid < NX) { xd = 1; xlx = tid - 2 * TYc - TXc; xly = TYc; }
+  // the cells just outside the patch whose slopes are needed: the 2 TY x-halo cells by the
+  // first lanes of wave 0, the 2 TX y-halo cells by the first lanes of wave 1
+  static_assert(2 * TYc <= 64 && 2 * TXc <= 64 && NTc >= 128, "halo slope tasks: one wave per direction");
+  const bool xtask = tid < 2 * TYc, ytask = tid >= 64 && tid < 64 + 2 * TXc;
+  const bool xplus = tid < TYc, yplus = tid - 64 < TXc;
+  const int xlx = xplus ? -1 : TXc, xly = xtask ? (xplus ? tid : tid - TYc) : 0;
+  const int yly = yplus ? -1 : TYc, ylx = ytask ? (yplus ? tid - 64 : tid - 64 - TXc) : 0;
 
   __shared__ int sTile;
   if (tid == 0) sTile = claim_tile(a.tq, a.nq, a.ntiles);
@@ -379,8 +383,24 @@ __global__ void __launch_bounds__(TXc* TYc)
     const int hgi = T.i0 + hx, hgj = T.j0 + hy;
     const bool hval = tid < NH && hgi <= ni + 2 && hgj <= nj + 2;
     const unsigned hbase = off + (unsigned)(hgi + 1) + (unsigned)sy_ * (unsigned)(hgj + 1);
-    const int xci = T.i0 + xlx, xcj = T.j0 + xly;
-    const bool xval = tid < NX && (xd == 0 ? (xcj <= nj && xci <= ni + 1) : (xci <= ni && xcj <= nj + 1));
+    const int xci = T.i0 + xlx, xcj = T.j0 + xly, yci = T.i0 + ylx, ycj = T.j0 + yly;
+    const bool xval = xtask && xcj <= nj && xci <= ni + 1;
+    const bool yval = ytask && yci <= ni && ycj <= nj + 1;
+    // spacing data of the patch's cells and of its halo cells along x and y (constant over the
+    // tile): sRL[d][q] = reciprocal centre distance of the face on the low side of the cell at
+    // offset q - 1 from the patch's first cell, sLP[d][q] its half length; read in the k-loop
+    if constexpr (GEO != 1) {
+      for (int q = tid; q < TXc + 3; q += NTc) {
+        const bool in = T.i0 + q <= ni + 3;
+        sRL[0][q] = in ? FT(Lx[T.i0 + q].rl1) : FT(0);
+        sLP[0][q] = in ? FT(Lx[T.i0 + q].lp) : FT(0);
+      }
+      for (int q = tid; q < TYc + 3; q += NTc) {
+        const bool in = T.j0 + q <= nj + 3;
+        sRL[1][q] = in ? FT(Ly[T.j0 + q].rl1) : FT(0);
+        sLP[1][q] = in ? FT(Ly[T.j0 + q].lp) : FT(0);
+      }
+    }
 
 
     // column queue P(k), P(k+1), P(k+2); "+" k-state of cell k; flux of the k face below;
@@ -477,101 +497,80 @@ __global__ void __launch_bounds__(TXc* TYc)
         if constexpr (VIS) stage_visc(k + 1);
         __syncthreads();       // plane k staged
 
-        // ---- i and j face states: own cell, then the cells just outside the patch
+        // ---- i and j face states: the own cell in each direction, then the cells just outside
+        // the patch (x halo on wave 0, y halo on wave 1); every call has a fixed direction
         FT Smx[5], Smy[5];
-        const int npass = tid < NX ? 3 : 2;
-#pragma unroll 1
-        for (int pass = 0; pass < npass; ++pass) {
-          int d, lx, ly;
-          bool ok;
-          if (pass < 2) {
-            d = pass; lx = tx; ly = ty; ok = d == 0 ? iface : jface;
-          } else {
-            d = xd; lx = xlx; ly = xly; ok = xval;
+        auto states = [&](const int d, int lx, int ly, FT* Sm_, FT* Sp_) {
+          const int q = (d == 0 ? lx : ly) + 1;     // sRL / sLP index of the cell's low face
+          FT rlm = FT(0), rlp = FT(0), hdl = FT(0);
+          if constexpr (GEO != 1) {
+            rlm = sRL[d][q];
+            rlp = sRL[d][q + 1];
+            hdl = sLP[d][q];
           }
-          if (ok) {
-            const int sx = d == 0 ? 1 : 0, sy = d == 0 ? 0 : 1;
-            FT m[5], c[5], p[5], Sm_[5], Sp_[5];
+          const int sx = d == 0 ? 1 : 0, sy = d == 0 ? 0 : 1;
+          FT m[5], c[5], p[5];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
-              m[v] = sP[v][ly + 2 - sy][lx + 2 - sx];
-              c[v] = sP[v][ly + 2][lx + 2];
-              p[v] = sP[v][ly + 2 + sy][lx + 2 + sx];
-            }
-            FT bm = FT(1), bp = FT(1);
-            if constexpr (SDR) {
-              const int cc = d == 0 ? T.i0 + lx : T.j0 + ly;
-              const unsigned pc = off + (unsigned)(T.i0 + lx + 1) + (unsigned)sy_ * (unsigned)(T.j0 + ly + 1) +
-                                  sz * (unsigned)(k + 1);
-              const unsigned st = d == 0 ? 1u : (unsigned)sy_;
-              bp = FT(beta[cc >= 1 ? pc : pc + st]);
-              bm = FT(beta[cc >= 2 ? pc - st : pc]);
-            }
-            if constexpr (GEO != 1) {
-              const LineEnt* L = d == 0 ? Lx : Ly;
-              const int cc = d == 0 ? T.i0 + lx : T.j0 + ly;
-              cell_states_b<LIM, SDR>(m, c, p, FT(L[cc + 1].rl1), FT(L[cc + 2].rl1), FT(L[cc + 1].lp), bm, bp, Sm_, Sp_);
-            } else {
-              const unsigned pc = off + (unsigned)(T.i0 + lx + 1) + (unsigned)sy_ * (unsigned)(T.j0 + ly + 1) +
-                                  sz * (unsigned)(k + 1);
-              const unsigned st = d == 0 ? 1u : (unsigned)sy_;
-              cell_states_b<LIM, SDR>(m, c, p, FT(GA(GRLO + d, pc)), FT(GA(GRLO + d, pc + st)), FT(0.5 * GA(GDL + d, pc)),
-                                      bm, bp, Sm_, Sp_);
-            }
-            if (pass == 0) {
+          for (int v = 0; v < 5; ++v) {
+            m[v] = sP[v][ly + 2 - sy][lx + 2 - sx];
+            c[v] = sP[v][ly + 2][lx + 2];
+            p[v] = sP[v][ly + 2 + sy][lx + 2 + sx];
+          }
+          FT bm = FT(1), bp = FT(1);
+          const unsigned pc = off + (unsigned)(T.i0 + lx + 1) + (unsigned)sy_ * (unsigned)(T.j0 + ly + 1) +
+                              sz * (unsigned)(k + 1);
+          const unsigned st = d == 0 ? 1u : (unsigned)sy_;
+          if constexpr (SDR) {
+            const int cc = d == 0 ? T.i0 + lx : T.j0 + ly;
+            bp = FT(beta[cc >= 1 ? pc : pc + st]);
+            bm = FT(beta[cc >= 2 ? pc - st : pc]);
+          }
+          if constexpr (GEO == 1) {
+            rlm = FT(GA(GRLO + d, pc));
+            rlp = FT(GA(GRLO + d, pc + st));
+            hdl = FT(0.5 * GA(GDL + d, pc));
+          }
+          cell_states_b<LIM, SDR>(m, c, p, rlm, rlp, hdl, bm, bp, Sm_, Sp_);
+        };
+        if (iface) {
+          FT Sp_[5];
+          states(0, tx, ty, Smx, Sp_);
 #pragma unroll
-              for (int v = 0; v < 5; ++v) { Smx[v] = Sm_[v]; sSx[v][ly][lx + 1] = Sp_[v]; }
-            } else if (pass == 1) {
+          for (int v = 0; v < 5; ++v) sSx[v][ty][tx + 1] = Sp_[v];
+        }
+        if (jface) {
+          FT Sp_[5];
+          states(1, tx, ty, Smy, Sp_);
 #pragma unroll
-              for (int v = 0; v < 5; ++v) { Smy[v] = Sm_[v]; sSy[v][ly + 1][lx] = Sp_[v]; }
-            } else if (d == 0) {
-              if (xplus) {
+          for (int v = 0; v < 5; ++v) sSy[v][ty + 1][tx] = Sp_[v];
+        }
+        if (xval) {      // wave 0: the x-halo cells (low: its "+" i-state, high: its "-" i-state)
+          FT Sm_[5], Sp_[5];
+          states(0, xlx, xly, Sm_, Sp_);
+          if (xplus) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) sSx[v][ly][0] = Sp_[v];
-              } else {
+            for (int v = 0; v < 5; ++v) sSx[v][xly][0] = Sp_[v];
+          } else {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) sMx[v][ly] = Sm_[v];
-              }
-            } else {
-              if (xplus) {
+            for (int v = 0; v < 5; ++v) sMx[v][xly] = Sm_[v];
+          }
+        }
+        if (yval) {      // wave 1: the y-halo cells
+          FT Sm_[5], Sp_[5];
+          states(1, ylx, yly, Sm_, Sp_);
+          if (yplus) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) sSy[v][0][lx] = Sp_[v];
-              } else {
+            for (int v = 0; v < 5; ++v) sSy[v][0][ylx] = Sp_[v];
+          } else {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) sMy[v][lx] = Sm_[v];
-              }
-            }
+            for (int v = 0; v < 5; ++v) sMy[v][ylx] = Sm_[v];
           }
         }
         __syncthreads();
 
-        // ---- one Riemann problem per i / j face: the own low faces, then the patch edges
-        const int e = (tid + 64 * (k & (NW - 1))) % NTc;     // rotating patch-edge assignment
-        const int nr = e < NE ? 3 : 2;
-#pragma unroll 1
-        for (int r = 0; r < nr; ++r) {
-          FT Lq[5], Rq[5], F[5];
-          int d, fi, fj;
-          bool ok;
-          if (r == 0) {
-            d = 0; fi = i; fj = j; ok = iface;
-#pragma unroll
-            for (int v = 0; v < 5; ++v) { Lq[v] = sSx[v][ty][tx]; Rq[v] = Smx[v]; }
-          } else if (r == 1) {
-            d = 1; fi = i; fj = j; ok = jface;
-#pragma unroll
-            for (int v = 0; v < 5; ++v) { Lq[v] = sSy[v][ty][tx]; Rq[v] = Smy[v]; }
-          } else if (e < TYc) {
-            d = 0; fi = T.i0 + TXc; fj = T.j0 + e; ok = fi <= ni + 1 && fj <= nj;
-#pragma unroll
-            for (int v = 0; v < 5; ++v) { Lq[v] = sSx[v][e][TXc]; Rq[v] = sMx[v][e]; }
-          } else {
-            const int x = e - TYc;
-            d = 1; fi = T.i0 + x; fj = T.j0 + TYc; ok = fi <= ni && fj <= nj + 1;
-#pragma unroll
-            for (int v = 0; v < 5; ++v) { Lq[v] = sSy[v][TYc][x]; Rq[v] = sMy[v][x]; }
-          }
-          if (!ok) continue;
+        // ---- one Riemann problem per i / j face: the own low faces (fixed directions), then
+        // the faces on the patch's high edges (a rotating wave)
+        auto solve = [&](const int d, int fi, int fj, const FT* Lq, const FT* Rq, FT* F) {
           stage_face<RS, FT, GEO>(d, fi, fj, k, Lq, Rq, F, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g, beta);
           if constexpr (VIS) {
             if (Bp->visc) {
@@ -603,18 +602,43 @@ __global__ void __launch_bounds__(TXc* TYc)
               }
             }
           }
-          if (r == 0) {
+        };
+        if (iface) {
+          FT Lq[5], F[5];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) sFi[v][ty][tx] = F[v];
-          } else if (r == 1) {
+          for (int v = 0; v < 5; ++v) Lq[v] = sSx[v][ty][tx];
+          solve(0, i, j, Lq, Smx, F);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) sFj[v][ty][tx] = F[v];
-          } else if (d == 0) {
+          for (int v = 0; v < 5; ++v) sFi[v][ty][tx] = F[v];
+        }
+        if (jface) {
+          FT Lq[5], F[5];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) sFi[v][e][TXc] = F[v];
-          } else {
+          for (int v = 0; v < 5; ++v) Lq[v] = sSy[v][ty][tx];
+          solve(1, i, j, Lq, Smy, F);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) sFj[v][TYc][e - TYc] = F[v];
+          for (int v = 0; v < 5; ++v) sFj[v][ty][tx] = F[v];
+        }
+        const int e = (tid + 64 * (k & (NW - 1))) % NTc;     // rotating patch-edge assignment
+        if (e < NE) {
+          FT Lq[5], Rq[5], F[5];
+          const bool ex = e < TYc;
+          const int d = ex ? 0 : 1;
+          const int fi = ex ? T.i0 + TXc : T.i0 + e - TYc, fj = ex ? T.j0 + e : T.j0 + TYc;
+          if (ex ? (fi <= ni + 1 && fj <= nj) : (fi <= ni && fj <= nj + 1)) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+              Lq[v] = ex ? sSx[v][e][TXc] : sSy[v][TYc][e - TYc];
+              Rq[v] = ex ? sMx[v][e] : sMy[v][e - TYc];
+            }
+            solve(d, fi, fj, Lq, Rq, F);
+            if (ex) {
+#pragma unroll
+              for (int v = 0; v < 5; ++v) sFi[v][e][TXc] = F[v];
+            } else {
+#pragma unroll
+              for (int v = 0; v < 5; ++v) sFj[v][TYc][e - TYc] = F[v];
+            }
           }
         }
         __syncthreads();
