@@ -30,7 +30,8 @@ def run(cmd, cwd=None):
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
-NLIM = 6      # limiters; mose_lim.hip is compiled once for each (MOSE_LIM_ID)
+NLIM = 6      # limiters; mose_lim.hip is compiled once per limiter and precision
+              # (MOSE_LIM_ID, MOSE_PREC_ID)
 BUILD = os.path.join(HERE, "_build")
 
 
@@ -42,12 +43,13 @@ def _jobs():
 
 
 def build_hip(extra=()):
-    """Compile the host/ABI unit and one unit per limiter in parallel, then link."""
+    """Compile the host/ABI unit and one unit per limiter and precision in parallel, then link."""
     os.makedirs(BUILD, exist_ok=True)
     inc = ["-I", os.path.join(ROOT, "include")]
     units = [(os.path.join(CSRC, "mose_hip.hip"), [], os.path.join(BUILD, "mose_hip.o"))]
-    units += [(os.path.join(CSRC, "mose_lim.hip"), ["-DMOSE_LIM_ID=%d" % k], os.path.join(BUILD, "mose_lim%d.o" % k))
-              for k in range(NLIM)]
+    units += [(os.path.join(CSRC, "mose_lim.hip"), ["-DMOSE_LIM_ID=%d" % k, "-DMOSE_PREC_ID=%d" % q],
+               os.path.join(BUILD, "mose_lim%d_%d.o" % (k, q)))
+              for k in range(NLIM) for q in range(2)]
     procs = []
     pending = list(units)
     failed = None

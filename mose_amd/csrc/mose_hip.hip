@@ -580,17 +580,23 @@ static double* g_scratch_ptr[kMaxDevices] = {};
 
 namespace {
 
-// every limiter's kernel variants (instantiated in mose_lim.hip, one unit per limiter)
+// every limiter's kernel variants (instantiated in mose_lim.hip, one unit per limiter and
+// precision)
+template <int LIM>
+void fill_both(KTable& k, ATable& a) {
+  fill_tables<LIM, 0>(k, a);
+  fill_tables<LIM, 1>(k, a);
+}
 struct Tables {
   KTable k;
   ATable a;
   Tables() {
-    fill_tables<LIM_MINMOD>(k, a);
-    fill_tables<LIM_VANLEER>(k, a);
-    fill_tables<LIM_VANALBADA>(k, a);
-    fill_tables<LIM_FIRST>(k, a);
-    fill_tables<LIM_MC>(k, a);
-    fill_tables<LIM_SUPERBEE>(k, a);
+    fill_both<LIM_MINMOD>(k, a);
+    fill_both<LIM_VANLEER>(k, a);
+    fill_both<LIM_VANALBADA>(k, a);
+    fill_both<LIM_FIRST>(k, a);
+    fill_both<LIM_MC>(k, a);
+    fill_both<LIM_SUPERBEE>(k, a);
   }
 };
 const Tables& tables() {
@@ -789,12 +795,13 @@ static void enqueue_bcfaces(mose_ctx* c, const StageArgs& a) {
   (void)hipLaunchKernel((const void*)fn, dim3(launch_blocks(n)), dim3(256), args, 0, c->stream);
 }
 
+// stage-kernel geometry kind: 0 rectilinear, 1 general metrics, 2 rectilinear viscous
+static int geo_kind(const mose_ctx* c) { return c->lay.curvi ? 1 : (c->lay.visc ? 2 : 0); }
+
 static StageFn stage_fn(mose_ctx* c, int mode) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
   if (c->cfg.muscl_sd) return ktable().s[c->lay.curvi ? 1 : 0][c->cfg.limiter][c->cfg.riemann][mode];
-  if (c->lay.curvi) return ktable().g[0][p][c->cfg.limiter][c->cfg.riemann][mode];
-  if (c->lay.visc) return ktable().g[1][p][c->cfg.limiter][c->cfg.riemann][mode];
-  return ktable().f[p][c->cfg.limiter][c->cfg.riemann][c->shape][mode];
+  return ktable().f[geo_kind(c)][p][c->cfg.limiter][c->cfg.riemann][c->shape][mode];
 }
 
 static void launch_dt(mose_ctx* c, const DtArgs& da) {
@@ -991,9 +998,9 @@ static int ensure_stage(mose_ctx* c, int64_t n) {
 
 static bool shape_compiled(const mose_ctx* c, int shape) {
   const int p = c->cfg.precision == MOSE_PREC_FP32_FLUX ? 1 : 0;
-  if (c->lay.curvi || c->lay.visc || c->cfg.muscl_sd) return shape == 0;
   if (shape < 0 || shape >= NSHAPE) return false;
-  return ktable().f[p][c->cfg.limiter][c->cfg.riemann][shape][0] != nullptr;
+  if (c->cfg.muscl_sd) return shape == 0;
+  return ktable().f[geo_kind(c)][p][c->cfg.limiter][c->cfg.riemann][shape][0] != nullptr;
 }
 
 // Every kernel launched with one thread per patch column takes tx * ty threads: refuse a

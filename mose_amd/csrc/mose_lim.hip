@@ -1,6 +1,7 @@
 // mose_lim.hip — the kernel variants of one limiter (MOSE_LIM_ID: 0 minmod, 1 van Leer,
-// 2 van Albada, 3 first order, 4 MC, 5 superbee) for every Riemann solver, precision, tile
-// shape, geometry kind and mode, and their table entries.  Compiled once per limiter.
+// 2 van Albada, 3 first order, 4 MC, 5 superbee) in one precision (MOSE_PREC_ID: 0 FP64,
+// 1 FP32 flux) for every Riemann solver, tile shape, geometry kind and mode, and their table
+// entries.  Compiled once per (limiter, precision) pair.
 #include <hip/hip_runtime.h>
 
 #include "../../include/mose_hip.h"
@@ -15,6 +16,9 @@ using namespace mose;
 #ifndef MOSE_LIM_ID
 #error "MOSE_LIM_ID must name the limiter this unit compiles"
 #endif
+#ifndef MOSE_PREC_ID
+#error "MOSE_PREC_ID must name the precision this unit compiles"
+#endif
 
 namespace {
 
@@ -25,48 +29,56 @@ void stage_modes(StageFn* t) {
   t[2] = k_stage<LIM, RS, FT, 2, TXc, TYc, GEO, SDR>;
 }
 
-template <int LIM, int RS>
+template <int LIM, int RS, int P, int GEO, int SH>
+void shape_entry(KTable& k) {
+  using FT = typename std::conditional<P == 0, double, float>::type;
+  if constexpr (shape_built(GEO, SH, rs_takes_beta(RS)))
+    stage_modes<LIM, RS, FT, kShape[P][SH][0], kShape[P][SH][1], GEO>(k.f[GEO][P][LIM][RS][SH]);
+}
+
+template <int LIM, int RS, int P, int GEO>
+void geo_entries(KTable& k) {
+  shape_entry<LIM, RS, P, GEO, 0>(k);
+  shape_entry<LIM, RS, P, GEO, 1>(k);
+  shape_entry<LIM, RS, P, GEO, 2>(k);
+  shape_entry<LIM, RS, P, GEO, 3>(k);
+}
+
+template <int LIM, int RS, int P>
 void solver_entries(KTable& k, ATable& a) {
-  stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1]>(k.f[0][LIM][RS][0]);
-  stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1]>(k.f[1][LIM][RS][0]);
-  stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 1>(k.g[0][0][LIM][RS]);
-  stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1], 1>(k.g[0][1][LIM][RS]);
-  stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 2>(k.g[1][0][LIM][RS]);
-  stage_modes<LIM, RS, float, kShape[1][0][0], kShape[1][0][1], 2>(k.g[1][1][LIM][RS]);
-  if constexpr (LIM != LIM_FIRST) {     // MUSCL-SD (no slopes to scale at first order)
-    stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 0, true>(k.s[0][LIM][RS]);
-    stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 1, true>(k.s[1][LIM][RS]);
-  }
-  if constexpr (LIM == LIM_VANLEER) {   // the tile tuner's alternative shapes
-    stage_modes<LIM, RS, double, kShape[0][1][0], kShape[0][1][1]>(k.f[0][LIM][RS][1]);
-    stage_modes<LIM, RS, double, kShape[0][2][0], kShape[0][2][1]>(k.f[0][LIM][RS][2]);
-    stage_modes<LIM, RS, float, kShape[1][1][0], kShape[1][1][1]>(k.f[1][LIM][RS][1]);
-    stage_modes<LIM, RS, float, kShape[1][2][0], kShape[1][2][1]>(k.f[1][LIM][RS][2]);
-    stage_modes<LIM, RS, double, kShape[0][3][0], kShape[0][3][1]>(k.f[0][LIM][RS][3]);
-    stage_modes<LIM, RS, float, kShape[1][3][0], kShape[1][3][1]>(k.f[1][LIM][RS][3]);
-  }
-  k.bc[0][LIM][RS] = k_bcface<LIM, RS, double>;
-  k.bc[1][LIM][RS] = k_bcface<LIM, RS, float>;
-  if constexpr (!rs_takes_beta(RS)) {   // no adjoint of the shock-detector solvers
-    a.face[LIM][RS][0] = k_adj_face<LIM, RS, 0>;
-    a.face[LIM][RS][1] = k_adj_face<LIM, RS, 1>;
-    a.bc[LIM][RS] = k_adj_bcface<LIM, RS>;
+  using FT = typename std::conditional<P == 0, double, float>::type;
+  geo_entries<LIM, RS, P, 0>(k);
+  geo_entries<LIM, RS, P, 1>(k);
+  geo_entries<LIM, RS, P, 2>(k);
+  k.bc[P][LIM][RS] = k_bcface<LIM, RS, FT>;
+  if constexpr (P == 0) {
+    if constexpr (LIM != LIM_FIRST) {     // MUSCL-SD (no slopes to scale at first order)
+      stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 0, true>(k.s[0][LIM][RS]);
+      stage_modes<LIM, RS, double, kShape[0][0][0], kShape[0][0][1], 1, true>(k.s[1][LIM][RS]);
+    }
+    if constexpr (!rs_takes_beta(RS)) {   // no adjoint of the shock-detector solvers
+      a.face[LIM][RS][0] = k_adj_face<LIM, RS, 0>;
+      a.face[LIM][RS][1] = k_adj_face<LIM, RS, 1>;
+      a.bc[LIM][RS] = k_adj_bcface<LIM, RS>;
+    }
   }
 }
 
 }  // namespace
 
-template <int LIM>
+template <int LIM, int P>
 void fill_tables(KTable& k, ATable& a) {
-  solver_entries<LIM, RS_HLLC>(k, a);
-  solver_entries<LIM, RS_AUSMP>(k, a);
-  solver_entries<LIM, RS_HLLE>(k, a);
-  solver_entries<LIM, RS_SLAU>(k, a);
-  solver_entries<LIM, RS_SLAU2>(k, a);
-  solver_entries<LIM, RS_HLLEPP>(k, a);
-  solver_entries<LIM, RS_HLLCP>(k, a);
-  a.cell[LIM][0] = k_adj_cell<LIM, 0>;
-  a.cell[LIM][1] = k_adj_cell<LIM, 1>;
+  solver_entries<LIM, RS_HLLC, P>(k, a);
+  solver_entries<LIM, RS_AUSMP, P>(k, a);
+  solver_entries<LIM, RS_HLLE, P>(k, a);
+  solver_entries<LIM, RS_SLAU, P>(k, a);
+  solver_entries<LIM, RS_SLAU2, P>(k, a);
+  solver_entries<LIM, RS_HLLEPP, P>(k, a);
+  solver_entries<LIM, RS_HLLCP, P>(k, a);
+  if constexpr (P == 0) {
+    a.cell[LIM][0] = k_adj_cell<LIM, 0>;
+    a.cell[LIM][1] = k_adj_cell<LIM, 1>;
+  }
 }
 
-template void fill_tables<MOSE_LIM_ID>(KTable&, ATable&);
+template void fill_tables<MOSE_LIM_ID, MOSE_PREC_ID>(KTable&, ATable&);

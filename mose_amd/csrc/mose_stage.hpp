@@ -305,11 +305,13 @@ __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const 
 // its direction fixed; the spacing data along x and y of a tile sit in LDS.
 // Occupancy floor: FP32 flux 3 waves per SIMD (2 for 512-thread patches, whose workgroup
 // brings two waves to every SIMD); FP64 2 (256 registers: without the floor some variants take
-// a few AGPRs beyond 256 and fall to one wave per SIMD); the viscous kernels, whose LDS allows
-// one workgroup per CU, none.
+// a few AGPRs beyond 256 and fall to one wave per SIMD); the viscous kernels 2 for the FP64
+// 128-thread patch (its 51 KB of LDS let three workgroups share a CU; the floor costs a few
+// spilled registers and is 16 % faster than one wave per SIMD), else none (the LDS of the
+// 256-thread viscous patches allows one workgroup per CU).
 template <int LIM, int RS, typename FT, int MODE, int TXc, int TYc, int GEO, bool SDR>
 __global__ void __launch_bounds__(TXc* TYc)
-    __attribute__((amdgpu_waves_per_eu(GEO == 2 ? 1 : (sizeof(FT) == 4 ? (TXc * TYc == 256 ? 3 : (TXc * TYc == 512 ? 2 : 1)) : 2))))
+    __attribute__((amdgpu_waves_per_eu(GEO == 2 ? (sizeof(FT) == 8 && TXc * TYc <= 128 ? 2 : 1) : (sizeof(FT) == 4 ? (TXc * TYc == 256 ? 3 : (TXc * TYc == 512 ? 2 : 1)) : 2))))
     k_stage(StageArgs a) {
   constexpr int NTc = TXc * TYc;
   constexpr int NW = NTc / 64;                   // waves per workgroup
