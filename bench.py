@@ -125,6 +125,7 @@ def steady_adjoint_bump(ni=512, nj=256, ncontrol=64, fd_steps=(1e-6, 1e-7)):
     mk = lambda h: cases.bump_channel(ni=ni, nj=nj, ncontrol=ncontrol, heights=h, limiter="minmod", alpha=0.03)
     case = mk(h0)
     g = GpuSolver(case)
+    g.autotune(3)
     conv = dict(max_steps=400000, tol=5e-15, chunk=2000, stall=6)
     t0 = time.perf_counter()
     steps, norms = sa.converge(g, **conv)
@@ -203,6 +204,7 @@ def taylor_green_batch(steps, warmup, n=192, nbox=8):
     for prec, name in ((abi.PREC_FP32_FLUX, "fp32_flux"), (abi.PREC_FP64, "fp64")):
         cfg = abi.make_cfg(cases.GAMMA, cases.RGAS_NONDIM, case.cfg.cfl, "vanleer", "AUSM+", 3, True, precision=prec)
         g = GpuSolver(case, cfg=cfg)
+        tiling, _ = g.autotune(2)
         g.advance(warmup)
         t0 = time.perf_counter()
         t_sim, _ = g.advance(steps)
@@ -210,7 +212,7 @@ def taylor_green_batch(steps, warmup, n=192, nbox=8):
         ms = g.time_kernels(nrep=5)
         ke[name] = np.array(cases.kinetic_energy([g.get_state(b + 1) for b in range(nbox)]))
         out[name] = {"value": case.ncells * steps / secs, "ms_per_step": 1e3 * secs / steps,
-                     "stage_kernel_ms": ms[2], "sim_time": t_sim}
+                     "stage_kernel_ms": ms[2], "sim_time": t_sim, "tiling": tiling}
         log("[bench] tgv %s: %.3f ms/step" % (name, 1e3 * secs / steps))
         g.close()
     out["speedup_fp32_vs_fp64"] = out["fp32_flux"]["value"] / out["fp64"]["value"]
@@ -229,6 +231,7 @@ def taylor_green_adjoint(nsteps, n=192, nbox=8, stride=0):
     os.environ["MOSE_PROGRESS"] = "1"      # one line per reversed segment (long sweeps)
     case = cases.taylor_green(n=n, nbox=nbox, precision=abi.PREC_FP64)
     g = GpuSolver(case)
+    g.autotune(2)
     dt = g.compute_dt()
     g.march_kinetic_energy(2, dt, adjoint=False)                 # warm-up
     t0 = time.perf_counter()

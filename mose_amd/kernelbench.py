@@ -6,7 +6,8 @@ HIP events (mose_time_kernels), then runs a few full steps.  Used under rocprofv
     rocprofv3 --kernel-trace --stats -d <dir> -- python3 -m mose_amd.kernelbench --n 256
     rocprofv3 --pmc FETCH_SIZE -d <dir> -- python3 -m mose_amd.kernelbench --n 128 --reps 3
 
-Options: --prec fp64|fp32, --riemann HLLC|AUSM+|HLLE|SLAU|SLAU2, --limiter vanleer|minmod|vanalbada,
+Options: --case vortex|tgv (tgv: --nbox viscous Taylor-Green boxes of n^3, van Leer + AUSM+),
+--prec fp64|fp32, --riemann HLLC|AUSM+|HLLE|SLAU|SLAU2, --limiter vanleer|minmod|vanalbada,
 --autotune picks the tile shape / k chunk / tile order by timing first,
 --copy also runs a plain device-to-device copy of 5 n^3 doubles (hipMemcpy) so the HBM
 byte counters (FETCH_SIZE / WRITE_SIZE) can be calibrated against a kernel of known size.
@@ -19,6 +20,8 @@ import time
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--case", default="vortex", choices=["vortex", "tgv"])
+    ap.add_argument("--nbox", type=int, default=8)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--riemann", default="HLLC")
@@ -31,7 +34,10 @@ def main():
     from mose_amd import abi, cases
     from mose_amd.lib import GpuSolver
     prec = abi.PREC_FP64 if args.prec == "fp64" else abi.PREC_FP32_FLUX
-    case = cases.vortex(n=args.n, riemann=args.riemann, limiter=args.limiter, precision=prec)
+    if args.case == "tgv":
+        case = cases.taylor_green(n=args.n, nbox=args.nbox, precision=prec)
+    else:
+        case = cases.vortex(n=args.n, riemann=args.riemann, limiter=args.limiter, precision=prec)
     g = GpuSolver(case)
     print("[kernelbench] setup done: %d cells" % case.ncells, flush=True)
     tuned = None
@@ -42,10 +48,11 @@ def main():
     t0 = time.perf_counter()
     g.advance(args.steps)
     dt = time.perf_counter() - t0
-    out = {"n": args.n, "prec": args.prec, "riemann": args.riemann, "limiter": args.limiter,
+    out = {"case": args.case, "cells": case.ncells, "n": args.n, "prec": args.prec, "riemann": case.cfg.riemann,
+           "limiter": case.cfg.limiter,
            "ms_dt": ms[0], "ms_ghost": ms[1], "ms_stage": ms[2], "ms_stage_last": ms[3],
            "steps": args.steps, "s_steps": dt,
-           "stage_GBs_algorithmic": 120.0 * case.ncells / (ms[2] * 1e-3) / 1e9,
+           "stage_GBs_algorithmic": 120.0 * case.ncells / (ms[2] * 1e-3) / 1e9, "tiling_in_use": g.tiling(),
            "tiling": tuned[0] if tuned else None}
     if args.copy:
         import ctypes as C
