@@ -286,22 +286,30 @@ def scheme_legs(n, steps, warmup, ref_ms_per_step):
     return out
 
 
-def cpu_baseline(n=96, steps=20):
-    """The C oracle (no fast-math; OpenMP over lines of cells like the reference's
-    collapse(2) loops, OMP_NUM_THREADS threads) on the same workload at n^3."""
+def cpu_baseline(n=128, steps=20):
+    """The C oracle on the same workload at n^3, on OMP_NUM_THREADS threads (OpenMP over lines of
+    cells like the reference's collapse(2) loops): the build with the reference's Release-style
+    flags (-O3 -funroll-loops -finline-functions -ffast-math -march=x86-64-v3
+    -fomit-frame-pointer) is the baseline; the parity-checker build (-O2, no fast-math) is
+    reported beside it."""
     from mose_amd import cases
     from oracle.oracle import OracleSolver, num_threads
-    case = cases.vortex(n=n)
-    o = OracleSolver(case)
-    o.step()
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    out = {}
+    for variant in ("release", "check"):
+        case = cases.vortex(n=n)
+        o = OracleSolver(case, variant=variant)
         o.step()
-    dt = time.perf_counter() - t0
-    nt = num_threads()
-    return {"value": n ** 3 * steps / dt, "unit": "cell-updates/s", "cores": nt, "kind": "port",
-            "sample": "vortex %d^3, %d steps after 1 warm-up step, C oracle (gcc -O2 -fopenmp, %d threads) on %s"
-                      % (n, steps, nt, platform.processor() or platform.machine())}
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.step()
+        out[variant] = n ** 3 * steps / (time.perf_counter() - t0)
+        o.close()
+    nt = num_threads("release")
+    return {"value": out["release"], "unit": "cell-updates/s", "cores": nt, "kind": "port",
+            "check_build_value": out["check"],
+            "sample": "vortex %d^3, %d steps after 1 warm-up step, C oracle built with -O3 -funroll-loops "
+                      "-finline-functions -ffast-math -march=x86-64-v3 -fopenmp (%d threads) on %s; "
+                      "check_build_value: the -O2 parity build" % (n, steps, nt, platform.processor() or platform.machine())}
 
 
 def main():

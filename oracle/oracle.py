@@ -11,13 +11,15 @@ import numpy as np
 from mose_amd import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+_LIBS = {}
+# "check": the parity checker (no fast-math); "release": the same source with the reference's
+# Release-style flags, used only to time the CPU baseline
+_FILES = {"check": "libmose_oracle.so", "release": "libmose_oracle_release.so"}
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "libmose_oracle.so")
+def lib(variant="check"):
+    if variant not in _LIBS:
+        path = os.path.join(_HERE, _FILES[variant])
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle`")
         L = C.CDLL(path)
@@ -54,8 +56,8 @@ def lib():
         L.ob_sd_tramel.argtypes = [C.c_double] * 7
         L.ob_prim2cons.argtypes = [P(abi.MoseCfg), P(C.c_double), P(C.c_double)]
         L.ob_cons2prim.argtypes = [P(abi.MoseCfg), P(C.c_double), P(C.c_double)]
-        _LIB = L
-    return _LIB
+        _LIBS[variant] = L
+    return _LIBS[variant]
 
 
 def _dp(a):
@@ -71,8 +73,8 @@ class OracleError(RuntimeError):
 class OracleSolver:
     """The CPU oracle domain for one Case."""
 
-    def __init__(self, case, cfg=None):
-        L = lib()
+    def __init__(self, case, cfg=None, variant="check"):
+        self.L = L = lib(variant)
         self.case = case
         self.cfg = cfg if cfg is not None else case.cfg
         self.h = L.ob_create(C.byref(self.cfg), len(case.blocks))
@@ -93,12 +95,12 @@ class OracleSolver:
     def _check(self, rc):
         if rc:
             e = abi.MoseErr()
-            lib().ob_last_error(self.h, C.byref(e))
+            self.L.ob_last_error(self.h, C.byref(e))
             raise OracleError(e)
 
     def close(self):
         if self.h:
-            lib().ob_destroy(self.h)
+            self.L.ob_destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -106,26 +108,26 @@ class OracleSolver:
     def put_state(self, b, P):
         P = np.ascontiguousarray(P, dtype=np.float64)
         assert P.shape == abi.padded_shape(self.dims[b - 1])
-        self._check(lib().ob_put_state(self.h, b, _dp(P)))
+        self._check(self.L.ob_put_state(self.h, b, _dp(P)))
 
     def get_state(self, b):
         P = np.zeros(abi.padded_shape(self.dims[b - 1]))
-        self._check(lib().ob_get_state(self.h, b, _dp(P)))
+        self._check(self.L.ob_get_state(self.h, b, _dp(P)))
         return P
 
     def fill_ghost(self):
-        self._check(lib().ob_fill_ghost(self.h))
+        self._check(self.L.ob_fill_ghost(self.h))
 
     def residual(self, b):
         R = np.zeros(abi.padded_shape(self.dims[b - 1]))
-        self._check(lib().ob_residual(self.h, b, _dp(R)))
+        self._check(self.L.ob_residual(self.h, b, _dp(R)))
         return R
 
     def irs_smooth(self, b, R, beta):
         """Residual_Smoothing of the field R (padded blk%R layout) of block b, the other
         blocks' residuals zero; returns the smoothed field."""
         R = np.ascontiguousarray(R, dtype=np.float64).copy()
-        self._check(lib().ob_irs_smooth(self.h, b, float(beta), _dp(R)))
+        self._check(self.L.ob_irs_smooth(self.h, b, float(beta), _dp(R)))
         return R
 
     def residuals(self):
@@ -133,25 +135,25 @@ class OracleSolver:
 
     def compute_dt(self, it=1):
         dt = C.c_double()
-        self._check(lib().ob_compute_dt(self.h, it, C.byref(dt)))
+        self._check(self.L.ob_compute_dt(self.h, it, C.byref(dt)))
         return dt.value
 
     def step(self):
         dt, t = C.c_double(), C.c_double()
         norms = np.zeros(5)
-        self._check(lib().ob_step(self.h, C.byref(dt), C.byref(t), _dp(norms)))
+        self._check(self.L.ob_step(self.h, C.byref(dt), C.byref(t), _dp(norms)))
         return dt.value, t.value, norms
 
     def time(self):
         t, it = C.c_double(), C.c_int32()
-        lib().ob_get_time(self.h, C.byref(t), C.byref(it))
+        self.L.ob_get_time(self.h, C.byref(t), C.byref(it))
         return t.value, it.value
 
     def metrics(self, b):
         npad = int(np.prod(abi.padded_shape(self.dims[b - 1])[:3]))
         fn = np.zeros((3, npad, 3)); fa = np.zeros((3, npad)); dl = np.zeros((npad, 3))
         M = np.zeros((npad, 9)); vol = np.zeros(npad)
-        lib().ob_get_metrics(self.h, b, _dp(fn), _dp(fa), _dp(dl), _dp(M), _dp(vol))
+        self.L.ob_get_metrics(self.h, b, _dp(fn), _dp(fa), _dp(dl), _dp(M), _dp(vol))
         sh = abi.padded_shape(self.dims[b - 1])[:3]
         return dict(fn=fn.reshape((3,) + sh + (3,)), fa=fa.reshape((3,) + sh), dl=dl.reshape(sh + (3,)),
                     M=M.reshape(sh + (3, 3)), vol=vol.reshape(sh))
@@ -159,11 +161,11 @@ class OracleSolver:
     def dtlocal(self, b):
         sh = abi.padded_shape(self.dims[b - 1])[:3]
         dt = np.zeros(int(np.prod(sh)))
-        lib().ob_get_dtlocal(self.h, b, _dp(dt))
+        self.L.ob_get_dtlocal(self.h, b, _dp(dt))
         return dt.reshape(sh)
 
     def ndir(self):
-        return lib().ob_ndir(self.h)
+        return self.L.ob_ndir(self.h)
 
 
 def limiter(kind, x, y):
@@ -202,6 +204,6 @@ def cons2prim(cfg, U):
     return P
 
 
-def num_threads():
+def num_threads(variant="check"):
     """OpenMP threads the oracle runs its sweeps on."""
-    return int(lib().ob_num_threads())
+    return int(lib(variant).ob_num_threads())
