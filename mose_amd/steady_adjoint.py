@@ -3,8 +3,9 @@
 Objective (the supersonic bump channel of cases.bump_channel, 2-D, nk = 1):
     J = sum over the bump's wall faces (j-min faces whose centre lies in 0 <= x <= 1) of
         p(cell next to the wall) * A(face),
-with A the face area (segment length in the x-y plane times the extrusion depth, the mean
-of the two node planes).  Its gradient with respect to the control-point heights h and the
+with A the face area: the segment length in the x-y plane (the mean of the two node planes)
+times the extrusion depth at the face's high node column.  Its gradient with respect to the
+control-point heights h and the
 405 inflow record (mach, T, p, alpha, beta) at a converged state U* of the local-time-step
 iteration (Mod_Explicit.f90:45-85) is
     dJ/dh = dJ/dX|_P . dX/dh - psi^T dR/dX . dX/dh,   dJ/d(inflow) = -psi^T dR/d(inflow),
@@ -65,12 +66,14 @@ def wall_objective(case, P):
             L = float(np.hypot(seg[0], seg[1]))
             area += 0.5 * L
             dA.append((k, seg / L))
-        dz = X[1, 0, i, 2] - X[0, 0, i, 2]
+        dz = X[1, 0, i, 2] - X[0, 0, i, 2]           # extrusion depth at node column i
         J += p * area * dz
         dJdP[2, 2, i + 1, 4] = area * dz
         for k, u in dA:                              # d(0.5 L_k dz)/d(node) along the segment
             dJdX[k, 0, i, :2] += p * 0.5 * dz * u
             dJdX[k, 0, i - 1, :2] -= p * 0.5 * dz * u
+        dJdX[1, 0, i, 2] += p * area
+        dJdX[0, 0, i, 2] -= p * area
     return J, dJdP, dJdX
 
 
