@@ -15,7 +15,6 @@ faces of BC_Symmetry_Eul (Lib_BC_Fluxes_Symmetry.f90:14-125).
 """
 import numpy as np
 
-from . import abi
 from .cases import bump_control_shape
 
 
