@@ -123,8 +123,8 @@ __global__ void __launch_bounds__(256) k_adj_vface(AdjArgs a) {
     double M[3];
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      const double mR = 1.0 / Ln[x][ijk[x] + 1].dl;
-      M[x] = x == d ? 0.5 * (1.0 / Ln[x][ijk[x]].dl + mR) : mR;
+      const double mR = Ln[x][ijk[x] + 1].rdl;
+      M[x] = x == d ? 0.5 * (Ln[x][ijk[x]].rdl + mR) : mR;
     }
     const double area = Ln[tb][ijk[tb] + 1].delta * Ln[tc][ijk[tc] + 1].delta;
     double Pm[5];

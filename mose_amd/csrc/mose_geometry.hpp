@@ -420,7 +420,10 @@ static bool build_layout(const std::vector<HostBlock>& hb, const std::vector<mos
     const HostBlock& B = hb[b];
     const int n[3] = {B.ni, B.nj, B.nk};
     for (int d = 0; d < 3; ++d) {
-      for (int q = -1; q <= n[d] + 2; ++q) LE(b, d, q).lp = 0.5 * LE(b, d, q).dl;
+      for (int q = -1; q <= n[d] + 2; ++q) {
+        LE(b, d, q).lp = 0.5 * LE(b, d, q).dl;
+        LE(b, d, q).rdl = 1.0 / LE(b, d, q).dl;    // correctly rounded, as the device's 1.0 / dl
+      }
       for (int q = 0; q <= n[d] + 2; ++q) {
         LineEnt& e = LE(b, d, q);
         e.rl1 = 1.0 / (0.5 * (LE(b, d, q - 1).dl + e.dl));

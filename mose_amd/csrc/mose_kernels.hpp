@@ -37,7 +37,7 @@ struct LineEnt {
   double dl;      // cell length along the line (Compute_Metric_Tensor dl)
   double delta;   // node spacing x_i - x_{i-1} (face areas / volume)
   double rl0, rl1, rl2, lm, lp;   // spacing of the face between cells i-1 and i
-  double pad;
+  double rdl;     // 1 / dl (the diagonal metric of the viscous terms)
 };
 
 struct BlockDev {

@@ -231,9 +231,9 @@ __device__ __forceinline__ void visc_kface(const BlockDev* __restrict__ Bp, int 
     }
   } else {
     FT M[3];
-    const double rzL = 1.0 / Lz[k + 1].dl, rzR = 1.0 / Lz[k + 2].dl;
-    M[0] = FT(1.0 / Lx[i + 1].dl);
-    M[1] = FT(1.0 / Ly[j + 1].dl);
+    const double rzL = Lz[k + 1].rdl, rzR = Lz[k + 2].rdl;
+    M[0] = FT(Lx[i + 1].rdl);
+    M[1] = FT(Ly[j + 1].rdl);
     M[2] = FT(0.5 * (rzL + rzR));
     FT Fv[5];
     visc_interior<FT, TXc, TYc>(2, tx, ty, sV, (k + 2) % 3, (k + 3) % 3, (k + 4) % 3, Q0[0], Q0[4], Q1[0], Q1[4], M, Rg,
@@ -593,9 +593,9 @@ __global__ void __launch_bounds__(TXc* TYc)
                 const FT rr = sP[0][lyL + 2 + (d == 1)][lxL + 2 + (d == 0)], pr = sP[4][lyL + 2 + (d == 1)][lxL + 2 + (d == 0)];
                 // averaged diagonal metric 1 / dl of the two cells
                 FT M[3];
-                const double rxL = 1.0 / Lx[fi - (d == 0) + 1].dl, rxR = 1.0 / Lx[fi + 1].dl;
-                const double ryL = 1.0 / Ly[fj - (d == 1) + 1].dl, ryR = 1.0 / Ly[fj + 1].dl;
-                const double rz = 1.0 / Lz[k + 1].dl;
+                const double rxL = Lx[fi - (d == 0) + 1].rdl, rxR = Lx[fi + 1].rdl;
+                const double ryL = Ly[fj - (d == 1) + 1].rdl, ryR = Ly[fj + 1].rdl;
+                const double rz = Lz[k + 1].rdl;
                 M[0] = FT(0.5 * (rxL + rxR)); M[1] = FT(0.5 * (ryL + ryR)); M[2] = FT(rz);
                 FT Fv[5];
                 visc_interior<FT, TXc, TYc>(d, lxL, lyL, sV, s0, s1, s2, rl, pl, rr, pr, M, Rg, vis, Fv);
