@@ -99,6 +99,19 @@ def adjoint_ratio(g, case, nrep=5):
     return out
 
 
+def tgv_adjoint_ratio(n=192, nbox=8):
+    """Adjoint/primal ratio on the viscous 8 x 192^3 Taylor-Green batch (FP64, tuned tiling)."""
+    from mose_amd import abi, cases
+    from mose_amd.lib import GpuSolver
+    case = cases.taylor_green(n=n, nbox=nbox, precision=abi.PREC_FP64)
+    g = GpuSolver(case)
+    g.autotune(2)
+    out = adjoint_ratio(g, case, nrep=5)
+    out["cells"] = case.ncells
+    g.close()
+    return out
+
+
 def bump_adjoint():
     """Adjoint/primal ratio on the 512 x 256 body-fitted bump channel (van Albada + HLLC,
     405 inflow, 400 outflow, 300 walls, local time stepping), after 20 steps."""
@@ -412,7 +425,7 @@ def main():
     if not args.no_schemes:
         result["schemes"] = scheme_legs(n, args.scheme_steps, min(args.warmup, 2), result["ms_per_step"])
     if adj_vortex is not None:
-        result["adjoint"] = {"vortex": adj_vortex, "bump_512x256": bump_adjoint(),
+        result["adjoint"] = {"vortex": adj_vortex, "bump_512x256": bump_adjoint(), "tgv_8x192": tgv_adjoint_ratio(),
                              "note": "device ms of one mose_residual_vjp / one mose_residual, FP64; "
                                      "*_geometry: mose_residual_vjp_geometry (state + node + inflow adjoints)"}
     if not args.no_steady_adjoint:

@@ -1029,6 +1029,7 @@ struct AdjArgs {
   double* __restrict__ VB;              // nvf: mu_ref adjoint of each connection viscous face
   double* __restrict__ VBC;             // 50 x nvf: each connection face's primitive adjoints of
                                         // its ten stencil cells, gathered by k_adj_vbc_gather
+  const double* __restrict__ Tp;        // npool: temperature p / (rho R) of every cell
   const int64_t* __restrict__ vbt;      // cells receiving connection-face adjoints
   const int64_t* __restrict__ vbrow;    // their entries vbent[vbrow[r] .. vbrow[r+1]) (face*10 + role)
   const int* __restrict__ vbent;

@@ -5,7 +5,7 @@
 
 // Primitive variables of every pool cell (ghost layers filled), read by the adjoint passes.
 __global__ void __launch_bounds__(256) k_adj_prim(const double* __restrict__ U, double* __restrict__ Pp, int64_t N,
-                                                  double gam) {
+                                                  double gam, double* __restrict__ Tp, double Rg) {
   const double gm1 = gam - 1.0;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
     double u[5], P[5] = {0, 0, 0, 0, 0};
@@ -14,6 +14,8 @@ __global__ void __launch_bounds__(256) k_adj_prim(const double* __restrict__ U, 
     if (u[0] != 0.0) cons2prim(u, gm1, P);
 #pragma unroll
     for (int v = 0; v < 5; ++v) Pp[v * N + c] = P[v];
+    // viscous layouts: the temperature the viscous adjoints read for every stencil cell
+    if (Tp) Tp[c] = P[0] != 0.0 ? P[4] / (P[0] * Rg) : 0.0;
   }
 }
 
@@ -109,7 +111,7 @@ __global__ void __launch_bounds__(256) k_adj_vface(AdjArgs a) {
 #pragma unroll
     for (int m = 0; m < 10; ++m) {
       prim_load(a.Pp, N, cell[m], P[m]);
-      V[m][0] = P[m][1]; V[m][1] = P[m][2]; V[m][2] = P[m][3]; V[m][3] = P[m][4] / (P[m][0] * Rg);
+      V[m][0] = P[m][1]; V[m][1] = P[m][2]; V[m][2] = P[m][3]; V[m][3] = a.Tp[cell[m]];
     }
     double G[4][3];
 #pragma unroll
@@ -226,7 +228,7 @@ __global__ void __launch_bounds__(256) k_adj_vbc(AdjArgs a) {
 #pragma unroll
     for (int m = 0; m < 10; ++m) {
       prim_load(a.Pp, N, cell[m], P[m]);
-      V[m][0] = P[m][1]; V[m][1] = P[m][2]; V[m][2] = P[m][3]; V[m][3] = P[m][4] / (P[m][0] * Rg);
+      V[m][0] = P[m][1]; V[m][1] = P[m][2]; V[m][2] = P[m][3]; V[m][3] = a.Tp[cell[m]];
     }
     const double m2 = (double)e.modfm2;
     double Gl[4][3], Gg[4][3];

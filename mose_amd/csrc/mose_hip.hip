@@ -682,6 +682,7 @@ struct mose_ctx {
   double *d_GF = nullptr, *d_GL = nullptr, *d_BG = nullptr, *d_GE = nullptr, *d_IB = nullptr;
   double *d_VW = nullptr, *d_VB = nullptr;   // viscous adjoint buffers (viscous layouts only)
   double* d_VBC = nullptr;                    // connection-face adjoints before their gather
+  double* d_Tp = nullptr;                     // temperature of every pool cell (viscous adjoints)
   int64_t *d_vbt = nullptr, *d_vbrow = nullptr;
   int* d_vbent = nullptr;
   int nvbt = 0;
@@ -1146,6 +1147,7 @@ static int ensure_adjoint(mose_ctx* c) {
     const size_t nvf = c->lay.vfaces.size();
     HIPCHK(c, hipMalloc((void**)&c->d_VB, std::max<size_t>(nvf, 1) * sizeof(double)));
     HIPCHK(c, hipMalloc((void**)&c->d_VBC, std::max<size_t>(50 * nvf, 1) * sizeof(double)));
+    HIPCHK(c, hipMalloc((void**)&c->d_Tp, c->npool * sizeof(double)));
     // receiving cells of the connection faces (the ten stencil cells of each, k_adj_vbc's
     // order), grouped per cell with the entries in (face, role) order
     std::vector<std::pair<int64_t, int>> te;
@@ -1182,7 +1184,7 @@ static AdjArgs adj_args(mose_ctx* c, const double* U = nullptr) {
   a.nseg = c->nseg; a.nbcf = (int)c->lay.bcfaces.size(); a.ngrows = c->ngrows; a.nblocks = c->nb;
   a.nents = std::max(c->nents, 1);
   a.VW = c->d_VW; a.VB = c->d_VB; a.vfaces = c->d_vfaces; a.nvf = (int)c->lay.vfaces.size();
-  a.VBC = c->d_VBC; a.vbt = c->d_vbt; a.vbrow = c->d_vbrow; a.vbent = c->d_vbent; a.nvbt = c->nvbt;
+  a.VBC = c->d_VBC; a.Tp = c->d_Tp; a.vbt = c->d_vbt; a.vbrow = c->d_vbrow; a.vbent = c->d_vbent; a.nvbt = c->nvbt;
   a.gam = c->cfg.gamma; a.R = c->cfg.Rgas;
   return a;
 }
@@ -1210,7 +1212,7 @@ static void enqueue_vjp(mose_ctx* c, bool geometry = false, double* U = nullptr)
   if (!U) U = c->U[0];
   enqueue_ghost(c, U);
   hipLaunchKernelGGL(k_adj_prim, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, U, c->d_Pp, c->npool,
-                     c->cfg.gamma);
+                     c->cfg.gamma, c->d_Tp, c->cfg.Rgas);
   AdjArgs a = adj_args(c, U);
   if (geometry) {
     a.GF = c->d_GF; a.GL = c->d_GL; a.BG = c->d_BG; a.GE = c->d_GE; a.IB = c->d_IB;
@@ -2186,7 +2188,7 @@ void mose_destroy(mose_ctx* c) {
                   c->d_iter, c->d_tq, c->d_Rbar, c->d_W, c->d_BB, c->d_Pbar, c->d_Pp, c->d_segs, c->d_grows, c->d_gents,
                   c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_pres, c->d_ghostp,
                   c->d_garc_cell, c->d_garc, c->d_rs1, c->d_irsg, c->d_gP, c->d_lam0, c->d_lam1, c->d_Dq,
-                  c->d_apart, c->d_ares, c->d_icell, c->d_VBC, c->d_vbt, c->d_vbrow, c->d_vbent, c->d_volq,
+                  c->d_apart, c->d_ares, c->d_icell, c->d_VBC, c->d_Tp, c->d_vbt, c->d_vbrow, c->d_vbent, c->d_volq,
                   c->d_upart, c->d_ke, c->d_mu, c->d_icbeg, c->d_vfbeg, c->d_vfi};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
