@@ -40,6 +40,12 @@ struct LineEnt {
   double rdl;     // 1 / dl (the diagonal metric of the viscous terms)
 };
 
+// Read-only data through the constant address space: loads from a wave-uniform address
+// become scalar loads (s_load, scalar data cache) instead of vector loads into VGPRs.  Only
+// for arrays no kernel writes while the loading kernel runs.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+
 struct BlockDev {
   int64_t off;       // pool index of padded cell (-1,-1,-1)
   int64_t sz;        // plane stride (ni+4)(nj+4)
@@ -155,6 +161,16 @@ __device__ __forceinline__ void finish_tiles(int* tq) {
   if (atomicAdd(tq + kTileQueues, 1) == (int)gridDim.x - 1) {
     for (int x = 0; x <= kTileQueues; ++x) atomicExch(tq + x, 0);
   }
+}
+
+// Workgroup barrier ordering LDS accesses only.  __syncthreads() also orders global memory,
+// which makes every wave wait for all of its outstanding global loads before the barrier;
+// kernels whose threads exchange data only through LDS use this one, so that loads issued
+// ahead of time (the next plane's cells) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // First error of a launch wins (code, block, i, j, k); later ones are dropped.

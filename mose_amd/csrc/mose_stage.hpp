@@ -330,6 +330,7 @@ __global__ void __launch_bounds__(TXc* TYc)
   __shared__ FT sV[VR][4][VY][VX];       // viscous variables, ring over planes k-1, k, k+1
   constexpr int NL = (TXc > TYc ? TXc : TYc) + 3;
   __shared__ FT sRL[2][NL], sLP[2][NL];  // per-tile spacing data along x and y (rectilinear)
+  __shared__ double sDel[2][NL], sDL[2][NL];   // node spacing delta and cell length dl along x, y
   // residual-norm sums: LDS for the FP32-flux kernels (register-bound), registers for FP64
   // (LDS-bound); reduced in a fixed order either way
   constexpr bool NLDS = MODE == 1 && sizeof(FT) == 4;
@@ -366,8 +367,10 @@ __global__ void __launch_bounds__(TXc* TYc)
 
   __shared__ int sTile;
   if (tid == 0) sTile = claim_tile(a.tq, a.nq, a.ntiles);
-  __syncthreads();
-  for (int t = sTile; t >= 0;) {
+  lds_barrier();
+  // the tile index is the same for every lane: held in a scalar register, so the tile's block
+  // fields, line-table entries and plane offsets are scalar loads and scalar arithmetic
+  for (int t = __builtin_amdgcn_readfirstlane(sTile); t >= 0;) {
     const Tile T = a.tiles[t];
     const BlockDev* __restrict__ Bp = a.blocks + T.b;
     const int ni = Bp->ni, nj = Bp->nj, nk = Bp->nk, sy_ = Bp->sy;
@@ -381,6 +384,7 @@ __global__ void __launch_bounds__(TXc* TYc)
     const LineEnt* Lx = a.lines + Bp->line[0];
     const LineEnt* Ly = a.lines + Bp->line[1];
     const LineEnt* Lz = a.lines + Bp->line[2];
+    const cptr<LineEnt> Lzc = (cptr<LineEnt>)Lz;     // Lz[k] is the same for every lane
     const int k0 = T.k0, k1 = T.k1;
     const int hgi = T.i0 + hx, hgj = T.j0 + hy;
     const bool hval = tid < NH && hgi <= ni + 2 && hgj <= nj + 2;
@@ -396,11 +400,15 @@ __global__ void __launch_bounds__(TXc* TYc)
         const bool in = T.i0 + q <= ni + 3;
         sRL[0][q] = in ? FT(Lx[T.i0 + q].rl1) : FT(0);
         sLP[0][q] = in ? FT(Lx[T.i0 + q].lp) : FT(0);
+        sDel[0][q] = in ? Lx[T.i0 + q].delta : 0.0;
+        sDL[0][q] = in ? Lx[T.i0 + q].dl : 0.0;
       }
       for (int q = tid; q < TYc + 3; q += NTc) {
         const bool in = T.j0 + q <= nj + 3;
         sRL[1][q] = in ? FT(Ly[T.j0 + q].rl1) : FT(0);
         sLP[1][q] = in ? FT(Ly[T.j0 + q].lp) : FT(0);
+        sDel[1][q] = in ? Ly[T.j0 + q].delta : 0.0;
+        sDL[1][q] = in ? Ly[T.j0 + q].dl : 0.0;
       }
     }
 
@@ -426,7 +434,7 @@ __global__ void __launch_bounds__(TXc* TYc)
       FT bp = FT(1);
       if constexpr (SDR) bp = FT(beta[k0 >= 2 ? cm : cm + sz]);
       if constexpr (GEO != 1) {
-        cell_states_b<LIM, SDR>(Qm, Q0, Q1, FT(Lz[k0].rl1), FT(Lz[k0 + 1].rl1), FT(Lz[k0].lp), FT(1), bp, Sm_, Skp);   // P+(k0-1)
+        cell_states_b<LIM, SDR>(Qm, Q0, Q1, FT(Lzc[k0].rl1), FT(Lzc[k0 + 1].rl1), FT(Lzc[k0].lp), FT(1), bp, Sm_, Skp);   // P+(k0-1)
       } else {
         cell_states_b<LIM, SDR>(Qm, Q0, Q1, FT(GA(GRLO + 2, cm)), FT(GA(GRLO + 2, cm + sz)), FT(0.5 * GA(GDL + 2, cm)),
                                 FT(1), bp, Sm_, Skp);
@@ -467,7 +475,7 @@ __global__ void __launch_bounds__(TXc* TYc)
       stage_visc(k0);
 #pragma unroll
       for (int x = 0; x < 4; ++x) vis[x] = FT(Bp->vis[x]);
-      __syncthreads();
+      lds_barrier();
     }
 
     // k = k0-1 only produces the face k0-1/2; planes k0..k1-1 are updated
@@ -497,7 +505,7 @@ __global__ void __launch_bounds__(TXc* TYc)
 
       if (plane) {
         if constexpr (VIS) stage_visc(k + 1);
-        __syncthreads();       // plane k staged
+        lds_barrier();       // plane k staged
 
         // ---- i and j face states: the own cell in each direction, then the cells just outside
         // the patch (x halo on wave 0, y halo on wave 1); every call has a fixed direction
@@ -568,7 +576,7 @@ __global__ void __launch_bounds__(TXc* TYc)
             for (int v = 0; v < 5; ++v) sMy[v][ylx] = Sm_[v];
           }
         }
-        __syncthreads();
+        lds_barrier();
 
         // ---- one Riemann problem per i / j face: the own low faces (fixed directions), then
         // the faces on the patch's high edges (a rotating wave)
@@ -595,7 +603,7 @@ __global__ void __launch_bounds__(TXc* TYc)
                 FT M[3];
                 const double rxL = Lx[fi - (d == 0) + 1].rdl, rxR = Lx[fi + 1].rdl;
                 const double ryL = Ly[fj - (d == 1) + 1].rdl, ryR = Ly[fj + 1].rdl;
-                const double rz = Lz[k + 1].rdl;
+                const double rz = Lzc[k + 1].rdl;
                 M[0] = FT(0.5 * (rxL + rxR)); M[1] = FT(0.5 * (ryL + ryR)); M[2] = FT(rz);
                 FT Fv[5];
                 visc_interior<FT, TXc, TYc>(d, lxL, lyL, sV, s0, s1, s2, rl, pl, rr, pr, M, Rg, vis, Fv);
@@ -643,7 +651,21 @@ __global__ void __launch_bounds__(TXc* TYc)
             }
           }
         }
-        __syncthreads();
+        lds_barrier();
+
+        // operands of the update of plane k, loaded before the k face is solved so that the
+        // solve covers their latency (rectilinear inviscid stages except the last, whose
+        // residual-norm terms leave no registers for them)
+        const unsigned c = cbase + sz * (unsigned)(k + 1);
+        constexpr bool EARLY = GEO == 0 && MODE == 0;
+        double un[5], ui[5], dt = 0.0;
+        if constexpr (EARLY) {
+          if (active) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) { un[v] = a.Un[v * N + c]; ui[v] = Uin[v * N + c]; }
+            dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
+          }
+        }
 
         // ---- k face k+1/2 of the own column, entirely in registers
         FT Fk[5];
@@ -656,7 +678,7 @@ __global__ void __launch_bounds__(TXc* TYc)
             bm = FT(beta[k + 1 >= 2 ? c1 - sz : c1]);
           }
           if constexpr (GEO != 1) {
-            cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), bm, bp, Sm_, Sp_);
+            cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(Lzc[k + 2].rl1), FT(Lzc[k + 3].rl1), FT(Lzc[k + 2].lp), bm, bp, Sm_, Sp_);
           } else {
             cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)),
                                     bm, bp, Sm_, Sp_);
@@ -672,32 +694,33 @@ __global__ void __launch_bounds__(TXc* TYc)
 
         // ---- residual of the own cell and the RK update of plane k
         if (active) {
-          const double Az = GEO != 1 ? Lx[i + 1].delta * Ly[j + 1].delta : 1.0;
-          const double Ax = GEO != 1 ? Ly[j + 1].delta * Lz[k + 1].delta : 1.0;
-          const double Ay = GEO != 1 ? Lx[i + 1].delta * Lz[k + 1].delta : 1.0;
+          const double dx = GEO != 1 ? sDel[0][tx + 1] : 1.0, dy = GEO != 1 ? sDel[1][ty + 1] : 1.0;
+          const double dz = GEO != 1 ? Lzc[k + 1].delta : 1.0;
+          const double Az = dx * dy, Ax = dy * dz, Ay = dx * dz;
           double Racc[5];
 #pragma unroll
           for (int v = 0; v < 5; ++v)
             Racc[v] = Az * ((double)Fk[v] - (double)Fkm[v]) +
                       Ax * ((double)sFi[v][ty][tx + 1] - (double)sFi[v][ty][tx]) +
                       Ay * ((double)sFj[v][ty + 1][tx] - (double)sFj[v][ty][tx]);
-          const unsigned c = cbase + sz * (unsigned)(k + 1);
           if constexpr (MODE == 2) {
             if (a.irs_scale) {
-              const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
-              const double vol = GEO != 1 ? Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta : GA(GVOL, c);
+              const double dtc = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
+              const double vol = GEO != 1 ? dx * dy * dz : GA(GVOL, c);
 #pragma unroll
-              for (int v = 0; v < 5; ++v) a.Rout[v * N + c] = -Racc[v] / vol * dt;
+              for (int v = 0; v < 5; ++v) a.Rout[v * N + c] = -Racc[v] / vol * dtc;
             } else {
 #pragma unroll
               for (int v = 0; v < 5; ++v) a.Rout[v * N + c] = Racc[v];
             }
           } else {
-            double un[5], ui[5], unew[5], P[5];
+            if constexpr (!EARLY) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) { un[v] = a.Un[v * N + c]; ui[v] = Uin[v * N + c]; }
-            const double dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
-            const double vol = GEO != 1 ? Lx[i + 1].delta * Ly[j + 1].delta * Lz[k + 1].delta : GA(GVOL, c);
+              for (int v = 0; v < 5; ++v) { un[v] = a.Un[v * N + c]; ui[v] = Uin[v * N + c]; }
+              dt = a.local_dt ? a.dtloc[c] : a.scal[S_DT];
+            }
+            double unew[5], P[5];
+            const double vol = GEO != 1 ? dx * dy * dz : GA(GVOL, c);
             const double rv = dt / vol;
 #pragma unroll
             for (int v = 0; v < 5; ++v) unew[v] = un[v] + a.rkc * (ui[v] - un[v] - Racc[v] * rv);
@@ -712,12 +735,12 @@ __global__ void __launch_bounds__(TXc* TYc)
               const double snd = sqrt(a.gam * P[4] / P[0]);
               double dtc = 1e8;
               if constexpr (GEO != 1) {
-                dtc = fmin(Lx[i + 1].dl / (fabs(P[1]) + snd) * a.cfl, dtc);
-                if (a.ndir > 1) dtc = fmin(Ly[j + 1].dl / (fabs(P[2]) + snd) * a.cfl, dtc);
-                if (a.ndir > 2) dtc = fmin(Lz[k + 1].dl / (fabs(P[3]) + snd) * a.cfl, dtc);
+                dtc = fmin(sDL[0][tx + 1] / (fabs(P[1]) + snd) * a.cfl, dtc);
+                if (a.ndir > 1) dtc = fmin(sDL[1][ty + 1] / (fabs(P[2]) + snd) * a.cfl, dtc);
+                if (a.ndir > 2) dtc = fmin(Lzc[k + 1].dl / (fabs(P[3]) + snd) * a.cfl, dtc);
                 if (GEO == 2 && Bp->visc) {   // VNN condition (Mod_dt.f90:88-103)
                   const double mi = sutherland(P[4] / (P[0] * a.R), Bp->vis[0], Bp->vis[1], Bp->vis[2]);
-                  const double dls[3] = {Lx[i + 1].dl, Ly[j + 1].dl, Lz[k + 1].dl};
+                  const double dls[3] = {sDL[0][tx + 1], sDL[1][ty + 1], Lzc[k + 1].dl};
                   for (int dd = 0; dd < a.ndir; ++dd) dtc = fmin(P[0] * dls[dd] * dls[dd] * Bp->vis[4] / mi, dtc);
                 }
               } else {
@@ -759,7 +782,7 @@ __global__ void __launch_bounds__(TXc* TYc)
             bm = FT(beta[k + 1 >= 2 ? c1 - sz : c1]);
           }
           if constexpr (GEO != 1) {
-            cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(Lz[k + 2].rl1), FT(Lz[k + 3].rl1), FT(Lz[k + 2].lp), bm, bp, Sm_, Sp_);
+            cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(Lzc[k + 2].rl1), FT(Lzc[k + 3].rl1), FT(Lzc[k + 2].lp), bm, bp, Sm_, Sp_);
           } else {
             cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)),
                                     bm, bp, Sm_, Sp_);
@@ -785,7 +808,7 @@ __global__ void __launch_bounds__(TXc* TYc)
         for (int v = 0; v < 5; ++v) { Q0[v] = Q1[v]; Q1[v] = Q2[v]; Q2[v] = (FT)p[v]; }
       }
     }
-    __syncthreads();
+    lds_barrier();
 
     if constexpr (MODE == 1) {   // fixed-order reductions over the workgroup: deterministic
       double tot = 0.0;
@@ -795,7 +818,7 @@ __global__ void __launch_bounds__(TXc* TYc)
 #pragma unroll
             for (int v = 0; v < 5; ++v) sN[v][tid] += sN[v][tid + w];
           }
-          __syncthreads();
+          lds_barrier();
         }
         if (tid < 5) tot = sN[tid][0];
       } else {
@@ -807,7 +830,7 @@ __global__ void __launch_bounds__(TXc* TYc)
           for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
           if (lane == 0) sW[5 * wave + v] = x;
         }
-        __syncthreads();
+        lds_barrier();
         if (tid < 5)
           for (int w = 0; w < NW; ++w) tot += sW[5 * w + tid];
       }
@@ -815,12 +838,12 @@ __global__ void __launch_bounds__(TXc* TYc)
       for (int o = 32; o > 0; o >>= 1) dtmin_t = fmin(dtmin_t, __shfl_xor(dtmin_t, o, 64));
       if ((tid & 63) == 0 && dtmin_t < 1e300)
         atomicMin(a.dtmin, (unsigned long long)__double_as_longlong(dtmin_t));
-      __syncthreads();
+      lds_barrier();
     }
     // every thread has read sTile before the barriers above; claim the next tile
     if (tid == 0) sTile = claim_tile(a.tq, a.nq, a.ntiles);
-    __syncthreads();
-    t = sTile;
+    lds_barrier();
+    t = __builtin_amdgcn_readfirstlane(sTile);
   }
   if (tid == 0) finish_tiles(a.tq);
 }
