@@ -121,6 +121,30 @@ __device__ __forceinline__ void reconstruct(T qm1, T q0, T q1, T q2, const FaceS
   qr = q1 - limiter<LIM>(s2, s1) * s.lp;
 }
 
+// ------------------------------------------------------------------------- face normals
+// A general unit normal (curvilinear faces) or a coordinate axis known at compile time (the
+// faces of rectilinear blocks): dot(u, v, w) = u.n, madd(a, d, p) = a + p n_d.  For an axis,
+// the products with the zero components are left out (the compiler cannot drop x * 0 itself);
+// the values are those of the general form with that unit normal.
+template <typename T>
+struct NGen {
+  T nx, ny, nz;
+  __device__ __forceinline__ T x() const { return nx; }
+  __device__ __forceinline__ T y() const { return ny; }
+  __device__ __forceinline__ T z() const { return nz; }
+  __device__ __forceinline__ T dot(T u, T v, T w) const { return u * nx + v * ny + w * nz; }
+  __device__ __forceinline__ T madd(T a, int d, T p) const { return a + p * (d == 0 ? nx : (d == 1 ? ny : nz)); }
+};
+template <typename T, int AX>
+struct NAx {
+  static_assert(AX >= 0 && AX < 3, "axis");
+  __device__ __forceinline__ T x() const { return AX == 0 ? T(1) : T(0); }
+  __device__ __forceinline__ T y() const { return AX == 1 ? T(1) : T(0); }
+  __device__ __forceinline__ T z() const { return AX == 2 ? T(1) : T(0); }
+  __device__ __forceinline__ T dot(T u, T v, T w) const { return AX == 0 ? u : (AX == 1 ? v : w); }
+  __device__ __forceinline__ T madd(T a, int d, T p) const { return d == AX ? a + p : a; }
+};
+
 // ------------------------------------------------------------------------- Riemann flux
 // Face flux per unit area from left/right primitive states (rho,u,v,w,p), unit normal n.
 // Returns false when a state is non-physical (check_gas_state: rho<=0 or p<=0).
@@ -129,14 +153,14 @@ __device__ __forceinline__ bool gas_ok(const T* P) {
   return (P[0] > T(0)) && (P[4] > T(0));
 }
 
-template <typename T>
-__device__ __forceinline__ void hllc(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F) {
+template <typename T, typename NT>
+__device__ __forceinline__ void hllc(const T* L, const T* R, const NT& n, const GasT<T>& g, T* F) {
   const T rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   const T rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
   const T irl = rcp_(rl), irr = rcp_(rr);
   const T al = sqrtp_(g.gam * pl * irl), ar = sqrtp_(g.gam * pr * irr);
-  const T unl = ul * nx + vl * ny + wl * nz;
-  const T unr = ur * nx + vr * ny + wr * nz;
+  const T unl = n.dot(ul, vl, wl);
+  const T unr = n.dot(ur, vr, wr);
   // total enthalpy h0 = cp T + |u|^2/2 = gamma/(gamma-1) p/rho + |u|^2/2
   const T kgl = g.gam * g.rgm1;
   const T h0l = kgl * pl * irl + T(0.5) * (ul * ul + vl * vl + wl * wl);
@@ -149,7 +173,7 @@ __device__ __forceinline__ void hllc(const T* L, const T* R, T nx, T ny, T nz, c
   const T w_roe = (srr * wr + srl * wl) * isr;
   const T h0_roe = (srr * h0r + srl * h0l) * isr;
   const T a_roe = sqrtp_(g.gm1 * (h0_roe - T(0.5) * (u_roe * u_roe + v_roe * v_roe + w_roe * w_roe)));
-  const T un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+  const T un_roe = n.dot(u_roe, v_roe, w_roe);
   const T e0l = h0l - pl * irl, e0r = h0r - pr * irr;
   // Batten signal speeds, clamped to include 0
   const T S1 = min_(T(0), min_(unl - al, un_roe - a_roe));
@@ -164,15 +188,15 @@ __device__ __forceinline__ void hllc(const T* L, const T* R, T nx, T ny, T nz, c
   T S = left ? S1 : S4;
   const bool star = left ? (S1 < T(0)) : (S4 > T(0));
   const T Fr = r_ * un_;
-  T f0 = Fr, f1 = Fr * u_ + p_ * nx, f2 = Fr * v_ + p_ * ny, f3 = Fr * w_ + p_ * nz, f4 = Fr * h0_;
+  T f0 = Fr, f1 = n.madd(Fr * u_, 0, p_), f2 = n.madd(Fr * v_, 1, p_), f3 = n.madd(Fr * w_, 2, p_), f4 = Fr * h0_;
   if (star) {
     const T inv = rcp_(S - Sstar);
     const T sd = (S - un_) * r_;
     const T dp = pstar - p_;
     const T U1S = sd * inv;
-    const T U2S = (sd * u_ + dp * nx) * inv;
-    const T U3S = (sd * v_ + dp * ny) * inv;
-    const T U4S = (sd * w_ + dp * nz) * inv;
+    const T U2S = n.madd(sd * u_, 0, dp) * inv;
+    const T U3S = n.madd(sd * v_, 1, dp) * inv;
+    const T U4S = n.madd(sd * w_, 2, dp) * inv;
     const T U5S = (sd * e0_ - p_ * un_ + pstar * Sstar) * inv;
     f0 += S * (U1S - r_);
     f1 += S * (U2S - r_ * u_);
@@ -183,15 +207,15 @@ __device__ __forceinline__ void hllc(const T* L, const T* R, T nx, T ny, T nz, c
   F[0] = f0; F[1] = f1; F[2] = f2; F[3] = f3; F[4] = f4;
 }
 
-template <typename T>
-__device__ __forceinline__ void ausmp(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F) {
+template <typename T, typename NT>
+__device__ __forceinline__ void ausmp(const T* L, const T* R, const NT& n, const GasT<T>& g, T* F) {
   const T beta = T(0.125), alpha = T(0.1875);
   const T rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   const T rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
   const T irl = rcp_(rl), irr = rcp_(rr);
   const T al2 = g.gam * pl * irl, ar2 = g.gam * pr * irr;
-  const T unl = ul * nx + vl * ny + wl * nz;
-  const T unr = ur * nx + vr * ny + wr * nz;
+  const T unl = n.dot(ul, vl, wl);
+  const T unr = n.dot(ur, vr, wr);
   const T kgl = g.gam * g.rgm1;
   const T H_L = kgl * pl * irl + T(0.5) * (ul * ul + vl * vl + wl * wl);
   const T H_R = kgl * pr * irr + T(0.5) * (ur * ur + vr * vr + wr * wr);
@@ -220,22 +244,22 @@ __device__ __forceinline__ void ausmp(const T* L, const T* R, T nx, T ny, T nz, 
   const T mdotp = rl * a_half * max_(Mhalf, T(0));
   const T mdotm = rr * a_half * min_(Mhalf, T(0));
   F[0] = mdotp + mdotm;
-  F[1] = mdotp * ul + mdotm * ur + nx * phalf;
-  F[2] = mdotp * vl + mdotm * vr + ny * phalf;
-  F[3] = mdotp * wl + mdotm * wr + nz * phalf;
+  F[1] = n.madd(mdotp * ul + mdotm * ur, 0, phalf);
+  F[2] = n.madd(mdotp * vl + mdotm * vr, 1, phalf);
+  F[3] = n.madd(mdotp * wl + mdotm * wr, 2, phalf);
   F[4] = mdotp * H_L + mdotm * H_R;
 }
 
 // HLLE: the two-wave HLL flux with Batten's (Einfeldt's) speed estimates, riemann_HLLE
 // (Lib_Riemann_HLL.f90:261-320):  F = (S4 F_L - S1 F_R + S1 S4 (U_R - U_L)) / (S4 - S1).
-template <typename T>
-__device__ __forceinline__ void hlle(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F) {
+template <typename T, typename NT>
+__device__ __forceinline__ void hlle(const T* L, const T* R, const NT& n, const GasT<T>& g, T* F) {
   const T rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   const T rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
   const T irl = rcp_(rl), irr = rcp_(rr);
   const T al = sqrtp_(g.gam * pl * irl), ar = sqrtp_(g.gam * pr * irr);
-  const T unl = ul * nx + vl * ny + wl * nz;
-  const T unr = ur * nx + vr * ny + wr * nz;
+  const T unl = n.dot(ul, vl, wl);
+  const T unr = n.dot(ur, vr, wr);
   const T kgl = g.gam * g.rgm1;
   const T h0l = kgl * pl * irl + T(0.5) * (ul * ul + vl * vl + wl * wl);
   const T h0r = kgl * pr * irr + T(0.5) * (ur * ur + vr * vr + wr * wr);
@@ -246,16 +270,16 @@ __device__ __forceinline__ void hlle(const T* L, const T* R, T nx, T ny, T nz, c
   const T w_roe = (srr * wr + srl * wl) * isr;
   const T h0_roe = (srr * h0r + srl * h0l) * isr;
   const T a_roe = sqrtp_(g.gm1 * (h0_roe - T(0.5) * (u_roe * u_roe + v_roe * v_roe + w_roe * w_roe)));
-  const T un_roe = u_roe * nx + v_roe * ny + w_roe * nz;
+  const T un_roe = n.dot(u_roe, v_roe, w_roe);
   const T S1 = min_(T(0), min_(unl - al, un_roe - a_roe));
   const T S4 = max_(T(0), max_(unr + ar, un_roe + a_roe));
   const T inv = rcp_(S4 - S1);
   const T a = S4 * inv, b = -S1 * inv, c = S1 * S4 * inv;
   const T FL = rl * unl, FR = rr * unr;
   F[0] = a * FL + b * FR + c * (rr - rl);
-  F[1] = a * (FL * ul + pl * nx) + b * (FR * ur + pr * nx) + c * (rr * ur - rl * ul);
-  F[2] = a * (FL * vl + pl * ny) + b * (FR * vr + pr * ny) + c * (rr * vr - rl * vl);
-  F[3] = a * (FL * wl + pl * nz) + b * (FR * wr + pr * nz) + c * (rr * wr - rl * wl);
+  F[1] = a * n.madd(FL * ul, 0, pl) + b * n.madd(FR * ur, 0, pr) + c * (rr * ur - rl * ul);
+  F[2] = a * n.madd(FL * vl, 1, pl) + b * n.madd(FR * vr, 1, pr) + c * (rr * vr - rl * vl);
+  F[3] = a * n.madd(FL * wl, 2, pl) + b * n.madd(FR * wr, 2, pr) + c * (rr * wr - rl * wl);
   F[4] = a * (FL * h0l) + b * (FR * h0r) + c * (rr * h0r - pr - (rl * h0l - pl));
 }
 
@@ -264,14 +288,14 @@ __device__ __forceinline__ void hlle(const T* L, const T* R, T nx, T ny, T nz, c
 // low-Mach weight chi = (1 - min(1, |u|_rms/aF))^2, a density-weighted |u_n| blended with
 // each side's by f = -clip(mL, -1, 0) clip(mR, 0, 1), cubic pressure weights BetaL/R; SLAU2
 // replaces the chi term of the pressure flux by |u|_rms (BetaL + BetaR - 1) aF (rhoL + rhoR)/2.
-template <bool V2, typename T>
-__device__ __forceinline__ void slau(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F) {
+template <bool V2, typename T, typename NT>
+__device__ __forceinline__ void slau(const T* L, const T* R, const NT& n, const GasT<T>& g, T* F) {
   const T rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   const T rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
   const T irl = rcp_(rl), irr = rcp_(rr);
   const T al = sqrtp_(g.gam * pl * irl), ar = sqrtp_(g.gam * pr * irr);
-  const T unl = ul * nx + vl * ny + wl * nz;
-  const T unr = ur * nx + vr * ny + wr * nz;
+  const T unl = n.dot(ul, vl, wl);
+  const T unr = n.dot(ur, vr, wr);
   const T sql = ul * ul + vl * vl + wl * wl, sqr = ur * ur + vr * vr + wr * wr;
   const T aF = T(0.5) * (al + ar);
   const T iaF = rcp_(aF);
@@ -294,9 +318,9 @@ __device__ __forceinline__ void slau(const T* L, const T* R, T nx, T ny, T nz, c
   const T u_ = left ? ul : ur, v_ = left ? vl : vr, w_ = left ? wl : wr;
   const T h0 = left ? kgl * pl * irl + T(0.5) * sql : kgl * pr * irr + T(0.5) * sqr;
   F[0] = Fr;
-  F[1] = Fr * u_ + nx * ph;
-  F[2] = Fr * v_ + ny * ph;
-  F[3] = Fr * w_ + nz * ph;
+  F[1] = n.madd(Fr * u_, 0, ph);
+  F[2] = n.madd(Fr * v_, 1, ph);
+  F[3] = n.madd(Fr * w_, 2, ph);
   F[4] = Fr * h0;
 }
 
@@ -310,8 +334,9 @@ __device__ __forceinline__ T harten_hyman(T v, T e) {
   return av <= e ? (v * v + e * e) * rcp_(T(2) * e) : av;
 }
 
-template <typename T>
-__device__ __forceinline__ void hllepp(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T beta, T* F) {
+template <typename T, typename NT>
+__device__ __forceinline__ void hllepp(const T* L, const T* R, const NT& n, const GasT<T>& g, T beta, T* F) {
+  const T nx = n.x(), ny = n.y(), nz = n.z();
   const T rl = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   const T rr = R[0], ur = R[1], vr = R[2], wr = R[3], pr = R[4];
   const T irl = rcp_(rl), irr = rcp_(rr);
@@ -372,26 +397,30 @@ __device__ __forceinline__ void hllepp(const T* L, const T* R, T nx, T ny, T nz,
 // beta: the shock detector's weight of the face, read by HLLE++ and HLLC+ only.  HLLC+ is
 // riemann_HLLCSD (Lib_Riemann_HLL.f90:486-534): HLLE at beta = 0, HLLC at beta >= 1, else
 // beta F_HLLC + (1 - beta) F_HLLE.
-template <int RS, typename T>
-__device__ __forceinline__ void riemann(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F,
-                                        T beta = T(1)) {
-  if constexpr (RS == RS_HLLEPP) hllepp(L, R, nx, ny, nz, g, beta, F);
+template <int RS, typename T, typename NT>
+__device__ __forceinline__ void riemann(const T* L, const T* R, const NT& n, const GasT<T>& g, T* F, T beta = T(1)) {
+  if constexpr (RS == RS_HLLEPP) hllepp(L, R, n, g, beta, F);
   else if constexpr (RS == RS_HLLCP) {
-    if (beta == T(0)) hlle(L, R, nx, ny, nz, g, F);
-    else if (beta >= T(1)) hllc(L, R, nx, ny, nz, g, F);
+    if (beta == T(0)) hlle(L, R, n, g, F);
+    else if (beta >= T(1)) hllc(L, R, n, g, F);
     else {
       T Fe[5];
-      hlle(L, R, nx, ny, nz, g, Fe);
-      hllc(L, R, nx, ny, nz, g, F);
+      hlle(L, R, n, g, Fe);
+      hllc(L, R, n, g, F);
 #pragma unroll
       for (int v = 0; v < 5; ++v) F[v] = beta * F[v] + (T(1) - beta) * Fe[v];
     }
   }
-  else if constexpr (RS == RS_AUSMP) ausmp(L, R, nx, ny, nz, g, F);
-  else if constexpr (RS == RS_HLLE) hlle(L, R, nx, ny, nz, g, F);
-  else if constexpr (RS == RS_SLAU) slau<false>(L, R, nx, ny, nz, g, F);
-  else if constexpr (RS == RS_SLAU2) slau<true>(L, R, nx, ny, nz, g, F);
-  else hllc(L, R, nx, ny, nz, g, F);
+  else if constexpr (RS == RS_AUSMP) ausmp(L, R, n, g, F);
+  else if constexpr (RS == RS_HLLE) hlle(L, R, n, g, F);
+  else if constexpr (RS == RS_SLAU) slau<false>(L, R, n, g, F);
+  else if constexpr (RS == RS_SLAU2) slau<true>(L, R, n, g, F);
+  else hllc(L, R, n, g, F);
+}
+template <int RS, typename T>
+__device__ __forceinline__ void riemann(const T* L, const T* R, T nx, T ny, T nz, const GasT<T>& g, T* F,
+                                        T beta = T(1)) {
+  riemann<RS>(L, R, NGen<T>{nx, ny, nz}, g, F, beta);
 }
 
 // Interior / connection face: reconstruction + Riemann, flux per unit area (Convective_Flux).

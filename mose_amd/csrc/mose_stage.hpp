@@ -247,7 +247,9 @@ __device__ __forceinline__ void visc_kface(const BlockDev* __restrict__ Bp, int 
 // offset off, strides sy / sz): the boundary-face buffer on non-connection block faces,
 // else the Riemann problem of (L, R) — a non-physical state raises MOSE_E_INTERFACE at the
 // cell on the low side.  GEO 1: normal from the geometry arrays, flux scaled by the area.
-template <int RS, typename FT, int GEO>
+// AX >= 0: the direction d = AX is known at compile time (the solver drops the products with
+// the normal's zero components); -1: d at run time.
+template <int RS, typename FT, int GEO, int AX = -1>
 __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const FT* L, const FT* R, FT* F,
                                            const BlockDev* __restrict__ Bp, int bnum, int ni, int nj, int nk,
                                            unsigned off, unsigned sy, unsigned sz, const double* __restrict__ bcbuf,
@@ -267,18 +269,22 @@ __device__ __forceinline__ void stage_face(int d, int fi, int fj, int fk, const 
   } else {
     if (!(gas_ok(L) && gas_ok(R)))
       set_error(err, MOSE_E_INTERFACE, bnum, d == 0 ? fi - 1 : fi, d == 1 ? fj - 1 : fj, d == 2 ? fk - 1 : fk);
-    FT nx = d == 0 ? FT(1) : FT(0), ny = d == 1 ? FT(1) : FT(0), nz = d == 2 ? FT(1) : FT(0);
-    if constexpr (GEO == 1) {
-      nx = FT(geo[(size_t)(GN + 4 * d) * N + pf]);
-      ny = FT(geo[(size_t)(GN + 4 * d + 1) * N + pf]);
-      nz = FT(geo[(size_t)(GN + 4 * d + 2) * N + pf]);
-    }
+    FT bw = FT(1);
     if constexpr (rs_takes_beta(RS)) {
       // the low cell's weight; on the block's low face, that of its own boundary cell
       const unsigned st = d == 0 ? 1u : (d == 1 ? sy : sz);
-      riemann<RS>(L, R, nx, ny, nz, g, F, FT(beta[idx == 1 ? pf : pf - st]));
+      bw = FT(beta[idx == 1 ? pf : pf - st]);
+    }
+    if constexpr (GEO != 1 && AX >= 0) {
+      riemann<RS>(L, R, NAx<FT, AX>{}, g, F, bw);
     } else {
-      riemann<RS>(L, R, nx, ny, nz, g, F);
+      FT nx = d == 0 ? FT(1) : FT(0), ny = d == 1 ? FT(1) : FT(0), nz = d == 2 ? FT(1) : FT(0);
+      if constexpr (GEO == 1) {
+        nx = FT(geo[(size_t)(GN + 4 * d) * N + pf]);
+        ny = FT(geo[(size_t)(GN + 4 * d + 1) * N + pf]);
+        nz = FT(geo[(size_t)(GN + 4 * d + 2) * N + pf]);
+      }
+      riemann<RS>(L, R, NGen<FT>{nx, ny, nz}, g, F, bw);
     }
   }
   if constexpr (GEO == 1) {
@@ -580,8 +586,9 @@ __global__ void __launch_bounds__(TXc* TYc)
 
         // ---- one Riemann problem per i / j face: the own low faces (fixed directions), then
         // the faces on the patch's high edges (a rotating wave)
-        auto solve = [&](const int d, int fi, int fj, const FT* Lq, const FT* Rq, FT* F) {
-          stage_face<RS, FT, GEO>(d, fi, fj, k, Lq, Rq, F, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g, beta);
+        // ax: std::integral_constant of the face axis (own faces) or of -1 (edge faces, d at run time)
+        auto solve = [&](auto ax, const int d, int fi, int fj, const FT* Lq, const FT* Rq, FT* F) {
+          stage_face<RS, FT, GEO, decltype(ax)::value>(d, fi, fj, k, Lq, Rq, F, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g, beta);
           if constexpr (VIS) {
             if (Bp->visc) {
               // cell on the low side of the face, patch coordinates
@@ -617,7 +624,7 @@ __global__ void __launch_bounds__(TXc* TYc)
           FT Lq[5], F[5];
 #pragma unroll
           for (int v = 0; v < 5; ++v) Lq[v] = sSx[v][ty][tx];
-          solve(0, i, j, Lq, Smx, F);
+          solve(std::integral_constant<int, 0>{}, 0, i, j, Lq, Smx, F);
 #pragma unroll
           for (int v = 0; v < 5; ++v) sFi[v][ty][tx] = F[v];
         }
@@ -625,7 +632,7 @@ __global__ void __launch_bounds__(TXc* TYc)
           FT Lq[5], F[5];
 #pragma unroll
           for (int v = 0; v < 5; ++v) Lq[v] = sSy[v][ty][tx];
-          solve(1, i, j, Lq, Smy, F);
+          solve(std::integral_constant<int, 1>{}, 1, i, j, Lq, Smy, F);
 #pragma unroll
           for (int v = 0; v < 5; ++v) sFj[v][ty][tx] = F[v];
         }
@@ -641,7 +648,7 @@ __global__ void __launch_bounds__(TXc* TYc)
               Lq[v] = ex ? sSx[v][e][TXc] : sSy[v][TYc][e - TYc];
               Rq[v] = ex ? sMx[v][e] : sMy[v][e - TYc];
             }
-            solve(d, fi, fj, Lq, Rq, F);
+            solve(std::integral_constant<int, -1>{}, d, fi, fj, Lq, Rq, F);
             if (ex) {
 #pragma unroll
               for (int v = 0; v < 5; ++v) sFi[v][e][TXc] = F[v];
@@ -683,7 +690,7 @@ __global__ void __launch_bounds__(TXc* TYc)
             cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)),
                                     bm, bp, Sm_, Sp_);
           }
-          stage_face<RS, FT, GEO>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g, beta);
+          stage_face<RS, FT, GEO, 2>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g, beta);
           if constexpr (VIS) {
             if (Bp->visc)
               visc_kface<FT, TXc, TYc>(Bp, i, j, k, ni, nk, tx, ty, sV, Q0, Q1, Lx, Ly, Lz, a.vbuf, Rg, vis, Fk);
@@ -787,7 +794,7 @@ __global__ void __launch_bounds__(TXc* TYc)
             cell_states_b<LIM, SDR>(Q0, Q1, Q2, FT(GA(GRLO + 2, c1)), FT(GA(GRLO + 2, c1 + sz)), FT(0.5 * GA(GDL + 2, c1)),
                                     bm, bp, Sm_, Sp_);
           }
-          stage_face<RS, FT, GEO>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g, beta);
+          stage_face<RS, FT, GEO, 2>(2, i, j, k + 1, Skp, Sm_, Fk, Bp, T.b + 1, ni, nj, nk, off, (unsigned)sy_, sz, bcbuf, geo, N, err, g, beta);
           if constexpr (VIS) {
             if (Bp->visc)
               visc_kface<FT, TXc, TYc>(Bp, i, j, k, ni, nk, tx, ty, sV, Q0, Q1, Lx, Ly, Lz, a.vbuf, Rg, vis, Fk);
