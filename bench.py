@@ -69,13 +69,20 @@ def l2_vs_exact(g, case, t):
     return float(np.sqrt(np.mean((P[..., 0] - ex[..., 0]) ** 2)))
 
 
+NO_AUTOTUNE = False
+
+
 def run_gpu(case, steps, warmup, label):
     from mose_amd.lib import GpuSolver
     t0 = time.perf_counter()
     g = GpuSolver(case)
     log("[bench] %s: setup %.1f s, %d cells" % (label, time.perf_counter() - t0, case.ncells))
-    tiling, tune_ms = g.autotune(3)
-    log("[bench] %s: autotune %s (%.3f ms for 3 stages)" % (label, tiling, tune_ms))
+    if NO_AUTOTUNE:     # the tiling the library took from MOSE_TILE_SHAPE / MOSE_KCHUNK / MOSE_TILE_XCD
+        tiling = g.tiling()
+        log("[bench] %s: tiling from the environment %s" % (label, tiling))
+    else:
+        tiling, tune_ms = g.autotune(3)
+        log("[bench] %s: autotune %s (%.3f ms for 3 stages)" % (label, tiling, tune_ms))
     g.tiling = tiling
     g.advance(warmup)
     t0 = time.perf_counter()
@@ -333,6 +340,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--riemann", default="HLLC")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="headline legs: keep the tiling of MOSE_TILE_SHAPE / MOSE_KCHUNK / MOSE_TILE_XCD (profiling)")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=50)
@@ -348,6 +357,8 @@ def main():
     ap.add_argument("--no-schemes", action="store_true")
     ap.add_argument("--scheme-steps", type=int, default=30)
     args = ap.parse_args()
+    global NO_AUTOTUNE
+    NO_AUTOTUNE = args.no_autotune
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; sharding blocks over several GPUs is not "
               "built. Run with --gpus 1.", file=sys.stderr)

@@ -1087,13 +1087,14 @@ __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a) {
     const AdjSeg S = a.segs[lo];
     const BlockDev& B = a.blocks[S.b];
     const int d = S.d;
-    // i fastest for every direction, so that a wavefront reads consecutive cells
-    int64_t r = q - S.start;
+    // i fastest for every direction, so that a wavefront reads consecutive cells (32-bit
+    // arithmetic: a segment holds fewer than 2^32 faces)
+    unsigned r = (unsigned)(q - S.start);
     int ijk[3];
-    ijk[0] = (int)(r % S.n0) + 1;
-    r /= S.n0;
-    ijk[1] = (int)(r % S.n1) + 1;
-    ijk[2] = (int)(r / S.n1) + 1;
+    ijk[0] = (int)(r % (unsigned)S.n0) + 1;
+    r /= (unsigned)S.n0;
+    ijk[1] = (int)(r % (unsigned)S.n1) + 1;
+    ijk[2] = (int)(r / (unsigned)S.n1) + 1;
     const int qd = ijk[d];                       // cell index along d whose low face this is
     const int nd = d == 0 ? B.ni : (d == 1 ? B.nj : B.nk);
     const bool blo = qd == 1, bhi = qd == nd + 1;
@@ -1241,9 +1242,11 @@ __global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a) {
       if (a.blocks[mid].off <= c) lo = mid; else hi = mid - 1;
     }
     const BlockDev& B = a.blocks[lo];
-    const int64_t r = c - B.off;
-    if (r >= B.sz * (int64_t)(B.nk + 4)) continue;     // alignment padding between blocks
-    const int pi = (int)(r % B.sy) - 1, pj = (int)((r / B.sy) % (B.nj + 4)) - 1, pk = (int)(r / B.sz) - 1;
+    const int64_t r64 = c - B.off;
+    if (r64 >= B.sz * (int64_t)(B.nk + 4)) continue;     // alignment padding between blocks
+    const unsigned r = (unsigned)r64, sy = (unsigned)B.sy, rj = r / sy;
+    const int pi = (int)(r - rj * sy) - 1, pj = (int)(rj % (unsigned)(B.nj + 4)) - 1,
+              pk = (int)(rj / (unsigned)(B.nj + 4)) - 1;
     const int n3[3] = {B.ni, B.nj, B.nk};
     const int p3[3] = {pi, pj, pk};
     // cells outside the face-ghost cross (edges and corners) are never read

@@ -91,12 +91,12 @@ __global__ void __launch_bounds__(256) k_adj_vface(AdjArgs a) {
     const BlockDev& B = a.blocks[S.b];
     if (!B.visc) continue;
     const int d = S.d;
-    int64_t r = q - S.start;
+    unsigned r = (unsigned)(q - S.start);
     int ijk[3];
-    ijk[0] = (int)(r % S.n0) + 1;
-    r /= S.n0;
-    ijk[1] = (int)(r % S.n1) + 1;
-    ijk[2] = (int)(r / S.n1) + 1;
+    ijk[0] = (int)(r % (unsigned)S.n0) + 1;
+    r /= (unsigned)S.n0;
+    ijk[1] = (int)(r % (unsigned)S.n1) + 1;
+    ijk[2] = (int)(r / (unsigned)S.n1) + 1;
     const int n3[3] = {B.ni, B.nj, B.nk};
     if (ijk[d] < 2 || ijk[d] > n3[d]) continue;          // block faces: k_adj_vbc
     int tb, tc;
@@ -162,9 +162,10 @@ __global__ void __launch_bounds__(256) k_adj_vcell(AdjArgs a) {
     }
     const BlockDev& B = a.blocks[lo];
     if (!B.visc) continue;
-    const int64_t r = c - B.off;
-    if (r >= B.sz * (int64_t)(B.nk + 4)) continue;
-    const int q3[3] = {(int)(r % B.sy) - 1, (int)((r / B.sy) % (B.nj + 4)) - 1, (int)(r / B.sz) - 1};
+    const int64_t r64 = c - B.off;
+    if (r64 >= B.sz * (int64_t)(B.nk + 4)) continue;
+    const unsigned r = (unsigned)r64, sy = (unsigned)B.sy, rj = r / sy;
+    const int q3[3] = {(int)(r - rj * sy) - 1, (int)(rj % (unsigned)(B.nj + 4)) - 1, (int)(rj / (unsigned)(B.nj + 4)) - 1};
     const int n3[3] = {B.ni, B.nj, B.nk};
     const int64_t st[3] = {1, (int64_t)B.sy, B.sz};
     double Vb[4] = {0, 0, 0, 0}, rhob = 0.0, pb = 0.0;

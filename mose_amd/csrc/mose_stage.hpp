@@ -662,9 +662,10 @@ __global__ void __launch_bounds__(TXc* TYc)
 
         // operands of the update of plane k, loaded before the k face is solved so that the
         // solve covers their latency (rectilinear inviscid stages except the last, whose
-        // residual-norm terms leave no registers for them)
+        // residual-norm terms leave no registers for them, and the HLLE++ / HLLC+ solvers,
+        // whose registers are taken too: early, the loads would spill)
         const unsigned c = cbase + sz * (unsigned)(k + 1);
-        constexpr bool EARLY = GEO == 0 && MODE == 0;
+        constexpr bool EARLY = GEO == 0 && MODE == 0 && !rs_takes_beta(RS);
         double un[5], ui[5], dt = 0.0;
         if constexpr (EARLY) {
           if (active) {
