@@ -686,6 +686,8 @@ struct mose_ctx {
   int64_t *d_vbt = nullptr, *d_vbrow = nullptr;
   int* d_vbent = nullptr;
   int nvbt = 0;
+  Tile* d_vtiles = nullptr;                   // k_adj_visc's patches of the viscous blocks
+  int nvtiles = 0;
   bool vjp_valid = false;                    // the last vjp left Pbar and the viscous buffers
   bool geo_valid = false;           // the last vjp wrote the geometry adjoints
   bool xbar_valid = false;
@@ -1170,6 +1172,17 @@ static int ensure_adjoint(mose_ctx* c) {
     row.push_back((int64_t)te.size());
     c->nvbt = (int)tgt.size();
     if (upload(c, &c->d_vbt, tgt) || upload(c, &c->d_vbrow, row) || upload(c, &c->d_vbent, ent)) return c->err.code;
+    // k_adj_visc: kVX x kVY patches of every viscous block, 32-plane chunks, i fastest
+    std::vector<Tile> vt;
+    for (int b = 0; b < c->nb; ++b) {
+      const BlockDev& B = c->lay.blocks[b];
+      if (!B.visc) continue;
+      for (int k0 = 1; k0 <= B.nk; k0 += 32)
+        for (int j0 = 1; j0 <= B.nj; j0 += kVY)
+          for (int i0 = 1; i0 <= B.ni; i0 += kVX) vt.push_back(Tile{b, i0, j0, k0, std::min(k0 + 32, B.nk + 1), 0, 0, 0});
+    }
+    c->nvtiles = (int)vt.size();
+    if (upload(c, &c->d_vtiles, vt)) return c->err.code;
   }
   c->adj_ready = true;
   return 0;
@@ -1226,7 +1239,8 @@ static void enqueue_vjp(mose_ctx* c, bool geometry = false, double* U = nullptr)
   (void)hipLaunchKernel((const void*)atable().cell[lim][geo], dim3(launch_blocks(c->npool)), dim3(256), args, 0,
                         c->stream);
   if (c->lay.visc) {   // viscous terms: interior faces (gathered per cell), then connection faces
-    hipLaunchKernelGGL(k_adj_vface, dim3(launch_blocks(c->nwork)), dim3(256), 0, c->stream, a);
+    if (c->nvtiles > 0)
+      hipLaunchKernelGGL(k_adj_visc, dim3(c->nvtiles), dim3(kVT), 0, c->stream, a, (const Tile*)c->d_vtiles);
     hipLaunchKernelGGL(k_adj_vcell, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, a);
     if (a.nvf > 0) {
       hipLaunchKernelGGL(k_adj_vbc, dim3(launch_blocks(a.nvf)), dim3(256), 0, c->stream, a);
@@ -2189,7 +2203,7 @@ void mose_destroy(mose_ctx* c) {
                   c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_pres, c->d_ghostp,
                   c->d_garc_cell, c->d_garc, c->d_rs1, c->d_irsg, c->d_gP, c->d_lam0, c->d_lam1, c->d_Dq,
                   c->d_apart, c->d_ares, c->d_icell, c->d_VBC, c->d_Tp, c->d_vbt, c->d_vbrow, c->d_vbent, c->d_volq,
-                  c->d_upart, c->d_ke, c->d_mu, c->d_icbeg, c->d_vfbeg, c->d_vfi};
+                  c->d_upart, c->d_ke, c->d_mu, c->d_icbeg, c->d_vfbeg, c->d_vfi, c->d_vtiles};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
