@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256) k_adj_ghostT(AdjArgs a) {
 // converting (u, v, w, T) adjoints to primitive ones; it stores per face the mu_ref adjoint at
 // the face's low cell (row 17 of VW) and, for the faces whose stencils reach ghost cells, the
 // adjoints of the index-space gradients Gb[4][3] and of the face state Pmb[5] (rows 0-16),
-// which k_adj_vcell (one thread per ghost cell) gathers the same way.  Connection faces (one per
+// which k_adj_vcell (one thread per first-layer ghost cell) gathers the same way.  Connection faces (one per
 // boundary face cell) reach cells of the donor block: k_adj_vbc stores the primitive adjoints of
 // each face's ten stencil cells, and k_adj_vbc_gather adds them to every receiving cell in a
 // fixed order (face, then role), so the result does not depend on the schedule.
@@ -136,8 +136,12 @@ __device__ __forceinline__ double vface_core(const AdjArgs& a, const BlockDev& B
 // written out only by the patch that holds its high cell: the mu_ref adjoint (row 17) always,
 // and the whole record for faces whose stencil reaches a ghost cell, which k_adj_vcell gathers
 // for the ghost cells.
+// 16 x 16 patches; threads: the face positions of the direction with the most (17 x 18,
+// 18 x 17, 2 x 16 + 18 x 16).  (A 14 x 14 patch, whose directions fit 256 threads, measured slower.)
 constexpr int kVX = 16, kVY = 16;
-constexpr int kVT = 320;       // threads: the face positions of one direction (at most 18 x 17 + 14)
+constexpr int kVT = 320;
+static_assert((kVX + 1) * (kVY + 2) <= kVT && (kVX + 2) * (kVY + 1) <= kVT && 2 * kVX + (kVX + 2) * kVY <= kVT,
+              "one face position per thread");
 __global__ void __launch_bounds__(kVT) k_adj_visc(AdjArgs a, const Tile* __restrict__ tiles) {
   constexpr int CX = kVX + 4, CY = kVY + 4;    // cell region x in [-2, kVX + 1], y in [-2, kVY + 1]
   constexpr int FX = kVX + 2, FY = kVY + 2;    // face positions x in [-1, kVX], y in [-1, kVY]
@@ -358,9 +362,12 @@ __global__ void __launch_bounds__(kVT) k_adj_visc(AdjArgs a, const Tile* __restr
   }
 }
 
-__global__ void __launch_bounds__(256) k_adj_vcell(AdjArgs a) {
+// gl: the first ghost layer of the viscous blocks (cells with one index 0 or n + 1 and the others
+// interior), the only ghost cells a tangential role reaches
+__global__ void __launch_bounds__(256) k_adj_vcell(AdjArgs a, const int64_t* __restrict__ gl, int64_t ngl) {
   const int64_t N = a.npool;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < N; c += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ngl; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = gl[e];
     int lo = 0, hi = a.nblocks - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;

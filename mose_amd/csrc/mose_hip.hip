@@ -688,6 +688,8 @@ struct mose_ctx {
   int nvbt = 0;
   Tile* d_vtiles = nullptr;                   // k_adj_visc's patches of the viscous blocks
   int nvtiles = 0;
+  int64_t* d_vghost = nullptr;                // first ghost layer of the viscous blocks (k_adj_vcell)
+  int64_t nvghost = 0;
   bool vjp_valid = false;                    // the last vjp left Pbar and the viscous buffers
   bool geo_valid = false;           // the last vjp wrote the geometry adjoints
   bool xbar_valid = false;
@@ -1183,6 +1185,23 @@ static int ensure_adjoint(mose_ctx* c) {
     }
     c->nvtiles = (int)vt.size();
     if (upload(c, &c->d_vtiles, vt)) return c->err.code;
+    std::vector<int64_t> gl;
+    for (int b = 0; b < c->nb; ++b) {
+      const BlockDev& B = c->lay.blocks[b];
+      if (!B.visc) continue;
+      const int n3[3] = {B.ni, B.nj, B.nk};
+      const int64_t st[3] = {1, (int64_t)B.sy, B.sz};
+      for (int d = 0; d < 3; ++d) {          // the two layers across axis d, the other indices interior
+        const int a1 = (d + 1) % 3, a2 = (d + 2) % 3;
+        for (int side = 0; side < 2; ++side)
+          for (int u = 1; u <= n3[a2]; ++u)
+            for (int t = 1; t <= n3[a1]; ++t)
+              gl.push_back(B.off + st[d] * (side ? n3[d] + 2 : 1) + st[a1] * (t + 1) + st[a2] * (u + 1));
+      }
+    }
+    std::sort(gl.begin(), gl.end());
+    c->nvghost = (int64_t)gl.size();
+    if (upload(c, &c->d_vghost, gl)) return c->err.code;
   }
   c->adj_ready = true;
   return 0;
@@ -1241,7 +1260,9 @@ static void enqueue_vjp(mose_ctx* c, bool geometry = false, double* U = nullptr)
   if (c->lay.visc) {   // viscous terms: interior faces (gathered per cell), then connection faces
     if (c->nvtiles > 0)
       hipLaunchKernelGGL(k_adj_visc, dim3(c->nvtiles), dim3(kVT), 0, c->stream, a, (const Tile*)c->d_vtiles);
-    hipLaunchKernelGGL(k_adj_vcell, dim3(launch_blocks(c->npool)), dim3(256), 0, c->stream, a);
+    if (c->nvghost > 0)
+      hipLaunchKernelGGL(k_adj_vcell, dim3(launch_blocks(c->nvghost)), dim3(256), 0, c->stream, a,
+                         (const int64_t*)c->d_vghost, c->nvghost);
     if (a.nvf > 0) {
       hipLaunchKernelGGL(k_adj_vbc, dim3(launch_blocks(a.nvf)), dim3(256), 0, c->stream, a);
       hipLaunchKernelGGL(k_adj_vbc_gather, dim3(launch_blocks(a.nvbt)), dim3(256), 0, c->stream, a);
@@ -2203,7 +2224,7 @@ void mose_destroy(mose_ctx* c) {
                   c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_pres, c->d_ghostp,
                   c->d_garc_cell, c->d_garc, c->d_rs1, c->d_irsg, c->d_gP, c->d_lam0, c->d_lam1, c->d_Dq,
                   c->d_apart, c->d_ares, c->d_icell, c->d_VBC, c->d_Tp, c->d_vbt, c->d_vbrow, c->d_vbent, c->d_volq,
-                  c->d_upart, c->d_ke, c->d_mu, c->d_icbeg, c->d_vfbeg, c->d_vfi, c->d_vtiles};
+                  c->d_upart, c->d_ke, c->d_mu, c->d_icbeg, c->d_vfbeg, c->d_vfi, c->d_vtiles, c->d_vghost};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
