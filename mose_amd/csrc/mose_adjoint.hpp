@@ -1030,6 +1030,8 @@ struct AdjArgs {
   double* __restrict__ VBC;             // 50 x nvf: each connection face's primitive adjoints of
                                         // its ten stencil cells, gathered by k_adj_vbc_gather
   const double* __restrict__ Tp;        // npool: temperature p / (rho R) of every cell
+  double* VPb;                          // 5 x npool: k_adj_visc's part of Pbar (interior cells of
+                                        // viscous blocks), folded in by k_adj_cell; null without
   const int64_t* __restrict__ vbt;      // cells receiving connection-face adjoints
   const int64_t* __restrict__ vbrow;    // their entries vbent[vbrow[r] .. vbrow[r+1]) (face*10 + role)
   const int* __restrict__ vbent;
@@ -1315,6 +1317,10 @@ __global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a) {
           }
         }
       }
+    }
+    if (a.VPb && B.visc && outside == 0) {     // the viscous terms' part (k_adj_visc, on a second stream)
+#pragma unroll
+      for (int v = 0; v < 5; ++v) acc[v] += a.VPb[v * N + c];
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) a.Pbar[v * N + c] = acc[v];

@@ -131,7 +131,8 @@ __device__ __forceinline__ double vface_core(const AdjArgs& a, const BlockDev& B
 // thread adds the roles its own cell plays in those faces (low / high cell: the normal gradient
 // and half the face state; the four tangential neighbours: a quarter of a tangential gradient)
 // to accumulators of planes q-1, q and q+1 in registers, in a fixed order.  Plane q-1 is then
-// complete: its (u, v, w, T) adjoints are turned into primitive ones and added to Pbar.  Each
+// complete: its (u, v, w, T) adjoints are turned into primitive ones and stored in VPb, which
+// k_adj_cell adds to the convective part (the two run on separate streams, concurrently).  Each
 // face is formed by every patch whose cells it reaches (the halo faces twice or more) but
 // written out only by the patch that holds its high cell: the mu_ref adjoint (row 17) always,
 // and the whole record for faces whose stencil reaches a ghost cell, which k_adj_vcell gathers
@@ -344,7 +345,8 @@ __global__ void __launch_bounds__(kVT) k_adj_visc(AdjArgs a, const Tile* __restr
       }
       lds_barrier();
     }
-    // ---- plane q - 1 is complete: (u, v, w, T) adjoints to primitive ones, into Pbar
+    // ---- plane q - 1 is complete: (u, v, w, T) adjoints to primitive ones, into VPb (which
+    // k_adj_cell adds to the convective part of Pbar)
     const int p = q - 1;
     if (own && p >= T.k0 && p < T.k1) {
       const int sl = (p + 3) % 3;
@@ -354,7 +356,7 @@ __global__ void __launch_bounds__(kVT) k_adj_visc(AdjArgs a, const Tile* __restr
       if (P[0] != 0.0) visc_vars_adj(P, a.R, am, Pb);
       const int64_t c = B.off + (oi + 1) + sy * (oj + 1) + sz * (int64_t)(p + 1);
 #pragma unroll
-      for (int v = 0; v < 5; ++v) a.Pbar[v * N + c] += Pb[v];
+      for (int v = 0; v < 5; ++v) a.VPb[v * N + c] = Pb[v];
     }
 #pragma unroll
     for (int v = 0; v < 6; ++v) { am[v] = a0[v]; a0[v] = ap[v]; ap[v] = 0.0; }

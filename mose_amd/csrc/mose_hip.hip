@@ -690,6 +690,9 @@ struct mose_ctx {
   int nvtiles = 0;
   int64_t* d_vghost = nullptr;                // first ghost layer of the viscous blocks (k_adj_vcell)
   int64_t nvghost = 0;
+  double* d_VPb = nullptr;                    // k_adj_visc's part of Pbar
+  hipStream_t vstream = nullptr;              // k_adj_visc runs here, beside the convective passes
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool vjp_valid = false;                    // the last vjp left Pbar and the viscous buffers
   bool geo_valid = false;           // the last vjp wrote the geometry adjoints
   bool xbar_valid = false;
@@ -1202,6 +1205,10 @@ static int ensure_adjoint(mose_ctx* c) {
     std::sort(gl.begin(), gl.end());
     c->nvghost = (int64_t)gl.size();
     if (upload(c, &c->d_vghost, gl)) return c->err.code;
+    HIPCHK(c, hipMalloc((void**)&c->d_VPb, 5 * c->npool * sizeof(double)));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->vstream, hipStreamNonBlocking));     // default priority
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   }
   c->adj_ready = true;
   return 0;
@@ -1217,6 +1224,7 @@ static AdjArgs adj_args(mose_ctx* c, const double* U = nullptr) {
   a.nents = std::max(c->nents, 1);
   a.VW = c->d_VW; a.VB = c->d_VB; a.vfaces = c->d_vfaces; a.nvf = (int)c->lay.vfaces.size();
   a.VBC = c->d_VBC; a.Tp = c->d_Tp; a.vbt = c->d_vbt; a.vbrow = c->d_vbrow; a.vbent = c->d_vbent; a.nvbt = c->nvbt;
+  a.VPb = c->d_VPb;
   a.gam = c->cfg.gamma; a.R = c->cfg.Rgas;
   return a;
 }
@@ -1251,15 +1259,26 @@ static void enqueue_vjp(mose_ctx* c, bool geometry = false, double* U = nullptr)
   }
   const int lim = c->cfg.limiter, rs = c->cfg.riemann, geo = c->lay.curvi ? 1 : 0;
   void* args[] = {&a};
+  // viscous interior faces on the second stream, concurrent with the convective face and
+  // boundary passes (k_adj_visc keeps one workgroup per CU for its LDS; the memory-bound face
+  // pass fills the rest); k_adj_cell waits for it and adds its part
+  const bool fork = c->lay.visc && c->nvtiles > 0;
+  if (fork) {
+    (void)hipEventRecord(c->ev_fork, c->stream);
+    (void)hipStreamWaitEvent(c->vstream, c->ev_fork, 0);
+    hipLaunchKernelGGL(k_adj_visc, dim3(c->nvtiles), dim3(kVT), 0, c->vstream, a, (const Tile*)c->d_vtiles);
+    (void)hipEventRecord(c->ev_join, c->vstream);
+  } else {
+    a.VPb = nullptr;
+  }
   (void)hipLaunchKernel((const void*)atable().face[lim][rs][geo], dim3(launch_blocks(c->nwork)), dim3(256), args, 0,
                         c->stream);
   if (a.nbcf > 0)
     (void)hipLaunchKernel((const void*)atable().bc[lim][rs], dim3(launch_blocks(a.nbcf)), dim3(256), args, 0, c->stream);
+  if (fork) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
   (void)hipLaunchKernel((const void*)atable().cell[lim][geo], dim3(launch_blocks(c->npool)), dim3(256), args, 0,
                         c->stream);
-  if (c->lay.visc) {   // viscous terms: interior faces (gathered per cell), then connection faces
-    if (c->nvtiles > 0)
-      hipLaunchKernelGGL(k_adj_visc, dim3(c->nvtiles), dim3(kVT), 0, c->stream, a, (const Tile*)c->d_vtiles);
+  if (c->lay.visc) {   // viscous terms: ghost cells of the interior faces, then connection faces
     if (c->nvghost > 0)
       hipLaunchKernelGGL(k_adj_vcell, dim3(launch_blocks(c->nvghost)), dim3(256), 0, c->stream, a,
                          (const int64_t*)c->d_vghost, c->nvghost);
@@ -2224,9 +2243,12 @@ void mose_destroy(mose_ctx* c) {
                   c->d_GF, c->d_GL, c->d_BG, c->d_GE, c->d_IB, c->d_VW, c->d_VB, c->d_beta, c->d_pres, c->d_ghostp,
                   c->d_garc_cell, c->d_garc, c->d_rs1, c->d_irsg, c->d_gP, c->d_lam0, c->d_lam1, c->d_Dq,
                   c->d_apart, c->d_ares, c->d_icell, c->d_VBC, c->d_Tp, c->d_vbt, c->d_vbrow, c->d_vbent, c->d_volq,
-                  c->d_upart, c->d_ke, c->d_mu, c->d_icbeg, c->d_vfbeg, c->d_vfi, c->d_vtiles, c->d_vghost};
+                  c->d_upart, c->d_ke, c->d_mu, c->d_icbeg, c->d_vfbeg, c->d_vfi, c->d_vtiles, c->d_vghost, c->d_VPb};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->vstream) (void)hipStreamDestroy(c->vstream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
