@@ -10,7 +10,8 @@ Options: --case vortex|tgv (tgv: --nbox viscous Taylor-Green boxes of n^3, van L
 --prec fp64|fp32, --riemann HLLC|AUSM+|HLLE|SLAU|SLAU2, --limiter vanleer|minmod|vanalbada,
 --autotune picks the tile shape / k chunk / tile order by timing first,
 --copy also runs a plain device-to-device copy of 5 n^3 doubles (hipMemcpy) so the HBM
-byte counters (FETCH_SIZE / WRITE_SIZE) can be calibrated against a kernel of known size.
+byte counters (FETCH_SIZE / WRITE_SIZE) can be calibrated against a kernel of known size,
+--vjp N times N residual adjoints (mose_residual_vjp) after one with Rbar = 1.
 """
 import argparse
 import json
@@ -30,6 +31,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--copy", action="store_true", help="also time a plain copy kernel of 5 n^3 doubles")
     ap.add_argument("--autotune", action="store_true", help="run the tile autotuner first")
+    ap.add_argument("--vjp", type=int, default=0,
+                    help="also time N residual adjoints (mose_residual_vjp, Rbar = 1 on every interior cell)")
     args = ap.parse_args()
     from mose_amd import abi, cases
     from mose_amd.lib import GpuSolver
@@ -54,6 +57,13 @@ def main():
            "steps": args.steps, "s_steps": dt,
            "stage_GBs_algorithmic": 120.0 * case.ncells / (ms[2] * 1e-3) / 1e9, "tiling_in_use": g.tiling(),
            "tiling": tuned[0] if tuned else None}
+    if args.vjp > 0:
+        import numpy as np
+        Rb = [np.pad(np.ones(tuple(b.dims[::-1]) + (5,)), ((2, 2), (2, 2), (2, 2), (0, 0))) for b in case.blocks]
+        g.residual_vjp(Rb)
+        res_ms, vjp_ms = g.time_residual_vjp(args.vjp)[:2]
+        out["vjp_residual_ms"], out["vjp_ms"], out["vjp_ratio"] = res_ms, vjp_ms, vjp_ms / res_ms
+        print("[kernelbench] residual %.3f ms, vjp %.3f ms, ratio %.2f" % (res_ms, vjp_ms, vjp_ms / res_ms), flush=True)
     if args.copy:
         import ctypes as C
         hip = C.CDLL("libamdhip64.so")
