@@ -149,3 +149,39 @@ def test_viscous_adjoint_is_deterministic():
     for x, y in zip(a, b_):
         assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
     assert mua == mub
+
+
+# Boxes of 36^3: three 16-wide patches of the interior-face kernel along i and j (the last one
+# partial) and two 32-plane chunks along k, so faces on patch and chunk edges (formed by every
+# patch they reach, written out by one), partial patches and the first ghost layer are exercised.
+def test_viscous_state_adjoint_several_patches():
+    _state_check(_tgv(n=36, nbox=2, seed=21), seed=5)
+
+
+def test_viscous_state_adjoint_several_patches_oracle_residual():
+    _state_check(_tgv(n=36, nbox=1, seed=22), solver_cls=OracleSolver, seed=6)
+
+
+def test_mu_ref_adjoint_several_patches():
+    case = _tgv(n=36, nbox=1, seed=23)
+    rng = np.random.default_rng(8)
+    Rbar = [rng.standard_normal(abi.padded_shape(b.dims)) * _mask(b.dims) for b in case.blocks]
+    g = GpuSolver(case)
+    g.residual_vjp(Rbar)
+    grad = g.viscous_bar(1)
+    a = g.residual_vjp(Rbar)
+    again = g.viscous_bar(1)
+    b_ = g.residual_vjp(Rbar)
+    g.close()
+    assert grad == again
+    for x, y in zip(a, b_):
+        assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    h = 1e-6 * case.meta["viscous"][0][1][0]
+
+    def with_mu(dmu):
+        c = copy.deepcopy(case)
+        c.meta["viscous"] = [(bb, [p[0] + dmu] + list(p[1:])) for bb, p in c.meta["viscous"]]
+        return c
+
+    fd = (_J(with_mu(h), Rbar, GpuSolver) - _J(with_mu(-h), Rbar, GpuSolver)) / (2 * h)
+    assert abs(grad - fd) <= 1e-6 * abs(fd) + 1e-10, (grad, fd)
