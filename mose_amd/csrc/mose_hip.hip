@@ -657,6 +657,8 @@ struct mose_ctx {
   int* d_iter = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph_n = nullptr;               // kStepsPerGraph steps in one graph (mose_advance)
+  hipGraphExec_t gexec_n = nullptr;
   // discrete adjoint (allocated on first use)
   bool adj_ready = false;
   // steady adjoint solve (allocated on first use)
@@ -955,20 +957,29 @@ static int sync_and_check(mose_ctx* c) {
   return 0;
 }
 
-static int ensure_graph(mose_ctx* c) {
-  if (c->gexec) return 0;
+// Graphs of one step and of kStepsPerGraph steps (the same kernels in the same order, so a run
+// of steps gives the same bits either way; the longer graph saves host launches on small cases,
+// whose steps are a few microseconds of work each).
+constexpr int kStepsPerGraph = 8;
+static int capture_steps(mose_ctx* c, int nsteps, hipGraph_t* graph, hipGraphExec_t* exec) {
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_step(c);
+  for (int s = 0; s < nsteps; ++s) enqueue_step(c);
   const hipError_t le = hipGetLastError();
-  HIPCHK(c, hipStreamEndCapture(c->stream, &c->graph));
+  HIPCHK(c, hipStreamEndCapture(c->stream, graph));
   if (le != hipSuccess) {
     fprintf(stderr, "[mose_hip] kernel launch failed during step capture: %s\n", hipGetErrorString(le));
-    (void)hipGraphDestroy(c->graph);
-    c->graph = nullptr;
+    (void)hipGraphDestroy(*graph);
+    *graph = nullptr;
     return fail(c, MOSE_E_HIP);
   }
-  HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  HIPCHK(c, hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
   return 0;
+}
+
+static int ensure_graph(mose_ctx* c) {
+  if (c->gexec) return 0;
+  if (capture_steps(c, 1, &c->graph, &c->gexec)) return c->err.code;
+  return capture_steps(c, kStepsPerGraph, &c->graph_n, &c->gexec_n);
 }
 
 template <typename V>
@@ -1056,8 +1067,12 @@ static int retile(mose_ctx* c, int shape, int kchunk, int xcd) {
     if (StageFn fn = stage_fn(c, m)) (void)resident_per_cu(c, fn);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->gexec_n) (void)hipGraphExecDestroy(c->gexec_n);
+  if (c->graph_n) (void)hipGraphDestroy(c->graph_n);
   c->gexec = nullptr;
   c->graph = nullptr;
+  c->gexec_n = nullptr;
+  c->graph_n = nullptr;
   if (c->agexec) (void)hipGraphExecDestroy(c->agexec);
   if (c->agraph) (void)hipGraphDestroy(c->agraph);
   c->agexec = nullptr;
@@ -1575,7 +1590,9 @@ int mose_advance(mose_ctx* c, int32_t nsteps, double* time_out, double* norms) {
     enqueue_prime_dt(c);
     c->dt_primed = true;
   }
-  for (int s = 0; s < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  int s = 0;
+  for (; s + kStepsPerGraph <= nsteps; s += kStepsPerGraph) HIPCHK(c, hipGraphLaunch(c->gexec_n, c->stream));
+  for (; s < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
   int rc = sync_and_check(c);
   double sc[S_COUNT];
   HIPCHK(c, hipMemcpy(sc, c->d_scal, sizeof sc, hipMemcpyDeviceToHost));
@@ -2235,6 +2252,8 @@ void mose_destroy(mose_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->gexec_n) (void)hipGraphExecDestroy(c->gexec_n);
+  if (c->graph_n) (void)hipGraphDestroy(c->graph_n);
   if (c->agexec) (void)hipGraphExecDestroy(c->agexec);
   if (c->agraph) (void)hipGraphDestroy(c->agraph);
   void* bufs[] = {c->U[0], c->U[1], c->U[2], c->Rbuf, c->stage, c->d_blocks, c->d_lines, c->d_geo, c->d_tiles,
