@@ -129,6 +129,24 @@ def test_advance_equals_steps():
     assert g1.time() == g2.time()
 
 
+@pytest.mark.parametrize("local_dt_irs", [False, True])
+def test_advance_multi_step_graphs_equal_steps(local_dt_irs):
+    """mose_advance launches eight-step graphs and single steps for the rest: 21 steps through
+    it (two eight-step graphs, five single ones) give the bits, time and iteration count of 21
+    single steps, time-accurate and with local time steps and residual smoothing."""
+    case = perturb(cases.vortex(dims=(16, 12, 8)), seed=9)
+    if local_dt_irs:
+        case.cfg = abi.make_cfg(cases.GAMMA, cases.RGAS_NONDIM, 0.8, "vanleer", "HLLC", 3, False, irs_beta=0.5)
+    g1, g2 = GpuSolver(case), GpuSolver(case)
+    for _ in range(21):
+        g1.step()
+    g2.advance(21)
+    assert np.array_equal(g1.get_state(1), g2.get_state(1))
+    assert g1.time() == g2.time() and g2.time()[1] == 21
+    g1.close()
+    g2.close()
+
+
 def test_sod_exact_solution():
     case = cases.sod(n=1000)
     g = GpuSolver(case)
