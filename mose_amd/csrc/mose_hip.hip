@@ -1276,13 +1276,19 @@ static void enqueue_vjp(mose_ctx* c, bool geometry = false, double* U = nullptr)
   void* args[] = {&a};
   // viscous interior faces on the second stream, concurrent with the convective face and
   // boundary passes (k_adj_visc keeps one workgroup per CU for its LDS; the memory-bound face
-  // pass fills the rest); k_adj_cell waits for it and adds its part
-  const bool fork = c->lay.visc && c->nvtiles > 0;
+  // pass fills the rest); k_adj_cell waits for it and adds its part.  While the stream is being
+  // captured into a graph (the steady adjoint iteration) the kernel runs in order on it instead.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(c->stream, &cap);
+  const bool visc = c->lay.visc && c->nvtiles > 0;
+  const bool fork = visc && cap == hipStreamCaptureStatusNone;
   if (fork) {
     (void)hipEventRecord(c->ev_fork, c->stream);
     (void)hipStreamWaitEvent(c->vstream, c->ev_fork, 0);
     hipLaunchKernelGGL(k_adj_visc, dim3(c->nvtiles), dim3(kVT), 0, c->vstream, a, (const Tile*)c->d_vtiles);
     (void)hipEventRecord(c->ev_join, c->vstream);
+  } else if (visc) {
+    hipLaunchKernelGGL(k_adj_visc, dim3(c->nvtiles), dim3(kVT), 0, c->stream, a, (const Tile*)c->d_vtiles);
   } else {
     a.VPb = nullptr;
   }
