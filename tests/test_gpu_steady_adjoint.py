@@ -129,6 +129,32 @@ def test_height_gradient_against_oracle_differences():
     assert abs(adj - fd) <= TOL * abs(fd)
 
 
+def test_height_gradient_128x64_16_controls():
+    """A larger grid (128 x 64, 16 control points): the adjoint gradient along a random height
+    direction against central differences (step 1e-7) of re-converged GPU states."""
+    nc, eps = 16, 1e-7
+    h0 = np.full(nc, 0.002)
+    mk = lambda h: cases.bump_channel(ni=128, nj=64, ncontrol=nc, heights=h, limiter="minmod", alpha=ALPHA0)
+    case = mk(h0)
+    g = GpuSolver(case)
+    steps, norms = sa.converge(g, max_steps=150000, tol=5e-15, chunk=1000)
+    assert np.max(norms) < 1e-13, (steps, norms)
+    res = sa.gradient(case, g, rtol=1e-12, check_every=500, max_iter=400000)
+    P = g.get_state(1)
+    g.close()
+    v = np.random.default_rng(13).standard_normal(nc)
+    Js = []
+    for sgn in (1.0, -1.0):
+        J, steps, norms = sa.objective_at(mk(h0 + sgn * eps * v), GpuSolver, warm=P, max_steps=150000, tol=5e-15,
+                                          chunk=1000)
+        assert np.max(norms) < 1e-13, (steps, norms)
+        Js.append(J)
+    fd = (Js[0] - Js[1]) / (2 * eps)
+    adj = float(res["dJdh"] @ v)
+    print("128x64, dJ/dh . v: adjoint %.10e  fd %.10e" % (adj, fd))
+    assert abs(adj - fd) <= TOL * abs(fd)
+
+
 def test_adjoint_solve_refuses_bad_arguments():
     case = _case(16, 8)
     g = GpuSolver(case)
