@@ -110,15 +110,17 @@ def test_device_march_adjoint_checkpoint_stride_invariance():
         assert np.array_equal(a, r)
 
 
-def test_device_march_adjoint_reynolds_against_central_differences():
-    """dKE/dmu_ref of each of two 12^3 boxes over 16 steps (checkpoint stride 5) against central
-    differences of the device march; d/dRe = -mu_ref^2 d/dmu_ref."""
-    case = cases.taylor_green(n=12, nbox=2, mach=0.2, precision=abi.PREC_FP64)
+@pytest.mark.parametrize("n,nsteps,stride", [(12, 16, 5), (36, 12, 4)])
+def test_device_march_adjoint_reynolds_against_central_differences(n, nsteps, stride):
+    """dKE/dmu_ref of each of two n^3 boxes over nsteps steps (checkpoint stride) against central
+    differences of the device march; d/dRe = -mu_ref^2 d/dmu_ref.  (36^3: several patches of
+    the viscous adjoint's interior-face kernel and two k chunks.)"""
+    case = cases.taylor_green(n=n, nbox=2, mach=0.2, precision=abi.PREC_FP64)
     case.meta["viscous"] = [(b, [0.02 * b] + list(p[1:])) for b, p in case.meta["viscous"]]
     case = perturb(case, amp=0.01, seed=7)
     g = GpuSolver(case)
     dt = 0.5 * g.compute_dt()
-    ke, mub, _ = g.march_kinetic_energy(16, dt, stride=5)
+    ke, mub, _ = g.march_kinetic_energy(nsteps, dt, stride=stride)
     g.close()
     for b in (1, 2):
         mu = case.meta["viscous"][b - 1][1][0]
@@ -128,7 +130,7 @@ def test_device_march_adjoint_reynolds_against_central_differences():
             c = copy.deepcopy(case)
             c.meta["viscous"] = [(bb, [p[0] + (sgn * h if bb == b else 0.0)] + list(p[1:])) for bb, p in c.meta["viscous"]]
             gg = GpuSolver(c)
-            kes.append(float(np.sum(gg.march_kinetic_energy(16, dt, adjoint=False))))
+            kes.append(float(np.sum(gg.march_kinetic_energy(nsteps, dt, adjoint=False))))
             gg.close()
         fd = (kes[0] - kes[1]) / (2 * h)
         print("box %d: dKE/dmu_ref adjoint %.10e fd %.10e; dKE/dRe %.6e" % (b, mub[b - 1], fd, -mu * mu * mub[b - 1]))
