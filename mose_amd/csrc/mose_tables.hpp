@@ -18,7 +18,7 @@ constexpr int NRS = 7;      // HLLC, AUSM+, HLLE, SLAU, SLAU2, HLLE++, HLLC+
 // viscous blocks shape 2 (128 threads: half the LDS of the others), the last three for the
 // solvers without the shock detector.
 constexpr int NSHAPE = 4;
-constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 16}, {8, 16}, {32, 16}}, {{32, 8}, {16, 16}, {16, 8}, {32, 16}}};
+constexpr int kShape[2][NSHAPE][2] = {{{32, 8}, {16, 16}, {16, 8}, {32, 16}}, {{32, 8}, {16, 16}, {16, 8}, {32, 16}}};
 constexpr bool shapes_within_bound() {
   for (int p = 0; p < 2; ++p)
     for (int s = 0; s < NSHAPE; ++s) {
